@@ -1,0 +1,152 @@
+// Oracle (test infrastructure only): a flat C ABI over the restatement so the
+// Python tests can call it through ctypes.  Output buffers are malloc'ed and
+// released with orc_free.
+#include <cstdlib>
+#include <cstring>
+
+#include "oracle.hpp"
+
+extern "C" {
+
+struct orc_err {
+  int kind;
+  size_t offset;
+  char message[512];
+};
+
+// One nd_lift step as four ints: dimension, kind (0 delta, 1 haar, 2 lift53),
+// levels, group.
+}
+
+namespace {
+
+template <class F> int guard(orc_err* err, F&& f) {
+  try {
+    f();
+    if (err) { err->kind = 0; err->offset = 0; err->message[0] = 0; }
+    return 0;
+  } catch (orc::Error& e) {
+    if (err) {
+      err->kind = e.kind;
+      err->offset = e.offset;
+      std::strncpy(err->message, e.message.c_str(), sizeof(err->message) - 1);
+      err->message[sizeof(err->message) - 1] = 0;
+    }
+    return e.kind;
+  }
+}
+
+void emit(const std::vector<uint8_t>& v, uint8_t** out, size_t* out_len) {
+  *out = (uint8_t*)std::malloc(v.size() ? v.size() : 1);
+  if (!v.empty()) std::memcpy(*out, v.data(), v.size());
+  *out_len = v.size();
+}
+
+std::vector<orc::AxisStep> steps_of(const int* s, size_t n) {
+  std::vector<orc::AxisStep> v;
+  for (size_t i = 0; i < n; ++i)
+    v.push_back({"", (size_t)s[4 * i], s[4 * i + 1], (unsigned)s[4 * i + 2], (uint32_t)s[4 * i + 3]});
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_free(void* p) { std::free(p); }
+
+int orc_lift_chunk(int forward, const uint8_t* in, size_t len, const size_t* shape, size_t ndim, int dtype,
+                   const char* version, const int* steps, size_t nsteps, uint8_t** out, size_t* out_len, orc_err* err) {
+  return guard(err, [&] {
+    std::vector<size_t> sh(shape, shape + ndim);
+    auto st = steps_of(steps, nsteps);
+    emit(forward ? orc::lift_forward_chunk(in, len, sh, dtype, version, st)
+                 : orc::lift_inverse_chunk(in, len, sh, dtype, version, st),
+         out, out_len);
+  });
+}
+
+int orc_lift_plane_i64(int forward, int64_t* data, const size_t* shape, size_t ndim, const int* steps, size_t nsteps,
+                       orc_err* err) {
+  return guard(err, [&] {
+    size_t n = 1;
+    for (size_t i = 0; i < ndim; ++i) n *= shape[i];
+    std::vector<int64_t> v(data, data + n);
+    std::vector<size_t> sh(shape, shape + ndim);
+    auto st = steps_of(steps, nsteps);
+    if (forward) orc::lift_forward_i64(v, sh, st);
+    else orc::lift_inverse_i64(v, sh, st);
+    std::memcpy(data, v.data(), n * 8);
+  });
+}
+
+int orc_dwt53(int forward, int32_t* plane, size_t width, size_t height, unsigned levels, orc_err* err) {
+  return guard(err, [&] {
+    if (forward) orc::dwt53_forward(plane, width, height, levels);
+    else orc::dwt53_inverse(plane, width, height, levels);
+  });
+}
+
+void orc_ht_tables(uint16_t* dec0, uint16_t* dec1, uint16_t* enc0, uint16_t* enc1, uint16_t* uvlc0, uint16_t* uvlc1) {
+  const orc::HtTables& t = orc::ht_tables();
+  std::memcpy(dec0, t.dec0, sizeof(t.dec0));
+  std::memcpy(dec1, t.dec1, sizeof(t.dec1));
+  std::memcpy(enc0, t.enc0, sizeof(t.enc0));
+  std::memcpy(enc1, t.enc1, sizeof(t.enc1));
+  std::memcpy(uvlc0, t.uvlc0, sizeof(t.uvlc0));
+  std::memcpy(uvlc1, t.uvlc1, sizeof(t.uvlc1));
+}
+
+int orc_ht_encode_block(const uint32_t* buf, size_t width, size_t height, uint32_t missing_msbs, uint8_t** out,
+                        size_t* out_len, orc_err* err) {
+  return guard(err, [&] { emit(orc::ht_encode_block(buf, width, height, width, missing_msbs), out, out_len); });
+}
+
+int orc_ht_decode_block(const uint8_t* coded, size_t coded_len, uint32_t* out, size_t width, size_t height,
+                        uint32_t missing_msbs, uint32_t num_passes, size_t len1, size_t len2, int stripe_causal,
+                        orc_err* err) {
+  return guard(err, [&] {
+    orc::ht_decode_block(coded, coded_len, out, width, height, width, missing_msbs, num_passes, len1, len2,
+                         stripe_causal != 0);
+  });
+}
+
+int orc_encode_image(const int32_t* samples, size_t width, size_t height, unsigned depth, int is_signed,
+                     unsigned levels, uint8_t** out, size_t* out_len, orc_err* err) {
+  return guard(err, [&] {
+    emit(orc::encode_image_with_depth(samples, width, height, depth, is_signed != 0, levels, 2, true), out, out_len);
+  });
+}
+
+int orc_decode_codestream(const uint8_t* data, size_t len, int max_res, int prefix, int32_t** out, size_t* width,
+                          size_t* height, orc_err* err) {
+  return guard(err, [&] {
+    orc::Decoded d = orc::decode_codestream(data, len, max_res, prefix != 0);
+    *out = (int32_t*)std::malloc(d.samples.size() * 4 + 4);
+    std::memcpy(*out, d.samples.data(), d.samples.size() * 4);
+    *width = d.width;
+    *height = d.height;
+  });
+}
+
+int orc_htj2k_encode_chunk(const uint8_t* in, size_t len, const size_t* shape, size_t ndim, int dtype,
+                           unsigned xy_levels, int reversible, int progression, int index, uint8_t** out,
+                           size_t* out_len, orc_err* err) {
+  return guard(err, [&] {
+    orc::Htj2kConfig c;
+    c.xy_levels = xy_levels;
+    c.reversible = reversible != 0;
+    c.progression = progression;
+    c.index = index != 0;
+    emit(orc::htj2k_encode_chunk(in, len, std::vector<size_t>(shape, shape + ndim), dtype, c), out, out_len);
+  });
+}
+
+int orc_htj2k_decode_chunk(const uint8_t* in, size_t len, const size_t* shape, size_t ndim, int dtype, uint8_t** out,
+                           size_t* out_len, orc_err* err) {
+  return guard(err, [&] {
+    emit(orc::htj2k_decode_chunk(in, len, std::vector<size_t>(shape, shape + ndim), dtype), out, out_len);
+  });
+}
+
+}  // extern "C"
