@@ -1,0 +1,281 @@
+#!/usr/bin/env python3
+"""Benchmark: bit-exact nd-lift-ht decode throughput on one MI355X.
+
+Workload (default): the uneven batch of 4,096 chunks -- plane y, x
+log-uniform in [64, 1024], z in 1..64, dtype u8:u16:i16 = 2:5:3, nd_lift kind
+delta/haar/lift53 along z (2 levels), DWT levels 0..5, blob + Poisson content
+scaled to the dtype range, 5% all-zero and 5% uniform-random chunks.  Samples
+are generated on the GPU (seeded), encoded once on the GPU, then decoded K
+times in the timed region with every chunk of the batch in one launch
+sequence.  The plain one-chunk-per-launch path is timed beside it.  Every
+decode is checked bit-exact against the original samples.
+
+Prints one JSON line (see the repository docs for the fields).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import importlib
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def gen_chunk(torch, rng, shape, dtype, kind_of_content, dev):
+    """Blob + Poisson content of config 0 (12-bit) scaled to the dtype range."""
+    z, y, x = shape
+    if kind_of_content == "zero":
+        return torch.zeros(shape, dtype=torch.int32, device=dev)
+    info = np.iinfo(dtype)
+    if kind_of_content == "random":
+        g = torch.Generator(device=dev)
+        g.manual_seed(int(rng.integers(0, 2**31)))
+        return torch.randint(int(info.min), int(info.max) + 1, shape, generator=g, device=dev, dtype=torch.int64).to(torch.int32)
+    vol = torch.full(shape, 1000.0, dtype=torch.float32, device=dev)
+    zz = torch.arange(z, device=dev, dtype=torch.float32)
+    yy = torch.arange(y, device=dev, dtype=torch.float32)
+    xx = torch.arange(x, device=dev, dtype=torch.float32)
+    for _ in range(8):
+        cz, cy, cx = rng.uniform(0, 1, 3) * np.array([z, y, x])
+        a = float(rng.uniform(200, 2500))
+        s = float(rng.uniform(4, 20))
+        gz = torch.exp(-(zz - float(cz)) ** 2 / (2 * s * s))
+        gy = torch.exp(-(yy - float(cy)) ** 2 / (2 * s * s))
+        gx = torch.exp(-(xx - float(cx)) ** 2 / (2 * s * s))
+        vol += a * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(rng.integers(0, 2**31)))
+    vol = torch.poisson(vol, generator=g)
+    v12 = torch.clamp(torch.round(vol), 0, 4095).to(torch.int32)
+    if dtype == np.uint8:
+        return v12 >> 4
+    if dtype == np.uint16:
+        return v12 << 4
+    return (v12 << 4) - 32768
+
+
+def torch_dtype(torch, dt):
+    return {np.uint8: torch.uint8, np.uint16: torch.int16, np.int16: torch.int16}[dt]
+
+
+def make_workload(torch, ndic, args, dev):
+    rng = np.random.default_rng(args.seed)
+    kinds = ["delta", "haar", "lift53"]
+    specs = []
+    for i in range(args.chunks):
+        if args.workload == "c2":
+            y = int(round(math.exp(rng.uniform(math.log(64), math.log(1024)))))
+            x = int(round(math.exp(rng.uniform(math.log(64), math.log(1024)))))
+            z = int(rng.integers(1, 65))
+            dt = rng.choice([np.uint8, np.uint16, np.int16], p=[0.2, 0.5, 0.3])
+            kind = kinds[int(rng.integers(0, 3))]
+            lv = int(rng.integers(0, 6))
+            r = rng.uniform()
+            content = "zero" if r < 0.05 else ("random" if r < 0.10 else "blobs")
+            steps = [{"dimension": 0, "kind": kind, "levels": 2, "group": 0}]
+            shape = (z, y, x)
+        else:  # c0: the 32x256x256 u16 chunk of the CPU-path parity config
+            dt, lv, content = np.uint16, 5, "blobs"
+            steps = [{"dimension": 0, "kind": "lift53", "levels": 2, "group": 0}]
+            shape = (32, 256, 256)
+        specs.append((shape, dt, steps, lv, content))
+    sizes = [int(np.prod(s[0])) * np.dtype(s[1]).itemsize for s in specs]
+    raw_total = sum(sizes)
+    padded = sum((n + 15) // 16 * 16 for n in sizes)
+    log(f"workload {args.workload}: {len(specs)} chunks, {raw_total / 1e9:.3f} GB raw")
+    d_raw = torch.zeros(padded, dtype=torch.uint8, device=dev)
+    off = 0
+    offsets = []
+    t0 = time.time()
+    for shape, dt, steps, lv, content in specs:
+        v = gen_chunk(torch, rng, shape, dt, content, dev)
+        nb = int(np.prod(shape)) * np.dtype(dt).itemsize
+        if dt == np.uint8:
+            d_raw[off:off + nb] = v.to(torch.uint8).reshape(-1)
+        else:
+            d_raw[off:off + nb].view(torch.int16).copy_(v.to(torch.int16).reshape(-1))
+        offsets.append(off)
+        off += (nb + 15) // 16 * 16
+    torch.cuda.synchronize()
+    log(f"generated in {time.time() - t0:.1f}s")
+    # encode on the GPU
+    cap = int(raw_total * 2.5) + (256 << 20)
+    d_enc = torch.empty(cap, dtype=torch.uint8, device=dev)
+    edescs = (ndic.ChunkDesc * len(specs))(*[
+        ndic.make_desc(s[0], np.dtype(s[1]).name, s[2], s[3], offsets[i], 0, 0) for i, s in enumerate(specs)])
+    t0 = time.time()
+    ndic.encode_batch(d_raw.data_ptr(), d_enc.data_ptr(), cap, edescs)
+    torch.cuda.synchronize()
+    enc_s = time.time() - t0
+    comp_total = sum(d.out_len for d in edescs)
+    log(f"encoded in {enc_s:.1f}s: {comp_total / 1e9:.3f} GB compressed ({comp_total / raw_total:.3f} of raw)")
+    ddescs = (ndic.ChunkDesc * len(specs))(*[
+        ndic.make_desc(s[0], np.dtype(s[1]).name, s[2], s[3], edescs[i].out_offset, edescs[i].out_len, offsets[i])
+        for i, s in enumerate(specs)])
+    return specs, d_raw, d_enc, ddescs, raw_total, comp_total, enc_s, padded
+
+
+def dev_hooks(ndic):
+    lib = ndic.lib()
+    lib.ndic_dev_profile.argtypes = [C.c_int32]
+    lib.ndic_dev_profile.restype = None
+    lib.ndic_dev_stage_times.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_size_t]
+    lib.ndic_dev_stage_times.restype = C.c_size_t
+    return lib
+
+
+def cpu_baseline(d_enc, ddescs, specs, budget_s):
+    """The oracle (single-threaded C++ restatement of the reference path)
+    decoding chunks of the same workload until the time budget is spent."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    enc = None
+    done_bytes, t_used, n = 0, 0.0, 0
+    order = np.random.default_rng(1).permutation(len(specs))
+    for i in order:
+        d = ddescs[int(i)]
+        shape, dt, steps, lv, content = specs[int(i)]
+        chunk = d_enc[d.in_offset:d.in_offset + d.in_len].cpu().numpy().tobytes()
+        code = {np.uint8: "u8", np.uint16: "u16", np.int16: "i16"}[dt]
+        t0 = time.perf_counter()
+        out = oracle.lift_ht_decode(chunk, shape, code, steps)
+        t_used += time.perf_counter() - t0
+        done_bytes += out.nbytes
+        n += 1
+        if t_used > budget_s:
+            break
+    return {"value": round(done_bytes / t_used / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"{n} chunks of the same workload ({done_bytes / 1e6:.1f} MB decoded) by the C++ oracle, "
+                      f"one thread, {t_used:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c0"])
+    ap.add_argument("--chunks", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X (got --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "c0":
+        args.chunks = min(args.chunks, 64)
+    import torch
+
+    dev = torch.device("cuda:0")
+    ndic = importlib.import_module("nd-image-codecs_amd")
+    lib = dev_hooks(ndic)
+    specs, d_raw, d_enc, ddescs, raw_total, comp_total, enc_s, padded = make_workload(torch, ndic, args, dev)
+    d_out = torch.zeros(padded, dtype=torch.uint8, device=dev)
+
+    def run(plain=False):
+        ndic.decode_batch(d_enc.data_ptr(), d_out.data_ptr(), ddescs, plain=plain)
+
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    exact = bool(torch.equal(d_out, d_raw))
+    log(f"bit-exact after warmup: {exact}")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    gbps = raw_total / dt / 1e9
+    exact &= bool(torch.equal(d_out, d_raw))
+    log(f"batched decode: {dt * 1e3:.1f} ms/step, {gbps:.2f} GB/s, bit-exact {exact}")
+    # plain path
+    d_out.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        run(plain=True)
+    torch.cuda.synchronize()
+    pdt = (time.perf_counter() - t0) / args.plain_steps
+    plain_gbps = raw_total / pdt / 1e9
+    exact &= bool(torch.equal(d_out, d_raw))
+    log(f"plain decode: {pdt * 1e3:.1f} ms/step, {plain_gbps:.2f} GB/s")
+    # stage profile (one extra pass, HIP events on the library's stream)
+    lib.ndic_dev_profile(1)
+    run()
+    torch.cuda.synchronize()
+    ms = (C.c_double * 5)()
+    by = (C.c_double * 5)()
+    lib.ndic_dev_stage_times(ms, by, 5)
+    lib.ndic_dev_profile(0)
+    names = ["parse", "ht_decode", "dwt_inverse", "dc_unshift", "lift_inverse_narrow"]
+    stages = {names[i]: {"ms": round(ms[i], 3), "GBps": round(by[i] / ms[i] / 1e6, 1) if ms[i] > 0 else None}
+              for i in range(5)}
+    dom = max(range(5), key=lambda i: ms[i])
+    achieved = by[dom] / ms[dom] / 1e6 if ms[dom] > 0 else 0.0
+    log("stages:", json.dumps(stages))
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(d_enc, ddescs, specs, args.cpu_seconds)
+        log("cpu baseline:", cpu)
+    line = {
+        "metric": "decoded GB/s lossless uint16 nd-lift-ht (bit-exact) vs plain GPU; p50/p99 latency",
+        "value": round(gbps, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded blobs + Poisson noise, generated on the GPU)",
+        "bit_exact": exact,
+        "config": {
+            "workload": ("c2: 4096 uneven chunks, y/x log-uniform 64..1024, z 1..64, u8:u16:i16 2:5:3, "
+                         "nd_lift delta/haar/lift53 along z, DWT levels 0..5, 64x64 HT code-blocks")
+            if args.workload == "c2" else "c0: u16 32x256x256, lift53 z L2, DWT L5",
+            "chunks": len(specs),
+            "raw_bytes": raw_total,
+            "compressed_bytes": comp_total,
+            "plain_gbps": round(plain_gbps, 3),
+            "batched_over_plain": round(gbps / plain_gbps, 2) if plain_gbps > 0 else None,
+            "encode_seconds": round(enc_s, 2),
+            "stages": stages,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": names[dom],
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

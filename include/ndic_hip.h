@@ -1,0 +1,149 @@
+/*
+ * ndic_hip.h — C ABI of the MI355X-native nd-lift-ht / nd-zfp codec path.
+ *
+ * Every entry point takes plain pointers and sizes.  The single-chunk calls
+ * mirror the reference's feature-free chunk cores, which every ecosystem
+ * (zarrs plugins, PyO3, WASM) calls; a binding swaps the Rust core for these
+ * functions and keeps its codec metadata, stream formats and errors.
+ *
+ * Errors mirror ndic_core::Error (crates/ndic-core/src/error.rs:14-43): the
+ * status is the variant, ndic_error carries the byte offset (Codestream) and
+ * the message.  Output buffers are allocated by the library and released with
+ * ndic_buffer_free.  All calls are safe to make from several host threads; the
+ * library serialises device work onto its own HIP stream.
+ */
+#ifndef NDIC_HIP_H
+#define NDIC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  NDIC_OK = 0,
+  NDIC_INVALID_ARGUMENT = 1, /* Error::InvalidArgument */
+  NDIC_CODESTREAM = 2,       /* Error::Codestream { offset } */
+  NDIC_UNSUPPORTED = 3,      /* Error::Unsupported */
+  NDIC_IO = 4,               /* Error::Io (device/runtime failures) */
+} ndic_status;
+
+typedef struct {
+  int32_t status;
+  uint64_t offset;
+  char message[256];
+} ndic_error;
+
+typedef struct {
+  uint8_t* data;
+  size_t len;
+} ndic_buffer;
+
+/* Element types: SampleType (crates/ndic-core/src/dtype.rs:12) and LiftDtype
+ * (crates/ndic-zarr/src/lift.rs:18-35) share these codes. */
+typedef enum {
+  NDIC_U8 = 0, NDIC_I8 = 1, NDIC_U16 = 2, NDIC_I16 = 3,
+  NDIC_U32 = 4, NDIC_I32 = 5, NDIC_U64 = 6, NDIC_I64 = 7,
+  NDIC_F32 = 8, NDIC_F64 = 9,
+} ndic_dtype;
+
+/* One nd_lift step: AxisTransform (crates/ndic-lift/src/lib.rs:82-101).
+ * kind: 0 delta, 1 haar, 2 lift53.  group 0 = whole extent. */
+typedef struct {
+  uint32_t dimension;
+  uint32_t kind;
+  uint32_t levels;
+  uint32_t group;
+} ndic_lift_step;
+
+/* Htj2kConfig (crates/ndic-zarr/src/htj2k.rs:22-38).  progression: 0 LRCP,
+ * 1 RLCP, 2 RPCL, 3 PCRL, 4 CPRL. */
+typedef struct {
+  uint32_t xy_levels;
+  int32_t reversible;
+  int32_t progression;
+  int32_t index;
+} ndic_htj2k_config;
+
+void ndic_buffer_free(ndic_buffer* buf);
+/* Library version string. */
+const char* ndic_version(void);
+
+/* Replaces ndic_zarr::lift::forward_chunk (crates/ndic-zarr/src/lift.rs:323):
+ * little-endian dtype elements in C order -> int32 (int64 for 64-bit dtypes)
+ * coefficient plane, with the encode-time overflow budget. */
+ndic_status ndic_lift_forward_chunk(const uint8_t* bytes, size_t len, const size_t* shape, size_t ndim,
+                                    ndic_dtype dtype, const char* version, const ndic_lift_step* steps,
+                                    size_t nsteps, ndic_buffer* out, ndic_error* err);
+
+/* Replaces ndic_zarr::lift::inverse_chunk (crates/ndic-zarr/src/lift.rs:339):
+ * coefficient plane -> dtype elements, refusing coefficients that do not
+ * narrow back. */
+ndic_status ndic_lift_inverse_chunk(const uint8_t* bytes, size_t len, const size_t* shape, size_t ndim,
+                                    ndic_dtype dtype, const char* version, const ndic_lift_step* steps,
+                                    size_t nsteps, ndic_buffer* out, ndic_error* err);
+
+/* Replaces ndic_zarr::htj2k::encode_chunk (crates/ndic-zarr/src/htj2k.rs:286):
+ * chunk bytes -> ndht container of per-plane Part 1/15 codestreams. */
+ndic_status ndic_htj2k_encode_chunk(const uint8_t* bytes, size_t len, const size_t* shape, size_t ndim,
+                                    ndic_dtype dtype, const ndic_htj2k_config* config, ndic_buffer* out,
+                                    ndic_error* err);
+
+/* Replaces ndic_zarr::htj2k::decode_chunk (crates/ndic-zarr/src/htj2k.rs:417). */
+ndic_status ndic_htj2k_decode_chunk(const uint8_t* chunk, size_t len, const size_t* shape, size_t ndim,
+                                    ndic_dtype dtype, ndic_buffer* out, ndic_error* err);
+
+/* Replaces Codestream::parse / parse_prefix + decode_to_resolution
+ * (crates/ndic-codestream/src/reader.rs:103-117, 478): decodes resolutions
+ * 0..=max_res (max_res < 0: all) of one plane codestream, or of a byte-range
+ * prefix of it (prefix != 0), to int32 samples (width*height of the reduced
+ * image, DC level shift undone for unsigned planes).  The thumbnail path of
+ * the ndht byte-range index. */
+ndic_status ndic_htj2k_decode_plane(const uint8_t* codestream, size_t len, int32_t max_res, int32_t prefix,
+                                    ndic_buffer* out, size_t* width, size_t* height, ndic_error* err);
+
+/* ---- fused nd-lift-ht chain over device-resident batches --------------- */
+
+/* One chunk of a batch: the codec chain transpose(done by the caller) ->
+ * nd_lift -> htj2k over the int32 coefficient plane, as zarrs runs it for
+ * codec_series output (crates/ndic-zarr/src/series.rs:203-404). */
+#define NDIC_MAX_NDIM 8
+#define NDIC_MAX_STEPS 4
+typedef struct {
+  uint64_t in_offset;  /* decode: compressed chunk offset in the input arena;
+                          encode: sample offset (bytes) in the input arena */
+  uint64_t in_len;
+  uint64_t out_offset; /* decode: decoded samples (bytes) in the output arena;
+                          encode: filled by the library */
+  uint64_t out_len;    /* encode: filled by the library */
+  uint32_t ndim;
+  uint32_t shape[NDIC_MAX_NDIM];
+  int32_t dtype;
+  uint32_t xy_levels;
+  uint32_t nsteps;
+  ndic_lift_step steps[NDIC_MAX_STEPS];
+  int32_t status;       /* filled by the library: ndic_status of this chunk */
+} ndic_chunk_desc;
+
+#define NDIC_BATCH_PLAIN 1 /* one chunk per launch sequence (the comparison bar) */
+
+/* Decodes every chunk of a batch on the device: d_in holds the compressed
+ * chunks, d_out receives the samples.  stream is a hipStream_t (NULL: the
+ * library's stream).  Returns after the work is complete; per-chunk status in
+ * descs[i].status, the first failure in err. */
+ndic_status ndic_lift_ht_decode_batch(const uint8_t* d_in, uint8_t* d_out, ndic_chunk_desc* descs, size_t n,
+                                      uint32_t flags, void* stream, ndic_error* err);
+
+/* Encodes every chunk of a batch: d_in holds the samples, the codestreams are
+ * written packed into d_out (capacity bytes); descs[i].out_offset/out_len
+ * report where. */
+ndic_status ndic_lift_ht_encode_batch(const uint8_t* d_in, uint8_t* d_out, size_t capacity, ndic_chunk_desc* descs,
+                                      size_t n, uint32_t flags, void* stream, ndic_error* err);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NDIC_HIP_H */

@@ -1,0 +1,100 @@
+// Device-side Part 1/15 codestream and ndht container handling (definitions
+// in codestream.hip).
+#pragma once
+
+#include "common.hpp"
+#include "ht.hpp"
+
+namespace ndic {
+
+// Geometry of one resolution's packet and its bands, shared by every plane of
+// a chunk (crates/ndic-codestream/src/geometry.rs:77-127: Mallat positions,
+// 64x64 code-blocks, maximal precincts).
+struct GeoBand {
+  uint32_t x0, y0, w, h;  // band rectangle in the Mallat plane
+  uint32_t nx, ny;        // code-block grid
+  uint32_t kidx;          // index into the plane's K_max table
+  uint32_t block_first;   // first block of the band within the plane
+};
+struct GeoPacket {
+  uint32_t band_first, nbands;
+  uint32_t block_first, nblocks;
+};
+
+// Decoded per-plane facts (written by the parse kernel / the encode planner).
+struct DevPlane {
+  int32_t depth;          // SIZ bit depth
+  int32_t is_signed;
+  uint8_t kmax[100];      // K_max per band index (1 + 3 * levels)
+  uint32_t ok;
+  uint64_t cs_off;        // codestream offset within the chunk
+  uint64_t cs_len;
+  uint32_t packet_len[33];    // encode: bytes per packet (header + bodies)
+  uint32_t packet_hdr[33];    // encode: packet header bytes
+  uint32_t head_len;          // encode: main + tile-part header bytes
+  uint64_t total_len;         // encode: codestream bytes
+  int64_t lo, hi;             // encode: sample range (depth_for)
+};
+
+// One chunk of a codestream batch.
+struct DevChunk {
+  const uint8_t* in;      // decode: the compressed chunk
+  uint64_t in_len;
+  uint32_t ndim;
+  uint32_t dims[32];
+  uint32_t W, H, P, L;    // plane width/height, planes, decomposition levels
+  int32_t max_res;        // decode: highest resolution decoded (L = all)
+  uint32_t geo_packet;    // first GeoPacket of this chunk's geometry
+  uint32_t geo_band;      // first GeoBand
+  uint32_t blocks_per_plane;
+  uint64_t task_base;     // first HT task
+  uint64_t plane_base;    // first global plane
+  int32_t* coef;          // Mallat planes (P x W x H)
+  int32_t* err;           // {code, offset_lo, offset_hi, plane}
+  uint8_t* tag;           // tag-tree scratch, tag_stride bytes per plane
+  uint32_t tag_stride;
+  uint32_t no_container;  // decode: bit 0 `in` is one bare codestream, bit 1 it may be a prefix
+  // encode
+  int32_t* planes_in;     // the int32 spatial planes (P x W x H)
+  int32_t sample_signed;  // the sample dtype is signed (SIZ signedness)
+  int32_t progression;
+  int32_t index;
+  uint8_t* slots;         // HT slots base (per-task offsets in slot_off)
+  uint8_t* head;          // per-plane header bytes, head_stride apart
+  uint32_t head_stride;
+  uint8_t* pkt_hdr;       // per-packet header bytes, pkt_stride apart (per plane: L+1 packets)
+  uint32_t pkt_stride;
+  uint8_t* chunk_head;    // container header + index bytes
+  uint64_t chunk_head_len;
+  uint64_t out_off;       // encode: byte offset of the chunk in the output
+  uint64_t out_len;
+};
+
+// Tag-tree scratch bytes a plane needs for a band grid of nx x ny blocks.
+uint32_t tag_scratch_bytes(uint32_t nx, uint32_t ny);
+void launch_parse_planes(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, DevPlane* planes,
+                         HtDecTask* tasks, const GeoPacket* gp, const GeoBand* gb, hipStream_t stream);
+void launch_walk_chunks(const DevChunk* d_chunks, uint32_t nchunks, DevPlane* planes, hipStream_t stream);
+// Unsigned planes: DC un-shift with clamping (reader.rs:626-638), in place on
+// each chunk's result planes (d_data[c], region d_dims[2c] x d_dims[2c+1]).
+void launch_unshift(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, const DevPlane* planes,
+                    int32_t* const* d_data, const uint64_t* d_dims, hipStream_t stream);
+void launch_plane_minmax(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, DevPlane* planes,
+                         hipStream_t stream);
+void launch_plane_setup(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, DevPlane* planes,
+                        hipStream_t stream);
+void launch_dc_shift(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, const DevPlane* planes,
+                     hipStream_t stream);
+void launch_build_enc_tasks(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, const DevPlane* planes,
+                            const GeoPacket* gp, const GeoBand* gb, HtEncTask* tasks, uint32_t* lens,
+                            uint64_t slot_stride, hipStream_t stream);
+void launch_write_planes(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, DevPlane* planes,
+                         const HtEncTask* tasks, const uint32_t* lens, const GeoPacket* gp, const GeoBand* gb,
+                         hipStream_t stream);
+void launch_write_chunks(const DevChunk* d_chunks, uint32_t nchunks, DevPlane* planes, uint64_t* chunk_len,
+                         hipStream_t stream);
+void launch_assemble(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, const DevPlane* planes,
+                     const HtEncTask* tasks, const uint32_t* lens, const GeoPacket* gp, uint8_t* out,
+                     hipStream_t stream);
+
+}  // namespace ndic

@@ -1,0 +1,86 @@
+// Host-side runtime of the library: the device context (stream + reusable
+// device buffers) and the orchestration of the codec kernels.
+#pragma once
+
+#include <functional>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "lift.hpp"
+
+namespace ndic {
+
+// A device allocation that grows on demand and is reused across calls.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  void* ensure(size_t bytes);
+  template <class T> T* as() const { return (T*)ptr; }
+  void release();
+};
+
+struct Context {
+  std::recursive_mutex mu;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, small;
+  static Context& get();
+};
+
+// One chunk of a lift launch sequence.
+struct LiftChunk {
+  void* plane = nullptr;   // coefficient plane (int32 or int64)
+  void* tmp = nullptr;     // chunk-sized temporary, for segments too long for LDS
+  const uint8_t* src = nullptr;
+  uint8_t* dst = nullptr;
+  int32_t* flag = nullptr;
+  std::vector<size_t> shape;
+  int dtype = NDIC_U8;
+  std::vector<ndic_lift_step> steps;
+};
+
+// One chunk of a DWT launch sequence: nplanes planes of W x H, each buffer
+// holding them back to back.
+struct DwtChunk {
+  int32_t* X = nullptr;   // forward input (spatial planes)
+  int32_t* A = nullptr;   // Mallat coefficients
+  int32_t* T = nullptr;   // pass temporary
+  int32_t* O = nullptr;   // inverse output
+  int32_t* result = nullptr;  // set by the run: where the result planes are
+  uint64_t nplanes = 0, W = 0, H = 0;  // region transformed (top-left of each plane)
+  uint64_t stride = 0;                 // plane row pitch (0: W)
+  uint64_t plane_elems = 0;            // plane spacing (0: W * H)
+  uint32_t levels = 0;
+};
+void level_dims(uint64_t w, uint64_t h, uint32_t level, uint64_t* ow, uint64_t* oh);
+void run_dwt_forward(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t stream);
+void run_dwt_inverse(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t stream);
+
+// Runs body, mapping a Failure to the C-ABI status and error record.
+ndic_status guard_call(ndic_error* err, const std::function<void()>& body);
+// Copies n device bytes into a fresh host ndic_buffer.
+void emit_host(const void* d_src, size_t n, ndic_buffer* out, hipStream_t stream);
+
+struct DecodeItem;
+struct EncodeItem;
+// Decode / encode batches of chunks on the device (engine_htj2k.cpp).
+// window_elems bounds the samples whose intermediate planes are resident at
+// once.
+void decode_items(Context& ctx, std::vector<DecodeItem>& items, hipStream_t s, uint64_t window_elems);
+void encode_items(Context& ctx, std::vector<EncodeItem>& items, uint8_t* d_out, uint64_t capacity, hipStream_t s,
+                  uint64_t window_elems);
+
+bool lift_wide(int dtype);
+// Validates a configuration against a chunk dimensionality the way
+// NdLiftConfig::validate does (crates/ndic-lift/src/lib.rs:122-184).
+void lift_validate(const char* version, const ndic_lift_step* steps, size_t nsteps, size_t ndim);
+// Budget check of the encode direction on the input's [lo, hi]
+// (crates/ndic-lift/src/chunk.rs:188-302); throws the reference's refusal.
+void lift_check_budget(int64_t lo, int64_t hi, const std::vector<size_t>& shape,
+                       const std::vector<ndic_lift_step>& steps, bool wide);
+// Runs every step of every chunk, forward (widening src into plane first)
+// or inverse (narrowing plane into dst last).
+void run_lift(Context& ctx, std::vector<LiftChunk>& chunks, bool fwd, hipStream_t stream);
+
+}  // namespace ndic

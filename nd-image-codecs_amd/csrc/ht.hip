@@ -1,0 +1,941 @@
+// HT (FBCOT) cleanup-pass block coder on the device, one lane per code-block.
+//
+// Decode restates crates/ndic-htj2k/src/block/decode.rs:54-396 with its two
+// steps fused per quad pair: the VLC/MEL stream (read backward from the end
+// of the cleanup segment) and the MagSgn stream (read forward from its start)
+// are independent, so decoding a pair's VLC codewords and then its MagSgn
+// bits gives the reference's output while keeping only one quad row of state
+// per lane: the row above's significance patterns (context c_q) and the
+// bit-lengths of its bottom-sample magnitudes (exponent bound E_max).  Both
+// rows live in LDS laid out [entry][lane], so a wave touching entry k hits
+// consecutive bytes.  The pair loop is uniform across the wave; lanes whose
+// block is narrower or shorter are masked.
+//
+// Encode restates crates/ndic-htj2k/src/block/encode.rs:113-331 with the MEL,
+// VLC and MagSgn packers of mel.rs:175-262 and streams.rs:267-441, writing
+// the three byte streams into a per-block slot; the codestream assembler
+// stitches them together (ht_seg_byte).
+//
+// The CxtVLC/UVLC tables are derived on the host from the T.814 rows and
+// live in constant memory.
+#include <vector>
+
+#include "ht.hpp"
+
+namespace ndic {
+
+namespace {
+#include "tables/ht_cxtvlc_rows.inc"
+}
+
+__constant__ uint16_t c_dec0[1024], c_dec1[1024], c_enc0[2048], c_enc1[2048], c_uvlc0[320], c_uvlc1[256];
+
+namespace {
+
+__constant__ uint8_t c_mel_exp[13] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 4, 5};
+
+// ---------------------------------------------------------------------------
+// host: table derivation (T.814 Annex C rows -> lookup tables)
+// ---------------------------------------------------------------------------
+template <size_t N> void derive_dec(const unsigned char (&src)[N][7], uint16_t* tbl) {
+  for (int i = 0; i < 1024; ++i) {
+    int cwd = i & 0x7F, cq = i >> 7;
+    tbl[i] = 0;
+    for (size_t r = 0; r < N; ++r)
+      if (src[r][0] == cq && src[r][5] == (cwd & ((1 << src[r][6]) - 1)))
+        tbl[i] = uint16_t((src[r][1] << 4) | (src[r][2] << 3) | (src[r][3] << 12) | (src[r][4] << 8) | src[r][6]);
+  }
+}
+
+template <size_t N> void derive_enc(const unsigned char (&src)[N][7], uint16_t* tbl) {
+  for (int i = 0; i < 2048; ++i) {
+    int cq = i >> 8, rho = (i >> 4) & 0xF, emb = i & 0xF;
+    tbl[i] = 0;
+    if ((emb & rho) != emb || (rho == 0 && cq == 0)) continue;
+    const unsigned char* best = nullptr;
+    int best_ones = -1;
+    for (size_t r = 0; r < N; ++r) {
+      const unsigned char* e = src[r];
+      if (e[0] != cq || e[1] != rho) continue;
+      if (emb) {
+        if (e[2] == 1 && (emb & e[3]) == e[4] && __builtin_popcount(e[3]) >= best_ones) {
+          best = e;
+          best_ones = __builtin_popcount(e[3]);
+        }
+      } else if (e[2] == 0) {
+        best = e;
+        break;
+      }
+    }
+    if (best) tbl[i] = uint16_t((best[5] << 8) | (best[6] << 4) | best[3]);
+  }
+}
+
+// UVLC prefixes of T.814: "1" -> 1, "01" -> 2, "001"+1 bit -> 3..4, "000"+5 bits -> 5..36.
+void uvlc_pfx(int b, int* len, int* suf, int* u) {
+  if (b & 1) { *len = 1; *suf = 0; *u = 1; }
+  else if (b & 2) { *len = 2; *suf = 0; *u = 2; }
+  else if (b & 4) { *len = 3; *suf = 1; *u = 3; }
+  else { *len = 3; *suf = 5; *u = 5; }
+}
+
+void derive_uvlc(uint16_t* tbl, int modes, bool initial) {
+  auto ent = [](int tp, int ts, int s0, int u0, int u1) {
+    return uint16_t(tp | (ts << 3) | (s0 << 7) | (u0 << 10) | (u1 << 13));
+  };
+  for (int i = 0; i < modes * 64; ++i) {
+    int mode = i >> 6, v = i & 0x3F;
+    tbl[i] = 0;
+    if (mode == 0) continue;
+    int l0, s0, u0, l1, s1, u1;
+    uvlc_pfx(v & 7, &l0, &s0, &u0);
+    if (mode <= 2) {
+      tbl[i] = mode == 1 ? ent(l0, s0, s0, u0, 0) : ent(l0, s0, 0, 0, u0);
+      continue;
+    }
+    uvlc_pfx((v >> l0) & 7, &l1, &s1, &u1);
+    if (mode == 3 && initial && l0 == 3) tbl[i] = ent(l0 + 1, s0, s0, u0, ((v >> l0) & 1) + 1);
+    else if (mode == 4) tbl[i] = ent(l0 + l1, s0 + s1, s0, u0 + 2, u1 + 2);
+    else tbl[i] = ent(l0 + l1, s0 + s1, s0, u0, u1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// device: stream readers (mel.rs:14-173, streams.rs:20-265)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ld32(const uint8_t* p) {
+  uint32_t v;
+  __builtin_memcpy(&v, p, 4);
+  return v;
+}
+
+struct MelDec {
+  const uint8_t* data;
+  int size, bits, k;
+  bool unstuff;
+  uint64_t tmp, runs;
+  uint32_t num_runs;
+  __device__ void init(const uint8_t* coded, uint32_t lcup, uint32_t scup) {
+    data = coded + lcup - scup;
+    size = (int)scup - 1;
+    bits = 0;
+    k = 0;
+    unstuff = false;
+    tmp = 0;
+    runs = 0;
+    num_runs = 0;
+  }
+  __device__ void read() {
+    if (bits > 32) return;
+    uint32_t val = 0xFFFFFFFFu;
+    if (size > 4) {
+      val = ld32(data);
+      data += 4;
+      size -= 4;
+    } else if (size > 0) {
+      uint32_t i = 0;
+      while (size > 1) {
+        uint32_t v = *data++;
+        uint32_t m = ~(0xFFu << i);
+        val = (val & m) | (v << i);
+        size -= 1;
+        i += 8;
+      }
+      uint32_t v = uint32_t(*data++) | 0xF;
+      uint32_t m = ~(0xFFu << i);
+      val = (val & m) | (v << i);
+      size -= 1;
+    }
+    int nb = 32 - int(unstuff);
+    uint32_t t = val & 0xFF;
+    bool us = (val & 0xFF) == 0xFF;
+    nb -= int(us);
+    t <<= 8 - int(us);
+    t |= (val >> 8) & 0xFF;
+    us = ((val >> 8) & 0xFF) == 0xFF;
+    nb -= int(us);
+    t <<= 8 - int(us);
+    t |= (val >> 16) & 0xFF;
+    us = ((val >> 16) & 0xFF) == 0xFF;
+    nb -= int(us);
+    t <<= 8 - int(us);
+    t |= (val >> 24) & 0xFF;
+    unstuff = ((val >> 24) & 0xFF) == 0xFF;
+    tmp |= uint64_t(t) << (64 - nb - bits);
+    bits += nb;
+  }
+  __device__ void decode() {
+    if (bits < 6) read();
+    while (bits >= 6 && num_runs < 8) {
+      uint32_t e = c_mel_exp[k];
+      uint32_t run;
+      if (tmp & (1ull << 63)) {
+        run = ((1u << e) - 1) << 1;
+        k = k + 1 < 12 ? k + 1 : 12;
+        tmp <<= 1;
+        bits -= 1;
+      } else {
+        uint32_t count = uint32_t(tmp >> (63 - e)) & ((1u << e) - 1);
+        run = (count << 1) | 1;
+        k = k > 0 ? k - 1 : 0;
+        tmp <<= e + 1;
+        bits -= int(e) + 1;
+      }
+      uint32_t sh = num_runs * 7;
+      runs &= ~(0x3Full << sh);
+      runs |= uint64_t(run) << sh;
+      num_runs += 1;
+    }
+  }
+  __device__ int get_run() {
+    if (num_runs == 0) decode();
+    int t = int(runs & 0x7F);
+    runs >>= 7;
+    num_runs -= 1;
+    return t;
+  }
+};
+
+struct RevDec {
+  const uint8_t* data;
+  long pos;
+  int size;
+  uint64_t tmp;
+  uint32_t bits;
+  bool unstuff;
+  __device__ void init_vlc(const uint8_t* coded, uint32_t lcup, uint32_t scup) {
+    data = coded;
+    uint8_t d = coded[lcup - 2];
+    tmp = d >> 4;
+    pos = (long)lcup - 3;
+    size = (int)scup - 2;
+    bits = 4 - uint32_t((tmp & 7) == 7);
+    unstuff = (d | 0xF) > 0x8F;
+    read();
+  }
+  __device__ void init_mrp(const uint8_t* coded, uint32_t lcup, uint32_t len2) {
+    data = coded;
+    pos = (long)(lcup + len2) - 1;
+    size = (int)len2;
+    tmp = 0;
+    bits = 0;
+    unstuff = true;
+    read();
+  }
+  __device__ void read() {
+    if (bits > 32) return;
+    uint32_t val = 0;
+    if (size > 3 && pos >= 3) {
+      val = ld32(data + pos - 3);
+      pos -= 4;
+      size -= 4;
+    } else if (size > 0) {
+      int i = 24;
+      while (size > 0 && i >= 0) {
+        uint32_t v = pos >= 0 ? data[pos] : 0;
+        pos -= 1;
+        val |= v << i;
+        size -= 1;
+        i -= 8;
+      }
+    }
+    uint32_t t = val >> 24;
+    uint32_t nb = 8 - uint32_t(unstuff && ((val >> 24) & 0x7F) == 0x7F);
+    bool us = (val >> 24) > 0x8F;
+    t |= ((val >> 16) & 0xFF) << nb;
+    nb += 8 - uint32_t(us && ((val >> 16) & 0x7F) == 0x7F);
+    us = ((val >> 16) & 0xFF) > 0x8F;
+    t |= ((val >> 8) & 0xFF) << nb;
+    nb += 8 - uint32_t(us && ((val >> 8) & 0x7F) == 0x7F);
+    us = ((val >> 8) & 0xFF) > 0x8F;
+    t |= (val & 0xFF) << nb;
+    nb += 8 - uint32_t(us && (val & 0x7F) == 0x7F);
+    us = (val & 0xFF) > 0x8F;
+    tmp |= uint64_t(t) << bits;
+    bits += nb;
+    unstuff = us;
+  }
+  __device__ uint32_t fetch() {
+    if (bits < 32) {
+      read();
+      if (bits < 32) read();
+    }
+    return uint32_t(tmp);
+  }
+  __device__ uint32_t advance(uint32_t n) {
+    tmp = n >= 64 ? 0 : tmp >> n;
+    bits = bits > n ? bits - n : 0;
+    return uint32_t(tmp);
+  }
+};
+
+template <uint32_t FILL> struct FwdDec {
+  const uint8_t* data;
+  int size;
+  uint64_t tmp;
+  uint32_t bits;
+  bool unstuff;
+  __device__ void init(const uint8_t* d, uint32_t len) {
+    data = d;
+    size = (int)len;
+    tmp = 0;
+    bits = 0;
+    unstuff = false;
+    read();
+  }
+  __device__ void read() {
+    if (bits > 32) return;
+    uint32_t val;
+    if (size > 3) {
+      val = ld32(data);
+      data += 4;
+      size -= 4;
+    } else if (size > 0) {
+      uint32_t v = FILL, i = 0;
+      while (size > 0) {
+        uint32_t b = *data++;
+        uint32_t m = ~(0xFFu << i);
+        v = (v & m) | (b << i);
+        size -= 1;
+        i += 8;
+      }
+      val = v;
+    } else {
+      val = FILL;
+    }
+    uint32_t nb = 8 - uint32_t(unstuff);
+    uint32_t t = val & 0xFF;
+    bool us = (val & 0xFF) == 0xFF;
+    t |= ((val >> 8) & 0xFF) << nb;
+    nb += 8 - uint32_t(us);
+    us = ((val >> 8) & 0xFF) == 0xFF;
+    t |= ((val >> 16) & 0xFF) << nb;
+    nb += 8 - uint32_t(us);
+    us = ((val >> 16) & 0xFF) == 0xFF;
+    t |= ((val >> 24) & 0xFF) << nb;
+    nb += 8 - uint32_t(us);
+    unstuff = ((val >> 24) & 0xFF) == 0xFF;
+    tmp |= uint64_t(t) << bits;
+    bits += nb;
+  }
+  __device__ uint32_t fetch() {
+    if (bits < 32) {
+      read();
+      if (bits < 32) read();
+    }
+    return uint32_t(tmp);
+  }
+  __device__ void advance(uint32_t n) {
+    tmp = n >= 64 ? 0 : tmp >> n;
+    bits = bits > n ? bits - n : 0;
+  }
+};
+
+__device__ __forceinline__ uint32_t bitlen(uint32_t v) { return 32 - __clz(v); }
+
+// Store one decoded sign-magnitude sample (as a coefficient, or raw).
+__device__ __forceinline__ void put(const HtDecTask& t, uint32_t x, uint32_t y, uint32_t sm, uint32_t shift) {
+  if (t.flags & kTaskSignMag) {
+    ((uint32_t*)t.out)[(uint64_t)y * t.stride + x] = sm;
+  } else {
+    int32_t mag = (int32_t)((sm & 0x7FFFFFFFu) >> shift);
+    ((int32_t*)t.out)[(uint64_t)y * t.stride + x] = (sm & 0x80000000u) ? -mag : mag;
+  }
+}
+
+__device__ void fill_zero(const HtDecTask& t) {
+  for (uint32_t y = 0; y < t.h; ++y)
+    for (uint32_t x = 0; x < t.w; ++x) put(t, x, y, 0, 0);
+}
+
+__device__ void report(const HtDecTask& t, int32_t code) {
+  if (t.err) atomicCAS(t.err, 0, code);
+}
+
+constexpr int kRowCap = 34;  // quads per row (64 columns) + terminators
+
+// decode.rs:54-396 (cleanup pass) for one block.
+__device__ void decode_one(const HtDecTask& t, uint8_t* lrho, uint8_t* lbl) {
+  const uint32_t width = t.w, height = t.h;
+  // per-lane rows, [entry][lane]
+  auto RHO = [&](int k) -> uint8_t& { return lrho[k * 64]; };
+  auto BL = [&](int k) -> uint8_t& { return lbl[k * 64]; };
+  if (!(t.flags & kTaskIncluded)) {
+    fill_zero(t);
+    return;
+  }
+  uint32_t num_passes = t.passes;
+  uint32_t missing_msbs = t.mmsb;
+  uint32_t lengths1 = t.len1, lengths2 = t.len2;
+  if (num_passes > 1 && lengths2 == 0) num_passes = 1;
+  if (num_passes > 3 || missing_msbs >= 30 || lengths1 < 2 || width == 0 || height == 0 || width > 64 ||
+      height > 64) {
+    report(t, kErrBlock);
+    fill_zero(t);
+    return;
+  }
+  if (missing_msbs == 29) num_passes = 1;
+  if (num_passes > 1) {
+    report(t, kErrCsUnsupported);
+    fill_zero(t);
+    return;
+  }
+  const uint32_t p = 30 - missing_msbs;
+  const uint8_t* coded = t.seg;
+  const uint32_t lcup = lengths1;
+  const uint32_t scup = (uint32_t(coded[lcup - 1]) << 4) + uint32_t(coded[lcup - 2] & 0xF);
+  if (scup < 2 || scup > lcup || scup > 4079) {
+    report(t, kErrBlock);
+    fill_zero(t);
+    return;
+  }
+  const uint32_t shift = 31 - t.kmax;
+  const uint32_t mmsbp2 = missing_msbs + 2;
+
+  MelDec mel;
+  mel.init(coded, lcup, scup);
+  RevDec vlc;
+  vlc.init_vlc(coded, lcup, scup);
+  FwdDec<0xFFFFFFFFu> ms;
+  ms.init(coded, lcup - scup);
+  int run = mel.get_run();
+  bool bad = false;
+
+  for (uint32_t y = 0; y < height; y += 2) {
+    const bool initial = y == 0;
+    uint32_t c_q = 0;
+    uint32_t prev_v_n = 0;
+    uint32_t vp = 0;
+    uint32_t q = 0;  // quad index of the pair's first quad
+    for (uint32_t x = 0; x < width; x += 4, q += 2) {
+      uint32_t a0 = 0, a1 = 0, a2 = 0;
+      if (!initial) {
+        a0 = uint32_t(RHO(q)) << 4;
+        a1 = uint32_t(RHO(q + 1)) << 4;
+        a2 = uint32_t(RHO(q + 2)) << 4;
+        c_q |= (a0 & 0xA0) << 2;
+        c_q |= (a1 & 0x20) << 4;
+      }
+      // ---- VLC / MEL for the pair (decode.rs:115-189, 192-273)
+      uint32_t vlc_val = vlc.fetch();
+      uint32_t t0 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
+      if (c_q == 0) {
+        run -= 2;
+        t0 = run == -1 ? t0 : 0;
+        if (run < 0) run = mel.get_run();
+      }
+      if (initial) {
+        c_q = ((t0 & 0x10) << 3) | ((t0 & 0xE0) << 2);
+      } else {
+        c_q = ((t0 & 0x40) << 2) | ((t0 & 0x80) << 1);
+        c_q |= a0 & 0x80;
+        c_q |= (a1 & 0xA0) << 2;
+        c_q |= (a2 & 0x20) << 4;
+      }
+      vlc_val = vlc.advance(t0 & 0x7);
+      uint32_t t1 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
+      if (c_q == 0 && x + 2 < width) {
+        run -= 2;
+        t1 = run == -1 ? t1 : 0;
+        if (run < 0) run = mel.get_run();
+      }
+      if (x + 2 >= width) t1 = 0;
+      if (initial) {
+        c_q = ((t1 & 0x10) << 3) | ((t1 & 0xE0) << 2);
+      } else {
+        c_q = ((t1 & 0x40) << 2) | ((t1 & 0x80) << 1);
+        c_q |= a1 & 0x80;
+      }
+      vlc_val = vlc.advance(t1 & 0x7);
+      uint32_t uvlc_mode = ((t0 & 0x8) << 3) | ((t1 & 0x8) << 4);
+      if (initial && uvlc_mode == 0xC0) {
+        run -= 2;
+        uvlc_mode += run == -1 ? 0x40 : 0;
+        if (run < 0) run = mel.get_run();
+      }
+      uint32_t e = initial ? c_uvlc0[uvlc_mode + (vlc_val & 0x3F)] : c_uvlc1[uvlc_mode + (vlc_val & 0x3F)];
+      vlc_val = vlc.advance(e & 0x7);
+      e >>= 3;
+      uint32_t len = e & 0xF;
+      uint32_t tmp = vlc_val & ((1u << len) - 1);
+      vlc.advance(len);
+      e >>= 4;
+      uint32_t len0 = e & 0x7;
+      e >>= 3;
+      uint32_t base = initial ? 1 : 0;
+      uint32_t uq[2] = {base + (e & 7) + (tmp & ~(0xFFu << len0)), base + (e >> 3) + (tmp >> len0)};
+      uint32_t tq[2] = {t0, t1};
+      // this row's patterns become the next row's context (reads above done)
+      RHO(q) = uint8_t(t0 >> 4);
+      RHO(q + 1) = uint8_t(t1 >> 4);
+
+      // ---- MagSgn for the two quads (decode.rs:300-396)
+      for (int qi = 0; qi < 2; ++qi) {
+        uint32_t cx = x + 2 * qi;
+        if (cx >= width) break;
+        uint32_t inf = tq[qi], u_q = uq[qi];
+        if (!initial) {
+          uint32_t gamma = inf & 0xF0;
+          gamma &= gamma - 0x10;
+          uint32_t emax = max(max(uint32_t(BL(vp)), uint32_t(BL(vp + 1))), 2u) - 1;
+          u_q += gamma != 0 ? emax : 1;
+        }
+        if (u_q > mmsbp2) bad = true;
+        uint32_t v_n[4];
+        v_n[1] = v_n[3] = 0;
+        for (uint32_t bit = 0; bit < 4; ++bit) {
+          uint32_t sx = cx + (bit >> 1), sy = y + (bit & 1);
+          if (bit >= 2 && sx >= width) break;
+          v_n[bit] = 0;
+          uint32_t val = 0;
+          if (inf & (1u << (4 + bit))) {
+            uint32_t ms_val = ms.fetch();
+            uint32_t m_n = u_q - ((inf >> (12 + bit)) & 1);
+            ms.advance(m_n);
+            uint32_t vv = ms_val & ((1u << (m_n & 31)) - 1);
+            vv |= ((inf >> (8 + bit)) & 1) << (m_n & 31);
+            vv |= 1;
+            v_n[bit] = vv;
+            val = (ms_val << 31) | ((vv + 2) << ((p - 1) & 31));
+          }
+          if (sy < height) put(t, sx, sy, val, shift);
+        }
+        BL(vp) = uint8_t(bitlen(prev_v_n | v_n[1]));
+        prev_v_n = 0;
+        vp += 1;
+        if (cx + 1 >= width) break;
+        prev_v_n = v_n[3];
+      }
+    }
+    // terminators: the quad after the row reads as insignificant
+    RHO(q) = 0;
+    RHO(q + 1) = 0;
+    BL(vp) = uint8_t(bitlen(prev_v_n));
+  }
+  if (bad) {
+    report(t, kErrBlock);
+    fill_zero(t);
+  }
+}
+
+__global__ __launch_bounds__(64) void ht_decode_kernel(const HtDecTask* __restrict__ tasks, uint64_t n) {
+  __shared__ uint8_t lrho[kRowCap * 64];
+  __shared__ uint8_t lbl[kRowCap * 64];
+  uint64_t i = blockIdx.x * (uint64_t)64 + threadIdx.x;
+  if (i >= n) return;
+  HtDecTask t = tasks[i];
+  for (int k = 0; k < kRowCap; ++k) {
+    lrho[k * 64 + threadIdx.x] = 0;
+    lbl[k * 64 + threadIdx.x] = 0;
+  }
+  decode_one(t, lrho + threadIdx.x, lbl + threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// device: encoders (mel.rs:175-262, streams.rs:267-441, encode.rs:24-331)
+// ---------------------------------------------------------------------------
+struct MelEnc {
+  uint8_t* buf;
+  uint32_t len, tmp, remaining, run, threshold;
+  int k;
+  __device__ void init(uint8_t* b) {
+    buf = b;
+    len = 0;
+    tmp = 0;
+    remaining = 8;
+    run = 0;
+    k = 0;
+    threshold = 1;
+  }
+  __device__ void emit_bit(uint32_t v) {
+    tmp = (tmp << 1) + v;
+    remaining -= 1;
+    if (remaining == 0) {
+      if (len < kMelCap) buf[len] = (uint8_t)tmp;
+      len += 1;
+      remaining = tmp == 0xFF ? 7 : 8;
+      tmp = 0;
+    }
+  }
+  __device__ void encode(bool bit) {
+    if (bit) {
+      emit_bit(0);
+      int t = c_mel_exp[k];
+      while (t > 0) {
+        t -= 1;
+        emit_bit((run >> t) & 1);
+      }
+      run = 0;
+      k = k > 0 ? k - 1 : 0;
+      threshold = 1u << c_mel_exp[k];
+    } else {
+      run += 1;
+      if (run >= threshold) {
+        emit_bit(1);
+        run = 0;
+        k = k + 1 < 12 ? k + 1 : 12;
+        threshold = 1u << c_mel_exp[k];
+      }
+    }
+  }
+};
+
+struct VlcEnc {
+  uint8_t* buf;  // production order; buf[0] is the 0xFF placeholder
+  uint32_t len, tmp, used;
+  bool gt8f;
+  __device__ void init(uint8_t* b) {
+    buf = b;
+    buf[0] = 0xFF;
+    len = 1;
+    tmp = 0xF;
+    used = 4;
+    gt8f = true;
+  }
+  __device__ void encode(uint32_t cwd, uint32_t cwd_len) {
+    while (cwd_len > 0) {
+      uint32_t avail = 8 - (gt8f ? 1 : 0) - used;
+      uint32_t t = avail < cwd_len ? avail : cwd_len;
+      tmp |= (cwd & ((1u << t) - 1)) << used;
+      used += t;
+      avail -= t;
+      cwd_len -= t;
+      cwd >>= t;
+      if (avail == 0) {
+        if (gt8f && tmp != 0x7F) {
+          gt8f = false;
+          continue;
+        }
+        if (len < kVlcCap) buf[len] = (uint8_t)tmp;
+        len += 1;
+        gt8f = tmp > 0x8F;
+        tmp = 0;
+        used = 0;
+      }
+    }
+  }
+};
+
+struct MsEnc {
+  uint8_t* buf;
+  uint32_t len, cap, tmp, used, max_bits;
+  __device__ void init(uint8_t* b, uint32_t c) {
+    buf = b;
+    cap = c;
+    len = 0;
+    tmp = 0;
+    used = 0;
+    max_bits = 8;
+  }
+  __device__ void encode(uint32_t cwd, uint32_t cwd_len) {
+    while (cwd_len > 0) {
+      uint32_t room = max_bits - used;
+      uint32_t t = room < cwd_len ? room : cwd_len;
+      tmp |= (cwd & ((1u << t) - 1)) << used;
+      used += t;
+      cwd >>= t;
+      cwd_len -= t;
+      if (used >= max_bits) {
+        if (len < cap) buf[len] = (uint8_t)tmp;
+        len += 1;
+        max_bits = tmp == 0xFF ? 7 : 8;
+        tmp = 0;
+        used = 0;
+      }
+    }
+  }
+  __device__ void terminate() {
+    if (used > 0) {
+      uint32_t t = max_bits - used;
+      tmp |= (0xFFu & ((1u << t) - 1)) << used;
+      used += t;
+      if (tmp != 0xFF) {
+        if (len < cap) buf[len] = (uint8_t)tmp;
+        len += 1;
+      }
+    } else if (max_bits == 7) {
+      len -= 1;
+    }
+  }
+};
+
+struct EQuad {
+  uint32_t rho, e_q[4], e_qmax, s[4];
+};
+
+__device__ __forceinline__ EQuad load_quad(const HtEncTask& t, uint32_t x, uint32_t y, uint32_t p, uint32_t shift) {
+  EQuad q;
+  q.rho = 0;
+  q.e_qmax = 0;
+  for (int n = 0; n < 4; ++n) {
+    q.e_q[n] = 0;
+    q.s[n] = 0;
+    uint32_t sx = x + n / 2, sy = y + n % 2;
+    if (sx >= t.w || sy >= t.h) continue;
+    uint32_t raw;
+    if (t.flags & kTaskSignMag) {
+      raw = ((const uint32_t*)t.in)[(uint64_t)sy * t.stride + sx];
+    } else {
+      int32_t v = t.in[(uint64_t)sy * t.stride + sx];
+      raw = (v < 0 ? 0x80000000u : 0u) | ((v < 0 ? 0u - (uint32_t)v : (uint32_t)v) << shift);
+    }
+    uint32_t val = ((raw + raw) >> p) & ~1u;
+    if (val != 0) {
+      q.rho |= 1u << n;
+      val -= 1;
+      q.e_q[n] = 32 - __clz(val);
+      q.e_qmax = q.e_qmax > q.e_q[n] ? q.e_qmax : q.e_q[n];
+      q.s[n] = (val - 1) + (raw >> 31);
+    }
+  }
+  return q;
+}
+
+__device__ __forceinline__ uint32_t quad_eps(const EQuad& q, uint32_t u_q) {
+  if (u_q == 0) return 0;
+  uint32_t e = 0;
+  for (int n = 0; n < 4; ++n) e |= uint32_t((q.rho & (1u << n)) != 0 && q.e_q[n] == q.e_qmax) << n;
+  return e;
+}
+
+__device__ __forceinline__ void emit_ms(MsEnc& ms, const EQuad& q, uint32_t tuple, uint32_t uq) {
+  for (uint32_t n = 0; n < 4; ++n)
+    if (q.rho & (1u << n)) {
+      uint32_t m = uq - ((tuple >> n) & 1);
+      uint32_t mask = m >= 32 ? 0xFFFFFFFFu : (1u << m) - 1;
+      ms.encode(q.s[n] & mask, m);
+    }
+}
+
+// tables.rs:39-54 (u -> prefix, prefix_len, suffix, suffix_len)
+__device__ __forceinline__ void uvlc_code(uint32_t u, uint32_t* pf, uint32_t* pl, uint32_t* sf, uint32_t* sl) {
+  if (u == 0) { *pf = 0; *pl = 0; *sf = 0; *sl = 0; }
+  else if (u == 1) { *pf = 1; *pl = 1; *sf = 0; *sl = 0; }
+  else if (u == 2) { *pf = 2; *pl = 2; *sf = 0; *sl = 0; }
+  else if (u <= 4) { *pf = 4; *pl = 3; *sf = u - 3; *sl = 1; }
+  else { *pf = 0; *pl = 3; *sf = u - 5; *sl = 5; }
+}
+
+__device__ __forceinline__ void vlc_pair(VlcEnc& v, uint32_t u0, uint32_t u1) {
+  uint32_t p0, l0, s0, sl0, p1, l1, s1, sl1;
+  uvlc_code(u0, &p0, &l0, &s0, &sl0);
+  uvlc_code(u1, &p1, &l1, &s1, &sl1);
+  v.encode(p0, l0);
+  v.encode(p1, l1);
+  v.encode(s0, sl0);
+  v.encode(s1, sl1);
+}
+
+__device__ void encode_one(const HtEncTask& t, uint8_t* le, uint8_t* lc) {
+  auto EV = [&](uint32_t k) -> uint8_t& { return le[k * 64]; };
+  auto CX = [&](uint32_t k) -> uint8_t& { return lc[k * 64]; };
+  const uint32_t width = t.w, height = t.h;
+  const uint32_t missing_msbs = (uint32_t)t.kmax - 1;
+  const uint32_t p = 30 - missing_msbs, shift = 31 - t.kmax;
+  MelEnc mel;
+  mel.init(t.slot + t.ms_cap);
+  VlcEnc vlc;
+  vlc.init(t.slot + t.ms_cap + kMelCap);
+  MsEnc ms;
+  ms.init(t.slot, t.ms_cap);
+  uint32_t cols = (width + 1) / 2;
+  for (uint32_t k = 0; k < cols + 4; ++k) {
+    EV(k) = 0;
+    CX(k) = 0;
+  }
+  uint32_t any = 0;
+  // ---- initial row (kappa = 1), encode.rs:150-238
+  uint32_t c_q0 = 0, lep = 0, lcxp = 0;
+  for (uint32_t x = 0; x < width; x += 4) {
+    EQuad q0 = load_quad(t, x, 0, p, shift);
+    any |= q0.rho;
+    uint32_t uq0 = q0.e_qmax > 1 ? q0.e_qmax : 1;
+    uint32_t u_q0 = uq0 - 1, u_q1 = 0;
+    uint32_t eps0 = quad_eps(q0, u_q0);
+    EV(lep) = max(uint32_t(EV(lep)), q0.e_q[1]);
+    lep += 1;
+    EV(lep) = (uint8_t)q0.e_q[3];
+    CX(lcxp) |= (uint8_t)((q0.rho & 2) >> 1);
+    lcxp += 1;
+    CX(lcxp) = (uint8_t)((q0.rho & 8) >> 3);
+    uint32_t tuple0 = c_enc0[(c_q0 << 8) + (q0.rho << 4) + eps0];
+    vlc.encode(tuple0 >> 8, (tuple0 >> 4) & 7);
+    if (c_q0 == 0) mel.encode(q0.rho != 0);
+    emit_ms(ms, q0, tuple0, uq0);
+    if (x + 2 < width) {
+      EQuad q1 = load_quad(t, x + 2, 0, p, shift);
+      any |= q1.rho;
+      uint32_t c_q1 = (q0.rho >> 1) | (q0.rho & 1);
+      uint32_t uq1 = q1.e_qmax > 1 ? q1.e_qmax : 1;
+      u_q1 = uq1 - 1;
+      uint32_t eps1 = quad_eps(q1, u_q1);
+      EV(lep) = max(uint32_t(EV(lep)), q1.e_q[1]);
+      lep += 1;
+      EV(lep) = (uint8_t)q1.e_q[3];
+      CX(lcxp) |= (uint8_t)((q1.rho & 2) >> 1);
+      lcxp += 1;
+      CX(lcxp) = (uint8_t)((q1.rho & 8) >> 3);
+      uint32_t tuple1 = c_enc0[(c_q1 << 8) + (q1.rho << 4) + eps1];
+      vlc.encode(tuple1 >> 8, (tuple1 >> 4) & 7);
+      if (c_q1 == 0) mel.encode(q1.rho != 0);
+      emit_ms(ms, q1, tuple1, uq1);
+      c_q0 = (q1.rho >> 1) | (q1.rho & 1);
+    } else {
+      c_q0 = 0;
+    }
+    if (u_q0 > 0 && u_q1 > 0) mel.encode((u_q0 < u_q1 ? u_q0 : u_q1) > 2);
+    if (u_q0 > 2 && u_q1 > 2) {
+      vlc_pair(vlc, u_q0 - 2, u_q1 - 2);
+    } else if (u_q0 > 2 && u_q1 > 0) {
+      uint32_t pf, pl, sf, sl;
+      uvlc_code(u_q0, &pf, &pl, &sf, &sl);
+      vlc.encode(pf, pl);
+      vlc.encode(u_q1 - 1, 1);
+      vlc.encode(sf, sl);
+    } else {
+      vlc_pair(vlc, u_q0, u_q1);
+    }
+  }
+  EV(lep + 1) = 0;
+  // ---- non-initial rows, encode.rs:241-309
+  for (uint32_t y = 2; y < height; y += 2) {
+    lep = 0;
+    int max_e = int(max(uint32_t(EV(0)), uint32_t(EV(1)))) - 1;
+    EV(0) = 0;
+    lcxp = 0;
+    c_q0 = uint32_t(CX(0)) + (uint32_t(CX(1)) << 2);
+    CX(0) = 0;
+    for (uint32_t x = 0; x < width; x += 4) {
+      EQuad q0 = load_quad(t, x, y, p, shift);
+      any |= q0.rho;
+      uint32_t kappa = (q0.rho & (q0.rho - 1)) != 0 ? uint32_t(max_e > 1 ? max_e : 1) : 1;
+      uint32_t uq0 = q0.e_qmax > kappa ? q0.e_qmax : kappa;
+      uint32_t u_q0 = uq0 - kappa, u_q1 = 0;
+      uint32_t eps0 = quad_eps(q0, u_q0);
+      EV(lep) = max(uint32_t(EV(lep)), q0.e_q[1]);
+      lep += 1;
+      max_e = int(max(uint32_t(EV(lep)), uint32_t(EV(lep + 1)))) - 1;
+      EV(lep) = (uint8_t)q0.e_q[3];
+      CX(lcxp) |= (uint8_t)((q0.rho & 2) >> 1);
+      lcxp += 1;
+      uint32_t c_q1 = uint32_t(CX(lcxp)) + (uint32_t(CX(lcxp + 1)) << 2);
+      CX(lcxp) = (uint8_t)((q0.rho & 8) >> 3);
+      uint32_t tuple0 = c_enc1[(c_q0 << 8) + (q0.rho << 4) + eps0];
+      vlc.encode(tuple0 >> 8, (tuple0 >> 4) & 7);
+      if (c_q0 == 0) mel.encode(q0.rho != 0);
+      emit_ms(ms, q0, tuple0, uq0);
+      if (x + 2 < width) {
+        EQuad q1 = load_quad(t, x + 2, y, p, shift);
+        any |= q1.rho;
+        uint32_t kappa1 = (q1.rho & (q1.rho - 1)) != 0 ? uint32_t(max_e > 1 ? max_e : 1) : 1;
+        c_q1 |= ((q0.rho & 4) >> 1) | ((q0.rho & 8) >> 2);
+        uint32_t uq1 = q1.e_qmax > kappa1 ? q1.e_qmax : kappa1;
+        u_q1 = uq1 - kappa1;
+        uint32_t eps1 = quad_eps(q1, u_q1);
+        EV(lep) = max(uint32_t(EV(lep)), q1.e_q[1]);
+        lep += 1;
+        max_e = int(max(uint32_t(EV(lep)), uint32_t(EV(lep + 1)))) - 1;
+        EV(lep) = (uint8_t)q1.e_q[3];
+        CX(lcxp) |= (uint8_t)((q1.rho & 2) >> 1);
+        lcxp += 1;
+        c_q0 = uint32_t(CX(lcxp)) + (uint32_t(CX(lcxp + 1)) << 2);
+        CX(lcxp) = (uint8_t)((q1.rho & 8) >> 3);
+        uint32_t tuple1 = c_enc1[(c_q1 << 8) + (q1.rho << 4) + eps1];
+        vlc.encode(tuple1 >> 8, (tuple1 >> 4) & 7);
+        if (c_q1 == 0) mel.encode(q1.rho != 0);
+        emit_ms(ms, q1, tuple1, uq1);
+        c_q0 |= ((q1.rho & 4) >> 1) | ((q1.rho & 8) >> 2);
+      } else {
+        c_q0 = c_q1;
+      }
+      vlc_pair(vlc, u_q0, u_q1);
+    }
+  }
+  if (!any) {
+    t.lens[0] = t.lens[1] = t.lens[2] = t.lens[3] = 0;
+    return;
+  }
+  // ---- terminate (mel.rs:247-261, streams.rs:348-372, 421-440)
+  if (mel.run > 0) mel.emit_bit(1);
+  uint32_t mel_tmp = mel.tmp << mel.remaining;
+  uint32_t mel_mask = (0xFFu << mel.remaining) & 0xFF;
+  uint32_t vlc_mask = vlc.used == 0 ? 0 : (0xFFu >> (8 - vlc.used));
+  if ((mel_mask | vlc_mask) != 0) {
+    uint32_t fuse = mel_tmp | vlc.tmp;
+    if ((((fuse ^ mel_tmp) & mel_mask) | ((fuse ^ vlc.tmp) & vlc_mask)) == 0 && fuse != 0xFF && vlc.len > 1) {
+      if (mel.len < kMelCap) mel.buf[mel.len] = (uint8_t)fuse;
+      mel.len += 1;
+    } else {
+      if (mel.len < kMelCap) mel.buf[mel.len] = (uint8_t)mel_tmp;
+      mel.len += 1;
+      if (vlc.len < kVlcCap) vlc.buf[vlc.len] = (uint8_t)vlc.tmp;
+      vlc.len += 1;
+    }
+  }
+  ms.terminate();
+  uint32_t scup = mel.len + vlc.len;
+  if (scup < 2 || scup > 4079 || mel.len > kMelCap || vlc.len > kVlcCap || ms.len > ms.cap) {
+    if (t.err) atomicCAS(t.err, 0, kErrEncodeScup);
+    t.lens[0] = t.lens[1] = t.lens[2] = t.lens[3] = 0;
+    return;
+  }
+  // Scup patch on the last two bytes of [MagSgn][MEL][VLC reversed]: the last
+  // byte is the VLC placeholder (production index 0).
+  uint32_t lens[3] = {ms.len, mel.len, vlc.len};
+  uint32_t total = ms.len + mel.len + vlc.len;
+  vlc.buf[0] = uint8_t(scup >> 4);
+  uint32_t k2 = total - 2;
+  uint8_t* b2;
+  if (k2 < lens[0]) b2 = t.slot + k2;
+  else if (k2 < lens[0] + lens[1]) b2 = t.slot + t.ms_cap + (k2 - lens[0]);
+  else b2 = vlc.buf + (lens[2] - 1 - (k2 - lens[0] - lens[1]));
+  *b2 = uint8_t((*b2 & 0xF0) | (scup & 0xF));
+  t.lens[0] = ms.len;
+  t.lens[1] = mel.len;
+  t.lens[2] = vlc.len;
+  t.lens[3] = 1;
+}
+
+__global__ __launch_bounds__(64) void ht_encode_kernel(const HtEncTask* __restrict__ tasks, uint64_t n) {
+  __shared__ uint8_t le[36 * 64];
+  __shared__ uint8_t lc[36 * 64];
+  uint64_t i = blockIdx.x * (uint64_t)64 + threadIdx.x;
+  if (i >= n) return;
+  HtEncTask t = tasks[i];
+  encode_one(t, le + threadIdx.x, lc + threadIdx.x);
+}
+
+}  // namespace
+
+void ht_init_tables() {
+  static bool done = false;
+  if (done) return;
+  std::vector<uint16_t> d0(1024), d1(1024), e0(2048), e1(2048), u0(320), u1(256);
+  derive_dec(kCxtVlcRows0, d0.data());
+  derive_dec(kCxtVlcRows1, d1.data());
+  derive_enc(kCxtVlcRows0, e0.data());
+  derive_enc(kCxtVlcRows1, e1.data());
+  derive_uvlc(u0.data(), 5, true);
+  derive_uvlc(u1.data(), 4, false);
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dec0), d0.data(), 2048));
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dec1), d1.data(), 2048));
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_enc0), e0.data(), 4096));
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_enc1), e1.data(), 4096));
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_uvlc0), u0.data(), 640));
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_uvlc1), u1.data(), 512));
+  done = true;
+}
+
+void launch_ht_decode(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(ht_decode_kernel, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, n);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_ht_encode(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(ht_encode_kernel, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, n);
+  NDIC_HIP(hipGetLastError());
+}
+
+}  // namespace ndic

@@ -1,0 +1,54 @@
+// HT (FBCOT) block coder on the device (definitions in ht.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace ndic {
+
+// One code-block to decode: the packet parser fills these.
+struct HtDecTask {
+  const uint8_t* seg;   // codeword segment (cleanup then refinement bytes)
+  void* out;            // top-left output sample
+  int32_t* err;         // error slot of the owning chunk (first error wins)
+  uint32_t stride;      // output row pitch in samples
+  uint32_t len1, len2;  // cleanup / refinement lengths
+  uint16_t w, h;
+  uint8_t mmsb, passes, kmax;
+  uint8_t flags;        // kTaskIncluded | kTaskSignMag
+  uint32_t err_offset;  // byte offset reported with an error
+};
+
+// One code-block to encode.
+struct HtEncTask {
+  const int32_t* in;    // top-left coefficient in the Mallat plane
+  uint8_t* slot;        // [ms_cap bytes MagSgn][4096 MEL][4096 VLC, production order]
+  uint32_t* lens;       // out: {ms_len, mel_len, vlc_len, included}
+  int32_t* err;
+  uint32_t stride;
+  uint32_t ms_cap;
+  uint16_t w, h;
+  uint8_t kmax, flags;  // kTaskSignMag: `in` holds sign-magnitude words
+  uint16_t pad;
+};
+
+enum : uint8_t { kTaskIncluded = 1, kTaskSignMag = 2 };
+constexpr uint32_t kMelCap = 4096, kVlcCap = 4096;
+
+// Upper bound of the MagSgn bytes of a w x h block at K_max.
+__host__ __device__ inline uint32_t ht_ms_cap(uint32_t w, uint32_t h, uint32_t kmax) {
+  return (uint32_t)((uint64_t)w * h * (kmax + 1) / 7 + 16);
+}
+
+void ht_init_tables();
+void launch_ht_decode(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
+void launch_ht_encode(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream);
+// Byte k of an encoded segment from its three pieces (see HtEncTask::slot).
+__host__ __device__ inline uint8_t ht_seg_byte(const uint8_t* slot, uint32_t ms_cap, const uint32_t* lens, uint32_t k) {
+  if (k < lens[0]) return slot[k];
+  k -= lens[0];
+  if (k < lens[1]) return slot[ms_cap + k];
+  k -= lens[1];
+  return slot[ms_cap + kMelCap + (lens[2] - 1 - k)];
+}
+
+}  // namespace ndic

@@ -1,0 +1,276 @@
+// nd_lift on the device: delta / haar / 5-3 lifting along one axis of many
+// chunks per launch.
+//
+// Layout: a chunk's coefficient plane is row-major, viewed per step as
+// (outer, len, inner) (crates/ndic-lift/src/chunk.rs:110-150).  The axis
+// samples of one column j are `inner` elements apart, so a wave that takes
+// 64 consecutive columns reads and writes 256-byte contiguous rows: the sweep
+// is coalesced without any transpose.  Each lane owns one column of one group
+// segment and runs every level of the step on it; the segment is staged in
+// LDS (two buffers: samples and the level's subband output), or, for segments
+// longer than the LDS budget, in global memory with a chunk-sized temporary.
+// Lanes never touch each other's columns, so no barriers are needed.
+//
+// Arithmetic follows crates/ndic-lift/src/kernel.rs:66-128: plain arithmetic
+// forward (the budget check has run), wrapping arithmetic inverse.  The first
+// forward step can widen the dtype input on load; the last inverse step can
+// narrow to the dtype on store and flags out-of-range coefficients.
+#include "lift.hpp"
+
+namespace ndic {
+
+namespace {
+
+template <class T> struct U_;
+template <> struct U_<int32_t> { typedef uint32_t t; };
+template <> struct U_<int64_t> { typedef uint64_t t; };
+template <class T> __device__ __forceinline__ T wadd(T a, T b) { typedef typename U_<T>::t U; return (T)((U)a + (U)b); }
+template <class T> __device__ __forceinline__ T wsub(T a, T b) { typedef typename U_<T>::t U; return (T)((U)a - (U)b); }
+
+__device__ __forceinline__ int64_t load_dtype(const uint8_t* p, uint32_t dtype) {
+  switch (dtype) {
+    case NDIC_U8: return p[0];
+    case NDIC_I8: return (int8_t)p[0];
+    case NDIC_U16: return *(const uint16_t*)p;
+    case NDIC_I16: return *(const int16_t*)p;
+    case NDIC_U32: return *(const uint32_t*)p;
+    case NDIC_I32: return *(const int32_t*)p;
+    default: return *(const int64_t*)p;  // u64 / i64 (range-checked before)
+  }
+}
+
+// Narrowing store; returns false when v does not fit the dtype.
+template <class T> __device__ __forceinline__ bool store_dtype(uint8_t* p, uint32_t dtype, T v) {
+  switch (dtype) {
+    case NDIC_U8: *p = (uint8_t)v; return v >= 0 && v <= 255;
+    case NDIC_I8: *p = (uint8_t)v; return v >= -128 && v <= 127;
+    case NDIC_U16: *(uint16_t*)p = (uint16_t)v; return v >= 0 && v <= 65535;
+    case NDIC_I16: *(int16_t*)p = (int16_t)v; return v >= -32768 && v <= 32767;
+    case NDIC_U32: *(uint32_t*)p = (uint32_t)v; return v >= 0;
+    case NDIC_I32: *(int32_t*)p = (int32_t)v; return true;
+    case NDIC_U64: *(uint64_t*)p = (uint64_t)v; return v >= 0;
+    default: *(int64_t*)p = (int64_t)v; return true;
+  }
+}
+
+// A column of one segment: element k at base[k * stride].
+template <class T> struct Col {
+  T* p;
+  uint64_t stride;
+  __device__ __forceinline__ T& operator[](uint64_t k) const { return p[k * stride]; }
+};
+
+__device__ __forceinline__ uint32_t band_count(uint64_t n, uint32_t levels) {
+  uint32_t c = 0;
+  uint64_t m = n;
+  for (uint32_t l = 0; l < levels && m >= 2; ++l) {
+    ++c;
+    m = (m + 1) / 2;
+  }
+  return c;
+}
+
+// kernel.rs:167-183, 214-240 (one forward level over x[0..n), subband order)
+template <class T> __device__ void level_fwd(uint32_t kind, Col<T> x, Col<T> tmp, uint64_t n) {
+  uint64_t ns = (n + 1) / 2, nd = n / 2;
+  if (kind == 1) {
+    for (uint64_t i = 0; i < nd; ++i) {
+      T x0 = x[2 * i], x1 = x[2 * i + 1];
+      T d = x1 - x0;
+      tmp[ns + i] = d;
+      tmp[i] = x0 + (d >> 1);
+    }
+    if (n & 1) tmp[ns - 1] = x[n - 1];
+  } else {
+    for (uint64_t i = 0; i < nd; ++i) {
+      uint64_t r = 2 * i + 2 < n ? 2 * i + 2 : 2 * n - 2 - (2 * i + 2);
+      tmp[ns + i] = x[2 * i + 1] - ((x[2 * i] + x[r] + 1) >> 1);
+    }
+    for (uint64_t i = 0; i < ns; ++i) {
+      uint64_t il = i == 0 ? 0 : i - 1, ir = i < nd - 1 ? i : nd - 1;
+      tmp[i] = x[2 * i] + ((tmp[ns + il] + tmp[ns + ir] + 2) >> 2);
+    }
+  }
+  for (uint64_t i = 0; i < n; ++i) x[i] = tmp[i];
+}
+
+// kernel.rs:186-200, 243-272 (one inverse level, wrapping arithmetic)
+template <class T> __device__ void level_inv(uint32_t kind, Col<T> x, Col<T> tmp, uint64_t n) {
+  uint64_t ns = (n + 1) / 2, nd = n / 2;
+  if (kind == 1) {
+    for (uint64_t i = 0; i < nd; ++i) {
+      T d = x[ns + i];
+      T x0 = wsub(x[i], (T)(d >> 1));
+      tmp[2 * i] = x0;
+      tmp[2 * i + 1] = wadd(x0, d);
+    }
+    if (n & 1) tmp[n - 1] = x[ns - 1];
+  } else {
+    for (uint64_t i = 0; i < ns; ++i) {
+      uint64_t il = i == 0 ? 0 : i - 1, ir = i < nd - 1 ? i : nd - 1;
+      tmp[2 * i] = wsub(x[i], (T)(wadd(wadd(x[ns + il], x[ns + ir]), (T)2) >> 2));
+    }
+    for (uint64_t i = 0; i < nd; ++i) {
+      uint64_t r = 2 * i + 2 < n ? 2 * i + 2 : 2 * n - 2 - (2 * i + 2);
+      tmp[2 * i + 1] = wadd(x[ns + i], (T)(wadd(wadd(tmp[2 * i], tmp[r]), (T)1) >> 1));
+    }
+  }
+  for (uint64_t i = 0; i < n; ++i) x[i] = tmp[i];
+}
+
+template <class T> __device__ void transform_column(const LiftJob& job, bool fwd, Col<T> x, Col<T> tmp, uint64_t n) {
+  if (n < 2) return;
+  if (job.kind == 0) {
+    if (fwd) {
+      for (uint64_t k = n - 1; k >= 1; --k) x[k] = x[k] - x[k - 1];
+    } else {
+      for (uint64_t k = 1; k < n; ++k) x[k] = wadd(x[k], x[k - 1]);
+    }
+    return;
+  }
+  uint32_t bc = band_count(n, job.levels);
+  if (fwd) {
+    for (uint32_t k = 0; k < bc; ++k) level_fwd<T>(job.kind, x, tmp, (n + (1ull << k) - 1) >> k);
+  } else {
+    for (uint32_t k = bc; k-- > 0;) level_inv<T>(job.kind, x, tmp, (n + (1ull << k) - 1) >> k);
+  }
+}
+
+__device__ __forceinline__ uint32_t find_job(const LiftJob* jobs, uint32_t njobs, uint64_t item) {
+  uint32_t lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) / 2;
+    if (jobs[mid].item_begin <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// One wave (64 lanes) per work item: a tile of 64 columns of one group
+// segment of one outer slab of one chunk-step.
+template <class T, bool kLds>
+__global__ __launch_bounds__(64) void lift_step_kernel(const LiftJob* __restrict__ jobs, uint32_t njobs, bool fwd,
+                                                       uint32_t lds_n) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  uint64_t item = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  uint32_t ji = find_job(jobs, njobs, item);
+  const LiftJob job = jobs[ji];
+  uint64_t local = item - job.item_begin;
+  if (local >= job.items) return;
+  uint64_t t = local % job.tiles, rest = local / job.tiles;
+  uint64_t s = rest % job.segs, o = rest / job.segs;
+  uint64_t j = t * 64 + threadIdx.x;
+  if (j >= job.inner) return;
+  uint64_t g0 = s * job.group;
+  uint64_t n = job.len - g0 < job.group ? job.len - g0 : job.group;
+  uint64_t elem = o * job.len * job.inner + g0 * job.inner + j;
+  T* plane = (T*)job.plane;
+  Col<T> g{plane + elem, job.inner};
+
+  if (kLds) {
+    T* lx = (T*)lds_raw;
+    Col<T> x{lx + threadIdx.x, 64};
+    Col<T> tmp{lx + (uint64_t)lds_n * 64 + threadIdx.x, 64};
+    if (fwd && job.src) {
+      for (uint64_t k = 0; k < n; ++k) x[k] = (T)load_dtype(job.src + (elem + k * job.inner) * job.dsize, job.dtype);
+    } else {
+      for (uint64_t k = 0; k < n; ++k) x[k] = g[k];
+    }
+    if (job.kind != 3) transform_column<T>(job, fwd, x, tmp, n);
+    if (!fwd && job.dst) {
+      bool ok = true;
+      for (uint64_t k = 0; k < n; ++k) ok &= store_dtype<T>(job.dst + (elem + k * job.inner) * job.dsize, job.dtype, x[k]);
+      if (!ok) atomicOr(job.flag, 1);
+    } else {
+      for (uint64_t k = 0; k < n; ++k) g[k] = x[k];
+    }
+  } else {
+    Col<T> tmp{(T*)job.tmp + elem, job.inner};
+    if (fwd && job.src)
+      for (uint64_t k = 0; k < n; ++k) g[k] = (T)load_dtype(job.src + (elem + k * job.inner) * job.dsize, job.dtype);
+    if (job.kind != 3) transform_column<T>(job, fwd, g, tmp, n);
+    if (!fwd && job.dst) {
+      bool ok = true;
+      for (uint64_t k = 0; k < n; ++k) ok &= store_dtype<T>(job.dst + (elem + k * job.inner) * job.dsize, job.dtype, g[k]);
+      if (!ok) atomicOr(job.flag, 1);
+    }
+  }
+}
+
+// Grid-stride widen (dtype -> plane) with per-chunk min/max for the budget
+// check, or narrow (plane -> dtype) with a range flag.
+template <class T>
+__global__ void convert_kernel(const uint8_t* __restrict__ src, uint32_t src_dtype, T* __restrict__ plane,
+                               uint8_t* __restrict__ dst, uint32_t dst_dtype, uint64_t n, int64_t* minmax,
+                               int32_t* flag) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  int64_t lo = INT64_MAX, hi = INT64_MIN;
+  bool ok = true;
+  size_t ss = src ? (src_dtype == 0 || src_dtype == 1 ? 1 : src_dtype < 4 ? 2 : src_dtype < 6 ? 4 : 8) : 0;
+  size_t ds = dst ? (dst_dtype == 0 || dst_dtype == 1 ? 1 : dst_dtype < 4 ? 2 : dst_dtype < 6 ? 4 : 8) : 0;
+  for (; i < n; i += stride) {
+    T v;
+    if (src) {
+      int64_t w = load_dtype(src + i * ss, src_dtype);
+      if (src_dtype == NDIC_U64 && *(const uint64_t*)(src + i * 8) > (uint64_t)INT64_MAX) ok = false;
+      if (src_dtype == NDIC_U32 && w > (int64_t)INT32_MAX) ok = false;
+      v = (T)w;
+      if (plane) plane[i] = v;
+    } else {
+      v = plane[i];
+    }
+    lo = (int64_t)v < lo ? (int64_t)v : lo;
+    hi = (int64_t)v > hi ? (int64_t)v : hi;
+    if (dst) ok &= store_dtype<T>(dst + i * ds, dst_dtype, v);
+  }
+  if (minmax) {
+    // wave-level reduction then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+      int64_t a = __shfl_xor(lo, off), b = __shfl_xor(hi, off);
+      lo = a < lo ? a : lo;
+      hi = b > hi ? b : hi;
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin((long long*)&minmax[0], (long long)lo);
+      atomicMax((long long*)&minmax[1], (long long)hi);
+    }
+  }
+  if (!ok && flag) atomicOr(flag, 1);
+}
+
+}  // namespace
+
+size_t lift_lds_limit(bool wide) { return wide ? 64 : 128; }
+
+void launch_lift_step(const LiftJob* d_jobs, uint32_t njobs, uint64_t items, bool fwd, bool wide, uint32_t max_n,
+                      hipStream_t stream) {
+  if (items == 0) return;
+  uint32_t gx = (uint32_t)(items < 65535 ? items : 65535);
+  uint32_t gy = (uint32_t)ceil_div(items, gx);
+  bool lds = max_n <= lift_lds_limit(wide);
+  size_t shmem = lds ? (size_t)2 * max_n * 64 * (wide ? 8 : 4) : 0;
+  if (wide) {
+    if (lds) hipLaunchKernelGGL((lift_step_kernel<int64_t, true>), dim3(gx, gy), dim3(64), shmem, stream, d_jobs, njobs, fwd, max_n);
+    else hipLaunchKernelGGL((lift_step_kernel<int64_t, false>), dim3(gx, gy), dim3(64), 0, stream, d_jobs, njobs, fwd, max_n);
+  } else {
+    if (lds) hipLaunchKernelGGL((lift_step_kernel<int32_t, true>), dim3(gx, gy), dim3(64), shmem, stream, d_jobs, njobs, fwd, max_n);
+    else hipLaunchKernelGGL((lift_step_kernel<int32_t, false>), dim3(gx, gy), dim3(64), 0, stream, d_jobs, njobs, fwd, max_n);
+  }
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_convert(const uint8_t* src, uint32_t src_dtype, void* plane, bool wide, uint8_t* dst, uint32_t dst_dtype,
+                    uint64_t n, int64_t* minmax, int32_t* flag, hipStream_t stream) {
+  if (n == 0) return;
+  uint32_t blocks = (uint32_t)(ceil_div(n, 256) < 4096 ? ceil_div(n, 256) : 4096);
+  if (wide)
+    hipLaunchKernelGGL(convert_kernel<int64_t>, dim3(blocks), dim3(256), 0, stream, src, src_dtype, (int64_t*)plane,
+                       dst, dst_dtype, n, minmax, flag);
+  else
+    hipLaunchKernelGGL(convert_kernel<int32_t>, dim3(blocks), dim3(256), 0, stream, src, src_dtype, (int32_t*)plane,
+                       dst, dst_dtype, n, minmax, flag);
+  NDIC_HIP(hipGetLastError());
+}
+
+}  // namespace ndic

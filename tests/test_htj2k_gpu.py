@@ -1,0 +1,212 @@
+"""The htj2k path on the GPU against the oracle: byte-identical encode,
+bit-exact decode.
+
+Mirrors the reference's tests: block round trips
+(crates/ndic-htj2k/tests/block_roundtrip.rs:110-140), DWT round trips on odd
+and degenerate shapes (crates/ndic-htj2k/src/dwt/mod.rs:430-494), the
+byte-stable fixtures (crates/ndic-codestream/tests/fixtures.rs:25-73,
+crates/ndic-zarr/tests/htj2k_zarrs.rs:373-398), the dtype x lift matrix of
+crates/ndic-zarr/tests/htj2k_zarrs.rs:47-125, and prefix decodes equal to
+the downsampled full decode (crates/ndic-codestream/tests/range_plans.rs:33-116).
+"""
+
+from __future__ import annotations
+
+import struct
+
+import numpy as np
+import pytest
+
+import devhooks
+import datagen
+
+pytestmark = pytest.mark.gpu
+
+NPT = {"u8": np.uint8, "i8": np.int8, "u16": np.uint16, "i16": np.int16, "u32": np.uint32, "i32": np.int32}
+NAME = {"u8": "uint8", "i8": "int8", "u16": "uint16", "i16": "int16", "u32": "uint32", "i32": "int32"}
+
+
+def _sm_block(rng, w, h, kmax, dens):
+    mag = rng.integers(0, 2 ** kmax, size=(h, w)).astype(np.int64)
+    mag[rng.uniform(size=(h, w)) > dens] = 0
+    if mag.max() == 0:
+        mag[0, 0] = 1
+    sign = rng.integers(0, 2, size=(h, w)).astype(np.uint32) << 31
+    sm = (sign | (mag.astype(np.uint32) << (31 - kmax))).astype(np.uint32)
+    sm[mag == 0] = 0
+    return sm
+
+
+def test_ht_block_encode_matches_oracle(ndic, oracle):
+    rng = np.random.default_rng(5)
+    for i in range(120):
+        w, h = int(rng.integers(1, 65)), int(rng.integers(1, 65))
+        if i < 8:
+            w, h = 64, 64
+        kmax = int(rng.integers(2, 29))
+        sm = _sm_block(rng, w, h, kmax, rng.uniform(0, 1))
+        exp = oracle.ht_encode_block(sm, kmax - 1)
+        got = devhooks.ht_encode_block(ndic, sm, kmax - 1)
+        assert got == exp, (w, h, kmax)
+        dec = devhooks.ht_decode_block(ndic, exp, w, h, kmax - 1)
+        assert np.array_equal(dec, oracle.ht_decode_block(exp, w, h, kmax - 1)), (w, h, kmax)
+
+
+def test_dwt_matches_oracle(ndic, oracle):
+    rng = np.random.default_rng(2)
+    for (h, w) in [(1, 1), (1, 9), (9, 1), (2, 2), (3, 5), (33, 65), (64, 64), (97, 311), (256, 256), (130, 7)]:
+        for lv in [0, 1, 2, 3, 5, 7]:
+            p = rng.integers(-30000, 30000, size=(h, w)).astype(np.int32)
+            f = devhooks.dwt53(ndic, True, p, lv)
+            assert np.array_equal(f, oracle.dwt53(True, p, lv)), (h, w, lv)
+            assert np.array_equal(devhooks.dwt53(ndic, False, f, lv), p), (h, w, lv)
+
+
+def test_codestream_fixture(ndic, golden):
+    j2c = (golden / "tiny-gradient-8x8.j2c").read_bytes()
+    img = np.array([(7 * (i % 8) + 13 * (i // 8)) % 256 for i in range(64)], np.int32).reshape(8, 8)
+    assert np.array_equal(ndic.htj2k_decode_plane(j2c), img)
+    # re-encode through the chunk core: a 1-plane u8 chunk carries the same codestream
+    chunk = ndic.htj2k_encode(img.astype(np.uint8), (8, 8), "uint8", xy_levels=1)
+    hdr = (8 + 4 * 2) + (12 + 4 * 2)  # fixed header (ndim 2) + one index entry (levels 1)
+    assert chunk[hdr:] == j2c
+
+
+def test_ndht_fixture(ndic, golden):
+    nd = (golden / "tiny-chunk-4x8x8.ndht").read_bytes()
+    vals = np.array([(i * 7) % 4096 for i in range(256)], np.uint16)
+    assert ndic.htj2k_encode(vals, (4, 8, 8), "uint16", xy_levels=2) == nd
+    assert ndic.htj2k_decode(nd, (4, 8, 8), "uint16") == vals.tobytes()
+
+
+@pytest.mark.parametrize("dt", list(NPT))
+@pytest.mark.parametrize("levels", [0, 1, 3, 5])
+def test_chunk_roundtrip_matches_oracle(ndic, oracle, dt, levels):
+    rng = np.random.default_rng(levels * 10 + len(dt))
+    info = np.iinfo(NPT[dt])
+    lo, hi = max(info.min, -(2 ** 20)), min(info.max, 2 ** 20)
+    for shape in [(3, 33, 61), (1, 64, 64), (2, 97, 131), (5, 1, 9), (2, 7, 1)]:
+        arr = rng.integers(lo, hi, size=shape, endpoint=True).astype(NPT[dt])
+        if rng.uniform() < 0.3:
+            arr[:] = 0
+        raw = arr.tobytes()
+        exp = oracle.htj2k_encode_chunk(raw, shape, dt, xy_levels=levels)
+        got = ndic.htj2k_encode(raw, shape, NAME[dt], xy_levels=levels)
+        assert got == exp, (dt, levels, shape)
+        assert ndic.htj2k_decode(exp, shape, NAME[dt]) == raw, (dt, levels, shape)
+
+
+def test_config0_chunk(ndic, oracle):
+    vol = datagen.c0_volume((32, 256, 256), seed=0)
+    steps = [{"dimension": 0, "kind": "lift53", "levels": 2, "group": 0}]
+    coeffs = oracle.lift_chunk(True, vol.tobytes(), vol.shape, "u16", steps)
+    exp = oracle.htj2k_encode_chunk(coeffs, vol.shape, "i32", xy_levels=5)
+    got = ndic.htj2k_encode(coeffs, vol.shape, "int32", xy_levels=5)
+    assert got == exp
+    assert ndic.htj2k_decode(exp, vol.shape, "int32") == coeffs
+
+
+def test_no_index_and_progressions(ndic, oracle):
+    rng = np.random.default_rng(4)
+    arr = rng.integers(0, 4096, size=(3, 40, 70)).astype(np.uint16)
+    for prog in ["LRCP", "RLCP", "RPCL", "PCRL", "CPRL"]:
+        code = ["LRCP", "RLCP", "RPCL", "PCRL", "CPRL"].index(prog)
+        for index in [True, False]:
+            exp = oracle.htj2k_encode_chunk(arr.tobytes(), arr.shape, "u16", xy_levels=3, progression=code,
+                                            index=index)
+            got = ndic.htj2k_encode(arr, arr.shape, "uint16", xy_levels=3, progression=prog, index=index)
+            assert got == exp, (prog, index)
+            assert ndic.htj2k_decode(exp, arr.shape, "uint16") == arr.tobytes()
+
+
+def test_thumbnail_prefix_decode(ndic, oracle):
+    rng = np.random.default_rng(8)
+    arr = rng.integers(0, 4096, size=(2, 200, 300)).astype(np.uint16)
+    chunk = ndic.htj2k_encode(arr, arr.shape, "uint16", xy_levels=5)
+    ndim, levels = 3, 5
+    entry = 12 + 4 * (levels + 1)
+    off, ln, *prefix = struct.unpack_from("<QI6I", chunk, 8 + 4 * ndim)
+    plane = chunk[off:off + ln]
+    full = ndic.htj2k_decode_plane(plane)
+    assert np.array_equal(full, arr[0].astype(np.int32))
+    for r in range(levels + 1):
+        exp = oracle.decode_codestream(plane[:prefix[r]], max_res=r, prefix=True)
+        got = ndic.htj2k_decode_plane(plane[:prefix[r]], max_res=r, prefix=True)
+        assert np.array_equal(got, exp), r
+        assert np.array_equal(ndic.htj2k_decode_plane(plane, max_res=r), exp), r
+
+
+def test_corrupt_chunks_error_not_crash(ndic, oracle):
+    rng = np.random.default_rng(12)
+    arr = rng.integers(0, 4096, size=(2, 64, 64)).astype(np.uint16)
+    chunk = bytearray(ndic.htj2k_encode(arr, arr.shape, "uint16", xy_levels=3))
+    with pytest.raises(ndic.NdicError):
+        ndic.htj2k_decode(bytes(chunk[:40]), arr.shape, "uint16")
+    with pytest.raises(ndic.NdicError):
+        ndic.htj2k_decode(b"nope" + bytes(chunk[4:]), arr.shape, "uint16")
+    with pytest.raises(ndic.NdicError):
+        ndic.htj2k_decode(bytes(chunk), (2, 64, 65), "uint16")
+    # single-byte mutations: an error or a (garbage) decode, never a crash
+    for k in range(200):
+        m = bytearray(chunk)
+        pos = int(rng.integers(0, len(m)))
+        m[pos] ^= int(rng.integers(1, 256))
+        try:
+            ndic.htj2k_decode(bytes(m), arr.shape, "uint16")
+        except ndic.NdicError:
+            pass
+
+
+def test_lift_ht_batch_matches_oracle(ndic, oracle):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(21)
+    kinds = ["delta", "haar", "lift53"]
+    specs = []
+    for i in range(24):
+        z = int(rng.integers(1, 9))
+        y, x = int(rng.integers(8, 140)), int(rng.integers(8, 140))
+        dt = ["uint8", "uint16", "int16"][i % 3]
+        lv = int(rng.integers(0, 6))
+        kind = kinds[i % 3]
+        npt = np.dtype(dt)
+        info = np.iinfo(npt)
+        vol = datagen.blobs_volume((z, y, x), n_blobs=3, base=info.max / 8, amp=(info.max / 10, info.max / 3),
+                                   lo=info.min, hi=info.max, seed=i, dtype=npt)
+        steps = [{"dimension": 0, "kind": kind, "levels": 2, "group": 0}]
+        chunk = oracle.lift_ht_encode(vol, {"uint8": "u8", "uint16": "u16", "int16": "i16"}[dt], steps, xy_levels=lv)
+        specs.append((vol, dt, steps, lv, chunk))
+    blob = b"".join(s[4] for s in specs)
+    d_in = torch.tensor(np.frombuffer(blob, np.uint8).copy(), device="cuda")
+    al = lambda n: (n + 15) // 16 * 16
+    total_out = sum(al(s[0].nbytes) for s in specs)
+    d_out = torch.zeros(total_out, dtype=torch.uint8, device="cuda")
+    descs = []
+    ioff = ooff = 0
+    for vol, dt, steps, lv, chunk in specs:
+        descs.append(ndic.make_desc(vol.shape, dt, steps, lv, ioff, len(chunk), ooff))
+        ioff += len(chunk)
+        ooff += al(vol.nbytes)
+    for plain in [False, True]:
+        d_out.zero_()
+        ndic.decode_batch(d_in.data_ptr(), d_out.data_ptr(), descs, plain=plain)
+        torch.cuda.synchronize()
+        out = d_out.cpu().numpy().tobytes()
+        o = 0
+        for vol, *_ in specs:
+            assert out[o:o + vol.nbytes] == vol.tobytes()
+            o += al(vol.nbytes)
+    # encode batch: byte-identical chunks
+    raw = b"".join(s[0].tobytes() + bytes(al(s[0].nbytes) - s[0].nbytes) for s in specs)
+    d_raw = torch.tensor(np.frombuffer(raw, np.uint8).copy(), device="cuda")
+    cap = len(blob) * 2 + (1 << 20)
+    d_enc = torch.zeros(cap, dtype=torch.uint8, device="cuda")
+    edescs = []
+    roff = 0
+    for vol, dt, steps, lv, chunk in specs:
+        edescs.append(ndic.make_desc(vol.shape, dt, steps, lv, roff, vol.nbytes, 0))
+        roff += al(vol.nbytes)
+    arr = (ndic.ChunkDesc * len(edescs))(*edescs)
+    ndic.encode_batch(d_raw.data_ptr(), d_enc.data_ptr(), cap, arr)
+    enc = d_enc.cpu().numpy().tobytes()
+    for d, s in zip(arr, specs):
+        assert enc[d.out_offset:d.out_offset + d.out_len] == s[4]
