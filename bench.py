@@ -37,44 +37,7 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def gen_chunk(torch, rng, shape, dtype, kind_of_content, dev):
-    """Blob + Poisson content of config 0 (12-bit) scaled to the dtype range."""
-    z, y, x = shape
-    if kind_of_content == "zero":
-        return torch.zeros(shape, dtype=torch.int32, device=dev)
-    info = np.iinfo(dtype)
-    if kind_of_content == "random":
-        g = torch.Generator(device=dev)
-        g.manual_seed(int(rng.integers(0, 2**31)))
-        return torch.randint(int(info.min), int(info.max) + 1, shape, generator=g, device=dev, dtype=torch.int64).to(torch.int32)
-    vol = torch.full(shape, 1000.0, dtype=torch.float32, device=dev)
-    zz = torch.arange(z, device=dev, dtype=torch.float32)
-    yy = torch.arange(y, device=dev, dtype=torch.float32)
-    xx = torch.arange(x, device=dev, dtype=torch.float32)
-    for _ in range(8):
-        cz, cy, cx = rng.uniform(0, 1, 3) * np.array([z, y, x])
-        a = float(rng.uniform(200, 2500))
-        s = float(rng.uniform(4, 20))
-        gz = torch.exp(-(zz - float(cz)) ** 2 / (2 * s * s))
-        gy = torch.exp(-(yy - float(cy)) ** 2 / (2 * s * s))
-        gx = torch.exp(-(xx - float(cx)) ** 2 / (2 * s * s))
-        vol += a * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
-    g = torch.Generator(device=dev)
-    g.manual_seed(int(rng.integers(0, 2**31)))
-    vol = torch.poisson(vol, generator=g)
-    v12 = torch.clamp(torch.round(vol), 0, 4095).to(torch.int32)
-    if dtype == np.uint8:
-        return v12 >> 4
-    if dtype == np.uint16:
-        return v12 << 4
-    return (v12 << 4) - 32768
-
-
-def torch_dtype(torch, dt):
-    return {np.uint8: torch.uint8, np.uint16: torch.int16, np.int16: torch.int16}[dt]
-
-
-def make_workload(torch, ndic, args, dev):
+def make_workload(dev, ndic, args):
     rng = np.random.default_rng(args.seed)
     kinds = ["delta", "haar", "lift53"]
     specs = []
@@ -83,61 +46,52 @@ def make_workload(torch, ndic, args, dev):
             y = int(round(math.exp(rng.uniform(math.log(64), math.log(1024)))))
             x = int(round(math.exp(rng.uniform(math.log(64), math.log(1024)))))
             z = int(rng.integers(1, 65))
-            dt = rng.choice([np.uint8, np.uint16, np.int16], p=[0.2, 0.5, 0.3])
+            dt = ["uint8", "uint16", "int16"][int(rng.choice(3, p=[0.2, 0.5, 0.3]))]
             kind = kinds[int(rng.integers(0, 3))]
             lv = int(rng.integers(0, 6))
             r = rng.uniform()
-            content = "zero" if r < 0.05 else ("random" if r < 0.10 else "blobs")
+            content = dev.SYNTH_ZERO if r < 0.05 else (dev.SYNTH_RANDOM if r < 0.10 else dev.SYNTH_BLOBS)
             steps = [{"dimension": 0, "kind": kind, "levels": 2, "group": 0}]
             shape = (z, y, x)
         else:  # c0: the 32x256x256 u16 chunk of the CPU-path parity config
-            dt, lv, content = np.uint16, 5, "blobs"
+            dt, lv, content = "uint16", 5, dev.SYNTH_BLOBS
             steps = [{"dimension": 0, "kind": "lift53", "levels": 2, "group": 0}]
             shape = (32, 256, 256)
-        specs.append((shape, dt, steps, lv, content))
+        # 8 blobs: centre uniform in the volume, amplitude U(200, 2500), sigma U(4, 20)
+        blobs = [tuple(float(v) for v in (*(rng.uniform(0, 1, 3) * np.array(shape)), rng.uniform(200, 2500),
+                                          rng.uniform(4, 20))) for _ in range(8)]
+        seed = int(rng.integers(0, 2**63))
+        specs.append((shape, dt, steps, lv, content, blobs, seed))
     sizes = [int(np.prod(s[0])) * np.dtype(s[1]).itemsize for s in specs]
     raw_total = sum(sizes)
     padded = sum((n + 15) // 16 * 16 for n in sizes)
     log(f"workload {args.workload}: {len(specs)} chunks, {raw_total / 1e9:.3f} GB raw")
-    d_raw = torch.zeros(padded, dtype=torch.uint8, device=dev)
+    d_raw = dev.DeviceBuffer(padded, zero=True)
     off = 0
     offsets = []
     t0 = time.time()
-    for shape, dt, steps, lv, content in specs:
-        v = gen_chunk(torch, rng, shape, dt, content, dev)
+    for shape, dt, steps, lv, content, blobs, seed in specs:
+        dev.synth_chunk(d_raw.ptr + off, shape, dt, content, blobs, seed)
         nb = int(np.prod(shape)) * np.dtype(dt).itemsize
-        if dt == np.uint8:
-            d_raw[off:off + nb] = v.to(torch.uint8).reshape(-1)
-        else:
-            d_raw[off:off + nb].view(torch.int16).copy_(v.to(torch.int16).reshape(-1))
         offsets.append(off)
         off += (nb + 15) // 16 * 16
-    torch.cuda.synchronize()
+    dev.synchronize()
     log(f"generated in {time.time() - t0:.1f}s")
     # encode on the GPU
     cap = int(raw_total * 2.5) + (256 << 20)
-    d_enc = torch.empty(cap, dtype=torch.uint8, device=dev)
+    d_enc = dev.DeviceBuffer(cap)
     edescs = (ndic.ChunkDesc * len(specs))(*[
-        ndic.make_desc(s[0], np.dtype(s[1]).name, s[2], s[3], offsets[i], 0, 0) for i, s in enumerate(specs)])
+        ndic.make_desc(s[0], s[1], s[2], s[3], offsets[i], 0, 0) for i, s in enumerate(specs)])
     t0 = time.time()
-    ndic.encode_batch(d_raw.data_ptr(), d_enc.data_ptr(), cap, edescs)
-    torch.cuda.synchronize()
+    ndic.encode_batch(d_raw.ptr, d_enc.ptr, cap, edescs)
+    dev.synchronize()
     enc_s = time.time() - t0
     comp_total = sum(d.out_len for d in edescs)
     log(f"encoded in {enc_s:.1f}s: {comp_total / 1e9:.3f} GB compressed ({comp_total / raw_total:.3f} of raw)")
     ddescs = (ndic.ChunkDesc * len(specs))(*[
-        ndic.make_desc(s[0], np.dtype(s[1]).name, s[2], s[3], edescs[i].out_offset, edescs[i].out_len, offsets[i])
+        ndic.make_desc(s[0], s[1], s[2], s[3], edescs[i].out_offset, edescs[i].out_len, offsets[i])
         for i, s in enumerate(specs)])
     return specs, d_raw, d_enc, ddescs, raw_total, comp_total, enc_s, padded
-
-
-def dev_hooks(ndic):
-    lib = ndic.lib()
-    lib.ndic_dev_profile.argtypes = [C.c_int32]
-    lib.ndic_dev_profile.restype = None
-    lib.ndic_dev_stage_times.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_size_t]
-    lib.ndic_dev_stage_times.restype = C.c_size_t
-    return lib
 
 
 def cpu_baseline(d_enc, ddescs, specs, budget_s):
@@ -151,9 +105,9 @@ def cpu_baseline(d_enc, ddescs, specs, budget_s):
     order = np.random.default_rng(1).permutation(len(specs))
     for i in order:
         d = ddescs[int(i)]
-        shape, dt, steps, lv, content = specs[int(i)]
-        chunk = d_enc[d.in_offset:d.in_offset + d.in_len].cpu().numpy().tobytes()
-        code = {np.uint8: "u8", np.uint16: "u16", np.int16: "i16"}[dt]
+        shape, dt, steps = specs[int(i)][:3]
+        chunk = d_enc.download(d.in_offset, d.in_len)
+        code = {"uint8": "u8", "uint16": "u16", "int16": "i16"}[dt]
         t0 = time.perf_counter()
         out = oracle.lift_ht_decode(chunk, shape, code, steps)
         t_used += time.perf_counter() - t0
@@ -184,49 +138,46 @@ def main():
         sys.exit(2)
     if args.workload == "c0":
         args.chunks = min(args.chunks, 64)
-    import torch
-
-    dev = torch.device("cuda:0")
     ndic = importlib.import_module("nd-image-codecs_amd")
-    lib = dev_hooks(ndic)
-    specs, d_raw, d_enc, ddescs, raw_total, comp_total, enc_s, padded = make_workload(torch, ndic, args, dev)
-    d_out = torch.zeros(padded, dtype=torch.uint8, device=dev)
+    dev = importlib.import_module("nd-image-codecs_amd.device")
+    specs, d_raw, d_enc, ddescs, raw_total, comp_total, enc_s, padded = make_workload(dev, ndic, args)
+    d_out = dev.DeviceBuffer(padded, zero=True)
 
     def run(plain=False):
-        ndic.decode_batch(d_enc.data_ptr(), d_out.data_ptr(), ddescs, plain=plain)
+        ndic.decode_batch(d_enc.ptr, d_out.ptr, ddescs, plain=plain)
+
+    def exact_now():
+        return dev.count_mismatch(d_out.ptr, d_raw.ptr, padded) == 0
 
     for _ in range(args.warmup):
         run()
-    torch.cuda.synchronize()
-    exact = bool(torch.equal(d_out, d_raw))
+    dev.synchronize()
+    exact = exact_now()
     log(f"bit-exact after warmup: {exact}")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         run()
-    torch.cuda.synchronize()
+    dev.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     gbps = raw_total / dt / 1e9
-    exact &= bool(torch.equal(d_out, d_raw))
+    exact &= exact_now()
     log(f"batched decode: {dt * 1e3:.1f} ms/step, {gbps:.2f} GB/s, bit-exact {exact}")
     # plain path
-    d_out.zero_()
-    torch.cuda.synchronize()
+    d_out.memset(0)
     t0 = time.perf_counter()
     for _ in range(args.plain_steps):
         run(plain=True)
-    torch.cuda.synchronize()
+    dev.synchronize()
     pdt = (time.perf_counter() - t0) / args.plain_steps
     plain_gbps = raw_total / pdt / 1e9
-    exact &= bool(torch.equal(d_out, d_raw))
+    exact &= exact_now()
     log(f"plain decode: {pdt * 1e3:.1f} ms/step, {plain_gbps:.2f} GB/s")
     # stage profile (one extra pass, HIP events on the library's stream)
-    lib.ndic_dev_profile(1)
+    dev.profile(True)
     run()
-    torch.cuda.synchronize()
-    ms = (C.c_double * 5)()
-    by = (C.c_double * 5)()
-    lib.ndic_dev_stage_times(ms, by, 5)
-    lib.ndic_dev_profile(0)
+    dev.synchronize()
+    ms, by, _ = dev.stage_times(5)
+    dev.profile(False)
     names = ["parse", "ht_decode", "dwt_inverse", "dc_unshift", "lift_inverse_narrow"]
     stages = {names[i]: {"ms": round(ms[i], 3), "GBps": round(by[i] / ms[i] / 1e6, 1) if ms[i] > 0 else None}
               for i in range(5)}
@@ -249,7 +200,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
-        "data": "synthetic (seeded blobs + Poisson noise, generated on the GPU)",
+        "data": "synthetic (seeded blobs + Poisson-like noise, generated on the GPU)",
         "bit_exact": exact,
         "config": {
             "workload": ("c2: 4096 uneven chunks, y/x log-uniform 64..1024, z 1..64, u8:u16:i16 2:5:3, "

@@ -40,6 +40,35 @@ ndic_status ndic_dev_ht_decode_block(const uint8_t* coded, size_t len, uint32_t*
 void ndic_dev_profile(int32_t enable);
 size_t ndic_dev_stage_times(double* ms, double* bytes, size_t n);
 
+/* Device memory on the library's own HIP runtime and stream, for callers
+ * (tests, the benchmark) that hold no other GPU framework.  kind of
+ * ndic_dev_memcpy: 0 host->device, 1 device->host, 2 device->device; every
+ * helper returns after the stream has drained. */
+ndic_status ndic_dev_malloc(size_t bytes, void** out, ndic_error* err);
+void ndic_dev_free(void* p);
+ndic_status ndic_dev_memcpy(void* dst, const void* src, size_t n, int32_t kind, ndic_error* err);
+ndic_status ndic_dev_memset(void* dst, int32_t value, size_t n, ndic_error* err);
+ndic_status ndic_dev_synchronize(ndic_error* err);
+
+/* Synthetic z,y,x volumes on the device: all-zero, uniform random over the
+ * dtype, or 1000 + a sum of Gaussian blobs with Poisson-like noise (normal
+ * approximation), clipped to 12 bits and scaled to the dtype (u8 >> 4,
+ * u16 << 4, i16 (<< 4) - 32768).  Deterministic in (shape, blobs, seed). */
+#define NDIC_SYNTH_ZERO 0
+#define NDIC_SYNTH_RANDOM 1
+#define NDIC_SYNTH_BLOBS 2
+#define NDIC_SYNTH_MAX_BLOBS 8
+typedef struct ndic_synth_blobs {
+  int32_t count;
+  float cz[NDIC_SYNTH_MAX_BLOBS], cy[NDIC_SYNTH_MAX_BLOBS], cx[NDIC_SYNTH_MAX_BLOBS];
+  float amp[NDIC_SYNTH_MAX_BLOBS], sigma[NDIC_SYNTH_MAX_BLOBS];
+} ndic_synth_blobs;
+ndic_status ndic_dev_synth_chunk(void* d_out, uint32_t z, uint32_t y, uint32_t x, int32_t dtype, int32_t content,
+                                 const ndic_synth_blobs* blobs, uint64_t seed, ndic_error* err);
+
+/* Number of differing bytes of two 16-byte aligned device buffers. */
+ndic_status ndic_dev_count_mismatch(const void* a, const void* b, size_t n, uint64_t* count, ndic_error* err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -460,12 +460,13 @@ __global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __rest
     if (sop && dn >= 6 && d[0] == 0xFF && d[1] == 0x91) start = 6;
     BitReader bb;
     bb.init(d + start, dn - start);
-    bool empty = true;
+    bool empty = true, any_band = false;
     // first pass over the header: inclusion, missing MSBs, passes, lengths
     uint32_t nincl = 0;
     for (uint32_t b = 0; b < pk.nbands; ++b) {
       const GeoBand& bd = gb[ch.geo_band + pk.band_first + b];
       if (bd.nx == 0 || bd.ny == 0) continue;
+      any_band = true;
       if (empty) {
         if (bb.bit() == 0) break;
         empty = false;
@@ -547,7 +548,8 @@ __global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __rest
           nincl += 1;
         }
     }
-    if (empty && pk.nbands == 0) (void)bb.bit();
+    // a precinct without code-blocks still carries the one-bit empty header
+    if (!any_band) (void)bb.bit();
     if (bb.bad) return raise(ch, kErrPacket, pl.cs_off + bs + cursor, gid);
     uint64_t hl = start + bb.terminate();
     if (eph) {

@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "oracle.hpp"
+#include "zfp_oracle.hpp"
 
 extern "C" {
 
@@ -147,6 +148,54 @@ int orc_htj2k_decode_chunk(const uint8_t* in, size_t len, const size_t* shape, s
   return guard(err, [&] {
     emit(orc::htj2k_decode_chunk(in, len, std::vector<size_t>(shape, shape + ndim), dtype), out, out_len);
   });
+}
+
+// ZFP: mode = {kind (0 rate, 1 precision, 2 accuracy, 3 reversible), value}.
+// type: 1 int32, 2 int64, 3 float, 4 double (raw field) or, with chunk = 1,
+// a dtype code (narrow integers promoted).
+static orc::zfp::Mode zfp_mode_of(int kind, double value) {
+  orc::zfp::Mode m;
+  m.kind = (orc::zfp::Mode::Kind)kind;
+  m.rate = value;
+  m.tolerance = value;
+  m.precision = (uint32_t)value;
+  return m;
+}
+
+int orc_zfp_encode(int chunk, const uint8_t* in, size_t len, const size_t* shape, size_t ndim, int type, int kind,
+                   double value, uint8_t** out, size_t* out_len, uint64_t* header_bits, orc_err* err) {
+  return guard(err, [&] {
+    std::vector<size_t> sh(shape, shape + ndim);
+    orc::zfp::Mode m = zfp_mode_of(kind, value);
+    if (chunk) {
+      emit(orc::zfp::encode_chunk(in, len, sh, type, m), out, out_len);
+      if (header_bits) *header_bits = 0;
+    } else {
+      emit(orc::zfp::compress(in, type, sh, m, header_bits), out, out_len);
+    }
+  });
+}
+
+int orc_zfp_decode(int chunk, const uint8_t* in, size_t len, const size_t* shape, size_t ndim, int type, int kind,
+                   double value, uint8_t** out, size_t* out_len, orc_err* err) {
+  return guard(err, [&] {
+    std::vector<size_t> sh(shape, shape + ndim);
+    orc::zfp::Mode m = zfp_mode_of(kind, value);
+    if (chunk) {
+      emit(orc::zfp::decode_chunk(in, len, sh, type, m), out, out_len);
+    } else {
+      size_t n = 1;
+      for (size_t d : sh) n *= d;
+      std::vector<uint8_t> v(n * (type == 1 || type == 3 ? 4 : 8));
+      orc::zfp::decompress(in, len, type, sh, m, v.data());
+      emit(v, out, out_len);
+    }
+  });
+}
+
+void orc_zfp_perm(int dims, uint8_t* out) {
+  const uint8_t* p = orc::zfp::perm_of(dims);
+  std::memcpy(out, p, (size_t)1 << (2 * dims));
 }
 
 }  // extern "C"

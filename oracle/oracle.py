@@ -67,11 +67,14 @@ def _declare(L) -> None:
         "orc_decode_codestream": [C.c_char_p, S, I, I, P, P, P, E],
         "orc_htj2k_encode_chunk": [C.c_char_p, S, P, S, I, U, I, I, I, P, P, E],
         "orc_htj2k_decode_chunk": [C.c_char_p, S, P, S, I, P, P, E],
+        "orc_zfp_encode": [I, C.c_char_p, S, P, S, I, I, C.c_double, P, P, P, E],
+        "orc_zfp_decode": [I, C.c_char_p, S, P, S, I, I, C.c_double, P, P, E],
+        "orc_zfp_perm": [I, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
-        fn.restype = None if name in ("orc_free", "orc_ht_tables") else C.c_int
+        fn.restype = None if name in ("orc_free", "orc_ht_tables", "orc_zfp_perm") else C.c_int
 
 
 def _check(rc: int, err: _Err) -> None:
@@ -226,3 +229,81 @@ def lift_ht_decode(chunk: bytes, shape, dtype: str, steps) -> np.ndarray:
 
 if os.environ.get("NDIC_ORACLE_AUTOBUILD") == "1":
     build()
+
+
+# ---------------------------------------------------------------------------
+# ZFP
+# ---------------------------------------------------------------------------
+ZFP_MODES = {"fixed_rate": 0, "fixed_precision": 1, "fixed_accuracy": 2, "reversible": 3}
+ZFP_TYPES = {"i32": 1, "i64": 2, "f32": 3, "f64": 4}
+
+
+def zfp_mode(config: dict):
+    """(kind, value) for a zfp configuration object (mode + its parameter)."""
+    mode = config["mode"]
+    value = {"fixed_rate": config.get("rate"), "fixed_precision": config.get("precision"),
+             "fixed_accuracy": config.get("tolerance"), "reversible": 0.0}[mode]
+    return ZFP_MODES[mode], float(value)
+
+
+def zfp_effective_shape(shape, dims=None):
+    """Field shape of a chunk: the chunk shape itself, or (legacy ``dims``)
+    the non-singleton axes left-padded with 1s up to ``dims``."""
+    if dims is None:
+        return list(shape)
+    squeezed = [int(d) for d in shape if d > 1]
+    if len(squeezed) > dims:
+        raise OracleError(1, 0, "zfp: too many non-singleton dimensions")
+    return [1] * (dims - len(squeezed)) + squeezed
+
+
+def zfp_encode_chunk(data: bytes, shape, dtype: str, config: dict) -> bytes:
+    L = lib()
+    sh, nd = _shape(zfp_effective_shape(shape, config.get("dims")))
+    kind, value = zfp_mode(config)
+    out, n, err = C.c_void_p(), C.c_size_t(), _Err()
+    _check(L.orc_zfp_encode(1, data, len(data), sh, nd, DTYPES[dtype], kind, value, C.byref(out), C.byref(n), None,
+                            C.byref(err)), err)
+    return _take(out, n.value)
+
+
+def zfp_decode_chunk(stream: bytes, shape, dtype: str, config: dict) -> bytes:
+    L = lib()
+    sh, nd = _shape(zfp_effective_shape(shape, config.get("dims")))
+    kind, value = zfp_mode(config)
+    out, n, err = C.c_void_p(), C.c_size_t(), _Err()
+    _check(L.orc_zfp_decode(1, stream, len(stream), sh, nd, DTYPES[dtype], kind, value, C.byref(out), C.byref(n),
+                            C.byref(err)), err)
+    return _take(out, n.value)
+
+
+def zfp_compress(values: np.ndarray, config: dict):
+    """Raw ZFP stream of an int32/int64/float32/float64 array (row-major
+    shape, x = last axis); returns (stream, header_bits)."""
+    L = lib()
+    t = {np.dtype(np.int32): 1, np.dtype(np.int64): 2, np.dtype(np.float32): 3, np.dtype(np.float64): 4}[values.dtype]
+    values = np.ascontiguousarray(values)
+    sh, nd = _shape(values.shape)
+    kind, value = zfp_mode(config)
+    out, n, hb, err = C.c_void_p(), C.c_size_t(), C.c_uint64(), _Err()
+    _check(L.orc_zfp_encode(0, values.tobytes(), values.nbytes, sh, nd, t, kind, value, C.byref(out), C.byref(n),
+                            C.byref(hb), C.byref(err)), err)
+    return _take(out, n.value), hb.value
+
+
+def zfp_decompress(stream: bytes, shape, np_dtype, config: dict) -> np.ndarray:
+    L = lib()
+    t = {np.dtype(np.int32): 1, np.dtype(np.int64): 2, np.dtype(np.float32): 3, np.dtype(np.float64): 4}[
+        np.dtype(np_dtype)]
+    sh, nd = _shape(shape)
+    kind, value = zfp_mode(config)
+    out, n, err = C.c_void_p(), C.c_size_t(), _Err()
+    _check(L.orc_zfp_decode(0, stream, len(stream), sh, nd, t, kind, value, C.byref(out), C.byref(n), C.byref(err)),
+           err)
+    return np.frombuffer(_take(out, n.value), dtype=np_dtype).reshape(shape)
+
+
+def zfp_perm(dims: int) -> list:
+    buf = (C.c_uint8 * (1 << (2 * dims)))()
+    lib().orc_zfp_perm(dims, buf)
+    return list(buf)

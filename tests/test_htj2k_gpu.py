@@ -158,7 +158,9 @@ def test_corrupt_chunks_error_not_crash(ndic, oracle):
 
 
 def test_lift_ht_batch_matches_oracle(ndic, oracle):
-    torch = pytest.importorskip("torch")
+    import importlib
+
+    dev = importlib.import_module("nd-image-codecs_amd.device")
     rng = np.random.default_rng(21)
     kinds = ["delta", "haar", "lift53"]
     specs = []
@@ -176,10 +178,10 @@ def test_lift_ht_batch_matches_oracle(ndic, oracle):
         chunk = oracle.lift_ht_encode(vol, {"uint8": "u8", "uint16": "u16", "int16": "i16"}[dt], steps, xy_levels=lv)
         specs.append((vol, dt, steps, lv, chunk))
     blob = b"".join(s[4] for s in specs)
-    d_in = torch.tensor(np.frombuffer(blob, np.uint8).copy(), device="cuda")
+    d_in = dev.DeviceBuffer.from_bytes(blob)
     al = lambda n: (n + 15) // 16 * 16
     total_out = sum(al(s[0].nbytes) for s in specs)
-    d_out = torch.zeros(total_out, dtype=torch.uint8, device="cuda")
+    d_out = dev.DeviceBuffer(total_out)
     descs = []
     ioff = ooff = 0
     for vol, dt, steps, lv, chunk in specs:
@@ -187,26 +189,26 @@ def test_lift_ht_batch_matches_oracle(ndic, oracle):
         ioff += len(chunk)
         ooff += al(vol.nbytes)
     for plain in [False, True]:
-        d_out.zero_()
-        ndic.decode_batch(d_in.data_ptr(), d_out.data_ptr(), descs, plain=plain)
-        torch.cuda.synchronize()
-        out = d_out.cpu().numpy().tobytes()
+        d_out.memset(0)
+        ndic.decode_batch(d_in.ptr, d_out.ptr, descs, plain=plain)
+        dev.synchronize()
+        out = d_out.download()
         o = 0
         for vol, *_ in specs:
             assert out[o:o + vol.nbytes] == vol.tobytes()
             o += al(vol.nbytes)
     # encode batch: byte-identical chunks
     raw = b"".join(s[0].tobytes() + bytes(al(s[0].nbytes) - s[0].nbytes) for s in specs)
-    d_raw = torch.tensor(np.frombuffer(raw, np.uint8).copy(), device="cuda")
+    d_raw = dev.DeviceBuffer.from_bytes(raw)
     cap = len(blob) * 2 + (1 << 20)
-    d_enc = torch.zeros(cap, dtype=torch.uint8, device="cuda")
+    d_enc = dev.DeviceBuffer(cap, zero=True)
     edescs = []
     roff = 0
     for vol, dt, steps, lv, chunk in specs:
         edescs.append(ndic.make_desc(vol.shape, dt, steps, lv, roff, vol.nbytes, 0))
         roff += al(vol.nbytes)
     arr = (ndic.ChunkDesc * len(edescs))(*edescs)
-    ndic.encode_batch(d_raw.data_ptr(), d_enc.data_ptr(), cap, arr)
-    enc = d_enc.cpu().numpy().tobytes()
+    ndic.encode_batch(d_raw.ptr, d_enc.ptr, cap, arr)
+    enc = d_enc.download()
     for d, s in zip(arr, specs):
         assert enc[d.out_offset:d.out_offset + d.out_len] == s[4]
