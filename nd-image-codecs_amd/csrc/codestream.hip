@@ -797,7 +797,7 @@ __global__ void write_planes_kernel(const DevChunk* __restrict__ chunks, uint32_
   uint64_t body = 0;
   for (uint32_t r = 0; r <= ch.L; ++r) {
     const GeoPacket& pk = gp[ch.geo_packet + r];
-    uint8_t* hdr = ch.pkt_hdr + (gid * (ch.L + 1) + r) * (uint64_t)ch.pkt_stride;
+    uint8_t* hdr = ch.pkt_hdr + (gid * (uint64_t)ch.pkt_slots + r) * (uint64_t)ch.pkt_stride;
     BitWriter bw;
     bw.init(hdr, ch.pkt_stride);
     bool any = false;
@@ -1052,7 +1052,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const DevChunk* __restric
   uint64_t pos = pl.head_len;
   uint64_t tb = ch.task_base + p * ch.blocks_per_plane;
   for (uint32_t r = 0; r <= ch.L; ++r) {
-    const uint8_t* hdr = ch.pkt_hdr + (gid * (ch.L + 1) + r) * (uint64_t)ch.pkt_stride;
+    const uint8_t* hdr = ch.pkt_hdr + (gid * (uint64_t)ch.pkt_slots + r) * (uint64_t)ch.pkt_stride;
     uint32_t hl = pl.packet_hdr[r];
     for (uint32_t i = threadIdx.x; i < hl; i += 256) dst[pos + i] = hdr[i];
     pos += hl;

@@ -62,8 +62,9 @@ struct DevChunk {
   uint8_t* slots;         // HT slots base (per-task offsets in slot_off)
   uint8_t* head;          // per-plane header bytes, head_stride apart
   uint32_t head_stride;
-  uint8_t* pkt_hdr;       // per-packet header bytes, pkt_stride apart (per plane: L+1 packets)
+  uint8_t* pkt_hdr;       // per-packet header bytes, pkt_stride apart
   uint32_t pkt_stride;
+  uint32_t pkt_slots;     // packet slots per plane (the window's largest L + 1, shared by all chunks)
   uint8_t* chunk_head;    // container header + index bytes
   uint64_t chunk_head_len;
   uint64_t out_off;       // encode: byte offset of the chunk in the output

@@ -461,6 +461,7 @@ void encode_window(Context& ctx, std::vector<EncodeItem*>& items, uint8_t* d_out
     dc[i].head_stride = head_stride;
     dc[i].pkt_hdr = d_pkt;
     dc[i].pkt_stride = (uint32_t)pkt_stride_eff;
+    dc[i].pkt_slots = (uint32_t)(maxL + 1);
     dc[i].chunk_head = d_chead + i * chead_stride;
     dc[i].chunk_head_len = 0;
     co += (uint64_t)dc[i].P * dc[i].W * dc[i].H;
