@@ -31,6 +31,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
+# the kernel each decode stage times (one launch per batch window)
+KERNELS = ["parse_planes_kernel", "ht_decode_kernel", "dwt_inv_h + dwt_inv_v", "unshift_kernel",
+           "lift_step_kernel (inverse, narrowing)"]
 
 
 def log(*a):
@@ -94,6 +97,17 @@ def make_workload(dev, ndic, args):
     return specs, d_raw, d_enc, ddescs, raw_total, comp_total, enc_s, padded
 
 
+def traffic_of(kernel: str, args):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this same workload (scaled by a
+    1 GiB copy kernel), or None when no such measurement exists."""
+    path = ROOT / "profiles" / f"{kernel}_c2_traffic.json"
+    if args.workload != "c2" or args.chunks != 4096 or args.seed != 2 or not path.exists():
+        return None
+    rec = json.loads(path.read_text())
+    return round(rec["traffic_bytes_per_dispatch"])
+
+
 def cpu_baseline(d_enc, ddescs, specs, budget_s):
     """The oracle (single-threaded C++ restatement of the reference path)
     decoding chunks of the same workload until the time budget is spent."""
@@ -131,6 +145,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="first run one device copy kernel of this many bytes (traffic-counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (got --gpus {args.gpus})",
@@ -140,6 +156,15 @@ def main():
         args.chunks = min(args.chunks, 64)
     ndic = importlib.import_module("nd-image-codecs_amd")
     dev = importlib.import_module("nd-image-codecs_amd.device")
+    if args.copy_calibration:
+        # one plain device copy of known size: the scale of the HBM traffic
+        # counters (FETCH_SIZE / WRITE_SIZE read for it what it moves)
+        a = dev.DeviceBuffer(args.copy_calibration, zero=True)
+        b = dev.DeviceBuffer(args.copy_calibration)
+        dev.copy(b.ptr, a.ptr, args.copy_calibration)
+        dev.synchronize()
+        log(f"copy calibration: {args.copy_calibration} bytes")
+        del a, b
     specs, d_raw, d_enc, ddescs, raw_total, comp_total, enc_s, padded = make_workload(dev, ndic, args)
     d_out = dev.DeviceBuffer(padded, zero=True)
 
@@ -149,11 +174,13 @@ def main():
     def exact_now():
         return dev.count_mismatch(d_out.ptr, d_raw.ptr, padded) == 0
 
+    exact = True
     for _ in range(args.warmup):
         run()
     dev.synchronize()
-    exact = exact_now()
-    log(f"bit-exact after warmup: {exact}")
+    if args.warmup:
+        exact = exact_now()
+        log(f"bit-exact after warmup: {exact}")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         run()
@@ -163,20 +190,22 @@ def main():
     exact &= exact_now()
     log(f"batched decode: {dt * 1e3:.1f} ms/step, {gbps:.2f} GB/s, bit-exact {exact}")
     # plain path
-    d_out.memset(0)
-    t0 = time.perf_counter()
-    for _ in range(args.plain_steps):
-        run(plain=True)
-    dev.synchronize()
-    pdt = (time.perf_counter() - t0) / args.plain_steps
-    plain_gbps = raw_total / pdt / 1e9
-    exact &= exact_now()
-    log(f"plain decode: {pdt * 1e3:.1f} ms/step, {plain_gbps:.2f} GB/s")
+    plain_gbps = 0.0
+    if args.plain_steps > 0:
+        d_out.memset(0)
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            run(plain=True)
+        dev.synchronize()
+        pdt = (time.perf_counter() - t0) / args.plain_steps
+        plain_gbps = raw_total / pdt / 1e9
+        exact &= exact_now()
+        log(f"plain decode: {pdt * 1e3:.1f} ms/step, {plain_gbps:.2f} GB/s")
     # stage profile (one extra pass, HIP events on the library's stream)
     dev.profile(True)
     run()
     dev.synchronize()
-    ms, by, _ = dev.stage_times(5)
+    ms, by, windows = dev.stage_times(5)
     dev.profile(False)
     names = ["parse", "ht_decode", "dwt_inverse", "dc_unshift", "lift_inverse_narrow"]
     stages = {names[i]: {"ms": round(ms[i], 3), "GBps": round(by[i] / ms[i] / 1e6, 1) if ms[i] > 0 else None}
@@ -209,19 +238,22 @@ def main():
             "chunks": len(specs),
             "raw_bytes": raw_total,
             "compressed_bytes": comp_total,
-            "plain_gbps": round(plain_gbps, 3),
+            "plain_gbps": round(plain_gbps, 3) if plain_gbps > 0 else None,
             "batched_over_plain": round(gbps / plain_gbps, 2) if plain_gbps > 0 else None,
             "encode_seconds": round(enc_s, 2),
             "stages": stages,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": names[dom],
+            "kernel": KERNELS[dom],
+            "launches": windows,
+            "avg_launch_ms": round(ms[dom] / max(windows, 1), 3),
+            "bytes_per_launch": round(by[dom] / max(windows, 1)),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": traffic_of(KERNELS[dom], args),
         },
         "cpu_baseline": cpu,
     }

@@ -142,6 +142,67 @@ ndic_status ndic_lift_ht_decode_batch(const uint8_t* d_in, uint8_t* d_out, ndic_
 ndic_status ndic_lift_ht_encode_batch(const uint8_t* d_in, uint8_t* d_out, size_t capacity, ndic_chunk_desc* descs,
                                       size_t n, uint32_t flags, void* stream, ndic_error* err);
 
+/* ---- nd_zfp / zfp (LLNL ZFP streams, codec version 5) ------------------ */
+
+/* ZfpMode (crates/ndic-zfp/src/lib.rs:77-86) as carried by NdZfpConfig
+ * (crates/ndic-zfp/src/chunk.rs:101-128): the mode, its one parameter, and
+ * the legacy `dims` member (0 = absent: the registered `zfp` mapping, the
+ * chunk shape is the field shape; 1..4 = squeeze singletons, left-pad). */
+#define NDIC_ZFP_FIXED_RATE 0
+#define NDIC_ZFP_FIXED_ACCURACY 1
+#define NDIC_ZFP_FIXED_PRECISION 2
+#define NDIC_ZFP_REVERSIBLE 3
+typedef struct {
+  int32_t mode;
+  uint32_t precision; /* fixed_precision: bit planes, 1..=64 */
+  double rate;        /* fixed_rate: bits per value, positive and finite */
+  double tolerance;   /* fixed_accuracy: absolute error bound, >= 0 and finite */
+  int32_t dims;       /* legacy nd_zfp dims (0: none) */
+} ndic_zfp_config;
+
+/* Replaces ndic_zfp::encode_chunk (crates/ndic-zfp/src/chunk.rs:358): chunk
+ * bytes (u8/i8/u16/i16 promoted to int32, i32, i64, f32, f64) -> one full
+ * ZFP stream (header + blocks, padded to a 64-bit word). */
+ndic_status ndic_zfp_encode_chunk(const uint8_t* bytes, size_t len, const size_t* shape, size_t ndim,
+                                  ndic_dtype dtype, const ndic_zfp_config* config, ndic_buffer* out,
+                                  ndic_error* err);
+
+/* Replaces ndic_zfp::decode_chunk (crates/ndic-zfp/src/chunk.rs:383): the
+ * stream header must declare exactly this field, type and mode; fixed-rate
+ * streams must have the exact computed length. */
+ndic_status ndic_zfp_decode_chunk(const uint8_t* stream, size_t len, const size_t* shape, size_t ndim,
+                                  ndic_dtype dtype, const ndic_zfp_config* config, ndic_buffer* out,
+                                  ndic_error* err);
+
+/* Replaces ndic_zfp::decode_chunk_brick / NdZfpBrickDecoder::decode_brick
+ * (crates/ndic-zfp/src/chunk.rs:446-540): one 4^d brick of a fixed-rate
+ * stream at per-axis brick coordinates over the effective shape; brick_shape
+ * (nbrick entries) receives the clipped shape of the returned samples. */
+ndic_status ndic_zfp_decode_brick(const uint8_t* stream, size_t len, const size_t* shape, size_t ndim,
+                                  ndic_dtype dtype, const ndic_zfp_config* config, const size_t* brick,
+                                  size_t nbrick, ndic_buffer* out, size_t* brick_shape, ndic_error* err);
+
+/* One chunk of a device-resident ZFP batch. */
+typedef struct {
+  uint64_t in_offset;  /* encode: samples (element-aligned); decode: stream (8-byte aligned) */
+  uint64_t in_len;
+  uint64_t out_offset; /* decode: samples; encode: filled by the library */
+  uint64_t out_len;    /* encode: filled by the library */
+  uint32_t ndim;
+  uint32_t shape[NDIC_MAX_NDIM];
+  int32_t dtype;
+  ndic_zfp_config config;
+  int32_t status;      /* filled by the library */
+} ndic_zfp_desc;
+
+/* Batch forms of ndic_zfp_encode_chunk / ndic_zfp_decode_chunk over device
+ * memory (the zarrs codec calls them one chunk at a time,
+ * crates/ndic-zarr/src/zfp_codec.rs:174-520; the caller batches). */
+ndic_status ndic_zfp_encode_batch(const uint8_t* d_in, uint8_t* d_out, size_t capacity, ndic_zfp_desc* descs,
+                                  size_t n, void* stream, ndic_error* err);
+ndic_status ndic_zfp_decode_batch(const uint8_t* d_in, uint8_t* d_out, ndic_zfp_desc* descs, size_t n,
+                                  void* stream, ndic_error* err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,10 @@ typedef struct ndic_synth_blobs {
 ndic_status ndic_dev_synth_chunk(void* d_out, uint32_t z, uint32_t y, uint32_t x, int32_t dtype, int32_t content,
                                  const ndic_synth_blobs* blobs, uint64_t seed, ndic_error* err);
 
+/* A plain vectorised device-to-device copy kernel (16-byte aligned, n a
+ * multiple of 16): the known-size reference for HBM traffic counters. */
+ndic_status ndic_dev_copy(void* dst, const void* src, size_t n, ndic_error* err);
+
 /* Number of differing bytes of two 16-byte aligned device buffers. */
 ndic_status ndic_dev_count_mismatch(const void* a, const void* b, size_t n, uint64_t* count, ndic_error* err);
 

@@ -8,8 +8,10 @@ errors raised as ``ValueError`` like the binding's ``PyValueError``.  Every
 call runs on the GPU; there is no CPU fallback, and importing the module
 fails loudly when the HIP library is missing.
 
-The device-resident batch entry points (``decode_batch``/``encode_batch``)
-take ``torch`` CUDA(HIP) tensors only as plain device memory.
+ZFP follows the binding's ``nd_zfp_encode``/``nd_zfp_decode``
+(``lib.rs:117-166``) and ``ndic_zfp::decode_chunk_brick``.  The device-resident
+batch entry points take device pointers (integers); ``.device`` provides
+device buffers on the library's own HIP runtime.
 """
 
 from __future__ import annotations
@@ -66,11 +68,25 @@ class Htj2kConfig(C.Structure):
                 ("index", C.c_int32)]
 
 
+class ZfpConfig(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("precision", C.c_uint32), ("rate", C.c_double), ("tolerance", C.c_double),
+                ("dims", C.c_int32)]
+
+
 class ChunkDesc(C.Structure):
     _fields_ = [("in_offset", C.c_uint64), ("in_len", C.c_uint64), ("out_offset", C.c_uint64),
                 ("out_len", C.c_uint64), ("ndim", C.c_uint32), ("shape", C.c_uint32 * MAX_NDIM),
                 ("dtype", C.c_int32), ("xy_levels", C.c_uint32), ("nsteps", C.c_uint32),
                 ("steps", LiftStep * MAX_STEPS), ("status", C.c_int32)]
+
+
+class ZfpDesc(C.Structure):
+    _fields_ = [("in_offset", C.c_uint64), ("in_len", C.c_uint64), ("out_offset", C.c_uint64),
+                ("out_len", C.c_uint64), ("ndim", C.c_uint32), ("shape", C.c_uint32 * MAX_NDIM),
+                ("dtype", C.c_int32), ("config", ZfpConfig), ("status", C.c_int32)]
+
+
+ZFP_MODES = {"fixed_rate": 0, "fixed_accuracy": 1, "fixed_precision": 2, "reversible": 3}
 
 
 def _load() -> C.CDLL:
@@ -89,6 +105,12 @@ def _load() -> C.CDLL:
                                               C.POINTER(C.c_size_t), E]),
         "ndic_lift_ht_decode_batch": (C.c_int, [P, P, P, S, C.c_uint32, P, E]),
         "ndic_lift_ht_encode_batch": (C.c_int, [P, P, S, P, S, C.c_uint32, P, E]),
+        "ndic_zfp_encode_chunk": (C.c_int, [C.c_char_p, S, P, S, C.c_int, C.POINTER(ZfpConfig), B, E]),
+        "ndic_zfp_decode_chunk": (C.c_int, [C.c_char_p, S, P, S, C.c_int, C.POINTER(ZfpConfig), B, E]),
+        "ndic_zfp_decode_brick": (C.c_int, [C.c_char_p, S, P, S, C.c_int, C.POINTER(ZfpConfig), P, S, B,
+                                            P, E]),
+        "ndic_zfp_encode_batch": (C.c_int, [P, P, S, P, S, P, E]),
+        "ndic_zfp_decode_batch": (C.c_int, [P, P, P, S, P, E]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name, None)
@@ -266,13 +288,148 @@ def encode_batch(d_in_ptr: int, d_out_ptr: int, capacity: int, descs, plain: boo
     _check(rc, err)
 
 
+# ---------------------------------------------------------------------------
+# nd_zfp / zfp (bindings/python/nd-image-codecs/src/lib.rs:117-166,
+# crates/ndic-zfp/src/chunk.rs:101-233)
+# ---------------------------------------------------------------------------
+def zfp_config(mode: str = "reversible", rate=None, tolerance=None, precision=None, dims=None) -> ZfpConfig:
+    """NdZfpConfig with NdZfpConfig::zfp_mode's parameter rules (chunk.rs:162-215):
+    exactly the mode's own parameter may be present."""
+    def forbid(present, what):
+        if present:
+            raise NdicError(1, 0, f'zfp: "{mode}" mode does not take "{what}"')
+
+    if dims is not None and not 1 <= int(dims) <= 4:
+        raise NdicError(1, 0, f"zfp: dims must be 1..=4, got {dims}")
+    c = ZfpConfig()
+    c.dims = 0 if dims is None else int(dims)
+    if mode == "reversible":
+        forbid(rate is not None, "rate")
+        forbid(tolerance is not None, "tolerance")
+        forbid(precision is not None, "precision")
+    elif mode == "fixed_rate":
+        forbid(tolerance is not None, "tolerance")
+        forbid(precision is not None, "precision")
+        if rate is None:
+            raise NdicError(1, 0, 'zfp: fixed_rate mode needs a "rate"')
+        c.rate = float(rate)
+    elif mode == "fixed_accuracy":
+        forbid(rate is not None, "rate")
+        forbid(precision is not None, "precision")
+        if tolerance is None:
+            raise NdicError(1, 0, 'zfp: fixed_accuracy mode needs a "tolerance"')
+        c.tolerance = float(tolerance)
+    elif mode == "fixed_precision":
+        forbid(rate is not None, "rate")
+        forbid(tolerance is not None, "tolerance")
+        if precision is None:
+            raise NdicError(1, 0, 'zfp: fixed_precision mode needs a "precision"')
+        if int(precision) < 0:
+            raise NdicError(1, 0, f"nd_zfp: fixed-precision mode needs 1..=64 bit planes, got {precision}")
+        c.precision = int(precision)
+    else:
+        raise NdicError(3, 0, f"zfp: unknown mode {mode!r}")
+    c.mode = ZFP_MODES[mode]
+    return c
+
+
+_ZFP_DTYPES = {"uint8": "uint8", "|u1": "uint8", "u1": "uint8", "int8": "int8", "|i1": "int8", "i1": "int8",
+               "uint16": "uint16", "<u2": "uint16", "u2": "uint16", "int16": "int16", "<i2": "int16", "i2": "int16",
+               "int32": "int32", "<i4": "int32", "i4": "int32", "int64": "int64", "<i8": "int64", "i8": "int64",
+               "float32": "float32", "<f4": "float32", "f4": "float32", "float64": "float64", "<f8": "float64",
+               "f8": "float64"}
+
+
+def _zfp_dtype(dtype: Any) -> int:
+    """ZfpDtype::from_zarr_name (crates/ndic-zfp/src/chunk.rs:53-65)."""
+    name = dtype if isinstance(dtype, str) else np.dtype(dtype).name
+    if name not in _ZFP_DTYPES:
+        raise NdicError(1, 0, f"zfp has no path for dtype {name!r}")
+    return DTYPE_CODES[_ZFP_DTYPES[name]]
+
+
+def nd_zfp_encode(chunk: Any, shape: Sequence[int], dtype: Any, *, mode: str = "reversible", rate=None,
+                  tolerance=None, precision=None, dims=None) -> bytes:
+    cfg = zfp_config(mode, rate, tolerance, precision, dims)
+    data = _as_bytes(chunk)
+    sh, nd = _shape(shape)
+    out, err = Buffer(), Error()
+    rc = _lib.ndic_zfp_encode_chunk(data, len(data), sh, nd, _zfp_dtype(dtype), C.byref(cfg), C.byref(out),
+                                    C.byref(err))
+    _check(rc, err)
+    return _take(out)
+
+
+def nd_zfp_decode(chunk: Any, shape: Sequence[int], dtype: Any, *, mode: str = "reversible", rate=None,
+                  tolerance=None, precision=None, dims=None) -> bytes:
+    cfg = zfp_config(mode, rate, tolerance, precision, dims)
+    data = _as_bytes(chunk)
+    sh, nd = _shape(shape)
+    out, err = Buffer(), Error()
+    rc = _lib.ndic_zfp_decode_chunk(data, len(data), sh, nd, _zfp_dtype(dtype), C.byref(cfg), C.byref(out),
+                                    C.byref(err))
+    _check(rc, err)
+    return _take(out)
+
+
+def nd_zfp_decode_brick(chunk: Any, shape: Sequence[int], dtype: Any, brick: Sequence[int], *, rate,
+                        dims=None, mode: str = "fixed_rate"):
+    """One 4^d brick of a fixed-rate chunk (ndic_zfp::decode_chunk_brick):
+    returns (bytes, brick_shape)."""
+    cfg = zfp_config(mode, rate=rate if mode == "fixed_rate" else None, dims=dims)
+    data = _as_bytes(chunk)
+    sh, nd = _shape(shape)
+    br = (C.c_size_t * max(1, len(brick)))(*[int(b) for b in brick])
+    bs = (C.c_size_t * 4)()
+    out, err = Buffer(), Error()
+    rc = _lib.ndic_zfp_decode_brick(data, len(data), sh, nd, _zfp_dtype(dtype), C.byref(cfg), br, len(brick),
+                                    C.byref(out), bs, C.byref(err))
+    _check(rc, err)
+    ndim_eff = len(shape) if dims is None else int(dims)
+    return _take(out), [int(bs[i]) for i in range(ndim_eff)]
+
+
+def make_zfp_desc(shape: Sequence[int], dtype: Any, config: ZfpConfig, in_offset: int = 0, in_len: int = 0,
+                  out_offset: int = 0) -> ZfpDesc:
+    if len(shape) > MAX_NDIM:
+        raise NdicError(1, 0, "batch descriptors hold at most 8 dimensions")
+    d = ZfpDesc()
+    d.in_offset, d.in_len, d.out_offset = int(in_offset), int(in_len), int(out_offset)
+    d.ndim = len(shape)
+    for i, s in enumerate(shape):
+        d.shape[i] = int(s)
+    d.dtype = _zfp_dtype(dtype)
+    d.config = config
+    return d
+
+
+def zfp_encode_batch(d_in_ptr: int, d_out_ptr: int, capacity: int, descs, stream: int = 0):
+    """Encode device-resident chunks; descs[i].out_offset/out_len report the streams."""
+    arr = descs if isinstance(descs, C.Array) else (ZfpDesc * len(descs))(*descs)
+    err = Error()
+    rc = _lib.ndic_zfp_encode_batch(C.c_void_p(d_in_ptr), C.c_void_p(d_out_ptr), int(capacity), arr, len(arr),
+                                    C.c_void_p(stream), C.byref(err))
+    _check(rc, err)
+    return arr
+
+
+def zfp_decode_batch(d_in_ptr: int, d_out_ptr: int, descs, stream: int = 0):
+    arr = descs if isinstance(descs, C.Array) else (ZfpDesc * len(descs))(*descs)
+    err = Error()
+    rc = _lib.ndic_zfp_decode_batch(C.c_void_p(d_in_ptr), C.c_void_p(d_out_ptr), arr, len(arr),
+                                    C.c_void_p(stream), C.byref(err))
+    _check(rc, err)
+    return arr
+
+
 def version() -> str:
     return _lib.ndic_version().decode()
 
 
 __all__ = ["htj2k_encode", "htj2k_decode", "htj2k_decode_plane", "forward", "inverse", "lift_forward_bytes",
-           "lift_inverse_bytes", "decode_batch", "encode_batch", "make_desc", "NdicError", "lib", "version",
-           "LIB_PATH"]
+           "lift_inverse_bytes", "decode_batch", "encode_batch", "make_desc", "nd_zfp_encode", "nd_zfp_decode",
+           "nd_zfp_decode_brick", "zfp_config", "make_zfp_desc", "zfp_encode_batch", "zfp_decode_batch",
+           "NdicError", "lib", "version", "LIB_PATH"]
 
 if os.environ.get("NDIC_VERBOSE"):
     print(f"nd-image-codecs_amd: loaded {LIB_PATH}")

@@ -72,6 +72,11 @@ __global__ __launch_bounds__(256) void mismatch_kernel(const uint8_t* __restrict
   if (local) atomicAdd(count, (unsigned long long)local);
 }
 
+__global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 uint32_t grid_for(uint64_t n) {
   uint64_t g = (n + 255) / 256;
   return (uint32_t)(g < 8192 ? (g ? g : 1) : 8192);
@@ -161,5 +166,20 @@ extern "C" ndic_status ndic_dev_count_mismatch(const void* a, const void* b, siz
     NDIC_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, ctx.stream));
     NDIC_HIP(hipStreamSynchronize(ctx.stream));
     *count = h;
+  });
+}
+
+extern "C" ndic_status ndic_dev_copy(void* dst, const void* src, size_t n, ndic_error* err) {
+  return guard_call(err, [&] {
+    if (n % 16 || (uintptr_t)dst % 16 || (uintptr_t)src % 16)
+      fail(NDIC_INVALID_ARGUMENT, "ndic_dev_copy moves 16-byte aligned multiples of 16 bytes");
+    Context& ctx = Context::get();
+    std::lock_guard<std::recursive_mutex> lock(ctx.mu);
+    if (n) {
+      hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n / 16)), dim3(256), 0, ctx.stream, (uint4*)dst,
+                         (const uint4*)src, (uint64_t)(n / 16));
+      NDIC_HIP(hipGetLastError());
+    }
+    NDIC_HIP(hipStreamSynchronize(ctx.stream));
   });
 }

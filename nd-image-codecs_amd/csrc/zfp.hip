@@ -1,0 +1,1089 @@
+// ZFP block codec on the device, one wavefront per 4^d block.
+//
+// Layout of a block across the wave: lane l holds coefficient l (x = l & 3,
+// y = (l >> 2) & 3, z = l >> 4) for d <= 3; for d = 4 each lane holds the four
+// w-slices of its (x, y, z) position in registers v[0..3].  The separable
+// decorrelating transform runs along x, y, z with lane shuffles (each lane
+// rebuilds its 4-line and keeps its own output) and along w in registers.
+// After the negabinary reorder, lane r holds the coefficient of rank r
+// (ranks 64q + l in register q for d = 4); one ballot then yields a whole
+// bit plane, so the embedded group-test coder runs wave-uniform on 64-bit
+// plane words (ctz finds the next significant coefficient instead of a
+// per-position loop) and every lane deposits its own bit when decoding.
+//
+// The algorithm and the stream layout are the published ZFP ones (Lindstrom,
+// IEEE TVCG 2014; LLNL ZFP stream format, codec 5), which the reference
+// reaches through zfp-rs 0.1.1 (crates/ndic-zfp/src/lib.rs:249-376): block
+// order x fastest, partial blocks padded by the pad rule, block-floating-
+// point cast at 2^(p-2-emax), integer lifting, negabinary, embedded coder,
+// reversible variant (Lorenzo lifting, precision prefix, float reinterpret).
+#include "zfp.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+namespace ndic {
+
+namespace {
+
+// coefficient order per dimensionality (rank -> index) and its inverse
+__constant__ uint8_t c_perm[4][256];
+__constant__ uint8_t c_iperm[4][256];
+
+__device__ __forceinline__ uint32_t lane() { return __lane_id(); }
+
+template <class S> struct ZT;
+template <> struct ZT<int32_t> {
+  using Int = int32_t;
+  using UInt = uint32_t;
+  static constexpr bool fp = false;
+};
+template <> struct ZT<int64_t> {
+  using Int = int64_t;
+  using UInt = uint64_t;
+  static constexpr bool fp = false;
+};
+template <> struct ZT<float> {
+  using Int = int32_t;
+  using UInt = uint32_t;
+  static constexpr int ebits = 8, ebias = 127, mbits = 23;
+  static constexpr bool fp = true;
+};
+template <> struct ZT<double> {
+  using Int = int64_t;
+  using UInt = uint64_t;
+  static constexpr int ebits = 11, ebias = 1023, mbits = 52;
+  static constexpr bool fp = true;
+};
+
+template <class Int> __device__ __forceinline__ Int wadd(Int a, Int b) {
+  using U = typename std::make_unsigned<Int>::type;
+  return (Int)((U)a + (U)b);
+}
+template <class Int> __device__ __forceinline__ Int wsub(Int a, Int b) {
+  using U = typename std::make_unsigned<Int>::type;
+  return (Int)((U)a - (U)b);
+}
+template <class Int> __device__ __forceinline__ Int wshl1(Int a) {
+  using U = typename std::make_unsigned<Int>::type;
+  return (Int)((U)a << 1);
+}
+
+template <class UInt> __device__ __forceinline__ UInt nbmask() { return (UInt)0xaaaaaaaaaaaaaaaaull; }
+
+// ---------------------------------------------------------------------------
+// 4-point lifting steps (forward / inverse, lossy / reversible)
+// ---------------------------------------------------------------------------
+template <class Int, bool Fwd, bool Rev> __device__ __forceinline__ void lift4(Int& x, Int& y, Int& z, Int& w) {
+  if (Fwd && !Rev) {
+    x = wadd(x, w); x >>= 1; w = wsub(w, x);
+    z = wadd(z, y); z >>= 1; y = wsub(y, z);
+    x = wadd(x, z); x >>= 1; z = wsub(z, x);
+    w = wadd(w, y); w >>= 1; y = wsub(y, w);
+    w = wadd(w, (Int)(y >> 1)); y = wsub(y, (Int)(w >> 1));
+  } else if (!Fwd && !Rev) {
+    y = wadd(y, (Int)(w >> 1)); w = wsub(w, (Int)(y >> 1));
+    y = wadd(y, w); w = wshl1(w); w = wsub(w, y);
+    z = wadd(z, x); x = wshl1(x); x = wsub(x, z);
+    y = wadd(y, z); z = wshl1(z); z = wsub(z, y);
+    w = wadd(w, x); x = wshl1(x); x = wsub(x, w);
+  } else if (Fwd) {
+    w = wsub(w, z); z = wsub(z, y); y = wsub(y, x);
+    w = wsub(w, z); z = wsub(z, y);
+    w = wsub(w, z);
+  } else {
+    w = wadd(w, z); z = wadd(z, y); y = wadd(y, x);
+    w = wadd(w, z); z = wadd(z, y);
+    w = wadd(w, z);
+  }
+}
+
+template <class Int, bool Fwd, bool Rev> __device__ __forceinline__ Int lift_lanes(Int v, uint32_t axis) {
+  uint32_t l = lane(), sh = 2 * axis, s = 1u << sh;
+  uint32_t pos = (l >> sh) & 3, base = l - pos * s;
+  Int x = __shfl(v, (int)base), y = __shfl(v, (int)(base + s)), z = __shfl(v, (int)(base + 2 * s)),
+      w = __shfl(v, (int)(base + 3 * s));
+  lift4<Int, Fwd, Rev>(x, y, z, w);
+  return pos == 0 ? x : pos == 1 ? y : pos == 2 ? z : w;
+}
+
+template <int D, class Int, bool Fwd, bool Rev> __device__ __forceinline__ void xform(Int* v) {
+  constexpr int R = D == 4 ? 4 : 1;
+  if (Fwd) {
+#pragma unroll
+    for (int a = 0; a < (D < 3 ? D : 3); ++a)
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = lift_lanes<Int, Fwd, Rev>(v[j], a);
+    if (D == 4) lift4<Int, Fwd, Rev>(v[0], v[1], v[2], v[R - 1]);
+  } else {
+    if (D == 4) lift4<Int, Fwd, Rev>(v[0], v[1], v[2], v[R - 1]);
+#pragma unroll
+    for (int a = (D < 3 ? D : 3) - 1; a >= 0; --a)
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = lift_lanes<Int, Fwd, Rev>(v[j], a);
+  }
+}
+
+// coefficients -> ranked negabinary (lane r, register q holds rank 64q + r)
+template <int D, class Int, class UInt> __device__ __forceinline__ void to_ranks(const Int* v, UInt* u) {
+  constexpr uint32_t S = 1u << (2 * D);
+  const UInt m = nbmask<UInt>();
+  uint32_t l = lane();
+  if constexpr (D < 4) {
+    uint32_t c = l < S ? c_perm[D - 1][l] : 0;
+    Int t = __shfl(v[0], (int)c);
+    u[0] = l < S ? (((UInt)t + m) ^ m) : 0;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t c = c_perm[3][l + 64 * q], sl = c & 63, sr = c >> 6;
+      Int t0 = __shfl(v[0], (int)sl), t1 = __shfl(v[1], (int)sl), t2 = __shfl(v[2], (int)sl),
+          t3 = __shfl(v[3], (int)sl);
+      Int t = sr == 0 ? t0 : sr == 1 ? t1 : sr == 2 ? t2 : t3;
+      u[q] = ((UInt)t + m) ^ m;
+    }
+  }
+}
+
+template <int D, class Int, class UInt> __device__ __forceinline__ void from_ranks(const UInt* u, Int* v) {
+  constexpr uint32_t S = 1u << (2 * D);
+  const UInt m = nbmask<UInt>();
+  uint32_t l = lane();
+  if constexpr (D < 4) {
+    uint32_t r = l < S ? c_iperm[D - 1][l] : 0;
+    UInt t = __shfl(u[0], (int)r);
+    v[0] = (Int)((t ^ m) - m);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      uint32_t r = c_iperm[3][l + 64 * j], sl = r & 63, sq = r >> 6;
+      UInt t0 = __shfl(u[0], (int)sl), t1 = __shfl(u[1], (int)sl), t2 = __shfl(u[2], (int)sl),
+           t3 = __shfl(u[3], (int)sl);
+      UInt t = sq == 0 ? t0 : sq == 1 ? t1 : sq == 2 ? t2 : t3;
+      v[j] = (Int)((t ^ m) - m);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// wave-uniform bit I/O (LSB-first 64-bit words)
+// ---------------------------------------------------------------------------
+struct BitWriter {
+  uint64_t* out;
+  uint64_t word;
+  uint64_t acc;
+  uint32_t fill;
+  bool shared;  // edge words may be shared with neighbouring blocks (OR them in)
+  bool first;
+  uint64_t count;
+
+  __device__ void init(uint64_t* base, uint64_t bitpos, bool share) {
+    out = base;
+    word = bitpos >> 6;
+    fill = (uint32_t)(bitpos & 63);
+    acc = 0;
+    shared = share;
+    first = true;
+    count = 0;
+  }
+  __device__ void store_word() {
+    if (lane() == 0) {
+      if (shared && first) atomicOr((unsigned long long*)&out[word], (unsigned long long)acc);
+      else out[word] = acc;
+    }
+    first = false;
+    ++word;
+    acc = 0;
+  }
+  // appends the low n bits of v (v has no bits at or above n)
+  __device__ void put(uint64_t v, uint32_t n) {
+    if (n == 0) return;
+    count += n;
+    acc |= v << fill;
+    uint32_t f = fill + n;
+    if (f >= 64) {
+      uint32_t used = 64 - fill;
+      store_word();
+      acc = used < 64 ? v >> used : 0;
+      fill = f - 64;
+    } else {
+      fill = f;
+    }
+  }
+  __device__ void zeros(uint64_t n) {
+    while (n) {
+      uint32_t k = n < 64 ? (uint32_t)n : 64;
+      put(0, k);
+      n -= k;
+    }
+  }
+  __device__ void finish() {
+    if (fill && lane() == 0) {
+      if (shared) atomicOr((unsigned long long*)&out[word], (unsigned long long)acc);
+      else out[word] = acc;
+    }
+  }
+};
+
+struct BitReader {
+  const uint8_t* base;
+  uint64_t nbytes;
+  uint64_t next;
+  uint64_t cur;
+  uint32_t left;
+
+  __device__ uint64_t load(uint64_t w) const {
+    uint64_t b = w * 8;
+    if (b + 8 <= nbytes) return *(const uint64_t*)(base + b);
+    uint64_t v = 0;
+    for (uint64_t i = b; i < nbytes && i < b + 8; ++i) v |= (uint64_t)base[i] << (8 * (i - b));
+    return v;
+  }
+  __device__ void seek(uint64_t bit) {
+    next = bit >> 6;
+    uint32_t o = (uint32_t)(bit & 63);
+    cur = load(next++) >> o;
+    left = 64 - o;
+  }
+  __device__ uint64_t pos() const { return next * 64 - left; }
+  __device__ uint64_t get(uint32_t n) {
+    if (n == 0) return 0;
+    if (n <= left) {
+      uint64_t v = n == 64 ? cur : cur & ((1ull << n) - 1);
+      cur = n == 64 ? 0 : cur >> n;
+      left -= n;
+      return v;
+    }
+    uint64_t v = cur;
+    uint32_t got = left;
+    uint64_t w = load(next++);
+    uint32_t rem = n - got;
+    v |= w << got;
+    cur = rem == 64 ? 0 : w >> rem;
+    left = 64 - rem;
+    return n == 64 ? v : v & ((1ull << n) - 1);
+  }
+  __device__ uint32_t bit() { return (uint32_t)get(1); }
+  // consumes up to lim consecutive zero bits; stops before a one bit
+  __device__ uint32_t zeros(uint32_t lim) {
+    uint32_t z = 0;
+    while (z < lim) {
+      if (left == 0) {
+        cur = load(next++);
+        left = 64;
+      }
+      uint32_t t = cur ? (uint32_t)__builtin_ctzll(cur) : 64;
+      if (t > left) t = left;
+      uint32_t take = t < lim - z ? t : lim - z;
+      cur = take == 64 ? 0 : cur >> take;
+      left -= take;
+      z += take;
+      if (take < t || left > 0) break;
+    }
+    return z;
+  }
+  __device__ void skip(uint64_t n) { seek(pos() + n); }
+};
+
+// ---------------------------------------------------------------------------
+// embedded bit-plane coder over ranked coefficients
+// ---------------------------------------------------------------------------
+template <int R> __device__ __forceinline__ uint32_t next_one(const uint64_t* x, uint32_t n, uint32_t S) {
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (64u * q + 63 < n) continue;
+    uint64_t v = x[q];
+    if (64u * q < n) v &= ~0ull << (n - 64u * q);
+    if (v) return 64u * q + (uint32_t)__builtin_ctzll(v);
+  }
+  return S;
+}
+
+template <int D, class UInt>
+__device__ uint32_t encode_ints(BitWriter& w, uint32_t maxbits, uint32_t maxprec, const UInt* u) {
+  constexpr int R = D == 4 ? 4 : 1;
+  constexpr uint32_t S = 1u << (2 * D);
+  constexpr uint32_t intprec = sizeof(UInt) * 8;
+  uint32_t kmin = intprec > maxprec ? intprec - maxprec : 0;
+  uint32_t bits = maxbits, n = 0;
+  for (uint32_t k = intprec; bits && k-- > kmin;) {
+    uint64_t x[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) x[q] = __ballot((int)((u[q] >> k) & 1));
+    uint32_t m = n < bits ? n : bits;
+    bits -= m;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      if (m > 64u * q) {
+        uint32_t c = m - 64u * q;
+        if (c >= 64) w.put(x[q], 64);
+        else w.put(x[q] & ((1ull << c) - 1), c);
+      }
+    }
+    while (n < S && bits) {
+      bits--;
+      uint32_t p = next_one<R>(x, n, S);
+      if (p >= S) {
+        w.put(0, 1);
+        break;
+      }
+      w.put(1, 1);
+      uint32_t z = p - n;
+      if (bits <= z) {
+        w.zeros(bits);
+        bits = 0;
+        break;
+      }
+      w.zeros(z);
+      bits -= z;
+      n = p;
+      if (n < S - 1) {
+        bits--;
+        w.put(1, 1);
+      }
+      n++;
+    }
+  }
+  return maxbits - bits;
+}
+
+template <int D, class UInt, bool Parse>
+__device__ uint32_t decode_ints(BitReader& r, uint32_t maxbits, uint32_t maxprec, UInt* u) {
+  constexpr int R = D == 4 ? 4 : 1;
+  constexpr uint32_t S = 1u << (2 * D);
+  constexpr uint32_t intprec = sizeof(UInt) * 8;
+  uint32_t kmin = intprec > maxprec ? intprec - maxprec : 0;
+  uint32_t bits = maxbits, n = 0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) u[q] = 0;
+  uint32_t l = lane();
+  for (uint32_t k = intprec; bits && k-- > kmin;) {
+    uint64_t x[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) x[q] = 0;
+    uint32_t m = n < bits ? n : bits;
+    bits -= m;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      if (m > 64u * q) {
+        uint32_t c = m - 64u * q;
+        x[q] = r.get(c >= 64 ? 64 : c);
+      }
+    }
+    while (n < S && bits) {
+      bits--;
+      if (!r.bit()) break;
+      uint32_t lim = S - 1 - n;
+      if (lim > bits) lim = bits;
+      uint32_t z = r.zeros(lim);
+      if (z < lim) {
+        (void)r.get(1);
+        bits -= z + 1;
+      } else {
+        bits -= z;
+      }
+      n += z;
+      if (!Parse) {
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if ((n >> 6) == (uint32_t)q) x[q] |= 1ull << (n & 63);
+      }
+      n++;
+    }
+    if (!Parse) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) u[q] |= (UInt)((x[q] >> l) & 1) << k;
+    }
+  }
+  return maxbits - bits;
+}
+
+// ---------------------------------------------------------------------------
+// integer blocks
+// ---------------------------------------------------------------------------
+template <int D, class Int>
+__device__ uint32_t encode_int_block(BitWriter& w, int minbits, int maxbits, int maxprec, Int* v) {
+  using UInt = typename std::make_unsigned<Int>::type;
+  constexpr int R = D == 4 ? 4 : 1;
+  xform<D, Int, true, false>(v);
+  UInt u[R];
+  to_ranks<D, Int, UInt>(v, u);
+  int bits = (int)encode_ints<D, UInt>(w, maxbits > 0 ? (uint32_t)maxbits : 0u, (uint32_t)maxprec, u);
+  if (bits < minbits) {
+    w.zeros((uint64_t)(minbits - bits));
+    bits = minbits;
+  }
+  return (uint32_t)bits;
+}
+
+template <class UInt> __device__ __forceinline__ UInt wave_or(UInt v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+
+template <int D, class Int>
+__device__ uint32_t rev_encode_int_block(BitWriter& w, int minbits, int maxbits, int maxprec, Int* v) {
+  using UInt = typename std::make_unsigned<Int>::type;
+  constexpr int R = D == 4 ? 4 : 1;
+  constexpr int pbits = sizeof(Int) == 4 ? 5 : 6;
+  constexpr uint32_t intprec = sizeof(Int) * 8;
+  xform<D, Int, true, true>(v);
+  UInt u[R];
+  to_ranks<D, Int, UInt>(v, u);
+  UInt m = 0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) m |= u[q];
+  m = wave_or(m);
+  uint32_t prec = m ? intprec - (uint32_t)__builtin_ctzll((unsigned long long)m) : 0;
+  if (prec > (uint32_t)maxprec) prec = (uint32_t)maxprec;
+  if (prec < 1) prec = 1;
+  w.put(prec - 1, pbits);
+  int bits = pbits;
+  bits += (int)encode_ints<D, UInt>(w, maxbits - bits > 0 ? (uint32_t)(maxbits - bits) : 0u, prec, u);
+  if (bits < minbits) {
+    w.zeros((uint64_t)(minbits - bits));
+    bits = minbits;
+  }
+  return (uint32_t)bits;
+}
+
+template <int D, class Int, bool Parse>
+__device__ uint32_t decode_int_block(BitReader& r, int minbits, int maxbits, int maxprec, bool rev, Int* v) {
+  using UInt = typename std::make_unsigned<Int>::type;
+  constexpr int R = D == 4 ? 4 : 1;
+  constexpr int pbits = sizeof(Int) == 4 ? 5 : 6;
+  int bits = 0;
+  uint32_t prec = (uint32_t)maxprec;
+  if (rev) {
+    prec = (uint32_t)r.get(pbits) + 1;
+    bits += pbits;
+  }
+  UInt u[R];
+  bits += (int)decode_ints<D, UInt, Parse>(r, maxbits - bits > 0 ? (uint32_t)(maxbits - bits) : 0u, prec, u);
+  if (bits < minbits) {
+    r.skip((uint64_t)(minbits - bits));
+    bits = minbits;
+  }
+  if (!Parse) {
+    from_ranks<D, Int, UInt>(u, v);
+    if (rev) xform<D, Int, false, true>(v);
+    else xform<D, Int, false, false>(v);
+  }
+  return (uint32_t)bits;
+}
+
+// ---------------------------------------------------------------------------
+// floating-point blocks
+// ---------------------------------------------------------------------------
+template <class S> __device__ __forceinline__ typename ZT<S>::UInt fbits(S x) {
+  typename ZT<S>::UInt b;
+  __builtin_memcpy(&b, &x, sizeof(b));
+  return b;
+}
+template <class S> __device__ __forceinline__ S fromb(typename ZT<S>::UInt b) {
+  S x;
+  __builtin_memcpy(&x, &b, sizeof(b));
+  return x;
+}
+// 2^k: +inf above the normal range, subnormal or 0 below it
+template <class S> __device__ __forceinline__ S pow2(int k) {
+  using U = typename ZT<S>::UInt;
+  constexpr int bias = ZT<S>::ebias, mb = ZT<S>::mbits;
+  if (k > bias) return fromb<S>((U)(2 * bias + 1) << mb);
+  if (k >= 1 - bias) return fromb<S>((U)(k + bias) << mb);
+  if (k >= 1 - bias - mb) return fromb<S>((U)1 << (k - (1 - bias - mb)));
+  return (S)0;
+}
+
+template <class UInt> __device__ __forceinline__ UInt wave_max(UInt v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    UInt t = __shfl_xor(v, o);
+    v = t > v ? t : v;
+  }
+  return v;
+}
+
+// x = m 2^e, 0.5 <= m < 1, of the largest |value| (NaN ignored, infinity -> 0)
+template <int D, class S> __device__ __forceinline__ int block_exponent(const S* f) {
+  using U = typename ZT<S>::UInt;
+  constexpr int R = D == 4 ? 4 : 1;
+  constexpr int bias = ZT<S>::ebias, mb = ZT<S>::mbits;
+  constexpr U absmask = ~(U)0 >> 1, inf = (U)(2 * bias + 1) << mb;
+  U mx = 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    U a = fbits(f[j]) & absmask;
+    if (a > inf) a = 0;
+    mx = a > mx ? a : mx;
+  }
+  mx = wave_max(mx);
+  if (mx == 0) return -bias;
+  int be = (int)(mx >> mb);
+  if (be == 0) return 1 - bias;
+  if (be == 2 * bias + 1) return 0;
+  return be - (bias - 1);
+}
+
+template <class S> __device__ __forceinline__ typename ZT<S>::Int cast_int(S y) {
+  using Int = typename ZT<S>::Int;
+  constexpr int bits = sizeof(Int) * 8;
+  const S lim = pow2<S>(bits - 1);
+  if (!(y < lim && y >= -lim)) return (Int)((typename ZT<S>::UInt)1 << (bits - 1));
+  return (Int)y;
+}
+
+template <int D, class S> __device__ __forceinline__ void fwd_cast(const S* f, typename ZT<S>::Int* v, int emax) {
+  constexpr int R = D == 4 ? 4 : 1;
+  S s = pow2<S>((int)sizeof(S) * 8 - 2 - emax);
+#pragma unroll
+  for (int j = 0; j < R; ++j) v[j] = cast_int<S>(s * f[j]);
+}
+template <int D, class S> __device__ __forceinline__ void inv_cast(const typename ZT<S>::Int* v, S* f, int emax) {
+  constexpr int R = D == 4 ? 4 : 1;
+  S s = pow2<S>(emax - ((int)sizeof(S) * 8 - 2));
+#pragma unroll
+  for (int j = 0; j < R; ++j) f[j] = s * (S)v[j];
+}
+
+template <int D, class S> __device__ uint32_t encode_float_block(BitWriter& w, const ZfpChunk& c, const S* f) {
+  using Int = typename ZT<S>::Int;
+  using U = typename ZT<S>::UInt;
+  constexpr int R = D == 4 ? 4 : 1;
+  constexpr uint32_t SZ = 1u << (2 * D);
+  constexpr int ebits = ZT<S>::ebits, ebias = ZT<S>::ebias;
+  Int v[R];
+  int emax = block_exponent<D, S>(f);
+  if (c.minexp < kZfpMinExp) {  // reversible
+    uint32_t bits = 0;
+    if (emax != -ebias) fwd_cast<D, S>(f, v, emax);
+    else
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = 0;
+    S g[R];
+    if (emax != -ebias) inv_cast<D, S>(v, g, emax);
+    else
+#pragma unroll
+      for (int j = 0; j < R; ++j) g[j] = 0;
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < R; ++j) bad |= (D == 4 || lane() < SZ) && fbits(g[j]) != fbits(f[j]);
+    bool same = __ballot(bad) == 0;
+    if (same) {
+      uint32_t e = (uint32_t)(emax + ebias);
+      if (!e) {
+        w.put(0, 1);
+        return 1;
+      }
+      w.put(1, 2);
+      w.put(e, ebits);
+      bits = 2 + ebits;
+    } else {
+      const U mx = ~(U)0 >> 1;
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        U b = fbits(f[j]);
+        v[j] = (Int)b < 0 ? (Int)(b ^ mx) : (Int)b;
+      }
+      w.put(3, 2);
+      bits = 2;
+    }
+    return bits + rev_encode_int_block<D, Int>(w, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits,
+                                                (int)c.maxprec, v);
+  }
+  int dp = emax - c.minexp + 2 * (D + 1);
+  int maxprec = (int)c.maxprec < (dp > 0 ? dp : 0) ? (int)c.maxprec : (dp > 0 ? dp : 0);
+  uint32_t e = maxprec ? (uint32_t)(emax + ebias) : 0;
+  if (e) {
+    uint32_t bits = 1 + ebits;
+    w.put(2 * (uint64_t)e + 1, bits);
+    fwd_cast<D, S>(f, v, emax);
+    return bits + encode_int_block<D, Int>(w, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits, maxprec, v);
+  }
+  w.put(0, 1);
+  if (c.minbits > 1) {
+    w.zeros(c.minbits - 1);
+    return c.minbits;
+  }
+  return 1;
+}
+
+template <int D, class S, bool Parse> __device__ uint32_t decode_float_block(BitReader& r, const ZfpChunk& c, S* f) {
+  using Int = typename ZT<S>::Int;
+  using U = typename ZT<S>::UInt;
+  constexpr int R = D == 4 ? 4 : 1;
+  constexpr int ebits = ZT<S>::ebits, ebias = ZT<S>::ebias;
+  Int v[R];
+  if (c.minexp < kZfpMinExp) {  // reversible
+    uint32_t bits = 1;
+    if (r.bit()) {
+      bits++;
+      if (r.bit()) {
+        bits += decode_int_block<D, Int, Parse>(r, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits,
+                                                (int)c.maxprec, true, v);
+        if (!Parse) {
+          const U mx = ~(U)0 >> 1;
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            Int x = v[j] < 0 ? (Int)((U)v[j] ^ mx) : v[j];
+            f[j] = fromb<S>((U)x);
+          }
+        }
+      } else {
+        bits += ebits;
+        int emax = (int)r.get(ebits) - ebias;
+        bits += decode_int_block<D, Int, Parse>(r, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits,
+                                                (int)c.maxprec, true, v);
+        if (!Parse) inv_cast<D, S>(v, f, emax);
+      }
+      return bits;
+    }
+    if (!Parse)
+#pragma unroll
+      for (int j = 0; j < R; ++j) f[j] = 0;
+    return bits;
+  }
+  if (r.bit()) {
+    uint32_t bits = 1 + ebits;
+    int emax = (int)r.get(ebits) - ebias;
+    int dp = emax - c.minexp + 2 * (D + 1);
+    int maxprec = (int)c.maxprec < (dp > 0 ? dp : 0) ? (int)c.maxprec : (dp > 0 ? dp : 0);
+    bits += decode_int_block<D, Int, Parse>(r, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits, maxprec,
+                                            false, v);
+    if (!Parse) inv_cast<D, S>(v, f, emax);
+    return bits;
+  }
+  if (!Parse)
+#pragma unroll
+    for (int j = 0; j < R; ++j) f[j] = 0;
+  if (c.minbits > 1) {
+    r.skip(c.minbits - 1);
+    return c.minbits;
+  }
+  return 1;
+}
+
+template <int D, class S> __device__ uint32_t encode_block(BitWriter& w, const ZfpChunk& c, S* vals) {
+  if constexpr (ZT<S>::fp) {
+    return encode_float_block<D, S>(w, c, vals);
+  } else {
+    if (c.minexp < kZfpMinExp)
+      return rev_encode_int_block<D, S>(w, (int)c.minbits, (int)c.maxbits, (int)c.maxprec, vals);
+    return encode_int_block<D, S>(w, (int)c.minbits, (int)c.maxbits, (int)c.maxprec, vals);
+  }
+}
+
+template <int D, class S, bool Parse> __device__ uint32_t decode_block(BitReader& r, const ZfpChunk& c, S* vals) {
+  if constexpr (ZT<S>::fp) {
+    return decode_float_block<D, S, Parse>(r, c, vals);
+  } else {
+    return decode_int_block<D, S, Parse>(r, (int)c.minbits, (int)c.maxbits, (int)c.maxprec,
+                                         c.minexp < kZfpMinExp, vals);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// gather / scatter with the dtype promotions of the chunk layer
+// (crates/ndic-zfp/src/chunk.rs:309-356)
+// ---------------------------------------------------------------------------
+template <class S> __device__ __forceinline__ S load_elem(const uint8_t* p, uint64_t i, int32_t dtype) {
+  if constexpr (std::is_same<S, int32_t>::value) {
+    switch (dtype) {
+      case NDIC_U8: return ((int32_t)p[i] - 0x80) * (1 << 23);
+      case NDIC_I8: return (int32_t)(int8_t)p[i] * (1 << 23);
+      case NDIC_U16: return ((int32_t)((const uint16_t*)p)[i] - 0x8000) * (1 << 15);
+      case NDIC_I16: return (int32_t)((const int16_t*)p)[i] * (1 << 15);
+      default: return ((const int32_t*)p)[i];
+    }
+  } else {
+    return ((const S*)p)[i];
+  }
+}
+template <class S> __device__ __forceinline__ void store_elem(uint8_t* p, uint64_t i, int32_t dtype, S v) {
+  if constexpr (std::is_same<S, int32_t>::value) {
+    switch (dtype) {
+      case NDIC_U8: {
+        int32_t t = (v >> 23) + 0x80;
+        p[i] = (uint8_t)(t < 0 ? 0 : t > 0xff ? 0xff : t);
+        return;
+      }
+      case NDIC_I8: {
+        int32_t t = v >> 23;
+        p[i] = (uint8_t)(int8_t)(t < -0x80 ? -0x80 : t > 0x7f ? 0x7f : t);
+        return;
+      }
+      case NDIC_U16: {
+        int32_t t = (v >> 15) + 0x8000;
+        ((uint16_t*)p)[i] = (uint16_t)(t < 0 ? 0 : t > 0xffff ? 0xffff : t);
+        return;
+      }
+      case NDIC_I16: {
+        int32_t t = v >> 15;
+        ((int16_t*)p)[i] = (int16_t)(t < -0x8000 ? -0x8000 : t > 0x7fff ? 0x7fff : t);
+        return;
+      }
+      default: ((int32_t*)p)[i] = v; return;
+    }
+  } else {
+    ((S*)p)[i] = v;
+  }
+}
+
+__device__ __forceinline__ uint32_t pad_src(uint32_t c, uint32_t ext) {
+  // pad rule of a partial 4-vector: n=1 -> p0 p0 p0 p0, n=2 -> p0 p1 p1 p0, n=3 -> p0 p1 p2 p0
+  if (c < ext) return c;
+  if (ext == 2 && c == 2) return 1;
+  return 0;
+}
+
+__device__ __forceinline__ void block_coords(const ZfpChunk& c, uint64_t k, uint32_t* b) {
+  b[0] = (uint32_t)(k % c.nb[0]);
+  uint64_t t = k / c.nb[0];
+  b[1] = (uint32_t)(t % c.nb[1]);
+  t /= c.nb[1];
+  b[2] = (uint32_t)(t % c.nb[2]);
+  b[3] = (uint32_t)(t / c.nb[2]);
+}
+
+template <int D, class S> __device__ void gather(const ZfpChunk& c, const uint32_t* b, S* vals) {
+  constexpr int R = D == 4 ? 4 : 1;
+  constexpr uint32_t SZ = 1u << (2 * D);
+  uint32_t l = lane();
+  uint32_t cc[3] = {l & 3, (l >> 2) & 3, (l >> 4) & 3};
+  uint32_t ext[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    uint32_t e = a < D ? c.n[a] - 4 * b[a] : 1;
+    ext[a] = e < 4 ? e : 4;
+  }
+  uint64_t sx = 4ull * b[0] + pad_src(cc[0], ext[0]);
+  uint64_t sy = D > 1 ? 4ull * b[1] + pad_src(cc[1], ext[1]) : 0;
+  uint64_t sz = D > 2 ? 4ull * b[2] + pad_src(cc[2], ext[2]) : 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    if (D < 4 && l >= SZ) {
+      vals[j] = 0;
+      continue;
+    }
+    uint64_t sw = D > 3 ? 4ull * b[3] + pad_src((uint32_t)j, ext[3]) : 0;
+    uint64_t idx = ((sw * c.n[2] + sz) * c.n[1] + sy) * c.n[0] + sx;
+    vals[j] = load_elem<S>(c.src, idx, c.dtype);
+  }
+}
+
+// Writes the block's samples that fall inside the field (or, for a brick,
+// into its own clipped C-ordered array).
+template <int D, class S> __device__ void scatter(const ZfpChunk& c, const uint32_t* b, const S* vals, uint8_t* brick) {
+  constexpr int R = D == 4 ? 4 : 1;
+  constexpr uint32_t SZ = 1u << (2 * D);
+  uint32_t l = lane();
+  if (D < 4 && l >= SZ) return;
+  uint32_t cc[3] = {l & 3, (l >> 2) & 3, (l >> 4) & 3};
+  uint32_t ext[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    uint32_t e = a < D ? c.n[a] - 4 * b[a] : 1;
+    ext[a] = e < 4 ? e : 4;
+  }
+  for (int a = 0; a < (D < 3 ? D : 3); ++a)
+    if (cc[a] >= ext[a]) return;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    if (D == 4 && (uint32_t)j >= ext[3]) continue;
+    uint64_t idx;
+    if (brick) {
+      uint64_t ey = D > 1 ? ext[1] : 1, ez = D > 2 ? ext[2] : 1;
+      idx = ((uint64_t)(D > 3 ? j : 0) * ez + (D > 2 ? cc[2] : 0)) * ey * ext[0] + (D > 1 ? cc[1] : 0) * ext[0] +
+            cc[0];
+      store_elem<S>(brick, idx, c.dtype, vals[j]);
+    } else {
+      uint64_t x = 4ull * b[0] + cc[0], y = D > 1 ? 4ull * b[1] + cc[1] : 0, z = D > 2 ? 4ull * b[2] + cc[2] : 0,
+               w = D > 3 ? 4ull * b[3] + j : 0;
+      idx = ((w * c.n[2] + z) * c.n[1] + y) * c.n[0] + x;
+      store_elem<S>(c.dst, idx, c.dtype, vals[j]);
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t find_chunk(const ZfpChunk* chunks, uint32_t n, uint64_t g) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) / 2;
+    if (chunks[mid].block_begin <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// kernels: four wavefronts (blocks) per workgroup
+// ---------------------------------------------------------------------------
+template <int D, class S>
+__global__ __launch_bounds__(256) void zfp_encode_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks,
+                                                         uint64_t total) {
+  constexpr int R = D == 4 ? 4 : 1;
+  uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= total) return;
+  const ZfpChunk& c = chunks[find_chunk(chunks, nchunks, g)];
+  if (c.fixed ? !c.dst : !c.slots) return;  // refused chunk
+  uint64_t k = g - c.block_begin;
+  uint32_t b[4];
+  block_coords(c, k, b);
+  S vals[R];
+  gather<D, S>(c, b, vals);
+  BitWriter w;
+  if (c.fixed) w.init((uint64_t*)c.dst, c.header_bits + k * c.maxbits, true);
+  else w.init(c.slots + g * c.slot_words, 0, false);
+  encode_block<D, S>(w, c, vals);
+  w.finish();
+  if (!c.fixed && lane() == 0) c.offs[g] = w.count;
+}
+
+// Variable-size modes: moves block g's bits from its scratch slot to its
+// place in the stream (header + exclusive prefix sum of the lengths).
+__global__ __launch_bounds__(256) void zfp_place_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks,
+                                                        uint64_t total, const uint64_t* __restrict__ scan) {
+  uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= total) return;
+  const ZfpChunk& c = chunks[find_chunk(chunks, nchunks, g)];
+  if (c.fixed || !c.offs || !c.dst) return;  // fixed-rate (written in place) or refused
+  uint64_t len = c.offs[g];
+  if (len == 0) return;
+  uint64_t d0 = c.header_bits + scan[g] - scan[c.block_begin];
+  const uint64_t* src = c.slots + g * c.slot_words;
+  uint64_t* dst = (uint64_t*)c.dst;
+  uint64_t first = d0 >> 6, last = (d0 + len - 1) >> 6;
+  for (uint64_t dw = first + lane(); dw <= last; dw += 64) {
+    uint64_t wb = dw * 64;
+    uint64_t lo = wb > d0 ? wb : d0, hi = wb + 64 < d0 + len ? wb + 64 : d0 + len;
+    uint64_t sb = lo - d0;
+    uint32_t cnt = (uint32_t)(hi - lo);
+    uint64_t sw = sb >> 6;
+    uint32_t so = (uint32_t)(sb & 63);
+    uint64_t v = src[sw] >> so;
+    if (so && so + cnt > 64) v |= src[sw + 1] << (64 - so);
+    if (cnt < 64) v &= (1ull << cnt) - 1;
+    v <<= (uint32_t)(lo - wb);
+    if (cnt == 64) dst[dw] = v;
+    else atomicOr((unsigned long long*)&dst[dw], (unsigned long long)v);
+  }
+}
+
+// Variable-size modes: one wavefront per field walks its blocks in stream
+// order and records where each begins (the format has no block index).
+template <int D, class S>
+__global__ __launch_bounds__(64) void zfp_index_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks) {
+  constexpr int R = D == 4 ? 4 : 1;
+  uint32_t ci = blockIdx.x;
+  if (ci >= nchunks) return;
+  const ZfpChunk& c = chunks[ci];
+  if (c.fixed) return;
+  BitReader r;
+  r.base = c.src;
+  r.nbytes = c.src_len;
+  r.seek(c.header_bits);
+  S vals[R];
+  for (uint64_t k = 0; k < c.nblocks; ++k) {
+    if (lane() == 0) c.offs[c.block_begin + k] = r.pos();
+    decode_block<D, S, true>(r, c, vals);
+  }
+}
+
+template <int D, class S>
+__global__ __launch_bounds__(256) void zfp_decode_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks,
+                                                         uint64_t total, const ZfpBrick* __restrict__ bricks) {
+  constexpr int R = D == 4 ? 4 : 1;
+  uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= total) return;
+  uint32_t ci;
+  uint64_t k;
+  uint8_t* brick = nullptr;
+  if (bricks) {
+    ci = bricks[g].chunk;
+    k = bricks[g].block;
+    brick = bricks[g].out;
+  } else {
+    ci = find_chunk(chunks, nchunks, g);
+    k = g - chunks[ci].block_begin;
+  }
+  const ZfpChunk& c = chunks[ci];
+  BitReader r;
+  r.base = c.src;
+  r.nbytes = c.src_len;
+  r.seek(c.fixed ? c.header_bits + k * c.maxbits : c.offs[c.block_begin + k]);
+  S vals[R];
+  decode_block<D, S, false>(r, c, vals);
+  uint32_t b[4];
+  block_coords(c, k, b);
+  scatter<D, S>(c, b, vals, brick);
+}
+
+template <int D, class S> void enc(const ZfpChunk* d, uint32_t n, uint64_t total, hipStream_t s) {
+  hipLaunchKernelGGL((zfp_encode_kernel<D, S>), dim3((uint32_t)ceil_div(total, 4)), dim3(256), 0, s, d, n, total);
+}
+template <int D, class S> void dec(const ZfpChunk* d, uint32_t n, uint64_t total, const ZfpBrick* br, hipStream_t s) {
+  hipLaunchKernelGGL((zfp_decode_kernel<D, S>), dim3((uint32_t)ceil_div(total, 4)), dim3(256), 0, s, d, n, total,
+                     br);
+}
+template <int D, class S> void idx(const ZfpChunk* d, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL((zfp_index_kernel<D, S>), dim3(n), dim3(64), 0, s, d, n);
+}
+
+#define NDIC_ZFP_DISPATCH(FN, ...)                                                     \
+  switch (dims * 8 + scalar) {                                                         \
+    case 1 * 8 + kZfpInt32: FN<1, int32_t>(__VA_ARGS__); break;                        \
+    case 1 * 8 + kZfpInt64: FN<1, int64_t>(__VA_ARGS__); break;                        \
+    case 1 * 8 + kZfpFloat: FN<1, float>(__VA_ARGS__); break;                          \
+    case 1 * 8 + kZfpDouble: FN<1, double>(__VA_ARGS__); break;                        \
+    case 2 * 8 + kZfpInt32: FN<2, int32_t>(__VA_ARGS__); break;                        \
+    case 2 * 8 + kZfpInt64: FN<2, int64_t>(__VA_ARGS__); break;                        \
+    case 2 * 8 + kZfpFloat: FN<2, float>(__VA_ARGS__); break;                          \
+    case 2 * 8 + kZfpDouble: FN<2, double>(__VA_ARGS__); break;                        \
+    case 3 * 8 + kZfpInt32: FN<3, int32_t>(__VA_ARGS__); break;                        \
+    case 3 * 8 + kZfpInt64: FN<3, int64_t>(__VA_ARGS__); break;                        \
+    case 3 * 8 + kZfpFloat: FN<3, float>(__VA_ARGS__); break;                          \
+    case 3 * 8 + kZfpDouble: FN<3, double>(__VA_ARGS__); break;                        \
+    case 4 * 8 + kZfpInt32: FN<4, int32_t>(__VA_ARGS__); break;                        \
+    case 4 * 8 + kZfpInt64: FN<4, int64_t>(__VA_ARGS__); break;                        \
+    case 4 * 8 + kZfpFloat: FN<4, float>(__VA_ARGS__); break;                          \
+    case 4 * 8 + kZfpDouble: FN<4, double>(__VA_ARGS__); break;                        \
+    default: fail(NDIC_INVALID_ARGUMENT, "zfp: unsupported field class");              \
+  }
+
+// Coefficient orders: by total sequency i+j+k+l, then by sum of squares.
+// The 1D-3D orders are written out per (i, j, k); the 4D one is the order
+// the LLNL C library 0.5.5 emits (ranks measured from its streams).
+#define P2(i, j) ((i) + 4 * (j))
+#define P3(i, j, k) ((i) + 4 * (j) + 16 * (k))
+const uint8_t kPerm2[16] = {P2(0, 0), P2(1, 0), P2(0, 1), P2(1, 1), P2(2, 0), P2(0, 2), P2(2, 1), P2(1, 2),
+                            P2(3, 0), P2(0, 3), P2(2, 2), P2(3, 1), P2(1, 3), P2(3, 2), P2(2, 3), P2(3, 3)};
+const uint8_t kPerm3[64] = {
+    P3(0, 0, 0), P3(1, 0, 0), P3(0, 1, 0), P3(0, 0, 1), P3(0, 1, 1), P3(1, 0, 1), P3(1, 1, 0), P3(2, 0, 0),
+    P3(0, 2, 0), P3(0, 0, 2), P3(1, 1, 1), P3(2, 1, 0), P3(2, 0, 1), P3(0, 2, 1), P3(1, 2, 0), P3(1, 0, 2),
+    P3(0, 1, 2), P3(3, 0, 0), P3(0, 3, 0), P3(0, 0, 3), P3(2, 1, 1), P3(1, 2, 1), P3(1, 1, 2), P3(0, 2, 2),
+    P3(2, 0, 2), P3(2, 2, 0), P3(3, 1, 0), P3(3, 0, 1), P3(0, 3, 1), P3(1, 3, 0), P3(1, 0, 3), P3(0, 1, 3),
+    P3(1, 2, 2), P3(2, 1, 2), P3(2, 2, 1), P3(3, 1, 1), P3(1, 3, 1), P3(1, 1, 3), P3(3, 2, 0), P3(3, 0, 2),
+    P3(0, 3, 2), P3(2, 3, 0), P3(2, 0, 3), P3(0, 2, 3), P3(2, 2, 2), P3(3, 2, 1), P3(3, 1, 2), P3(1, 3, 2),
+    P3(2, 3, 1), P3(2, 1, 3), P3(1, 2, 3), P3(0, 3, 3), P3(3, 0, 3), P3(3, 3, 0), P3(3, 2, 2), P3(2, 3, 2),
+    P3(2, 2, 3), P3(1, 3, 3), P3(3, 1, 3), P3(3, 3, 1), P3(2, 3, 3), P3(3, 2, 3), P3(3, 3, 2), P3(3, 3, 3)};
+#undef P2
+#undef P3
+const uint8_t kPerm4[256] = {
+    0,   1,   4,   16,  64,  5,   80,  17,  68,  65,  20,  2,   8,   32,  128, 84,  81,  69,  21,  6,   18,  66,
+    24,  72,  9,   96,  33,  36,  129, 132, 144, 3,   12,  48,  192, 85,  82,  70,  22,  73,  25,  88,  37,  100,
+    97,  148, 145, 133, 10,  160, 34,  136, 130, 40,  7,   19,  67,  28,  76,  13,  112, 49,  52,  193, 196, 208,
+    86,  89,  101, 149, 161, 137, 41,  134, 38,  164, 26,  152, 146, 104, 98,  74,  83,  71,  23,  77,  29,  92,
+    53,  116, 113, 212, 209, 197, 11,  35,  131, 44,  140, 14,  176, 50,  56,  194, 200, 224, 90,  165, 102, 153,
+    150, 105, 168, 162, 138, 42,  87,  93,  117, 213, 27,  75,  99,  39,  135, 147, 108, 45,  141, 156, 30,  78,
+    177, 180, 54,  114, 120, 57,  198, 210, 216, 201, 225, 228, 15,  240, 51,  204, 195, 60,  169, 166, 154, 106,
+    91,  103, 151, 109, 157, 94,  181, 118, 121, 214, 217, 229, 163, 139, 43,  142, 46,  172, 58,  184, 178, 232,
+    226, 202, 241, 205, 61,  199, 55,  244, 31,  220, 211, 124, 115, 79,  170, 167, 155, 107, 158, 110, 173, 122,
+    185, 182, 233, 230, 218, 95,  245, 119, 221, 215, 125, 242, 206, 62,  203, 59,  248, 47,  236, 227, 188, 179,
+    143, 171, 174, 186, 234, 246, 222, 126, 219, 123, 249, 111, 237, 231, 189, 183, 159, 252, 243, 207, 63,  175,
+    250, 187, 238, 235, 190, 253, 247, 223, 127, 254, 251, 239, 191, 255};
+
+__global__ void zfp_headers_kernel(const ZfpHeader* __restrict__ h, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (uint32_t k = 0; k < h[i].nwords; ++k) h[i].dst[k] = h[i].w[k];
+}
+
+__global__ void zfp_totals_kernel(const ZfpChunk* __restrict__ chunks, uint32_t n, const uint64_t* __restrict__ scan,
+                                  uint64_t* totals) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const ZfpChunk& c = chunks[i];
+  if (c.fixed || c.nblocks == 0) {
+    totals[i] = 0;
+    return;
+  }
+  uint64_t last = c.block_begin + c.nblocks - 1;
+  totals[i] = scan[last] + c.offs[last] - scan[c.block_begin];
+}
+
+__global__ void zfp_heads_kernel(const ZfpHead* __restrict__ h, uint32_t n, uint8_t* out) {
+  uint32_t i = blockIdx.x, b = threadIdx.x;
+  if (i >= n || b >= kZfpHeadBytes) return;
+  out[(uint64_t)i * kZfpHeadBytes + b] = b < h[i].len ? h[i].src[b] : 0;
+}
+
+}  // namespace
+
+size_t zfp_scan_temp_bytes(uint64_t n) {
+  size_t bytes = 0;
+  NDIC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr, (int)n));
+  return bytes;
+}
+
+void zfp_exclusive_scan(void* temp, size_t temp_bytes, const uint64_t* d_in, uint64_t* d_out, uint64_t n,
+                        hipStream_t stream) {
+  if (n == 0) return;
+  if (n > 0x7fffffffull) fail(NDIC_INVALID_ARGUMENT, "zfp: too many blocks in one batch");
+  NDIC_HIP(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, d_in, d_out, (int)n, stream));
+}
+
+void launch_zfp_headers(const ZfpHeader* d_headers, uint32_t n, hipStream_t stream) {
+  if (!n) return;
+  hipLaunchKernelGGL(zfp_headers_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_headers, n);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_zfp_totals(const ZfpChunk* d_chunks, uint32_t nchunks, const uint64_t* d_scan, uint64_t* d_totals,
+                       hipStream_t stream) {
+  if (!nchunks) return;
+  hipLaunchKernelGGL(zfp_totals_kernel, dim3((nchunks + 63) / 64), dim3(64), 0, stream, d_chunks, nchunks, d_scan,
+                     d_totals);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_zfp_heads(const ZfpHead* d_heads, uint32_t n, uint8_t* d_out, hipStream_t stream) {
+  if (!n) return;
+  hipLaunchKernelGGL(zfp_heads_kernel, dim3(n), dim3(64), 0, stream, d_heads, n, d_out);
+  NDIC_HIP(hipGetLastError());
+}
+
+void zfp_init_tables() {
+  static bool done = false;
+  if (done) return;
+  uint8_t perm[4][256] = {}, iperm[4][256] = {};
+  for (int i = 0; i < 4; ++i) perm[0][i] = (uint8_t)i;
+  for (int i = 0; i < 16; ++i) perm[1][i] = kPerm2[i];
+  for (int i = 0; i < 64; ++i) perm[2][i] = kPerm3[i];
+  for (int i = 0; i < 256; ++i) perm[3][i] = kPerm4[i];
+  for (int d = 0; d < 4; ++d)
+    for (int r = 0; r < (1 << (2 * (d + 1))); ++r) iperm[d][perm[d][r]] = (uint8_t)r;
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_perm), perm, sizeof(perm)));
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_iperm), iperm, sizeof(iperm)));
+  done = true;
+}
+
+void launch_zfp_encode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, uint32_t dims, uint32_t scalar,
+                       hipStream_t stream) {
+  if (total == 0) return;
+  NDIC_ZFP_DISPATCH(enc, d_chunks, nchunks, total, stream);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_zfp_place(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, const uint64_t* d_scan,
+                      hipStream_t stream) {
+  if (total == 0) return;
+  hipLaunchKernelGGL(zfp_place_kernel, dim3((uint32_t)ceil_div(total, 4)), dim3(256), 0, stream, d_chunks, nchunks,
+                     total, d_scan);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_zfp_index(const ZfpChunk* d_chunks, uint32_t nchunks, uint32_t dims, uint32_t scalar,
+                      hipStream_t stream) {
+  if (nchunks == 0) return;
+  NDIC_ZFP_DISPATCH(idx, d_chunks, nchunks, stream);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_zfp_decode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, uint32_t dims, uint32_t scalar,
+                       const ZfpBrick* d_bricks, hipStream_t stream) {
+  if (total == 0) return;
+  NDIC_ZFP_DISPATCH(dec, d_chunks, nchunks, total, d_bricks, stream);
+  NDIC_HIP(hipGetLastError());
+}
+
+}  // namespace ndic

@@ -40,6 +40,7 @@ def _lib():
             ("ndic_dev_synth_chunk", C.c_int, [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
                                                C.POINTER(SynthBlobs), C.c_uint64, E]),
             ("ndic_dev_count_mismatch", C.c_int, [P, P, S, C.POINTER(C.c_uint64), E]),
+            ("ndic_dev_copy", C.c_int, [P, P, S, E]),
             ("ndic_dev_profile", None, [C.c_int32]),
             ("ndic_dev_stage_times", C.c_size_t, [C.POINTER(C.c_double), C.POINTER(C.c_double), S]),
         ]:
@@ -119,6 +120,11 @@ def synth_chunk(ptr: int, shape, dtype: str, content: int, blobs=(), seed: int =
         b.cz[i], b.cy[i], b.cx[i], b.amp[i], b.sigma[i] = cz, cy, cx, a, s
     code = _pkg._dtype_code(dtype)
     _call(_lib().ndic_dev_synth_chunk, ptr, z, y, x, code, int(content), C.byref(b), int(seed))
+
+
+def copy(dst: int, src: int, n: int) -> None:
+    """Device copy through a plain kernel (not the copy engine)."""
+    _call(_lib().ndic_dev_copy, dst, src, int(n))
 
 
 def count_mismatch(a: int, b: int, n: int) -> int:

@@ -294,6 +294,7 @@ template <class S> int block_exponent(const S* p, int n) {
   typename Fp<S>::Bits b = to_bits(mx);
   int be = (int)((b >> Fp<S>::mbits) & Fp<S>::emask);
   if (be == 0) return 1 - Fp<S>::bias;
+  if (be == Fp<S>::emask) return 0;  // infinity: frexp reports exponent 0
   return be - (Fp<S>::bias - 1);
 }
 // truncating conversion with the x86 out-of-range result
@@ -529,6 +530,8 @@ Cfg cfg_of(const Mode& m, int type, int dims) {
       uint32_t bits = (uint32_t)std::floor(n * m.rate + 0.5);
       if (type == kFloat) bits = std::max(bits, 1u + 8u);
       if (type == kDouble) bits = std::max(bits, 1u + 11u);
+      // the header's long mode form holds 1..32768 bits per block
+      if (bits < 1 || bits > 0x8000) fail(kInvalidArgument, "nd_zfp: fixed-rate mode needs 1..=32768 bits per block");
       c.minbits = c.maxbits = bits;
       c.maxprec = 64;
       c.minexp = kMinExp;
