@@ -333,202 +333,261 @@ template <uint32_t FILL> struct FwdDec {
 
 __device__ __forceinline__ uint32_t bitlen(uint32_t v) { return 32 - __clz(v); }
 
-// Store one decoded sign-magnitude sample (as a coefficient, or raw).
-__device__ __forceinline__ void put(const HtDecTask& t, uint32_t x, uint32_t y, uint32_t sm, uint32_t shift) {
-  if (t.flags & kTaskSignMag) {
-    ((uint32_t*)t.out)[(uint64_t)y * t.stride + x] = sm;
-  } else {
-    int32_t mag = (int32_t)((sm & 0x7FFFFFFFu) >> shift);
-    ((int32_t*)t.out)[(uint64_t)y * t.stride + x] = (sm & 0x80000000u) ? -mag : mag;
-  }
-}
-
-__device__ void fill_zero(const HtDecTask& t) {
-  for (uint32_t y = 0; y < t.h; ++y)
-    for (uint32_t x = 0; x < t.w; ++x) put(t, x, y, 0, 0);
-}
-
 __device__ void report(const HtDecTask& t, int32_t code) {
   if (t.err) atomicCAS(t.err, 0, code);
 }
 
 constexpr int kRowCap = 34;  // quads per row (64 columns) + terminators
+constexpr int kStageCols = 16;   // output columns staged per lane before a wave-wide flush
+constexpr int kStagePitch = 65;  // LDS pitch of a staged sample row (bank-conflict free)
 
-// decode.rs:54-396 (cleanup pass) for one block.
-__device__ void decode_one(const HtDecTask& t, uint8_t* lrho, uint8_t* lbl) {
-  const uint32_t width = t.w, height = t.h;
-  // per-lane rows, [entry][lane]
-  auto RHO = [&](int k) -> uint8_t& { return lrho[k * 64]; };
-  auto BL = [&](int k) -> uint8_t& { return lbl[k * 64]; };
-  if (!(t.flags & kTaskIncluded)) {
-    fill_zero(t);
-    return;
-  }
-  uint32_t num_passes = t.passes;
-  uint32_t missing_msbs = t.mmsb;
-  uint32_t lengths1 = t.len1, lengths2 = t.len2;
-  if (num_passes > 1 && lengths2 == 0) num_passes = 1;
-  if (num_passes > 3 || missing_msbs >= 30 || lengths1 < 2 || width == 0 || height == 0 || width > 64 ||
-      height > 64) {
-    report(t, kErrBlock);
-    fill_zero(t);
-    return;
-  }
-  if (missing_msbs == 29) num_passes = 1;
-  if (num_passes > 1) {
-    report(t, kErrCsUnsupported);
-    fill_zero(t);
-    return;
-  }
-  const uint32_t p = 30 - missing_msbs;
-  const uint8_t* coded = t.seg;
-  const uint32_t lcup = lengths1;
-  const uint32_t scup = (uint32_t(coded[lcup - 1]) << 4) + uint32_t(coded[lcup - 2] & 0xF);
-  if (scup < 2 || scup > lcup || scup > 4079) {
-    report(t, kErrBlock);
-    fill_zero(t);
-    return;
-  }
-  const uint32_t shift = 31 - t.kmax;
-  const uint32_t mmsbp2 = missing_msbs + 2;
-
+// Per-lane decoder of one code-block's cleanup pass (decode.rs:54-396).  The
+// caller drives it through uniform row-pair / column-pair loops so that the
+// whole wave can flush decoded rows together.
+struct BlockDec {
+  HtDecTask t;
   MelDec mel;
-  mel.init(coded, lcup, scup);
   RevDec vlc;
-  vlc.init_vlc(coded, lcup, scup);
   FwdDec<0xFFFFFFFFu> ms;
-  ms.init(coded, lcup - scup);
-  int run = mel.get_run();
-  bool bad = false;
+  uint32_t p, shift, mmsbp2;
+  int run;
+  bool active;  // decoding (else the block's samples are zero)
+  bool bad;
+  // row-pair state
+  uint32_t c_q, prev_v_n, vp, q;
 
-  for (uint32_t y = 0; y < height; y += 2) {
-    const bool initial = y == 0;
-    uint32_t c_q = 0;
-    uint32_t prev_v_n = 0;
-    uint32_t vp = 0;
-    uint32_t q = 0;  // quad index of the pair's first quad
-    for (uint32_t x = 0; x < width; x += 4, q += 2) {
-      uint32_t a0 = 0, a1 = 0, a2 = 0;
-      if (!initial) {
-        a0 = uint32_t(RHO(q)) << 4;
-        a1 = uint32_t(RHO(q + 1)) << 4;
-        a2 = uint32_t(RHO(q + 2)) << 4;
-        c_q |= (a0 & 0xA0) << 2;
-        c_q |= (a1 & 0x20) << 4;
-      }
-      // ---- VLC / MEL for the pair (decode.rs:115-189, 192-273)
-      uint32_t vlc_val = vlc.fetch();
-      uint32_t t0 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
-      if (c_q == 0) {
-        run -= 2;
-        t0 = run == -1 ? t0 : 0;
-        if (run < 0) run = mel.get_run();
-      }
-      if (initial) {
-        c_q = ((t0 & 0x10) << 3) | ((t0 & 0xE0) << 2);
-      } else {
-        c_q = ((t0 & 0x40) << 2) | ((t0 & 0x80) << 1);
-        c_q |= a0 & 0x80;
-        c_q |= (a1 & 0xA0) << 2;
-        c_q |= (a2 & 0x20) << 4;
-      }
-      vlc_val = vlc.advance(t0 & 0x7);
-      uint32_t t1 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
-      if (c_q == 0 && x + 2 < width) {
-        run -= 2;
-        t1 = run == -1 ? t1 : 0;
-        if (run < 0) run = mel.get_run();
-      }
-      if (x + 2 >= width) t1 = 0;
-      if (initial) {
-        c_q = ((t1 & 0x10) << 3) | ((t1 & 0xE0) << 2);
-      } else {
-        c_q = ((t1 & 0x40) << 2) | ((t1 & 0x80) << 1);
-        c_q |= a1 & 0x80;
-      }
-      vlc_val = vlc.advance(t1 & 0x7);
-      uint32_t uvlc_mode = ((t0 & 0x8) << 3) | ((t1 & 0x8) << 4);
-      if (initial && uvlc_mode == 0xC0) {
-        run -= 2;
-        uvlc_mode += run == -1 ? 0x40 : 0;
-        if (run < 0) run = mel.get_run();
-      }
-      uint32_t e = initial ? c_uvlc0[uvlc_mode + (vlc_val & 0x3F)] : c_uvlc1[uvlc_mode + (vlc_val & 0x3F)];
-      vlc_val = vlc.advance(e & 0x7);
-      e >>= 3;
-      uint32_t len = e & 0xF;
-      uint32_t tmp = vlc_val & ((1u << len) - 1);
-      vlc.advance(len);
-      e >>= 4;
-      uint32_t len0 = e & 0x7;
-      e >>= 3;
-      uint32_t base = initial ? 1 : 0;
-      uint32_t uq[2] = {base + (e & 7) + (tmp & ~(0xFFu << len0)), base + (e >> 3) + (tmp >> len0)};
-      uint32_t tq[2] = {t0, t1};
-      // this row's patterns become the next row's context (reads above done)
-      RHO(q) = uint8_t(t0 >> 4);
-      RHO(q + 1) = uint8_t(t1 >> 4);
-
-      // ---- MagSgn for the two quads (decode.rs:300-396)
-      for (int qi = 0; qi < 2; ++qi) {
-        uint32_t cx = x + 2 * qi;
-        if (cx >= width) break;
-        uint32_t inf = tq[qi], u_q = uq[qi];
-        if (!initial) {
-          uint32_t gamma = inf & 0xF0;
-          gamma &= gamma - 0x10;
-          uint32_t emax = max(max(uint32_t(BL(vp)), uint32_t(BL(vp + 1))), 2u) - 1;
-          u_q += gamma != 0 ? emax : 1;
-        }
-        if (u_q > mmsbp2) bad = true;
-        uint32_t v_n[4];
-        v_n[1] = v_n[3] = 0;
-        for (uint32_t bit = 0; bit < 4; ++bit) {
-          uint32_t sx = cx + (bit >> 1), sy = y + (bit & 1);
-          if (bit >= 2 && sx >= width) break;
-          v_n[bit] = 0;
-          uint32_t val = 0;
-          if (inf & (1u << (4 + bit))) {
-            uint32_t ms_val = ms.fetch();
-            uint32_t m_n = u_q - ((inf >> (12 + bit)) & 1);
-            ms.advance(m_n);
-            uint32_t vv = ms_val & ((1u << (m_n & 31)) - 1);
-            vv |= ((inf >> (8 + bit)) & 1) << (m_n & 31);
-            vv |= 1;
-            v_n[bit] = vv;
-            val = (ms_val << 31) | ((vv + 2) << ((p - 1) & 31));
-          }
-          if (sy < height) put(t, sx, sy, val, shift);
-        }
-        BL(vp) = uint8_t(bitlen(prev_v_n | v_n[1]));
-        prev_v_n = 0;
-        vp += 1;
-        if (cx + 1 >= width) break;
-        prev_v_n = v_n[3];
-      }
+  // decode.rs:54-110: validation and stream set-up
+  __device__ void start(const HtDecTask& task) {
+    t = task;
+    active = false;
+    bad = false;
+    if (!(t.flags & kTaskIncluded) || t.w == 0 || t.h == 0) return;
+    uint32_t num_passes = t.passes, missing_msbs = t.mmsb;
+    if (num_passes > 1 && t.len2 == 0) num_passes = 1;
+    if (num_passes > 3 || missing_msbs >= 30 || t.len1 < 2 || t.w > 64 || t.h > 64) {
+      report(t, kErrBlock);
+      return;
     }
-    // terminators: the quad after the row reads as insignificant
-    RHO(q) = 0;
-    RHO(q + 1) = 0;
-    BL(vp) = uint8_t(bitlen(prev_v_n));
+    if (missing_msbs == 29) num_passes = 1;
+    if (num_passes > 1) {
+      report(t, kErrCsUnsupported);
+      return;
+    }
+    const uint8_t* coded = t.seg;
+    const uint32_t lcup = t.len1;
+    const uint32_t scup = (uint32_t(coded[lcup - 1]) << 4) + uint32_t(coded[lcup - 2] & 0xF);
+    if (scup < 2 || scup > lcup || scup > 4079) {
+      report(t, kErrBlock);
+      return;
+    }
+    p = 30 - missing_msbs;
+    shift = 31 - t.kmax;
+    mmsbp2 = missing_msbs + 2;
+    mel.init(coded, lcup, scup);
+    vlc.init_vlc(coded, lcup, scup);
+    ms.init(coded, lcup - scup);
+    run = mel.get_run();
+    active = true;
   }
-  if (bad) {
-    report(t, kErrBlock);
-    fill_zero(t);
+
+  __device__ uint32_t value(uint32_t sm) const {
+    if (t.flags & kTaskSignMag) return sm;
+    int32_t mag = (int32_t)((sm & 0x7FFFFFFFu) >> shift);
+    return (uint32_t)((sm & 0x80000000u) ? -mag : mag);
   }
+
+  __device__ void row_begin() {
+    c_q = 0;
+    prev_v_n = 0;
+    vp = 0;
+    q = 0;
+  }
+
+  // One pair of quads (columns x..x+3, rows y, y+1): decode.rs:115-396.
+  // Samples go to stage[(r * kStageCols + col) * kStagePitch] (r = row - y,
+  // col = column - window base).
+  __device__ void pair(uint32_t x, uint32_t y, uint8_t* lrho, uint8_t* lbl, uint32_t* stage, uint32_t xbase) {
+    auto RHO = [&](int k) -> uint8_t& { return lrho[k * 64]; };
+    auto BL = [&](int k) -> uint8_t& { return lbl[k * 64]; };
+    const uint32_t width = t.w, height = t.h;
+    const bool initial = y == 0;
+    uint32_t a0 = 0, a1 = 0, a2 = 0;
+    if (!initial) {
+      a0 = uint32_t(RHO(q)) << 4;
+      a1 = uint32_t(RHO(q + 1)) << 4;
+      a2 = uint32_t(RHO(q + 2)) << 4;
+      c_q |= (a0 & 0xA0) << 2;
+      c_q |= (a1 & 0x20) << 4;
+    }
+    uint32_t vlc_val = vlc.fetch();
+    uint32_t t0 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
+    if (c_q == 0) {
+      run -= 2;
+      t0 = run == -1 ? t0 : 0;
+      if (run < 0) run = mel.get_run();
+    }
+    if (initial) {
+      c_q = ((t0 & 0x10) << 3) | ((t0 & 0xE0) << 2);
+    } else {
+      c_q = ((t0 & 0x40) << 2) | ((t0 & 0x80) << 1);
+      c_q |= a0 & 0x80;
+      c_q |= (a1 & 0xA0) << 2;
+      c_q |= (a2 & 0x20) << 4;
+    }
+    vlc_val = vlc.advance(t0 & 0x7);
+    uint32_t t1 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
+    if (c_q == 0 && x + 2 < width) {
+      run -= 2;
+      t1 = run == -1 ? t1 : 0;
+      if (run < 0) run = mel.get_run();
+    }
+    if (x + 2 >= width) t1 = 0;
+    if (initial) {
+      c_q = ((t1 & 0x10) << 3) | ((t1 & 0xE0) << 2);
+    } else {
+      c_q = ((t1 & 0x40) << 2) | ((t1 & 0x80) << 1);
+      c_q |= a1 & 0x80;
+    }
+    vlc_val = vlc.advance(t1 & 0x7);
+    uint32_t uvlc_mode = ((t0 & 0x8) << 3) | ((t1 & 0x8) << 4);
+    if (initial && uvlc_mode == 0xC0) {
+      run -= 2;
+      uvlc_mode += run == -1 ? 0x40 : 0;
+      if (run < 0) run = mel.get_run();
+    }
+    uint32_t e = initial ? c_uvlc0[uvlc_mode + (vlc_val & 0x3F)] : c_uvlc1[uvlc_mode + (vlc_val & 0x3F)];
+    vlc_val = vlc.advance(e & 0x7);
+    e >>= 3;
+    uint32_t len = e & 0xF;
+    uint32_t tmp = vlc_val & ((1u << len) - 1);
+    vlc.advance(len);
+    e >>= 4;
+    uint32_t len0 = e & 0x7;
+    e >>= 3;
+    uint32_t base = initial ? 1 : 0;
+    uint32_t uq[2] = {base + (e & 7) + (tmp & ~(0xFFu << len0)), base + (e >> 3) + (tmp >> len0)};
+    uint32_t tq[2] = {t0, t1};
+    // this row's patterns become the next row's context (reads above done)
+    RHO(q) = uint8_t(t0 >> 4);
+    RHO(q + 1) = uint8_t(t1 >> 4);
+
+    for (int qi = 0; qi < 2; ++qi) {
+      uint32_t cx = x + 2 * qi;
+      if (cx >= width) break;
+      uint32_t inf = tq[qi], u_q = uq[qi];
+      if (!initial) {
+        uint32_t gamma = inf & 0xF0;
+        gamma &= gamma - 0x10;
+        uint32_t emax = max(max(uint32_t(BL(vp)), uint32_t(BL(vp + 1))), 2u) - 1;
+        u_q += gamma != 0 ? emax : 1;
+      }
+      if (u_q > mmsbp2) bad = true;
+      uint32_t v_n[4];
+      v_n[1] = v_n[3] = 0;
+      for (uint32_t bit = 0; bit < 4; ++bit) {
+        uint32_t sx = cx + (bit >> 1), sy = y + (bit & 1);
+        if (bit >= 2 && sx >= width) break;
+        v_n[bit] = 0;
+        uint32_t val = 0;
+        if (inf & (1u << (4 + bit))) {
+          uint32_t ms_val = ms.fetch();
+          uint32_t m_n = u_q - ((inf >> (12 + bit)) & 1);
+          ms.advance(m_n);
+          uint32_t vv = ms_val & ((1u << (m_n & 31)) - 1);
+          vv |= ((inf >> (8 + bit)) & 1) << (m_n & 31);
+          vv |= 1;
+          v_n[bit] = vv;
+          val = (ms_val << 31) | ((vv + 2) << ((p - 1) & 31));
+        }
+        if (sy < height) stage[((bit & 1) * kStageCols + (sx - xbase)) * kStagePitch] = value(val);
+      }
+      BL(vp) = uint8_t(bitlen(prev_v_n | v_n[1]));
+      prev_v_n = 0;
+      vp += 1;
+      if (cx + 1 >= width) break;
+      prev_v_n = v_n[3];
+    }
+    q += 2;
+  }
+
+  // terminators: the quad after the row reads as insignificant
+  __device__ void row_end(uint8_t* lrho, uint8_t* lbl) {
+    lrho[q * 64] = 0;
+    lrho[(q + 1) * 64] = 0;
+    lbl[vp * 64] = uint8_t(bitlen(prev_v_n));
+  }
+};
+
+// Wave-wide store of the staged window (columns xbase..xbase+15 of rows y,
+// y+1) of every lane's block: each store instruction writes 16-column row
+// segments of two blocks, so lines are written whole instead of 4 bytes per
+// lane per instruction.
+__device__ __forceinline__ void flush_stage(const uint32_t* stage_all, uint64_t my_out, uint32_t my_stride,
+                                            uint32_t my_w, uint32_t my_h, uint32_t y, uint32_t xbase) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t r = (lane >> 4) & 1, col = lane & 15;
+#pragma unroll 4
+  for (uint32_t L0 = 0; L0 < 64; L0 += 2) {
+    uint32_t L = L0 + (lane >> 5);
+    uint64_t out = __shfl(my_out, (int)L);
+    uint32_t stride = __shfl(my_stride, (int)L), w = __shfl(my_w, (int)L), h = __shfl(my_h, (int)L);
+    uint32_t gx = xbase + col, gy = y + r;
+    if (out && gx < w && gy < h)
+      ((uint32_t*)out)[(uint64_t)gy * stride + gx] = stage_all[(r * kStageCols + col) * kStagePitch + L];
+  }
+}
+
+__device__ void fill_zero(const HtDecTask& t) {
+  for (uint32_t y = 0; y < t.h; ++y)
+    for (uint32_t x = 0; x < t.w; ++x) ((uint32_t*)t.out)[(uint64_t)y * t.stride + x] = 0;
 }
 
 __global__ __launch_bounds__(64) void ht_decode_kernel(const HtDecTask* __restrict__ tasks, uint64_t n) {
   __shared__ uint8_t lrho[kRowCap * 64];
   __shared__ uint8_t lbl[kRowCap * 64];
-  uint64_t i = blockIdx.x * (uint64_t)64 + threadIdx.x;
-  if (i >= n) return;
-  HtDecTask t = tasks[i];
+  __shared__ uint32_t stage[2 * kStageCols * kStagePitch];
+  const uint32_t lane = threadIdx.x;
+  uint64_t i = blockIdx.x * (uint64_t)64 + lane;
+  HtDecTask task{};
+  if (i < n) task = tasks[i];
   for (int k = 0; k < kRowCap; ++k) {
-    lrho[k * 64 + threadIdx.x] = 0;
-    lbl[k * 64 + threadIdx.x] = 0;
+    lrho[k * 64 + lane] = 0;
+    lbl[k * 64 + lane] = 0;
   }
-  decode_one(t, lrho + threadIdx.x, lbl + threadIdx.x);
+  BlockDec d;
+  d.start(task);
+  const uint32_t w = task.w, h = task.h;
+  const uint64_t out = i < n ? (uint64_t)task.out : 0;
+  uint32_t* my_stage = stage + lane;
+  for (uint32_t y = 0; y < 64; y += 2) {
+    if (__ballot(y < h) == 0) break;
+    if (d.active && y < h) d.row_begin();
+    for (uint32_t xb = 0; xb < 64; xb += kStageCols) {
+      if (__ballot(y < h && xb < w) == 0) break;
+      for (uint32_t x = xb; x < xb + kStageCols; x += 4) {
+        if (y < h && x < w) {
+          if (d.active) {
+            d.pair(x, y, lrho + lane, lbl + lane, my_stage, xb);
+          } else {
+            for (uint32_t s = 0; s < 8; ++s) {
+              uint32_t sx = x + (s >> 1), sy = y + (s & 1);
+              if (sx < w && sy < h) my_stage[((s & 1) * kStageCols + (sx - xb)) * kStagePitch] = 0;
+            }
+          }
+        }
+      }
+      __syncthreads();  // the staged window is complete (one wave per workgroup)
+      flush_stage(stage, out, task.stride, w, h, y, xb);
+      __syncthreads();  // reads done before the next window overwrites the stage
+    }
+    if (d.active && y < h) d.row_end(lrho + lane, lbl + lane);
+  }
+  // a magnitude bound violated mid-block: the whole block reads as zero
+  if (d.active && d.bad) {
+    report(task, kErrBlock);
+    __threadfence();  // the flushed rows land before they are overwritten
+    fill_zero(task);
+  }
 }
 
 // ---------------------------------------------------------------------------
