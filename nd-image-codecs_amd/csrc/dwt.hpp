@@ -5,7 +5,7 @@
 
 namespace ndic {
 
-enum DwtPass { kDwtInvH = 0, kDwtInvV = 1, kDwtFwdV = 2, kDwtFwdH = 3 };
+enum DwtPass { kDwtInvH = 0, kDwtInvV = 1, kDwtFwdV = 2, kDwtFwdH = 3, kDwtInvLevel = 4 };
 
 // One pass of one level over every plane of one chunk (all planes of a
 // chunk share their geometry).  Planes are plane_elems apart in each buffer.
@@ -17,8 +17,8 @@ struct DwtJob {
   uint64_t plane_elems;
   uint32_t w, h;          // level region
   uint32_t hl;            // rows of the LL half (inverse horizontal)
-  uint32_t pad;
-  uint64_t tiles;         // 64-wide tiles per line
+  uint32_t tiles_x;       // fused inverse level: 64x64 output tiles per tile row
+  uint64_t tiles;         // 64-wide tiles per line (fused inverse level: tiles per plane)
   uint64_t items, item_begin;
 };
 

@@ -46,6 +46,10 @@ void add_launch(std::vector<DwtJob>& all, std::vector<PassLaunch>& launches, std
 
 }  // namespace
 
+// Synthesis, one fused launch per level (dwt_inv_level).  A level reads its
+// LL rows from the previous level's reconstruction and writes the next one,
+// so reconstructions alternate between O and T (the coarsest level reads LL
+// from A); result names the buffer the finest level wrote.
 void run_dwt_inverse(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t stream) {
   uint32_t max_l = 0;
   for (auto& c : chunks) {
@@ -55,7 +59,7 @@ void run_dwt_inverse(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t st
   std::vector<DwtJob> all;
   std::vector<PassLaunch> launches;
   for (uint32_t l = max_l; l-- > 0;) {
-    std::vector<DwtJob> hj, vj;
+    std::vector<DwtJob> jobs;
     for (auto& c : chunks) {
       if (c.levels <= l || c.nplanes == 0) continue;
       uint64_t w, h;
@@ -68,25 +72,15 @@ void run_dwt_inverse(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t st
       j.h = (uint32_t)h;
       j.hl = (uint32_t)((h + 1) / 2);
       j.src = c.A;
-      j.src_ll = c.result;  // A for the coarsest level, else the previous reconstruction
-      j.dst = c.T;
-      j.tiles = ceil_div((w + 1) / 2, 64);
-      j.items = c.nplanes * h * j.tiles;
-      hj.push_back(j);
-      DwtJob v{};
-      v.stride = c.stride ? c.stride : c.W;
-      v.plane_elems = c.plane_elems ? c.plane_elems : c.W * c.H;
-      v.w = (uint32_t)w;
-      v.h = (uint32_t)h;
-      v.src = c.T;
-      v.dst = c.O;
-      v.tiles = ceil_div(w, 64);
-      v.items = c.nplanes * ((h + 1) / 2) * v.tiles;
-      vj.push_back(v);
-      c.result = c.O;
+      j.src_ll = c.result;
+      j.dst = c.result == c.O ? c.T : c.O;
+      j.tiles_x = (uint32_t)ceil_div(w, 64);
+      j.tiles = j.tiles_x * ceil_div(h, 64);
+      j.items = c.nplanes * j.tiles;
+      jobs.push_back(j);
+      c.result = j.dst;
     }
-    add_launch(all, launches, hj, kDwtInvH);
-    add_launch(all, launches, vj, kDwtInvV);
+    add_launch(all, launches, jobs, kDwtInvLevel);
   }
   submit(ctx, all, launches, stream);
 }

@@ -242,16 +242,35 @@ void run_lift(Context& ctx, std::vector<LiftChunk>& chunks, bool fwd, hipStream_
           glob_max = std::max(glob_max, n);
         }
       }
-      for (int path = 0; path < 2; ++path) {
-        std::vector<LiftJob>& jobs = path == 0 ? lds_jobs : glob_jobs;
+      // LDS jobs go out in segment-length classes so that short segments
+      // are not launched with the LDS (and occupancy) of the longest one
+      (void)lds_max;
+      const uint32_t classes[] = {4, 8, 16, 32, 64, 128};
+      uint32_t lo = 0;
+      for (uint32_t cap : classes) {
+        std::vector<LiftJob> jobs;
+        for (auto& j : lds_jobs) {
+          uint32_t n = (uint32_t)std::min<uint64_t>(j.group, 0xFFFFFFFFu);
+          if (n > lo && n <= cap) jobs.push_back(j);
+        }
+        lo = cap;
         if (jobs.empty()) continue;
         uint64_t items = 0;
         for (auto& j : jobs) {
           j.item_begin = items;
           items += j.items;
         }
-        launches.push_back({all.size(), jobs.size(), items, wide == 1, path == 0 ? lds_max : glob_max});
+        launches.push_back({all.size(), jobs.size(), items, wide == 1, cap});
         all.insert(all.end(), jobs.begin(), jobs.end());
+      }
+      if (!glob_jobs.empty()) {
+        uint64_t items = 0;
+        for (auto& j : glob_jobs) {
+          j.item_begin = items;
+          items += j.items;
+        }
+        launches.push_back({all.size(), glob_jobs.size(), items, wide == 1, glob_max});
+        all.insert(all.end(), glob_jobs.begin(), glob_jobs.end());
       }
     }
   }
