@@ -197,6 +197,116 @@ __global__ __launch_bounds__(64) void lift_step_kernel(const LiftJob* __restrict
   }
 }
 
+// LDS tile kernel: one 256-thread workgroup per 64-column tile of one group
+// segment (n <= the LDS limit).  The segment is loaded once with coalesced
+// 256-byte rows, every level runs with the four waves spread over the
+// segment's sample pairs (predict / update phases separated by barriers,
+// ping-pong between two LDS buffers), and the result is stored once (narrowed
+// to the dtype on the last inverse step).  Same arithmetic as the per-column
+// reference loops above.
+template <class T>
+__global__ __launch_bounds__(256) void lift_tile_kernel(const LiftJob* __restrict__ jobs, uint32_t njobs, bool fwd,
+                                                        uint32_t nmax) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* X = (T*)lds_raw;
+  T* Y = X + (size_t)nmax * 64;
+  uint64_t item = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  const LiftJob& job = jobs[find_job(jobs, njobs, item)];
+  uint64_t local = item - job.item_begin;
+  if (local >= job.items) return;
+  const uint32_t tid = threadIdx.x, col = tid & 63, grp = tid >> 6;
+  uint64_t t = local % job.tiles, rest = local / job.tiles;
+  uint64_t sg = rest % job.segs, o = rest / job.segs;
+  const uint64_t j = t * 64 + col;
+  const bool valid = j < job.inner;
+  const uint64_t g0 = sg * job.group;
+  const uint32_t n = (uint32_t)(job.len - g0 < job.group ? job.len - g0 : job.group);
+  const uint64_t elem0 = o * job.len * job.inner + g0 * job.inner + j;
+  T* plane = (T*)job.plane;
+  const uint64_t inner = job.inner;
+  // load
+  for (uint32_t k = grp; k < n; k += 4) {
+    T v = 0;
+    if (valid) {
+      if (fwd && job.src) v = (T)load_dtype(job.src + (elem0 + k * inner) * job.dsize, job.dtype);
+      else v = plane[elem0 + k * inner];
+    }
+    X[k * 64 + col] = v;
+  }
+  __syncthreads();
+  if (job.kind != 3 && n >= 2) {
+    if (job.kind == 0) {
+      if (fwd) {
+        for (uint32_t k = grp; k < n; k += 4) Y[k * 64 + col] = k == 0 ? X[col] : X[k * 64 + col] - X[(k - 1) * 64 + col];
+        __syncthreads();
+        for (uint32_t k = grp; k < n; k += 4) X[k * 64 + col] = Y[k * 64 + col];
+      } else if (grp == 0) {
+        for (uint32_t k = 1; k < n; ++k) X[k * 64 + col] = wadd(X[k * 64 + col], X[(k - 1) * 64 + col]);
+      }
+      __syncthreads();
+    } else {
+      const uint32_t bc = band_count(n, job.levels);
+      for (uint32_t step = 0; step < bc; ++step) {
+        const uint32_t lv = fwd ? step : bc - 1 - step;
+        const uint32_t len = (uint32_t)((n + (1u << lv) - 1) >> lv);
+        const uint32_t ns = (len + 1) / 2, nd = len / 2;
+        auto x = [&](uint32_t k) -> T& { return X[k * 64 + col]; };
+        auto y = [&](uint32_t k) -> T& { return Y[k * 64 + col]; };
+        if (job.kind == 1) {  // haar (kernel.rs:167-200)
+          for (uint32_t i = grp; i < nd; i += 4) {
+            if (fwd) {
+              T d = x(2 * i + 1) - x(2 * i);
+              y(ns + i) = d;
+              y(i) = x(2 * i) + (d >> 1);
+            } else {
+              T d = x(ns + i);
+              T x0 = wsub(x(i), (T)(d >> 1));
+              y(2 * i) = x0;
+              y(2 * i + 1) = wadd(x0, d);
+            }
+          }
+          if ((len & 1) && grp == 0) {
+            if (fwd) y(ns - 1) = x(len - 1);
+            else y(len - 1) = x(ns - 1);
+          }
+        } else if (fwd) {  // 5/3 analysis (kernel.rs:214-240)
+          for (uint32_t i = grp; i < nd; i += 4) {
+            uint32_t r = 2 * i + 2 < len ? 2 * i + 2 : 2 * len - 2 - (2 * i + 2);
+            y(ns + i) = x(2 * i + 1) - ((x(2 * i) + x(r) + 1) >> 1);
+          }
+          __syncthreads();
+          for (uint32_t i = grp; i < ns; i += 4) {
+            uint32_t il = i == 0 ? 0 : i - 1, ir = i < nd - 1 ? i : nd - 1;
+            y(i) = x(2 * i) + ((y(ns + il) + y(ns + ir) + 2) >> 2);
+          }
+        } else {  // 5/3 synthesis (kernel.rs:243-272), wrapping
+          for (uint32_t i = grp; i < ns; i += 4) {
+            uint32_t il = i == 0 ? 0 : i - 1, ir = i < nd - 1 ? i : nd - 1;
+            y(2 * i) = wsub(x(i), (T)(wadd(wadd(x(ns + il), x(ns + ir)), (T)2) >> 2));
+          }
+          __syncthreads();
+          for (uint32_t i = grp; i < nd; i += 4) {
+            uint32_t r = 2 * i + 2 < len ? 2 * i + 2 : 2 * len - 2 - (2 * i + 2);
+            y(2 * i + 1) = wadd(x(ns + i), (T)(wadd(wadd(y(2 * i), y(r)), (T)1) >> 1));
+          }
+        }
+        __syncthreads();
+        for (uint32_t k = grp; k < len; k += 4) x(k) = y(k);
+        __syncthreads();
+      }
+    }
+  }
+  // store
+  bool ok = true;
+  for (uint32_t k = grp; k < n; k += 4) {
+    if (!valid) break;
+    T v = X[k * 64 + col];
+    if (!fwd && job.dst) ok &= store_dtype<T>(job.dst + (elem0 + k * inner) * job.dsize, job.dtype, v);
+    else plane[elem0 + k * inner] = v;
+  }
+  if (!ok) atomicOr(job.flag, 1);
+}
+
 // Grid-stride widen (dtype -> plane) with per-chunk min/max for the budget
 // check, or narrow (plane -> dtype) with a range flag.
 template <class T>
@@ -250,7 +360,12 @@ void launch_lift_step(const LiftJob* d_jobs, uint32_t njobs, uint64_t items, boo
   uint32_t gy = (uint32_t)ceil_div(items, gx);
   bool lds = max_n <= lift_lds_limit(wide);
   size_t shmem = lds ? (size_t)2 * max_n * 64 * (wide ? 8 : 4) : 0;
-  if (wide) {
+  if (lds) {
+    if (wide)
+      hipLaunchKernelGGL((lift_tile_kernel<int64_t>), dim3(gx, gy), dim3(256), shmem, stream, d_jobs, njobs, fwd, max_n);
+    else
+      hipLaunchKernelGGL((lift_tile_kernel<int32_t>), dim3(gx, gy), dim3(256), shmem, stream, d_jobs, njobs, fwd, max_n);
+  } else if (wide) {
     if (lds) hipLaunchKernelGGL((lift_step_kernel<int64_t, true>), dim3(gx, gy), dim3(64), shmem, stream, d_jobs, njobs, fwd, max_n);
     else hipLaunchKernelGGL((lift_step_kernel<int64_t, false>), dim3(gx, gy), dim3(64), 0, stream, d_jobs, njobs, fwd, max_n);
   } else {
