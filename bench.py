@@ -155,6 +155,102 @@ def serve_bench(ndic, dev, d_raw, d_enc, ddescs, specs, args):
     return out
 
 
+def sine_field(rng, shape, drift_axis=None):
+    """Sum of 6 plane waves (random directions, <= 0.1 cycles/voxel, unit
+    amplitude) plus N(0, 1e-3), optionally with a linear drift along an axis."""
+    grids = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+    a = np.zeros(shape, np.float64)
+    for _ in range(6):
+        f = rng.uniform(-0.1, 0.1, len(shape))
+        a += np.sin(2 * np.pi * sum(fi * g for fi, g in zip(f, grids)) + rng.uniform(0, 2 * np.pi))
+    if drift_axis is not None:
+        a += 0.05 * grids[drift_axis]
+    return a + rng.normal(0, 1e-3, shape)
+
+
+def zfp_bench(ndic, dev, args):
+    """ZFP at the two field shapes (f32 64x256x256, f64 4x64x128x128), fixed
+    rate 8 / 16 and reversible: device-resident encode and decode rates, the
+    oracle on one core beside them, and a batch of 2,048 independent bricks
+    of 16^3..64^3 samples."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    rng = np.random.default_rng(4)
+    fields = [("f32 64x256x256", sine_field(rng, (64, 256, 256)).astype(np.float32)),
+              ("f64 4x64x128x128", sine_field(rng, (4, 64, 128, 128), drift_axis=0))]
+    modes = [("rate8", ndic.zfp_config("fixed_rate", rate=8.0)), ("rate16", ndic.zfp_config("fixed_rate", rate=16.0)),
+             ("reversible", ndic.zfp_config("reversible"))]
+    out = {}
+    for name, fld in fields:
+        d_in = dev.DeviceBuffer.from_bytes(fld.tobytes())
+        cap = 3 * fld.nbytes + (1 << 20)
+        d_enc = dev.DeviceBuffer(cap, zero=True)
+        d_out = dev.DeviceBuffer(fld.nbytes)
+        for mname, cfg in modes:
+            desc = [ndic.make_zfp_desc(fld.shape, fld.dtype.name, cfg, 0, fld.nbytes)]
+            arr = ndic.zfp_encode_batch(d_in.ptr, d_enc.ptr, cap, desc)  # warm-up
+            reps = 5
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                arr = ndic.zfp_encode_batch(d_in.ptr, d_enc.ptr, cap, desc)
+            enc_s = (time.perf_counter() - t0) / reps
+            clen = arr[0].out_len
+            ddesc = [ndic.make_zfp_desc(fld.shape, fld.dtype.name, cfg, arr[0].out_offset, clen, 0)]
+            ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, ddesc)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, ddesc)
+            dec_s = (time.perf_counter() - t0) / reps
+            exact = None
+            if mname == "reversible":
+                exact = dev.count_mismatch(d_out.ptr, d_in.ptr, fld.nbytes) == 0
+            out[f"{name} {mname}"] = {
+                "encode_GBps": round(fld.nbytes / enc_s / 1e9, 2), "decode_GBps": round(fld.nbytes / dec_s / 1e9, 2),
+                "decode_ms": round(dec_s * 1e3, 3), "compressed_bytes": clen,
+                "decode_algorithmic_GBps": round((fld.nbytes + clen) / dec_s / 1e9, 2),
+                **({"lossless_exact": exact} if exact is not None else {})}
+            log(f"zfp {name} {mname}: {out[f'{name} {mname}']}")
+        del d_in, d_enc, d_out
+    # 2,048 independent bricks (chunks) of 16^3..64^3 f32 at rate 8
+    sides = rng.integers(16, 65, size=(2048, 3))
+    sizes = [int(np.prod(sd)) * 4 for sd in sides]
+    offs = np.concatenate([[0], np.cumsum([(n + 255) // 256 * 256 for n in sizes])])
+    base = fields[0][1].reshape(-1)
+    raw = np.zeros(int(offs[-1]) // 4, np.float32)
+    for i, n in enumerate(sizes):
+        st = int(rng.integers(0, base.size - n // 4))
+        raw[int(offs[i]) // 4:int(offs[i]) // 4 + n // 4] = base[st:st + n // 4]
+    d_raw = dev.DeviceBuffer.from_bytes(raw.tobytes())
+    cfg = ndic.zfp_config("fixed_rate", rate=8.0)
+    descs = [ndic.make_zfp_desc(tuple(int(v) for v in sd), "float32", cfg, int(offs[i]), sizes[i])
+             for i, sd in enumerate(sides)]
+    cap = int(offs[-1]) + 2048 * 4096
+    d_enc = dev.DeviceBuffer(cap, zero=True)
+    arr = ndic.zfp_encode_batch(d_raw.ptr, d_enc.ptr, cap, descs)
+    ddescs = [ndic.make_zfp_desc(tuple(int(v) for v in sd), "float32", cfg, arr[i].out_offset, arr[i].out_len,
+                                 int(offs[i])) for i, sd in enumerate(sides)]
+    d_out = dev.DeviceBuffer(int(offs[-1]))
+    ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, ddescs)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, ddescs)
+    bs = (time.perf_counter() - t0) / 5
+    out["2048 bricks 16^3..64^3 f32 rate8 decode"] = {"GBps": round(sum(sizes) / bs / 1e9, 2),
+                                                      "ms": round(bs * 1e3, 3)}
+    log(f"zfp bricks: {out['2048 bricks 16^3..64^3 f32 rate8 decode']}")
+    # the oracle on one core, f32 field rate 8 (bounded sample: the first 8 z-slices)
+    import oracle
+    sample = fields[0][1][:8]
+    t0 = time.perf_counter()
+    s_ = oracle.zfp_encode_chunk(sample.tobytes(), sample.shape, "f32", {"mode": "fixed_rate", "rate": 8.0})
+    te = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.zfp_decode_chunk(s_, sample.shape, "f32", {"mode": "fixed_rate", "rate": 8.0})
+    td = time.perf_counter() - t0
+    out["cpu oracle f32 rate8 (8x256x256, 1 core)"] = {"encode_GBps": round(sample.nbytes / te / 1e9, 4),
+                                                        "decode_GBps": round(sample.nbytes / td / 1e9, 4)}
+    return out
+
+
 def traffic_of(kernel: str, args, avg_ms: float):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this same workload (scaled by a
@@ -209,6 +305,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="seconds of Poisson arrivals per serving rate (0: no serving run)")
     ap.add_argument("--serve-rates", default="500,2000,5000")
+    ap.add_argument("--no-zfp", dest="zfp", action="store_false", help="skip the ZFP section")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first run one device copy kernel of this many bytes (traffic-counter scale)")
     args = ap.parse_args()
@@ -277,6 +374,7 @@ def main():
     dom = max(range(5), key=lambda i: ms[i])
     achieved = by[dom] / ms[dom] / 1e6 if ms[dom] > 0 else 0.0
     log("stages:", json.dumps(stages))
+    zfp = zfp_bench(ndic, dev, args) if args.zfp else None
     serving = None
     if args.serve_seconds > 0 and args.workload == "c2":
         serving = serve_bench(ndic, dev, d_raw, d_enc, ddescs, specs, args)
@@ -324,6 +422,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "serving": serving,
+        "zfp": zfp,
     }
     print(json.dumps(line), flush=True)
 

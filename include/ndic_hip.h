@@ -142,6 +142,22 @@ ndic_status ndic_lift_ht_decode_batch(const uint8_t* d_in, uint8_t* d_out, ndic_
 ndic_status ndic_lift_ht_encode_batch(const uint8_t* d_in, uint8_t* d_out, size_t capacity, ndic_chunk_desc* descs,
                                       size_t n, uint32_t flags, void* stream, ndic_error* err);
 
+/* One plane codestream (or a byte-range prefix of it, e.g. up to the ndht
+ * index's prefix[r]) of a device-resident batch, decoded to resolution
+ * max_res: the serving form of ndic_htj2k_decode_plane.  width/height/levels
+ * are the plane's (known from the chunk it belongs to); out_width/out_height
+ * are filled with the reduced size, whose int32 samples land at out_offset. */
+typedef struct {
+  uint64_t in_offset, in_len, out_offset;
+  uint32_t width, height, levels;
+  int32_t max_res;      /* < 0: all resolutions */
+  int32_t prefix;       /* the bytes may stop after the last needed packet */
+  uint32_t out_width, out_height;
+  int32_t status;
+} ndic_plane_desc;
+ndic_status ndic_htj2k_decode_plane_batch(const uint8_t* d_in, uint8_t* d_out, ndic_plane_desc* descs, size_t n,
+                                          void* stream, ndic_error* err);
+
 /* ---- nd_zfp / zfp (LLNL ZFP streams, codec version 5) ------------------ */
 
 /* ZfpMode (crates/ndic-zfp/src/lib.rs:77-86) as carried by NdZfpConfig
@@ -202,6 +218,23 @@ ndic_status ndic_zfp_encode_batch(const uint8_t* d_in, uint8_t* d_out, size_t ca
                                   size_t n, void* stream, ndic_error* err);
 ndic_status ndic_zfp_decode_batch(const uint8_t* d_in, uint8_t* d_out, ndic_zfp_desc* descs, size_t n,
                                   void* stream, ndic_error* err);
+
+/* One 4^d brick of a device-resident fixed-rate stream (the serving form of
+ * NdZfpBrickDecoder::decode_brick, chunk.rs:484): shape is the effective
+ * field shape, brick the per-axis brick coordinates; the clipped brick's
+ * samples land at out_offset in C order. */
+typedef struct {
+  uint64_t in_offset;  /* 8-byte aligned */
+  uint64_t in_len, out_offset;
+  uint32_t ndim;
+  uint32_t shape[4];
+  uint32_t brick[4];
+  int32_t dtype;
+  double rate;
+  int32_t status;
+} ndic_zfp_brick_desc;
+ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp_brick_desc* descs, size_t n,
+                                   void* stream, ndic_error* err);
 
 #ifdef __cplusplus
 }

@@ -410,3 +410,43 @@ ndic_status ndic_dev_ht_decode_block(const uint8_t* coded, size_t len, uint32_t*
 }
 
 }  // extern "C"
+
+extern "C" ndic_status ndic_htj2k_decode_plane_batch(const uint8_t* d_in, uint8_t* d_out, ndic_plane_desc* descs,
+                                                     size_t n, void* stream, ndic_error* err) {
+  return ndic::guard_call(err, [&] {
+    using namespace ndic;
+    if (n && !descs) fail(NDIC_INVALID_ARGUMENT, "null descriptors");
+    Context& ctx = Context::get();
+    std::lock_guard<std::recursive_mutex> lock(ctx.mu);
+    hipStream_t s = stream ? (hipStream_t)stream : ctx.stream;
+    std::vector<DecodeItem> items(n);
+    for (size_t i = 0; i < n; ++i) {
+      ndic_plane_desc& d = descs[i];
+      DecodeItem& it = items[i];
+      if (d.width == 0 || d.height == 0 || d.levels > 32) fail(NDIC_INVALID_ARGUMENT, "plane descriptor out of range");
+      if (d.out_offset % 4) fail(NDIC_INVALID_ARGUMENT, "plane batch output offsets must be 4-byte aligned");
+      it.d_in = d_in + d.in_offset;
+      it.len = d.in_len;
+      it.shape = {d.height, d.width};
+      it.L = d.levels;
+      it.bare = true;
+      it.prefix = d.prefix != 0;
+      it.max_res = d.max_res < 0 ? -1 : d.max_res;
+      it.d_out = d_out + d.out_offset;
+    }
+    decode_items(ctx, items, s, window_elems());
+    NDIC_HIP(hipStreamSynchronize(s));
+    int first = -1;
+    for (size_t i = 0; i < n; ++i) {
+      descs[i].status = items[i].status;
+      descs[i].out_width = (uint32_t)items[i].out_w;
+      descs[i].out_height = (uint32_t)items[i].out_h;
+      if (items[i].status != NDIC_OK && first < 0) first = (int)i;
+    }
+    if (first >= 0) {
+      std::ostringstream m;
+      m << "plane " << first << ": " << items[first].message;
+      fail(items[first].status, m.str(), items[first].offset);
+    }
+  });
+}

@@ -274,7 +274,14 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   for (size_t i = 0; i < n; ++i) {
     DecodeItem& it = *items[i];
     uint64_t total = (uint64_t)dc[i].P * dc[i].W * dc[i].H;
-    if (it.bare) continue;  // plane decodes hand back the int32 region (copied by the caller)
+    if (it.bare) {
+      // plane decodes hand back the int32 region: into the caller's buffer
+      // when one is given (the window's planes are reused afterwards)
+      if (it.d_out && it.out_w && it.out_h)
+        NDIC_HIP(hipMemcpy2DAsync(it.d_out, it.out_w * 4, res[i], (size_t)dc[i].W * 4, it.out_w * 4, it.out_h,
+                                  hipMemcpyDeviceToDevice, s));
+      continue;
+    }
     if (it.lift) {
       LiftChunk lc;
       lc.plane = res[i];

@@ -1,0 +1,327 @@
+"""Serving loop over an HBM-resident store of compressed chunks.
+
+Requests arrive over time (thumbnail, full-chunk decode, ZFP brick, encode).
+Each pass of the loop admits every request that has arrived, as far as the
+output memory budget allows, and runs them as device batches: thumbnails
+through ``ndic_htj2k_decode_plane_batch`` (plane 0 cut at the ndht index's
+``prefix[r]``, resolutions 0..r), full decodes through
+``ndic_lift_ht_decode_batch``, bricks through ``ndic_zfp_decode_bricks`` and
+encodes through ``ndic_lift_ht_encode_batch``.  Requests that arrive while a
+pass is on the GPU join the next pass (continuous batching); latency is the
+time from a request's arrival to the end of the pass that served it.
+
+The thumbnail byte ranges follow ``RangeIndex::thumbnail``
+(crates/ndic-codestream/src/range.rs:128-200): plane 0's codestream up to
+the end of its resolution-r packets, read from the chunk's own index.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import importlib
+import struct
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+_pkg = importlib.import_module("nd-image-codecs_amd")
+dev = importlib.import_module("nd-image-codecs_amd.device")
+
+THUMB, DECODE, BRICK, ENCODE = "thumbnail", "decode", "brick", "encode"
+
+
+class PlaneDesc(C.Structure):
+    _fields_ = [("in_offset", C.c_uint64), ("in_len", C.c_uint64), ("out_offset", C.c_uint64),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("levels", C.c_uint32), ("max_res", C.c_int32),
+                ("prefix", C.c_int32), ("out_width", C.c_uint32), ("out_height", C.c_uint32), ("status", C.c_int32)]
+
+
+class BrickDesc(C.Structure):
+    _fields_ = [("in_offset", C.c_uint64), ("in_len", C.c_uint64), ("out_offset", C.c_uint64), ("ndim", C.c_uint32),
+                ("shape", C.c_uint32 * 4), ("brick", C.c_uint32 * 4), ("dtype", C.c_int32), ("rate", C.c_double),
+                ("status", C.c_int32)]
+
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    lib = _pkg.lib()
+    if not _bound:
+        P, S, E = C.c_void_p, C.c_size_t, C.POINTER(_pkg.Error)
+        lib.ndic_htj2k_decode_plane_batch.argtypes = [P, P, P, S, P, E]
+        lib.ndic_htj2k_decode_plane_batch.restype = C.c_int
+        lib.ndic_zfp_decode_bricks.argtypes = [P, P, P, S, P, E]
+        lib.ndic_zfp_decode_bricks.restype = C.c_int
+        _bound = True
+    return lib
+
+
+def reduced(n: int, levels: int) -> int:
+    for _ in range(levels):
+        n = (n + 1) // 2
+    return n
+
+
+@dataclass
+class HtChunk:
+    """A compressed nd-lift-ht chunk in the store."""
+    offset: int
+    length: int
+    shape: tuple
+    dtype: str
+    steps: list
+    levels: int
+    raw_offset: int = -1           # where the original samples live (for checks), if kept
+    plane0: tuple = (0, 0, [])     # (codestream offset in the chunk, length, prefix[r])
+
+
+@dataclass
+class ZfpField:
+    offset: int
+    length: int
+    shape: tuple
+    dtype: str
+    rate: float
+
+
+@dataclass
+class Request:
+    kind: str
+    arrival: float
+    key: int = 0
+    brick: tuple = ()
+    res: int = 3
+    out_bytes: int = 0
+    done: float = -1.0
+    out_offset: int = -1
+
+
+def parse_plane0(head: bytes):
+    """Plane 0's entry of an ndht coefficient-plane index
+    (crates/ndic-codestream/src/container.rs:14-30): (offset, len, prefix[])."""
+    if head[:4] != b"ndht":
+        raise ValueError("not an ndht chunk")
+    flags, levels, ndim = head[5], head[6], head[7]
+    if not flags & 1:
+        raise ValueError("ndht chunk has no coefficient-plane index")
+    pos = 8 + 4 * ndim
+    off, ln = struct.unpack_from("<QI", head, pos)
+    pre = list(struct.unpack_from("<%dI" % (levels + 1), head, pos + 12))
+    return off, ln, pre
+
+
+class Store:
+    """Compressed chunks kept in device memory for the server."""
+
+    def __init__(self):
+        self.ht: List[HtChunk] = []
+        self.zfp: List[ZfpField] = []
+        self.ht_arena: Optional[dev.DeviceBuffer] = None
+        self.zfp_arena: Optional[dev.DeviceBuffer] = None
+        self.raw_arena: Optional[dev.DeviceBuffer] = None
+
+    def add_htj2k_arena(self, arena, descs, specs, raw_arena=None, raw_offsets=None):
+        """Adopt an arena written by ndic_lift_ht_encode_batch (descs[i].out_offset/out_len);
+        specs[i] = (shape, dtype, steps, levels)."""
+        self.ht_arena = arena
+        self.raw_arena = raw_arena
+        for i, d in enumerate(descs):
+            shape, dt, steps, lv = specs[i][:4]
+            need = 8 + 4 * len(shape) + 12 + 4 * (lv + 1)
+            head = arena.download(d.out_offset, min(d.out_len, need))
+            self.ht.append(HtChunk(d.out_offset, d.out_len, tuple(shape), dt, steps, lv,
+                                   raw_offsets[i] if raw_offsets is not None else -1, parse_plane0(head)))
+
+    def add_zfp_arena(self, arena, fields: Sequence[ZfpField]):
+        self.zfp_arena = arena
+        self.zfp.extend(fields)
+
+
+@dataclass
+class Stats:
+    latencies_ms: Dict[str, List[float]] = field(default_factory=dict)
+    passes: int = 0
+    bytes_out: int = 0
+    wall_s: float = 0.0
+
+    def summary(self):
+        allv = sorted(v for vs in self.latencies_ms.values() for v in vs)
+
+        def pct(v, q):
+            if not v:
+                return None
+            return round(float(np.percentile(np.array(v), q)), 3)
+
+        out = {"requests": len(allv), "passes": self.passes, "p50_ms": pct(allv, 50), "p99_ms": pct(allv, 99),
+               "GBps": round(self.bytes_out / self.wall_s / 1e9, 3) if self.wall_s else None,
+               "by_kind": {k: {"n": len(v), "p50_ms": pct(sorted(v), 50), "p99_ms": pct(sorted(v), 99)}
+                           for k, v in self.latencies_ms.items()}}
+        return out
+
+
+class Server:
+    """Continuous-batching request loop on one GPU."""
+
+    def __init__(self, store: Store, out_budget: int = 4 << 30, encode_sources=None):
+        self.store = store
+        self.out_budget = int(out_budget)
+        self.out = dev.DeviceBuffer(self.out_budget)
+        # encode requests take samples from these (device buffer, shape, dtype, steps, levels) sources
+        self.encode_sources = encode_sources or []
+        self.enc_out: Optional[dev.DeviceBuffer] = None
+
+    # -- request sizing
+    def out_bytes(self, r: Request) -> int:
+        s = self.store
+        if r.kind == THUMB:
+            c = s.ht[r.key]
+            h, w = c.shape[-2], c.shape[-1]
+            rl = c.levels - min(r.res, c.levels)
+            return reduced(w, rl) * reduced(h, rl) * 4
+        if r.kind == DECODE:
+            c = s.ht[r.key]
+            return int(np.prod(c.shape)) * np.dtype(c.dtype).itemsize
+        if r.kind == BRICK:
+            f = s.zfp[r.key]
+            n = 1
+            for d, b in zip(f.shape, r.brick):
+                n *= min(4, d - 4 * b)
+            return n * np.dtype(f.dtype).itemsize
+        return 0
+
+    def _run_pass(self, reqs: List[Request]):
+        s = self.store
+        lib = _lib()
+        cursor = 0
+        for r in reqs:
+            r.out_offset = cursor
+            cursor += (self.out_bytes(r) + 255) // 256 * 256
+        thumbs = [r for r in reqs if r.kind == THUMB]
+        decs = [r for r in reqs if r.kind == DECODE]
+        bricks = [r for r in reqs if r.kind == BRICK]
+        encs = [r for r in reqs if r.kind == ENCODE]
+        if thumbs:
+            arr = (PlaneDesc * len(thumbs))()
+            for i, r in enumerate(thumbs):
+                c = s.ht[r.key]
+                off, ln, pre = c.plane0
+                res = min(r.res, c.levels)
+                d = arr[i]
+                d.in_offset = c.offset + off
+                d.in_len = pre[res] if pre else ln
+                d.out_offset = r.out_offset
+                d.width, d.height, d.levels = c.shape[-1], c.shape[-2], c.levels
+                d.max_res, d.prefix = res, 1
+            err = _pkg.Error()
+            _pkg._check(lib.ndic_htj2k_decode_plane_batch(C.c_void_p(s.ht_arena.ptr), C.c_void_p(self.out.ptr), arr,
+                                                          len(thumbs), None, C.byref(err)), err)
+        if decs:
+            descs = [_pkg.make_desc(s.ht[r.key].shape, s.ht[r.key].dtype, s.ht[r.key].steps, s.ht[r.key].levels,
+                                    s.ht[r.key].offset, s.ht[r.key].length, r.out_offset) for r in decs]
+            _pkg.decode_batch(s.ht_arena.ptr, self.out.ptr, descs)
+        if bricks:
+            arr = (BrickDesc * len(bricks))()
+            for i, r in enumerate(bricks):
+                f = s.zfp[r.key]
+                d = arr[i]
+                d.in_offset, d.in_len, d.out_offset = f.offset, f.length, r.out_offset
+                d.ndim = len(f.shape)
+                for a in range(len(f.shape)):
+                    d.shape[a] = f.shape[a]
+                    d.brick[a] = r.brick[a]
+                d.dtype = _pkg._zfp_dtype(f.dtype)
+                d.rate = f.rate
+            err = _pkg.Error()
+            _pkg._check(lib.ndic_zfp_decode_bricks(C.c_void_p(s.zfp_arena.ptr), C.c_void_p(self.out.ptr), arr,
+                                                   len(bricks), None, C.byref(err)), err)
+        if encs and self.encode_sources:
+            descs, total = [], 0
+            for r in encs:
+                buf, shape, dt, steps, lv = self.encode_sources[r.key % len(self.encode_sources)]
+                descs.append((buf, _pkg.make_desc(shape, dt, steps, lv, 0, int(np.prod(shape)) * np.dtype(dt).itemsize,
+                                                  0)))
+                total += int(np.prod(shape)) * np.dtype(dt).itemsize
+            cap = 2 * total + (64 << 20)
+            if self.enc_out is None or self.enc_out.nbytes < cap:
+                self.enc_out = dev.DeviceBuffer(cap)
+            # sources live in separate buffers: one batch per source buffer
+            by_src: Dict[int, list] = {}
+            for buf, d in descs:
+                by_src.setdefault(buf.ptr, []).append(d)
+            for ptr, ds in by_src.items():
+                _pkg.encode_batch(ptr, self.enc_out.ptr, self.enc_out.nbytes, ds)
+        dev.synchronize()
+
+    def run(self, requests: List[Request], verify=None) -> Stats:
+        """Serves requests (sorted by arrival, seconds from start) in real time."""
+        stats = Stats()
+        pending: List[Request] = []
+        i, n = 0, len(requests)
+        t0 = time.perf_counter()
+        done = 0
+        while done < n:
+            now = time.perf_counter() - t0
+            while i < n and requests[i].arrival <= now:
+                pending.append(requests[i])
+                i += 1
+            if not pending:
+                wait = requests[i].arrival - now
+                if wait > 0:
+                    time.sleep(min(wait, 0.001))
+                continue
+            # admission by output memory
+            batch, used = [], 0
+            for r in pending:
+                b = (self.out_bytes(r) + 255) // 256 * 256
+                if batch and used + b > self.out_budget:
+                    break
+                if b > self.out_budget:
+                    raise ValueError("a request's output exceeds the server's output budget")
+                batch.append(r)
+                used += b
+            pending = pending[len(batch):]
+            self._run_pass(batch)
+            t = time.perf_counter() - t0
+            stats.passes += 1
+            for r in batch:
+                r.done = t
+                stats.latencies_ms.setdefault(r.kind, []).append((t - r.arrival) * 1e3)
+                stats.bytes_out += self.out_bytes(r)
+            if verify is not None:
+                verify(self, batch)
+            done += len(batch)
+        stats.wall_s = time.perf_counter() - t0
+        return stats
+
+    def result(self, r: Request) -> bytes:
+        return self.out.download(r.out_offset, self.out_bytes(r))
+
+
+def poisson_requests(rng: np.random.Generator, rate: float, duration: float, store: Store,
+                     mix=(0.60, 0.25, 0.10, 0.05), thumb_res: int = 3, n_encode_sources: int = 1) -> List[Request]:
+    """Poisson arrivals at `rate` req/s for `duration` s with the given kind mix."""
+    n = int(rng.poisson(rate * duration))
+    arrivals = np.sort(rng.uniform(0, duration, n))
+    kinds = rng.choice(4, size=n, p=np.array(mix) / sum(mix))
+    out = []
+    for t, k in zip(arrivals, kinds):
+        if k == 2 and not store.zfp:
+            k = 0
+        if k == 3 and n_encode_sources == 0:
+            k = 0
+        if k == 0:
+            out.append(Request(THUMB, float(t), key=int(rng.integers(0, len(store.ht))), res=thumb_res))
+        elif k == 1:
+            out.append(Request(DECODE, float(t), key=int(rng.integers(0, len(store.ht)))))
+        elif k == 2:
+            key = int(rng.integers(0, len(store.zfp)))
+            f = store.zfp[key]
+            out.append(Request(BRICK, float(t), key=key,
+                               brick=tuple(int(rng.integers(0, (d + 3) // 4)) for d in f.shape)))
+        else:
+            out.append(Request(ENCODE, float(t), key=int(rng.integers(0, max(1, n_encode_sources)))))
+    return out
