@@ -25,11 +25,13 @@ ndic_status ndic_dev_dwt53(int32_t forward, int32_t* plane, size_t width, size_t
 ndic_status ndic_dev_ht_encode_block(const uint32_t* samples, size_t width, size_t height, uint32_t missing_msbs,
                                      ndic_buffer* out, ndic_error* err);
 
-/* HT decode of one code-block segment into sign-magnitude samples
- * (crates/ndic-htj2k/src/block/decode.rs:54). */
+/* HT decode of one code-block segment into sign-magnitude samples: the
+ * cleanup pass, then SigProp and MagRef when num_passes is 2 or 3
+ * (crates/ndic-htj2k/src/block/decode.rs:54, the stripe_causal argument as
+ * there). */
 ndic_status ndic_dev_ht_decode_block(const uint8_t* coded, size_t len, uint32_t* samples, size_t width,
                                      size_t height, uint32_t missing_msbs, uint32_t num_passes, size_t len_cleanup,
-                                     size_t len_refinement, ndic_error* err);
+                                     size_t len_refinement, int stripe_causal, ndic_error* err);
 
 /* Stage timing of the decode pipeline, for the roofline report: enable (and
  * reset) with ndic_dev_profile(1); ndic_dev_stage_times fills, per stage,

@@ -338,7 +338,7 @@ __device__ bool parse_main(const DevChunk& ch, const uint8_t* cs, uint64_t len, 
   // component, reversible 5/3, one layer, maximal precincts, 64x64 blocks.
   if (!(h.cb_style & 0x40) || h.wavelet != 1 || h.style != 0 || h.layers != 1)
     return raise(ch, kErrCsUnsupported, base, gid), false;
-  if ((h.scod & 1) || h.cbw != 4 || h.cbh != 4 || (h.cb_style & 0x08))
+  if ((h.scod & 1) || h.cbw != 4 || h.cbh != 4)
     return raise(ch, kErrCsUnsupported, base, gid), false;
   if (h.decomps != ch.L) return raise(ch, kErrCsLevels, base, gid), false;
   if (h.xsiz != ch.W || h.ysiz != ch.H) return raise(ch, kErrCsGeometry, base, gid), false;
@@ -542,9 +542,9 @@ __global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __rest
           t.len1 = len1;
           t.len2 = len2;
           t.mmsb = (uint8_t)min(mmsbs, 255u);
-          t.passes = (uint8_t)min(np, 3u);
+          t.passes = (uint8_t)min(np, 255u);  // more than 3 is refused by the block decoder
           t.kmax = (uint8_t)kmax;
-          t.flags = kTaskIncluded;
+          t.flags = kTaskIncluded | ((h.cb_style & 0x08) ? kTaskCausal : 0);
           nincl += 1;
         }
     }

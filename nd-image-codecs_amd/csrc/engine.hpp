@@ -25,6 +25,14 @@ struct Context {
   hipStream_t stream = nullptr;
   int device = 0;
   DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, small;
+  // Host copies of job tables whose uploads are still queued on the stream;
+  // released once the stream has been synchronised (stage_keep / release_kept).
+  std::vector<std::vector<char>> kept;
+  const void* stage_keep(const void* p, size_t n) {
+    kept.emplace_back((const char*)p, (const char*)p + n);
+    return kept.back().data();
+  }
+  void release_kept() { kept.clear(); }
   static Context& get();
 };
 

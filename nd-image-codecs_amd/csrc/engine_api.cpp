@@ -369,7 +369,7 @@ ndic_status ndic_dev_ht_encode_block(const uint32_t* samples, size_t width, size
 
 ndic_status ndic_dev_ht_decode_block(const uint8_t* coded, size_t len, uint32_t* samples, size_t width,
                                      size_t height, uint32_t missing_msbs, uint32_t num_passes, size_t len_cleanup,
-                                     size_t len_refinement, ndic_error* err) {
+                                     size_t len_refinement, int stripe_causal, ndic_error* err) {
   return guard_call(err, [&] {
     if (width == 0 || height == 0 || width > 64 || height > 64)
       fail(NDIC_INVALID_ARGUMENT, "device code-blocks are 1..=64 on each side");
@@ -397,14 +397,14 @@ ndic_status ndic_dev_ht_decode_block(const uint8_t* coded, size_t len, uint32_t*
     t.mmsb = (uint8_t)std::min<uint32_t>(missing_msbs, 255);
     t.passes = (uint8_t)std::min<uint32_t>(num_passes, 255);
     t.kmax = 31;
-    t.flags = kTaskIncluded | kTaskSignMag;
+    t.flags = kTaskIncluded | kTaskSignMag | (stripe_causal ? kTaskCausal : 0);
     NDIC_HIP(hipMemcpyAsync(d_task, &t, sizeof(t), hipMemcpyHostToDevice, s));
     launch_ht_decode(d_task, 1, s);
+    launch_ht_refine(d_task, 1, s);
     int32_t e = 0;
     NDIC_HIP(hipMemcpyAsync(&e, d_err, 4, hipMemcpyDeviceToHost, s));
     NDIC_HIP(hipMemcpyAsync(samples, d_out, width * height * 4, hipMemcpyDeviceToHost, s));
     NDIC_HIP(hipStreamSynchronize(s));
-    if (e == kErrCsUnsupported) fail(NDIC_UNSUPPORTED, "refinement passes are not decoded on the device");
     if (e) fail(NDIC_CODESTREAM, "malformed HT code-block segment");
   });
 }

@@ -27,10 +27,12 @@ struct PassLaunch {
 
 void submit(Context& ctx, std::vector<DwtJob>& all, const std::vector<PassLaunch>& launches, hipStream_t stream) {
   if (all.empty()) return;
+  // the job table buffer is reused by the next submit of this context: the
+  // caller synchronises before that (each window ends with a sync)
   DwtJob* d = (DwtJob*)ctx.dwtjobs.ensure(all.size() * sizeof(DwtJob));
-  NDIC_HIP(hipMemcpyAsync(d, all.data(), all.size() * sizeof(DwtJob), hipMemcpyHostToDevice, stream));
+  const void* h = ctx.stage_keep(all.data(), all.size() * sizeof(DwtJob));
+  NDIC_HIP(hipMemcpyAsync(d, h, all.size() * sizeof(DwtJob), hipMemcpyHostToDevice, stream));
   for (auto& l : launches) launch_dwt_pass(l.pass, d + l.first, (uint32_t)l.count, l.items, stream);
-  NDIC_HIP(hipStreamSynchronize(stream));
 }
 
 void add_launch(std::vector<DwtJob>& all, std::vector<PassLaunch>& launches, std::vector<DwtJob>& jobs, int pass) {
@@ -160,5 +162,6 @@ extern "C" ndic_status ndic_dev_dwt53(int32_t forward, int32_t* plane, size_t wi
     else run_dwt_inverse(ctx, cs, s);
     NDIC_HIP(hipMemcpyAsync(plane, cs[0].result, n * 4, hipMemcpyDeviceToHost, s));
     NDIC_HIP(hipStreamSynchronize(s));
+    ctx.release_kept();
   });
 }

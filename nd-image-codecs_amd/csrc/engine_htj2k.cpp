@@ -99,7 +99,7 @@ const char* err_text(int32_t code) {
     case kErrCsMarker: return "malformed codestream marker segment";
     case kErrCsUnsupported:
       return "codestream outside the device decoder's scope (one HT component, reversible 5/3, one layer, maximal "
-             "precincts, 64x64 code-blocks, one tile-part, cleanup-only blocks)";
+             "precincts, 64x64 code-blocks, one tile-part)";
     case kErrCsGeometry: return "plane decodes to a different geometry than the chunk header declares";
     case kErrCsPlt: return "PLT lengths do not cover the tile-part body";
     case kErrPacket: return "malformed packet header or truncated packet body";
@@ -167,6 +167,7 @@ uint64_t planes_of(const std::vector<size_t>& shape) {
 
 // Device memory for one window of a decode batch.
 void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s) {
+  ctx.release_kept();  // every earlier window of this context ended synchronised
   GeoTables geo;
   std::vector<DevChunk> dc(items.size());
   uint64_t nplanes = 0, ntasks = 0, coef = 0;
@@ -238,6 +239,7 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   launch_parse_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_gp, d_gb, s);
   timer.mark();
   launch_ht_decode(d_tasks, ntasks, s);
+  launch_ht_refine(d_tasks, ntasks, s);
   timer.mark();
   std::vector<DwtChunk> dw(n);
   for (size_t i = 0; i < n; ++i) {
@@ -381,6 +383,7 @@ namespace {
 
 void encode_window(Context& ctx, std::vector<EncodeItem*>& items, uint8_t* d_out, uint64_t capacity,
                    uint64_t* out_cursor, hipStream_t s) {
+  ctx.release_kept();  // every earlier window of this context ended synchronised
   GeoTables geo;
   size_t n = items.size();
   std::vector<DevChunk> dc(n);

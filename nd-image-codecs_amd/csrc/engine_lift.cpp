@@ -26,16 +26,21 @@ void DevBuf::release() {
   cap = 0;
 }
 
+// One context (stream + reusable device buffers) per host thread: calls
+// from different threads run on their own streams and overlap on the GPU;
+// calls from one thread are ordered on its stream.  Contexts live until the
+// process ends (the buffers are reused by the thread's later calls).
 Context& Context::get() {
-  static Context* ctx = [] {
-    auto* c = new Context();
+  static thread_local Context* ctx = nullptr;
+  if (!ctx) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
       fail(NDIC_IO, "no HIP device is visible: the nd-image-codecs MI355X path needs a GPU");
+    auto* c = new Context();
     NDIC_HIP(hipGetDevice(&c->device));
     NDIC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    return c;
-  }();
+    ctx = c;
+  }
   return *ctx;
 }
 
@@ -275,11 +280,12 @@ void run_lift(Context& ctx, std::vector<LiftChunk>& chunks, bool fwd, hipStream_
     }
   }
   if (all.empty()) return;
+  // the host job table outlives the asynchronous upload in the context's
+  // kept list; callers synchronise before the job buffer is reused
   LiftJob* d_jobs = (LiftJob*)ctx.jobs.ensure(all.size() * sizeof(LiftJob));
-  NDIC_HIP(hipMemcpyAsync(d_jobs, all.data(), all.size() * sizeof(LiftJob), hipMemcpyHostToDevice, stream));
+  const void* h = ctx.stage_keep(all.data(), all.size() * sizeof(LiftJob));
+  NDIC_HIP(hipMemcpyAsync(d_jobs, h, all.size() * sizeof(LiftJob), hipMemcpyHostToDevice, stream));
   for (auto& l : launches) launch_lift_step(d_jobs + l.first, (uint32_t)l.count, l.items, fwd, l.wide, l.max_n, stream);
-  // the host job table must outlive the asynchronous upload
-  NDIC_HIP(hipStreamSynchronize(stream));
 }
 
 namespace {
@@ -354,6 +360,7 @@ ndic_status lift_chunk_call(bool fwd, const uint8_t* bytes, size_t len, const si
     }
     Context& ctx = Context::get();
     std::lock_guard<std::recursive_mutex> lock(ctx.mu);
+    ctx.release_kept();  // earlier calls on this context ended synchronised
     hipStream_t s = ctx.stream;
     uint8_t* d_in = (uint8_t*)ctx.in.ensure(len + 16);
     void* d_plane = ctx.plane.ensure(n * psz + 16);

@@ -18,6 +18,7 @@
 //
 // The CxtVLC/UVLC tables are derived on the host from the T.814 rows and
 // live in constant memory.
+#include <mutex>
 #include <vector>
 
 #include "ht.hpp"
@@ -353,6 +354,7 @@ struct BlockDec {
   int run;
   bool active;  // decoding (else the block's samples are zero)
   bool bad;
+  bool refine;  // SigProp/MagRef follow: store raw sign-magnitude samples
   // row-pair state
   uint32_t c_q, prev_v_n, vp, q;
 
@@ -361,6 +363,7 @@ struct BlockDec {
     t = task;
     active = false;
     bad = false;
+    refine = false;
     if (!(t.flags & kTaskIncluded) || t.w == 0 || t.h == 0) return;
     uint32_t num_passes = t.passes, missing_msbs = t.mmsb;
     if (num_passes > 1 && t.len2 == 0) num_passes = 1;
@@ -369,10 +372,9 @@ struct BlockDec {
       return;
     }
     if (missing_msbs == 29) num_passes = 1;
-    if (num_passes > 1) {
-      report(t, kErrCsUnsupported);
-      return;
-    }
+    // blocks with SigProp/MagRef passes keep raw sign-magnitude samples
+    // here; ht_refine_kernel finishes them
+    refine = num_passes > 1;
     const uint8_t* coded = t.seg;
     const uint32_t lcup = t.len1;
     const uint32_t scup = (uint32_t(coded[lcup - 1]) << 4) + uint32_t(coded[lcup - 2] & 0xF);
@@ -391,7 +393,7 @@ struct BlockDec {
   }
 
   __device__ uint32_t value(uint32_t sm) const {
-    if (t.flags & kTaskSignMag) return sm;
+    if ((t.flags & kTaskSignMag) || refine) return sm;
     int32_t mag = (int32_t)((sm & 0x7FFFFFFFu) >> shift);
     return (uint32_t)((sm & 0x80000000u) ? -mag : mag);
   }
@@ -587,6 +589,178 @@ __global__ __launch_bounds__(64) void ht_decode_kernel(const HtDecTask* __restri
     report(task, kErrBlock);
     __threadfence();  // the flushed rows land before they are overwritten
     fill_zero(task);
+  }
+  // blocks the refinement kernel must finish: clear the mark of failed ones
+  if (i < n && task.passes > 1 && (!d.active || d.bad)) ((HtDecTask*)tasks)[i].passes = 1;
+}
+
+// SigProp and MagRef passes (decode.rs:398-600) of the blocks that carry
+// them, one lane per block, in place on the raw sign-magnitude samples the
+// cleanup decode left; then the samples become coefficients.  The cleanup
+// significance (what both passes read) is captured first, as one 64-bit row
+// mask per block row in LDS.
+__constant__ uint32_t c_spp_prop[4] = {0x33, 0x76, 0xEC, 0xC8};
+
+__global__ __launch_bounds__(64) void ht_refine_kernel(const HtDecTask* __restrict__ tasks, uint64_t n) {
+  __shared__ uint64_t sigrow[64 * 64];
+  __shared__ uint16_t prevrow[18 * 64];
+  const uint32_t lane = threadIdx.x;
+  uint64_t i = blockIdx.x * (uint64_t)64 + lane;
+  if (i >= n) return;
+  const HtDecTask t = tasks[i];
+  if (!(t.flags & kTaskIncluded) || t.passes < 2 || t.len2 == 0 || t.mmsb >= 29) return;
+  const uint32_t num_passes = t.passes;
+  const uint32_t width = t.w, height = t.h;
+  if (width == 0 || height == 0 || width > 64 || height > 64) return;
+  uint32_t* out = (uint32_t*)t.out;
+  const uint64_t stride = t.stride;
+  const uint32_t p = 30 - t.mmsb;
+  const bool causal = (t.flags & kTaskCausal) != 0;
+  auto SIG = [&](uint32_t y) -> uint64_t& { return sigrow[y * 64 + lane]; };
+  for (uint32_t y = 0; y < height; ++y) {
+    uint64_t m = 0;
+    for (uint32_t x = 0; x < width; ++x) m |= uint64_t(out[y * stride + x] != 0) << x;
+    SIG(y) = m;
+  }
+  // 16-bit group masks: bit 4c + r is sample (row 4s + r, column 4g + c)
+  auto group = [&](uint32_t s, uint32_t g) -> uint32_t {
+    uint32_t b = 0;
+    if (4 * g >= width) return 0;
+    for (uint32_t r = 0; r < 4; ++r) {
+      uint32_t y = 4 * s + r;
+      if (y >= height) break;
+      uint32_t bits = uint32_t(SIG(y) >> (4 * g)) & 0xF;
+      for (uint32_t c = 0; c < 4; ++c)
+        if (bits & (1u << c)) b |= 1u << (4 * c + r);
+    }
+    return b;
+  };
+  const uint32_t groups = (width + 3) / 4;
+  for (uint32_t g = 0; g < groups + 2; ++g) prevrow[g * 64 + lane] = 0;
+  // ---- SigProp (decode.rs:440-551)
+  {
+    FwdDec<0u> sp;
+    sp.init(t.seg + t.len1, t.len2);
+    for (uint32_t y = 0; y < height; y += 4) {
+      uint32_t pattern = 0xFFFF;
+      switch (height - y) {
+        case 1: pattern = 0x1111; break;
+        case 2: pattern = 0x3333; break;
+        case 3: pattern = 0x7777; break;
+        default: break;
+      }
+      uint32_t prev = 0;
+      const uint32_t s = y / 4;
+      for (uint32_t gi = 0, x = 0; x < width; x += 4, ++gi) {
+        uint32_t overhang = x + 4 > width ? x + 4 - width : 0;
+        pattern >>= overhang * 4;
+        uint32_t ps = uint32_t(prevrow[gi * 64 + lane]) | (uint32_t(prevrow[(gi + 1) * 64 + lane]) << 16);
+        uint32_t ns = group(s + 1, gi) | (group(s + 1, gi + 1) << 16);
+        uint32_t cs = group(s, gi) | (group(s, gi + 1) << 16);
+        uint32_t u = (ps & 0x88888888u) >> 3;
+        if (!causal) u |= (ns & 0x11111111u) << 3;
+        uint32_t mbr = cs;
+        mbr |= (cs & 0x77777777u) << 1;
+        mbr |= (cs & 0xEEEEEEEEu) >> 1;
+        mbr |= u;
+        uint32_t tt = mbr;
+        mbr |= tt << 4;
+        mbr |= tt >> 4;
+        mbr |= prev >> 12;
+        mbr &= pattern;
+        mbr &= ~cs;
+        uint32_t new_sig = mbr;
+        if (new_sig != 0) {
+          uint32_t cwd = sp.fetch();
+          uint32_t cnt = 0;
+          uint32_t inv_sig = ~cs & pattern;
+          uint32_t col_mask = 0xF;
+          for (uint32_t b = 0; b < 16; b += 4) {
+            if (col_mask & new_sig) {
+              uint32_t sample_mask = 0x1111 & col_mask;
+              for (uint32_t row = 0; row < 4; ++row) {
+                if (new_sig & sample_mask) {
+                  new_sig &= ~sample_mask;
+                  if (cwd & 1) new_sig |= (c_spp_prop[row] << b) & inv_sig;
+                  cwd >>= 1;
+                  cnt += 1;
+                }
+                sample_mask <<= 1;
+              }
+            }
+            col_mask <<= 4;
+          }
+          if (new_sig != 0) {
+            uint32_t val = 3u << (p - 2);
+            uint32_t cm = 0xF;
+            for (uint32_t col = 0; col < 4; ++col) {
+              if (cm & new_sig) {
+                uint32_t sample_mask = 0x1111 & cm;
+                for (uint32_t row = 0; row < 4; ++row) {
+                  if (new_sig & sample_mask) {
+                    out[(uint64_t)(y + row) * stride + x + col] = (cwd << 31) | val;
+                    cwd >>= 1;
+                    cnt += 1;
+                  }
+                  sample_mask <<= 1;
+                }
+              }
+              cm <<= 4;
+            }
+          }
+          sp.advance(cnt);
+        }
+        new_sig |= cs;
+        prevrow[gi * 64 + lane] = uint16_t(new_sig & 0xFFFF);
+        uint32_t t2 = new_sig;
+        new_sig |= (t2 & 0x7777) << 1;
+        new_sig |= (t2 & 0xEEEE) >> 1;
+        prev = (new_sig | u) & 0xF000;
+      }
+    }
+  }
+  // ---- MagRef (decode.rs:558-600)
+  if (num_passes > 2) {
+    RevDec mr;
+    mr.init_mrp(t.seg, t.len1, t.len2);
+    const uint32_t half = 1u << (p - 2);
+    for (uint32_t y = 0; y < height; y += 4) {
+      const uint32_t s = y / 4;
+      for (uint32_t i8 = 0, gi = 0; i8 < width; i8 += 8, gi += 2) {
+        uint32_t cwd = mr.fetch();
+        uint32_t sig = group(s, gi) | (group(s, gi + 1) << 16);
+        if (sig != 0) {
+          uint32_t col_mask = 0xF;
+          for (uint32_t j = 0; j < 8; ++j) {
+            if (sig & col_mask) {
+              uint32_t sample_mask = 0x11111111u & col_mask;
+              for (uint32_t k = 0; k < 4; ++k) {
+                if (sig & sample_mask) {
+                  if (i8 + j < width && y + k < height) {
+                    uint32_t sym = ((1 - (cwd & 1)) << (p - 1)) | half;
+                    out[(uint64_t)(y + k) * stride + i8 + j] ^= sym;
+                  }
+                  cwd >>= 1;
+                }
+                sample_mask <<= 1;
+              }
+            }
+            col_mask <<= 4;
+          }
+        }
+        mr.advance(__builtin_popcount(sig));
+      }
+    }
+  }
+  // sign-magnitude -> coefficients
+  if (!(t.flags & kTaskSignMag)) {
+    const uint32_t shift = 31 - t.kmax;
+    for (uint32_t y = 0; y < height; ++y)
+      for (uint32_t x = 0; x < width; ++x) {
+        uint32_t sm = out[y * stride + x];
+        int32_t mag = (int32_t)((sm & 0x7FFFFFFFu) >> shift);
+        out[y * stride + x] = (uint32_t)((sm & 0x80000000u) ? -mag : mag);
+      }
   }
 }
 
@@ -967,7 +1141,9 @@ __global__ __launch_bounds__(64) void ht_encode_kernel(const HtEncTask* __restri
 }  // namespace
 
 void ht_init_tables() {
+  static std::mutex mu;
   static bool done = false;
+  std::lock_guard<std::mutex> lock(mu);
   if (done) return;
   std::vector<uint16_t> d0(1024), d1(1024), e0(2048), e1(2048), u0(320), u1(256);
   derive_dec(kCxtVlcRows0, d0.data());
@@ -988,6 +1164,12 @@ void ht_init_tables() {
 void launch_ht_decode(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream) {
   if (n == 0) return;
   hipLaunchKernelGGL(ht_decode_kernel, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, n);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_ht_refine(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(ht_refine_kernel, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, n);
   NDIC_HIP(hipGetLastError());
 }
 

@@ -31,7 +31,9 @@ struct HtEncTask {
   uint16_t pad;
 };
 
-enum : uint8_t { kTaskIncluded = 1, kTaskSignMag = 2 };
+// kTaskCausal: the COD style's vertically causal bit (SigProp ignores the
+// stripe below)
+enum : uint8_t { kTaskIncluded = 1, kTaskSignMag = 2, kTaskCausal = 4 };
 constexpr uint32_t kMelCap = 4096, kVlcCap = 4096;
 
 // Upper bound of the MagSgn bytes of a w x h block at K_max.
@@ -41,6 +43,8 @@ __host__ __device__ inline uint32_t ht_ms_cap(uint32_t w, uint32_t h, uint32_t k
 
 void ht_init_tables();
 void launch_ht_decode(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
+// SigProp/MagRef of the tasks with more than one coding pass (after launch_ht_decode)
+void launch_ht_refine(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
 void launch_ht_encode(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream);
 // Byte k of an encoded segment from its three pieces (see HtEncTask::slot).
 __host__ __device__ inline uint8_t ht_seg_byte(const uint8_t* slot, uint32_t ms_cap, const uint32_t* lens, uint32_t k) {

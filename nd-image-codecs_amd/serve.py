@@ -8,7 +8,9 @@ through ``ndic_htj2k_decode_plane_batch`` (plane 0 cut at the ndht index's
 ``ndic_lift_ht_decode_batch``, bricks through ``ndic_zfp_decode_bricks`` and
 encodes through ``ndic_lift_ht_encode_batch``.  Requests that arrive while a
 pass is on the GPU join the next pass (continuous batching); latency is the
-time from a request's arrival to the end of the pass that served it.
+time from a request's arrival to the end of the pass that served it.  Encodes
+run on a second host thread, which the library gives its own HIP stream, so
+their work overlaps the decode passes on the device.
 
 The thumbnail byte ranges follow ``RangeIndex::thumbnail``
 (crates/ndic-codestream/src/range.rs:128-200): plane 0's codestream up to
@@ -19,7 +21,9 @@ from __future__ import annotations
 
 import ctypes as C
 import importlib
+import queue
 import struct
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
@@ -147,6 +151,14 @@ class Stats:
     passes: int = 0
     bytes_out: int = 0
     wall_s: float = 0.0
+    call_ms: Dict[str, float] = field(default_factory=dict)   # device-batch call time per kind
+    calls: Dict[str, int] = field(default_factory=dict)
+    lock: threading.Lock = field(default_factory=threading.Lock)
+
+    def record(self, kind: str, ms: float):
+        with self.lock:
+            self.call_ms[kind] = self.call_ms.get(kind, 0.0) + ms
+            self.calls[kind] = self.calls.get(kind, 0) + 1
 
     def summary(self):
         allv = sorted(v for vs in self.latencies_ms.values() for v in vs)
@@ -158,7 +170,8 @@ class Stats:
 
         out = {"requests": len(allv), "passes": self.passes, "p50_ms": pct(allv, 50), "p99_ms": pct(allv, 99),
                "GBps": round(self.bytes_out / self.wall_s / 1e9, 3) if self.wall_s else None,
-               "by_kind": {k: {"n": len(v), "p50_ms": pct(sorted(v), 50), "p99_ms": pct(sorted(v), 99)}
+               "by_kind": {k: {"n": len(v), "p50_ms": pct(sorted(v), 50), "p99_ms": pct(sorted(v), 99),
+                               "mean_call_ms": round(self.call_ms.get(k, 0.0) / max(1, self.calls.get(k, 0)), 3)}
                            for k, v in self.latencies_ms.items()}}
         return out
 
@@ -193,8 +206,9 @@ class Server:
             return n * np.dtype(f.dtype).itemsize
         return 0
 
-    def _run_pass(self, reqs: List[Request]):
+    def _run_pass(self, reqs: List[Request], stats: Optional[Stats] = None):
         s = self.store
+        tick = time.perf_counter
         lib = _lib()
         cursor = 0
         for r in reqs:
@@ -205,6 +219,7 @@ class Server:
         bricks = [r for r in reqs if r.kind == BRICK]
         encs = [r for r in reqs if r.kind == ENCODE]
         if thumbs:
+            t0 = tick()
             arr = (PlaneDesc * len(thumbs))()
             for i, r in enumerate(thumbs):
                 c = s.ht[r.key]
@@ -219,11 +234,17 @@ class Server:
             err = _pkg.Error()
             _pkg._check(lib.ndic_htj2k_decode_plane_batch(C.c_void_p(s.ht_arena.ptr), C.c_void_p(self.out.ptr), arr,
                                                           len(thumbs), None, C.byref(err)), err)
+            if stats:
+                stats.record(THUMB, (tick() - t0) * 1e3)
         if decs:
+            t0 = tick()
             descs = [_pkg.make_desc(s.ht[r.key].shape, s.ht[r.key].dtype, s.ht[r.key].steps, s.ht[r.key].levels,
                                     s.ht[r.key].offset, s.ht[r.key].length, r.out_offset) for r in decs]
             _pkg.decode_batch(s.ht_arena.ptr, self.out.ptr, descs)
+            if stats:
+                stats.record(DECODE, (tick() - t0) * 1e3)
         if bricks:
+            t0 = tick()
             arr = (BrickDesc * len(bricks))()
             for i, r in enumerate(bricks):
                 f = s.zfp[r.key]
@@ -238,62 +259,110 @@ class Server:
             err = _pkg.Error()
             _pkg._check(lib.ndic_zfp_decode_bricks(C.c_void_p(s.zfp_arena.ptr), C.c_void_p(self.out.ptr), arr,
                                                    len(bricks), None, C.byref(err)), err)
-        if encs and self.encode_sources:
-            descs, total = [], 0
-            for r in encs:
-                buf, shape, dt, steps, lv = self.encode_sources[r.key % len(self.encode_sources)]
-                descs.append((buf, _pkg.make_desc(shape, dt, steps, lv, 0, int(np.prod(shape)) * np.dtype(dt).itemsize,
-                                                  0)))
-                total += int(np.prod(shape)) * np.dtype(dt).itemsize
-            cap = 2 * total + (64 << 20)
-            if self.enc_out is None or self.enc_out.nbytes < cap:
-                self.enc_out = dev.DeviceBuffer(cap)
-            # sources live in separate buffers: one batch per source buffer
-            by_src: Dict[int, list] = {}
-            for buf, d in descs:
-                by_src.setdefault(buf.ptr, []).append(d)
-            for ptr, ds in by_src.items():
-                _pkg.encode_batch(ptr, self.enc_out.ptr, self.enc_out.nbytes, ds)
+            if stats:
+                stats.record(BRICK, (tick() - t0) * 1e3)
+        if encs:
+            self._run_encodes(encs, stats)
         dev.synchronize()
 
+    def _run_encodes(self, encs: List[Request], stats: Optional[Stats] = None):
+        if not self.encode_sources:
+            return
+        t0 = time.perf_counter()
+        by_src: Dict[int, list] = {}
+        total = 0
+        for r in encs:
+            buf, shape, dt, steps, lv = self.encode_sources[r.key % len(self.encode_sources)]
+            n = int(np.prod(shape)) * np.dtype(dt).itemsize
+            by_src.setdefault(buf.ptr, []).append(_pkg.make_desc(shape, dt, steps, lv, 0, n, 0))
+            total += n
+        cap = 2 * total + (64 << 20)
+        if self.enc_out is None or self.enc_out.nbytes < cap:
+            self.enc_out = dev.DeviceBuffer(cap)
+        for ptr, ds in by_src.items():
+            _pkg.encode_batch(ptr, self.enc_out.ptr, self.enc_out.nbytes, ds)
+        if stats:
+            stats.record(ENCODE, (time.perf_counter() - t0) * 1e3)
+
     def run(self, requests: List[Request], verify=None) -> Stats:
-        """Serves requests (sorted by arrival, seconds from start) in real time."""
+        """Serves requests (sorted by arrival, seconds from start) in real time.
+        Encodes go to an encoder thread (its own HIP stream); everything else
+        is served by passes of this thread."""
         stats = Stats()
         pending: List[Request] = []
         i, n = 0, len(requests)
+        encq: "queue.Queue[Optional[Request]]" = queue.Queue()
+        finished = [0]
         t0 = time.perf_counter()
-        done = 0
-        while done < n:
-            now = time.perf_counter() - t0
-            while i < n and requests[i].arrival <= now:
-                pending.append(requests[i])
-                i += 1
-            if not pending:
-                wait = requests[i].arrival - now
-                if wait > 0:
-                    time.sleep(min(wait, 0.001))
-                continue
-            # admission by output memory
-            batch, used = [], 0
-            for r in pending:
-                b = (self.out_bytes(r) + 255) // 256 * 256
-                if batch and used + b > self.out_budget:
-                    break
-                if b > self.out_budget:
-                    raise ValueError("a request's output exceeds the server's output budget")
-                batch.append(r)
-                used += b
-            pending = pending[len(batch):]
-            self._run_pass(batch)
-            t = time.perf_counter() - t0
-            stats.passes += 1
-            for r in batch:
-                r.done = t
-                stats.latencies_ms.setdefault(r.kind, []).append((t - r.arrival) * 1e3)
-                stats.bytes_out += self.out_bytes(r)
-            if verify is not None:
-                verify(self, batch)
-            done += len(batch)
+
+        def encoder():
+            while True:
+                r = encq.get()
+                if r is None:
+                    return
+                batch = [r]
+                while True:
+                    try:
+                        nxt = encq.get_nowait()
+                    except queue.Empty:
+                        break
+                    if nxt is None:
+                        encq.put(None)
+                        break
+                    batch.append(nxt)
+                self._run_encodes(batch, stats)
+                t = time.perf_counter() - t0
+                with stats.lock:
+                    for q in batch:
+                        q.done = t
+                        stats.latencies_ms.setdefault(q.kind, []).append((t - q.arrival) * 1e3)
+                    finished[0] += len(batch)
+
+        th = threading.Thread(target=encoder, daemon=True)
+        th.start()
+        try:
+            while True:
+                with stats.lock:
+                    if finished[0] >= n:
+                        break
+                now = time.perf_counter() - t0
+                while i < n and requests[i].arrival <= now:
+                    r = requests[i]
+                    i += 1
+                    if r.kind == ENCODE:
+                        encq.put(r)
+                    else:
+                        pending.append(r)
+                if not pending:
+                    wait = requests[i].arrival - now if i < n else 0.0005
+                    if wait > 0:
+                        time.sleep(min(wait, 0.0005))
+                    continue
+                # admission by output memory
+                batch, used = [], 0
+                for r in pending:
+                    b = (self.out_bytes(r) + 255) // 256 * 256
+                    if batch and used + b > self.out_budget:
+                        break
+                    if b > self.out_budget:
+                        raise ValueError("a request's output exceeds the server's output budget")
+                    batch.append(r)
+                    used += b
+                pending = pending[len(batch):]
+                self._run_pass(batch, stats)
+                t = time.perf_counter() - t0
+                with stats.lock:
+                    stats.passes += 1
+                    for r in batch:
+                        r.done = t
+                        stats.latencies_ms.setdefault(r.kind, []).append((t - r.arrival) * 1e3)
+                        stats.bytes_out += self.out_bytes(r)
+                    finished[0] += len(batch)
+                if verify is not None:
+                    verify(self, batch)
+        finally:
+            encq.put(None)
+            th.join()
         stats.wall_s = time.perf_counter() - t0
         return stats
 

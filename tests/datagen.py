@@ -36,3 +36,16 @@ def c0_volume(shape=(32, 256, 256), seed=0):
 
 def random_plane_ints(rng, shape, lo, hi, dtype):
     return rng.integers(lo, hi + 1, size=shape, dtype=np.int64).astype(dtype)
+
+
+def sm_block(rng, w, h, kmax, dens):
+    """A random HT code-block in sign-magnitude form: magnitudes below 2**kmax
+    in the top kmax bits under the sign, a fraction dens of them non-zero."""
+    mag = rng.integers(0, 2 ** kmax, size=(h, w)).astype(np.int64)
+    mag[rng.uniform(size=(h, w)) > dens] = 0
+    if mag.max() == 0:
+        mag[0, 0] = 1
+    sign = rng.integers(0, 2, size=(h, w)).astype(np.uint32) << 31
+    sm = (sign | (mag.astype(np.uint32) << (31 - kmax))).astype(np.uint32)
+    sm[mag == 0] = 0
+    return sm

@@ -15,7 +15,7 @@ def bind(ndic):
     lib.ndic_dev_ht_encode_block.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_uint32, B, E]
     lib.ndic_dev_ht_encode_block.restype = C.c_int
     lib.ndic_dev_ht_decode_block.argtypes = [C.c_char_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,
-                                             C.c_uint32, C.c_uint32, C.c_size_t, C.c_size_t, E]
+                                             C.c_uint32, C.c_uint32, C.c_size_t, C.c_size_t, C.c_int, E]
     lib.ndic_dev_ht_decode_block.restype = C.c_int
     return lib
 
@@ -38,11 +38,13 @@ def ht_encode_block(ndic, sm: np.ndarray, missing_msbs: int) -> bytes:
     return ndic._take(out)
 
 
-def ht_decode_block(ndic, coded: bytes, w: int, h: int, missing_msbs: int) -> np.ndarray:
+def ht_decode_block(ndic, coded: bytes, w: int, h: int, missing_msbs: int, num_passes: int = 1,
+                    len1: int | None = None, len2: int = 0, stripe_causal: bool = False) -> np.ndarray:
     lib = bind(ndic)
     out = np.zeros((h, w), np.uint32)
     err = ndic.Error()
-    rc = lib.ndic_dev_ht_decode_block(coded, len(coded), out.ctypes.data, w, h, missing_msbs, 1, len(coded), 0,
+    rc = lib.ndic_dev_ht_decode_block(coded, len(coded), out.ctypes.data, w, h, missing_msbs, num_passes,
+                                      len(coded) if len1 is None else len1, len2, int(stripe_causal),
                                       C.byref(err))
     ndic._check(rc, err)
     return out

@@ -26,17 +26,6 @@ NPT = {"u8": np.uint8, "i8": np.int8, "u16": np.uint16, "i16": np.int16, "u32": 
 NAME = {"u8": "uint8", "i8": "int8", "u16": "uint16", "i16": "int16", "u32": "uint32", "i32": "int32"}
 
 
-def _sm_block(rng, w, h, kmax, dens):
-    mag = rng.integers(0, 2 ** kmax, size=(h, w)).astype(np.int64)
-    mag[rng.uniform(size=(h, w)) > dens] = 0
-    if mag.max() == 0:
-        mag[0, 0] = 1
-    sign = rng.integers(0, 2, size=(h, w)).astype(np.uint32) << 31
-    sm = (sign | (mag.astype(np.uint32) << (31 - kmax))).astype(np.uint32)
-    sm[mag == 0] = 0
-    return sm
-
-
 def test_ht_block_encode_matches_oracle(ndic, oracle):
     rng = np.random.default_rng(5)
     for i in range(120):
@@ -44,12 +33,45 @@ def test_ht_block_encode_matches_oracle(ndic, oracle):
         if i < 8:
             w, h = 64, 64
         kmax = int(rng.integers(2, 29))
-        sm = _sm_block(rng, w, h, kmax, rng.uniform(0, 1))
+        sm = datagen.sm_block(rng, w, h, kmax, rng.uniform(0, 1))
         exp = oracle.ht_encode_block(sm, kmax - 1)
         got = devhooks.ht_encode_block(ndic, sm, kmax - 1)
         assert got == exp, (w, h, kmax)
         dec = devhooks.ht_decode_block(ndic, exp, w, h, kmax - 1)
         assert np.array_equal(dec, oracle.ht_decode_block(exp, w, h, kmax - 1)), (w, h, kmax)
+
+
+def test_ht_refinement_passes_match_oracle(ndic, oracle):
+    """SigProp and MagRef on the device (decode.rs:398-600) against the oracle,
+    whose scan is pinned by tests/test_ht_refine_oracle.py: random, all-zero
+    and all-one refinement segments (stuffing cases), both causal modes, and
+    the cases that fall back to the cleanup pass alone."""
+    rng = np.random.default_rng(21)
+    for i in range(80):
+        w, h = int(rng.integers(1, 65)), int(rng.integers(1, 65))
+        if i < 4:
+            w, h = [(64, 64), (1, 1), (64, 5), (3, 64)][i]
+        kmax = int(rng.integers(3, 29))
+        mmsb = kmax - 1
+        sm = datagen.sm_block(rng, w, h, kmax, float(rng.uniform(0.02, 0.6)))
+        cleanup = oracle.ht_encode_block(sm, mmsb)
+        causal = bool(i & 1)
+        for passes in (2, 3):
+            n2 = int(rng.integers(1, 500))
+            kind = i % 3
+            seg2 = bytes(n2) if kind == 1 else b"\xff" * n2 if kind == 2 else \
+                rng.integers(0, 256, size=n2, dtype=np.uint8).tobytes()
+            coded = cleanup + seg2
+            exp = oracle.ht_decode_block(coded, w, h, mmsb, passes, len(cleanup), len(seg2), causal)
+            got = devhooks.ht_decode_block(ndic, coded, w, h, mmsb, passes, len(cleanup), len(seg2), causal)
+            assert np.array_equal(got, exp), (i, w, h, kmax, passes, kind, causal)
+    # an empty refinement segment and 29 missing MSBs leave the cleanup result
+    sm = datagen.sm_block(rng, 20, 20, 1, 0.5)
+    cleanup = oracle.ht_encode_block(sm, 29)
+    for passes, extra in ((3, b""), (3, b"\x5a" * 9)):
+        exp = oracle.ht_decode_block(cleanup + extra, 20, 20, 29, passes, len(cleanup), len(extra))
+        got = devhooks.ht_decode_block(ndic, cleanup + extra, 20, 20, 29, passes, len(cleanup), len(extra))
+        assert np.array_equal(got, exp)
 
 
 def test_dwt_matches_oracle(ndic, oracle):
