@@ -75,6 +75,12 @@ ndic_status ndic_dev_copy(void* dst, const void* src, size_t n, ndic_error* err)
 /* Number of differing bytes of two 16-byte aligned device buffers. */
 ndic_status ndic_dev_count_mismatch(const void* a, const void* b, size_t n, uint64_t* count, ndic_error* err);
 
+/* The same count added to a device counter (*d_count, 8-byte aligned) on the
+ * calling thread's stream, without waiting: ordered after the thread's
+ * earlier library calls, read back by the caller when it synchronises. */
+ndic_status ndic_dev_count_mismatch_async(const void* a, const void* b, size_t n, uint64_t* d_count,
+                                          ndic_error* err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -169,6 +169,20 @@ extern "C" ndic_status ndic_dev_count_mismatch(const void* a, const void* b, siz
   });
 }
 
+extern "C" ndic_status ndic_dev_count_mismatch_async(const void* a, const void* b, size_t n, uint64_t* d_count,
+                                                     ndic_error* err) {
+  return guard_call(err, [&] {
+    bool aligned = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0) && ((uintptr_t)d_count % 8 == 0);
+    if (!aligned) fail(NDIC_INVALID_ARGUMENT, "ndic_dev_count_mismatch_async needs 16-byte aligned buffers");
+    if (!n) return;
+    Context& ctx = Context::get();
+    std::lock_guard<std::recursive_mutex> lock(ctx.mu);
+    hipLaunchKernelGGL(mismatch_kernel, dim3(grid_for(n / 16 + 1)), dim3(256), 0, ctx.stream, (const uint8_t*)a,
+                       (const uint8_t*)b, (uint64_t)n, (unsigned long long*)d_count);
+    NDIC_HIP(hipGetLastError());
+  });
+}
+
 extern "C" ndic_status ndic_dev_copy(void* dst, const void* src, size_t n, ndic_error* err) {
   return guard_call(err, [&] {
     if (n % 16 || (uintptr_t)dst % 16 || (uintptr_t)src % 16)

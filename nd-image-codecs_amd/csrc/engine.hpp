@@ -2,6 +2,7 @@
 // device buffers) and the orchestration of the codec kernels.
 #pragma once
 
+#include <cstring>
 #include <functional>
 #include <mutex>
 #include <vector>
@@ -20,19 +21,41 @@ struct DevBuf {
   void release();
 };
 
+// Pinned host staging for the small tables a call uploads (descriptors, job
+// lists) and the statuses it reads back: copies from pinned memory are
+// queued without the host waiting, where a copy from pageable memory would
+// wait for the stream to drain.  Space is handed out by a bump pointer and
+// reclaimed by reset() at the start of the next window: every window ends
+// with its stream synchronised, and a call that fails midway drains the
+// device before it returns (guard_call).
+struct PinnedArena {
+  char* base = nullptr;
+  size_t cap = 0, off = 0;
+  std::vector<char*> retired;  // outgrown blocks, freed at reset()
+  void* take(size_t n);
+  void reset();
+};
+
 struct Context {
   std::recursive_mutex mu;
   hipStream_t stream = nullptr;
   int device = 0;
   DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, small;
-  // Host copies of job tables whose uploads are still queued on the stream;
-  // released once the stream has been synchronised (stage_keep / release_kept).
-  std::vector<std::vector<char>> kept;
+  PinnedArena pin;
+  // A pinned copy of p[0..n) for an upload queued on the stream; valid until
+  // release_kept().
   const void* stage_keep(const void* p, size_t n) {
-    kept.emplace_back((const char*)p, (const char*)p + n);
-    return kept.back().data();
+    void* h = pin.take(n);
+    if (n) std::memcpy(h, p, n);
+    return h;
   }
-  void release_kept() { kept.clear(); }
+  // Queue an upload of host bytes through pinned staging.
+  void up(void* d, const void* h, size_t n, hipStream_t s);
+  // Queue a download into pinned staging; the bytes are there once s is
+  // synchronised.
+  void* down(const void* d, size_t n, hipStream_t s);
+  // Reclaims the staging; only between windows (no copy of it in flight).
+  void release_kept() { pin.reset(); }
   static Context& get();
 };
 

@@ -399,7 +399,7 @@ ndic_status ndic_dev_ht_decode_block(const uint8_t* coded, size_t len, uint32_t*
     t.kmax = 31;
     t.flags = kTaskIncluded | kTaskSignMag | (stripe_causal ? kTaskCausal : 0);
     NDIC_HIP(hipMemcpyAsync(d_task, &t, sizeof(t), hipMemcpyHostToDevice, s));
-    launch_ht_decode(d_task, 1, s);
+    launch_ht_decode(d_task, nullptr, 1, s);
     launch_ht_refine(d_task, 1, s);
     int32_t e = 0;
     NDIC_HIP(hipMemcpyAsync(&e, d_err, 4, hipMemcpyDeviceToHost, s));

@@ -30,8 +30,7 @@ void submit(Context& ctx, std::vector<DwtJob>& all, const std::vector<PassLaunch
   // the job table buffer is reused by the next submit of this context: the
   // caller synchronises before that (each window ends with a sync)
   DwtJob* d = (DwtJob*)ctx.dwtjobs.ensure(all.size() * sizeof(DwtJob));
-  const void* h = ctx.stage_keep(all.data(), all.size() * sizeof(DwtJob));
-  NDIC_HIP(hipMemcpyAsync(d, h, all.size() * sizeof(DwtJob), hipMemcpyHostToDevice, stream));
+  ctx.up(d, all.data(), all.size() * sizeof(DwtJob), stream);
   for (auto& l : launches) launch_dwt_pass(l.pass, d + l.first, (uint32_t)l.count, l.items, stream);
 }
 

@@ -166,6 +166,10 @@ uint64_t planes_of(const std::vector<size_t>& shape) {
 }
 
 // Device memory for one window of a decode batch.
+// Up to kSoloTasks code-blocks a window decodes one block per wavefront
+// (latency); above, a block per lane in size-class order (throughput).
+constexpr uint64_t kSoloTasks = 8192;
+
 void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s) {
   ctx.release_kept();  // every earlier window of this context ended synchronised
   GeoTables geo;
@@ -205,7 +209,11 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
            sz_gb = al(geo.bands.size() * sizeof(GeoBand)), sz_planes = al(nplanes * sizeof(DevPlane)),
            sz_tasks = al(ntasks * sizeof(HtDecTask)), sz_tag = al(nplanes * tag_stride), sz_err = al(n * 16),
            sz_ptr = al(n * sizeof(void*)), sz_dims = al(n * 16), sz_flag = al(n * 4);
-  uint64_t meta = sz_chunks + sz_gp + sz_gb + sz_planes + sz_tasks + sz_tag + sz_err + sz_ptr + sz_dims + sz_flag;
+  // large windows decode their code-blocks in size-class order
+  const bool ordered = ntasks > kSoloTasks;
+  uint64_t sz_order = ordered ? al(ht_order_scratch_bytes(ntasks)) : 0;
+  uint64_t meta =
+      sz_chunks + sz_gp + sz_gb + sz_planes + sz_tasks + sz_tag + sz_err + sz_ptr + sz_dims + sz_flag + sz_order;
   uint8_t* m = (uint8_t*)ctx.work.ensure(meta);
   DevChunk* d_chunks = (DevChunk*)m;
   GeoPacket* d_gp = (GeoPacket*)(m + sz_chunks);
@@ -217,6 +225,7 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   int32_t** d_ptr = (int32_t**)((uint8_t*)d_err + sz_err);
   uint64_t* d_dims = (uint64_t*)((uint8_t*)d_ptr + sz_ptr);
   int32_t* d_flag = (int32_t*)((uint8_t*)d_dims + sz_dims);
+  void* d_order = ordered ? (void*)((uint8_t*)d_flag + sz_flag) : nullptr;
   int32_t* A = (int32_t*)ctx.coef.ensure(al(3 * coef * 4 + 64));
   int32_t* T = A + coef;
   int32_t* O = A + 2 * coef;
@@ -230,15 +239,20 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   }
   NDIC_HIP(hipMemsetAsync(d_planes, 0, sz_planes, s));
   NDIC_HIP(hipMemsetAsync(d_err, 0, sz_err + sz_ptr + sz_dims + sz_flag, s));
-  NDIC_HIP(hipMemcpyAsync(d_chunks, dc.data(), n * sizeof(DevChunk), hipMemcpyHostToDevice, s));
-  NDIC_HIP(hipMemcpyAsync(d_gp, geo.packets.data(), geo.packets.size() * sizeof(GeoPacket), hipMemcpyHostToDevice, s));
-  NDIC_HIP(hipMemcpyAsync(d_gb, geo.bands.data(), geo.bands.size() * sizeof(GeoBand), hipMemcpyHostToDevice, s));
+  ctx.up(d_chunks, dc.data(), n * sizeof(DevChunk), s);
+  ctx.up(d_gp, geo.packets.data(), geo.packets.size() * sizeof(GeoPacket), s);
+  ctx.up(d_gb, geo.bands.data(), geo.bands.size() * sizeof(GeoBand), s);
   StageTimer timer(s);
   timer.mark();
   launch_walk_chunks(d_chunks, (uint32_t)n, d_planes, s);
   launch_parse_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_gp, d_gb, s);
   timer.mark();
-  launch_ht_decode(d_tasks, ntasks, s);
+  if (ordered) {
+    launch_ht_order(d_tasks, ntasks, d_order, s);
+    launch_ht_decode(d_tasks, (const uint32_t*)((uint8_t*)d_order + ht_order_offset()), ntasks, s);
+  } else {
+    launch_ht_decode_solo(d_tasks, ntasks, s);
+  }
   launch_ht_refine(d_tasks, ntasks, s);
   timer.mark();
   std::vector<DwtChunk> dw(n);
@@ -267,8 +281,8 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
     dims[2 * i] = dw[i].W;
     dims[2 * i + 1] = dw[i].H;
   }
-  NDIC_HIP(hipMemcpyAsync(d_ptr, res.data(), n * sizeof(void*), hipMemcpyHostToDevice, s));
-  NDIC_HIP(hipMemcpyAsync(d_dims, dims.data(), n * 16, hipMemcpyHostToDevice, s));
+  ctx.up(d_ptr, res.data(), n * sizeof(void*), s);
+  ctx.up(d_dims, dims.data(), n * 16, s);
   launch_unshift(d_chunks, (uint32_t)n, nplanes, d_planes, d_ptr, d_dims, s);
   timer.mark();
   // outputs
@@ -300,10 +314,10 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   }
   if (!lifts.empty()) run_lift(ctx, lifts, false, s);
   timer.mark();
-  std::vector<int32_t> herr(4 * n), hflag(n);
-  NDIC_HIP(hipMemcpyAsync(herr.data(), d_err, n * 16, hipMemcpyDeviceToHost, s));
-  NDIC_HIP(hipMemcpyAsync(hflag.data(), d_flag, n * 4, hipMemcpyDeviceToHost, s));
+  const int32_t* p_err = (const int32_t*)ctx.down(d_err, n * 16, s);
+  const int32_t* p_flag = (const int32_t*)ctx.down(d_flag, n * 4, s);
   NDIC_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> herr(p_err, p_err + 4 * n), hflag(p_flag, p_flag + n);
   if (g_prof.on) {
     // algorithmic bytes per stage: parse reads the streams; HT decode reads
     // the segments and writes 4 B per coefficient; the DWT reads and writes
@@ -489,16 +503,17 @@ void encode_window(Context& ctx, std::vector<EncodeItem*>& items, uint8_t* d_out
       init[2 * i] = INT64_MAX;
       init[2 * i + 1] = INT64_MIN;
     }
-    NDIC_HIP(hipMemcpyAsync(mm, init.data(), n * 16, hipMemcpyHostToDevice, s));
+    ctx.up(mm, init.data(), n * 16, s);
     for (size_t i = 0; i < n; ++i) {
       uint64_t total = (uint64_t)dc[i].P * dc[i].W * dc[i].H;
       launch_convert(items[i]->d_in, items[i]->in_dtype, dc[i].planes_in, false, nullptr, 0, total, mm + 2 * i,
                      d_flag + i, s);
     }
-    std::vector<int32_t> fl(n);
-    NDIC_HIP(hipMemcpyAsync(mm_h.data(), mm, n * 16, hipMemcpyDeviceToHost, s));
-    NDIC_HIP(hipMemcpyAsync(fl.data(), d_flag, n * 4, hipMemcpyDeviceToHost, s));
+    const int64_t* p_mm = (const int64_t*)ctx.down(mm, n * 16, s);
+    const int32_t* p_fl = (const int32_t*)ctx.down(d_flag, n * 4, s);
     NDIC_HIP(hipStreamSynchronize(s));
+    std::copy(p_mm, p_mm + 2 * n, mm_h.begin());
+    std::vector<int32_t> fl(p_fl, p_fl + n);
     std::vector<LiftChunk> lifts;
     for (size_t i = 0; i < n; ++i) {
       EncodeItem& it = *items[i];
@@ -527,9 +542,9 @@ void encode_window(Context& ctx, std::vector<EncodeItem*>& items, uint8_t* d_out
     }
     if (!lifts.empty()) run_lift(ctx, lifts, true, s);
   }
-  NDIC_HIP(hipMemcpyAsync(d_chunks, dc.data(), n * sizeof(DevChunk), hipMemcpyHostToDevice, s));
-  NDIC_HIP(hipMemcpyAsync(d_gp, geo.packets.data(), geo.packets.size() * sizeof(GeoPacket), hipMemcpyHostToDevice, s));
-  NDIC_HIP(hipMemcpyAsync(d_gb, geo.bands.data(), geo.bands.size() * sizeof(GeoBand), hipMemcpyHostToDevice, s));
+  ctx.up(d_chunks, dc.data(), n * sizeof(DevChunk), s);
+  ctx.up(d_gp, geo.packets.data(), geo.packets.size() * sizeof(GeoPacket), s);
+  ctx.up(d_gb, geo.bands.data(), geo.bands.size() * sizeof(GeoBand), s);
   launch_plane_minmax(d_chunks, (uint32_t)n, nplanes, d_planes, s);
   launch_plane_setup(d_chunks, (uint32_t)n, nplanes, d_planes, s);
   launch_dc_shift(d_chunks, (uint32_t)n, nplanes, d_planes, s);
@@ -554,16 +569,16 @@ void encode_window(Context& ctx, std::vector<EncodeItem*>& items, uint8_t* d_out
       dc[i].coef = dw[i].result;
       reupload = true;
     }
-  if (reupload) NDIC_HIP(hipMemcpyAsync(d_chunks, dc.data(), n * sizeof(DevChunk), hipMemcpyHostToDevice, s));
+  if (reupload) ctx.up(d_chunks, dc.data(), n * sizeof(DevChunk), s);
   launch_build_enc_tasks(d_chunks, (uint32_t)n, nplanes, d_planes, d_gp, d_gb, d_tasks, d_lens, slot_stride, s);
   launch_ht_encode(d_tasks, ntasks, s);
   launch_write_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_lens, d_gp, d_gb, s);
   launch_write_chunks(d_chunks, (uint32_t)n, d_planes, d_clen, s);
-  std::vector<uint64_t> clen(n);
-  std::vector<int32_t> herr(4 * n);
-  NDIC_HIP(hipMemcpyAsync(clen.data(), d_clen, n * 8, hipMemcpyDeviceToHost, s));
-  NDIC_HIP(hipMemcpyAsync(herr.data(), d_err, n * 16, hipMemcpyDeviceToHost, s));
+  const uint64_t* p_clen = (const uint64_t*)ctx.down(d_clen, n * 8, s);
+  const int32_t* p_err = (const int32_t*)ctx.down(d_err, n * 16, s);
   NDIC_HIP(hipStreamSynchronize(s));
+  std::vector<uint64_t> clen(p_clen, p_clen + n);
+  std::vector<int32_t> herr(p_err, p_err + 4 * n);
   for (size_t i = 0; i < n; ++i) {
     EncodeItem& it = *items[i];
     if (it.status == NDIC_OK && herr[4 * i]) {
@@ -590,7 +605,7 @@ void encode_window(Context& ctx, std::vector<EncodeItem*>& items, uint8_t* d_out
   }
   // refused chunks assemble nothing (out_len 0)
   for (size_t i = 0; i < n; ++i) dc[i].out_len = items[i]->status == NDIC_OK ? items[i]->out_len : 0;
-  NDIC_HIP(hipMemcpyAsync(d_chunks, dc.data(), n * sizeof(DevChunk), hipMemcpyHostToDevice, s));
+  ctx.up(d_chunks, dc.data(), n * sizeof(DevChunk), s);
   launch_assemble(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_lens, d_gp, d_out, s);
   NDIC_HIP(hipStreamSynchronize(s));
 }

@@ -14,7 +14,12 @@ namespace ndic {
 void* DevBuf::ensure(size_t bytes) {
   if (bytes <= cap && ptr) return ptr;
   release();
+  // grow to the next size of the form m * 2^k (8 <= m < 16): at most 1/8
+  // over, and a slowly growing sequence of requests reallocates (and
+  // synchronises the device in hipFree) only a logarithmic number of times
   size_t want = bytes < 256 ? 256 : bytes;
+  int k = 63 - __builtin_clzll((unsigned long long)want) - 3;
+  if (k > 0) want = ((want + ((size_t)1 << k) - 1) >> k) << k;
   NDIC_HIP(hipMalloc(&ptr, want));
   cap = want;
   return ptr;
@@ -24,6 +29,40 @@ void DevBuf::release() {
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   cap = 0;
+}
+
+void* PinnedArena::take(size_t n) {
+  n = (n + 63) & ~(size_t)63;
+  if (off + n > cap) {
+    if (base) retired.push_back(base);
+    size_t want = std::max(cap * 2, n + ((size_t)1 << 20));
+    base = nullptr;
+    cap = 0;
+    off = 0;
+    NDIC_HIP(hipHostMalloc((void**)&base, want, hipHostMallocDefault));
+    cap = want;
+  }
+  void* r = base + off;
+  off += n;
+  return r;
+}
+
+void PinnedArena::reset() {
+  for (char* p : retired) (void)hipHostFree(p);
+  retired.clear();
+  off = 0;
+}
+
+void Context::up(void* d, const void* h, size_t n, hipStream_t s) {
+  if (!n) return;
+  const void* p = stage_keep(h, n);
+  NDIC_HIP(hipMemcpyAsync(d, p, n, hipMemcpyHostToDevice, s));
+}
+
+void* Context::down(const void* d, size_t n, hipStream_t s) {
+  void* p = pin.take(n);
+  if (n) NDIC_HIP(hipMemcpyAsync(p, d, n, hipMemcpyDeviceToHost, s));
+  return p;
 }
 
 // One context (stream + reusable device buffers) per host thread: calls
@@ -283,8 +322,7 @@ void run_lift(Context& ctx, std::vector<LiftChunk>& chunks, bool fwd, hipStream_
   // the host job table outlives the asynchronous upload in the context's
   // kept list; callers synchronise before the job buffer is reused
   LiftJob* d_jobs = (LiftJob*)ctx.jobs.ensure(all.size() * sizeof(LiftJob));
-  const void* h = ctx.stage_keep(all.data(), all.size() * sizeof(LiftJob));
-  NDIC_HIP(hipMemcpyAsync(d_jobs, h, all.size() * sizeof(LiftJob), hipMemcpyHostToDevice, stream));
+  ctx.up(d_jobs, all.data(), all.size() * sizeof(LiftJob), stream);
   for (auto& l : launches) launch_lift_step(d_jobs + l.first, (uint32_t)l.count, l.items, fwd, l.wide, l.max_n, stream);
 }
 
@@ -310,9 +348,12 @@ ndic_status guard_call(ndic_error* err, const std::function<void()>& body) {
     }
     return NDIC_OK;
   } catch (const Failure& f) {
+    // work this call queued may still read its pinned staging: drain it
+    (void)hipDeviceSynchronize();
     set_error(err, f);
     return (ndic_status)f.status;
   } catch (const std::bad_alloc&) {
+    (void)hipDeviceSynchronize();
     set_error(err, Failure{NDIC_IO, 0, "host allocation failed"});
     return NDIC_IO;
   }

@@ -370,6 +370,7 @@ uint32_t class_of(const Plan& p) { return p.dims * 8 + p.scalar; }
 }  // namespace
 
 void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out, uint64_t capacity, hipStream_t s) {
+  ctx.release_kept();  // earlier calls on this context ended synchronised
   zfp_init_tables();
   for (auto& it : items) {
     if (it.status != NDIC_OK) continue;
@@ -451,18 +452,18 @@ void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out,
     }
     ZfpChunk* d_ch = (ZfpChunk*)meta;
     ZfpHeader* d_hdr = (ZfpHeader*)(meta + sz_ch);
-    NDIC_HIP(hipMemcpyAsync(d_ch, ch.data(), ch.size() * sizeof(ZfpChunk), hipMemcpyHostToDevice, s));
+    ctx.up(d_ch, ch.data(), ch.size() * sizeof(ZfpChunk), s);
     if (!hdr.empty()) {
-      NDIC_HIP(hipMemcpyAsync(d_hdr, hdr.data(), hdr.size() * sizeof(ZfpHeader), hipMemcpyHostToDevice, s));
+      ctx.up(d_hdr, hdr.data(), hdr.size() * sizeof(ZfpHeader), s);
       launch_zfp_headers(d_hdr, (uint32_t)hdr.size(), s);
     }
     launch_zfp_encode(d_ch, (uint32_t)ch.size(), total, cls / 8, cls % 8, s);
     if (vblocks) {
       zfp_exclusive_scan(scan_tmp, scan_bytes, offs, scan, total, s);
       launch_zfp_totals(d_ch, (uint32_t)ch.size(), scan, totals, s);
-      std::vector<uint64_t> tot(group.size());
-      NDIC_HIP(hipMemcpyAsync(tot.data(), totals, group.size() * 8, hipMemcpyDeviceToHost, s));
+      const uint64_t* p_tot = (const uint64_t*)ctx.down(totals, group.size() * 8, s);
       NDIC_HIP(hipStreamSynchronize(s));
+      std::vector<uint64_t> tot(p_tot, p_tot + group.size());
       hdr.clear();
       for (size_t i = 0; i < group.size(); ++i) {
         if (ch[i].fixed || group[i]->status != NDIC_OK) continue;
@@ -486,9 +487,9 @@ void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out,
       // fixed-rate chunks of the class take no part in placement
       for (size_t i = 0; i < ch.size(); ++i)
         if (ch[i].fixed) ch[i].offs = nullptr;
-      NDIC_HIP(hipMemcpyAsync(d_ch, ch.data(), ch.size() * sizeof(ZfpChunk), hipMemcpyHostToDevice, s));
+      ctx.up(d_ch, ch.data(), ch.size() * sizeof(ZfpChunk), s);
       if (!hdr.empty()) {
-        NDIC_HIP(hipMemcpyAsync(d_hdr, hdr.data(), hdr.size() * sizeof(ZfpHeader), hipMemcpyHostToDevice, s));
+        ctx.up(d_hdr, hdr.data(), hdr.size() * sizeof(ZfpHeader), s);
         launch_zfp_headers(d_hdr, (uint32_t)hdr.size(), s);
       }
       launch_zfp_place(d_ch, (uint32_t)ch.size(), total, scan, s);
@@ -498,6 +499,7 @@ void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out,
 }
 
 void zfp_decode_items(Context& ctx, std::vector<ZfpItem>& items, hipStream_t s) {
+  ctx.release_kept();  // earlier calls on this context ended synchronised
   zfp_init_tables();
   // header bytes of every stream, for the host-side checks
   std::vector<uint8_t> heads(items.size() * kZfpHeadBytes, 0);
@@ -516,10 +518,9 @@ void zfp_decode_items(Context& ctx, std::vector<ZfpItem>& items, hipStream_t s) 
     if (!hv.empty()) {
       uint64_t sz_h = al256(hv.size() * sizeof(ZfpHead));
       uint8_t* m = (uint8_t*)ctx.small.ensure(sz_h + hv.size() * kZfpHeadBytes);
-      NDIC_HIP(hipMemcpyAsync(m, hv.data(), hv.size() * sizeof(ZfpHead), hipMemcpyHostToDevice, s));
+      ctx.up(m, hv.data(), hv.size() * sizeof(ZfpHead), s);
       launch_zfp_heads((ZfpHead*)m, (uint32_t)hv.size(), m + sz_h, s);
-      std::vector<uint8_t> got(hv.size() * kZfpHeadBytes);
-      NDIC_HIP(hipMemcpyAsync(got.data(), m + sz_h, got.size(), hipMemcpyDeviceToHost, s));
+      const uint8_t* got = (const uint8_t*)ctx.down(m + sz_h, hv.size() * kZfpHeadBytes, s);
       NDIC_HIP(hipStreamSynchronize(s));
       for (size_t k = 0; k < which.size(); ++k)
         std::memcpy(&heads[which[k] * kZfpHeadBytes], &got[k * kZfpHeadBytes], kZfpHeadBytes);
@@ -558,7 +559,7 @@ void zfp_decode_items(Context& ctx, std::vector<ZfpItem>& items, hipStream_t s) 
     uint64_t* offs = variable ? (uint64_t*)(meta + sz_ch) : nullptr;
     for (auto& c : ch) c.offs = offs;
     ZfpChunk* d_ch = (ZfpChunk*)meta;
-    NDIC_HIP(hipMemcpyAsync(d_ch, ch.data(), ch.size() * sizeof(ZfpChunk), hipMemcpyHostToDevice, s));
+    ctx.up(d_ch, ch.data(), ch.size() * sizeof(ZfpChunk), s);
     if (variable) launch_zfp_index(d_ch, (uint32_t)ch.size(), cls / 8, cls % 8, s);
     launch_zfp_decode(d_ch, (uint32_t)ch.size(), total, cls / 8, cls % 8, nullptr, s);
     NDIC_HIP(hipStreamSynchronize(s));
@@ -817,6 +818,7 @@ extern "C" ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_ou
     Context& ctx = Context::get();
     std::lock_guard<std::recursive_mutex> lock(ctx.mu);
     zfp_init_tables();
+    ctx.release_kept();
     hipStream_t s = stream ? (hipStream_t)stream : ctx.stream;
     std::vector<Plan> plans(n);
     std::vector<uint64_t> block(n, 0);
@@ -827,10 +829,11 @@ extern "C" ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_ou
     uint8_t* m = (uint8_t*)ctx.small.ensure(sz_h + n * kZfpHeadBytes + 64);
     std::vector<uint8_t> heads(n * kZfpHeadBytes);
     if (n) {
-      NDIC_HIP(hipMemcpyAsync(m, hv.data(), n * sizeof(ZfpHead), hipMemcpyHostToDevice, s));
+      ctx.up(m, hv.data(), n * sizeof(ZfpHead), s);
       launch_zfp_heads((ZfpHead*)m, (uint32_t)n, m + sz_h, s);
-      NDIC_HIP(hipMemcpyAsync(heads.data(), m + sz_h, heads.size(), hipMemcpyDeviceToHost, s));
+      const uint8_t* got = (const uint8_t*)ctx.down(m + sz_h, heads.size(), s);
       NDIC_HIP(hipStreamSynchronize(s));
+      std::memcpy(heads.data(), got, heads.size());
     }
     for (size_t i = 0; i < n; ++i) {
       const ndic_zfp_brick_desc& d = descs[i];
@@ -878,8 +881,8 @@ extern "C" ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_ou
       if (ch.empty()) continue;
       uint64_t sz_ch = al256(ch.size() * sizeof(ZfpChunk));
       uint8_t* w = (uint8_t*)ctx.work.ensure(sz_ch + br.size() * sizeof(ZfpBrick));
-      NDIC_HIP(hipMemcpyAsync(w, ch.data(), ch.size() * sizeof(ZfpChunk), hipMemcpyHostToDevice, s));
-      NDIC_HIP(hipMemcpyAsync(w + sz_ch, br.data(), br.size() * sizeof(ZfpBrick), hipMemcpyHostToDevice, s));
+      ctx.up(w, ch.data(), ch.size() * sizeof(ZfpChunk), s);
+      ctx.up(w + sz_ch, br.data(), br.size() * sizeof(ZfpBrick), s);
       launch_zfp_decode((ZfpChunk*)w, (uint32_t)ch.size(), br.size(), cls / 8, cls % 8, (ZfpBrick*)(w + sz_ch), s);
       NDIC_HIP(hipStreamSynchronize(s));
     }

@@ -104,11 +104,27 @@ void derive_uvlc(uint16_t* tbl, int modes, bool initial) {
 // ---------------------------------------------------------------------------
 // device: stream readers (mel.rs:14-173, streams.rs:20-265)
 // ---------------------------------------------------------------------------
+// Segment reads: the segment is in global memory, or (one-block-per-wave
+// decode) staged in LDS, where a word is read as two aligned words.
+typedef const __attribute__((address_space(1))) uint32_t* gwords;
 __device__ __forceinline__ uint32_t ld32(const uint8_t* p) {
-  uint32_t v;
-  __builtin_memcpy(&v, p, 4);
-  return v;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const bool in_lds = __builtin_amdgcn_is_shared((const void*)p);
+#else
+  const bool in_lds = false;  // host pass: never executed
+#endif
+  if (in_lds) {
+    const uint32_t* a = (const uint32_t*)((uintptr_t)p & ~(uintptr_t)3);
+    const uint32_t o = (uint32_t)((uintptr_t)p & 3);
+    const uint32_t lo = a[0];
+    return o ? __builtin_amdgcn_alignbyte(a[1], lo, o) : lo;
+  }
+  return *(gwords)(uintptr_t)p;
 }
+__device__ __forceinline__ uint32_t ld8(const uint8_t* p) { return *p; }
+__device__ __forceinline__ void st8(uint8_t* p, uint8_t v) { *(__attribute__((address_space(1))) uint8_t*)(uintptr_t)p = v; }
+__device__ __forceinline__ void st32(void* p, uint32_t v) { *(__attribute__((address_space(1))) uint32_t*)(uintptr_t)p = v; }
+__device__ __forceinline__ int32_t ldi32(const int32_t* p) { return *(const __attribute__((address_space(1))) int32_t*)(uintptr_t)p; }
 
 struct MelDec {
   const uint8_t* data;
@@ -136,13 +152,13 @@ struct MelDec {
     } else if (size > 0) {
       uint32_t i = 0;
       while (size > 1) {
-        uint32_t v = *data++;
+        uint32_t v = ld8(data++);
         uint32_t m = ~(0xFFu << i);
         val = (val & m) | (v << i);
         size -= 1;
         i += 8;
       }
-      uint32_t v = uint32_t(*data++) | 0xF;
+      uint32_t v = ld8(data++) | 0xF;
       uint32_t m = ~(0xFFu << i);
       val = (val & m) | (v << i);
       size -= 1;
@@ -206,7 +222,7 @@ struct RevDec {
   bool unstuff;
   __device__ void init_vlc(const uint8_t* coded, uint32_t lcup, uint32_t scup) {
     data = coded;
-    uint8_t d = coded[lcup - 2];
+    uint8_t d = (uint8_t)ld8(coded + lcup - 2);
     tmp = d >> 4;
     pos = (long)lcup - 3;
     size = (int)scup - 2;
@@ -233,7 +249,7 @@ struct RevDec {
     } else if (size > 0) {
       int i = 24;
       while (size > 0 && i >= 0) {
-        uint32_t v = pos >= 0 ? data[pos] : 0;
+        uint32_t v = pos >= 0 ? ld8(data + pos) : 0;
         pos -= 1;
         val |= v << i;
         size -= 1;
@@ -294,7 +310,7 @@ template <uint32_t FILL> struct FwdDec {
     } else if (size > 0) {
       uint32_t v = FILL, i = 0;
       while (size > 0) {
-        uint32_t b = *data++;
+        uint32_t b = ld8(data++);
         uint32_t m = ~(0xFFu << i);
         v = (v & m) | (b << i);
         size -= 1;
@@ -338,8 +354,15 @@ __device__ void report(const HtDecTask& t, int32_t code) {
   if (t.err) atomicCAS(t.err, 0, code);
 }
 
-constexpr int kRowCap = 34;  // quads per row (64 columns) + terminators
-constexpr int kStageCols = 16;   // output columns staged per lane before a wave-wide flush
+constexpr int kRowCap = 34;
+constexpr uint32_t kHtClasses = 192;   // size classes of the decode order (170 used)
+constexpr uint32_t kSoloSegCap = 8192; // cleanup-segment bytes a solo decode stages in LDS
+constexpr uint32_t kOrderBlock = 256;  // quads per row (64 columns) + terminators
+#ifndef NDIC_HT_STAGE_COLS
+#define NDIC_HT_STAGE_COLS 8
+#endif
+// output columns staged per lane before a wave-wide flush (4, 8, 16 or 32)
+constexpr uint32_t kStageCols = NDIC_HT_STAGE_COLS;
 constexpr int kStagePitch = 65;  // LDS pitch of a staged sample row (bank-conflict free)
 
 // Per-lane decoder of one code-block's cleanup pass (decode.rs:54-396).  The
@@ -377,7 +400,7 @@ struct BlockDec {
     refine = num_passes > 1;
     const uint8_t* coded = t.seg;
     const uint32_t lcup = t.len1;
-    const uint32_t scup = (uint32_t(coded[lcup - 1]) << 4) + uint32_t(coded[lcup - 2] & 0xF);
+    const uint32_t scup = (ld8(coded + lcup - 1) << 4) + (ld8(coded + lcup - 2) & 0xF);
     if (scup < 2 || scup > lcup || scup > 4079) {
       report(t, kErrBlock);
       return;
@@ -520,38 +543,138 @@ struct BlockDec {
   }
 };
 
-// Wave-wide store of the staged window (columns xbase..xbase+15 of rows y,
-// y+1) of every lane's block: each store instruction writes 16-column row
-// segments of two blocks, so lines are written whole instead of 4 bytes per
-// lane per instruction.
+// Wave-wide store of the staged window (columns xbase..xbase+kStageCols-1
+// of rows y, y+1) of every lane's block: each store instruction writes
+// kStageCols-column row segments of 32 / kStageCols blocks, so lines are
+// written in runs instead of 4 bytes per lane per instruction.
 __device__ __forceinline__ void flush_stage(const uint32_t* stage_all, uint64_t my_out, uint32_t my_stride,
                                             uint32_t my_w, uint32_t my_h, uint32_t y, uint32_t xbase) {
+  constexpr uint32_t kSeg = 2 * kStageCols, kPer = 64 / kSeg;  // lanes per block, blocks per store
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t r = (lane >> 4) & 1, col = lane & 15;
+  const uint32_t r = (lane / kStageCols) & 1, col = lane % kStageCols;
 #pragma unroll 4
-  for (uint32_t L0 = 0; L0 < 64; L0 += 2) {
-    uint32_t L = L0 + (lane >> 5);
+  for (uint32_t L0 = 0; L0 < 64; L0 += kPer) {
+    uint32_t L = L0 + lane / kSeg;
     uint64_t out = __shfl(my_out, (int)L);
     uint32_t stride = __shfl(my_stride, (int)L), w = __shfl(my_w, (int)L), h = __shfl(my_h, (int)L);
     uint32_t gx = xbase + col, gy = y + r;
     if (out && gx < w && gy < h)
-      ((uint32_t*)out)[(uint64_t)gy * stride + gx] = stage_all[(r * kStageCols + col) * kStagePitch + L];
+      st32((uint32_t*)out + (uint64_t)gy * stride + gx, stage_all[(r * kStageCols + col) * kStagePitch + L]);
   }
 }
 
 __device__ void fill_zero(const HtDecTask& t) {
   for (uint32_t y = 0; y < t.h; ++y)
-    for (uint32_t x = 0; x < t.w; ++x) ((uint32_t*)t.out)[(uint64_t)y * t.stride + x] = 0;
+    for (uint32_t x = 0; x < t.w; ++x) st32((uint32_t*)t.out + (uint64_t)y * t.stride + x, 0);
 }
 
-__global__ __launch_bounds__(64) void ht_decode_kernel(const HtDecTask* __restrict__ tasks, uint64_t n) {
+// ---------------------------------------------------------------------------
+// Decode order: a wave runs its row-pair / column-window loop to its largest
+// block, so code-blocks are grouped by size class (rows in steps of 4,
+// columns in steps of 16), largest first, with the blocks that carry no data
+// (all zero) last.  A counting sort: per-class counts, their exclusive sum,
+// then each workgroup reserves a run per class and fills it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ht_size_class(const HtDecTask& t) {
+  uint32_t h = t.h > 64 ? 64 : t.h, w = t.w > 64 ? 64 : t.w;
+  uint32_t c = (16 - ((h + 3) >> 2)) * 5 + (4 - ((w + 15) >> 4));
+  return (t.flags & kTaskIncluded) ? c : 85 + c;
+}
+
+__global__ void ht_zero_counts(unsigned long long* counts) {
+  for (uint32_t c = threadIdx.x; c < 2 * kHtClasses; c += blockDim.x) counts[c] = 0;
+}
+
+__global__ __launch_bounds__(kOrderBlock) void ht_class_count(const HtDecTask* __restrict__ tasks, uint64_t n,
+                                                             unsigned long long* counts) {
+  __shared__ uint32_t hist[kHtClasses];
+  for (uint32_t c = threadIdx.x; c < kHtClasses; c += blockDim.x) hist[c] = 0;
+  __syncthreads();
+  uint64_t i = blockIdx.x * (uint64_t)kOrderBlock + threadIdx.x;
+  if (i < n) atomicAdd(&hist[ht_size_class(tasks[i])], 1u);
+  __syncthreads();
+  for (uint32_t c = threadIdx.x; c < kHtClasses; c += blockDim.x)
+    if (hist[c]) atomicAdd(&counts[c], (unsigned long long)hist[c]);
+}
+
+// counts[c] -> start of class c (counts[kHtClasses + c] becomes its cursor)
+__global__ void ht_class_scan(unsigned long long* counts) {
+  if (threadIdx.x == 0) {
+    unsigned long long run = 0;
+    for (uint32_t c = 0; c < kHtClasses; ++c) {
+      unsigned long long k = counts[c];
+      counts[kHtClasses + c] = run;
+      run += k;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kOrderBlock) void ht_class_scatter(const HtDecTask* __restrict__ tasks, uint64_t n,
+                                                               unsigned long long* counts, uint32_t* order) {
+  __shared__ uint32_t hist[kHtClasses];
+  __shared__ unsigned long long base[kHtClasses];
+  for (uint32_t c = threadIdx.x; c < kHtClasses; c += blockDim.x) hist[c] = 0;
+  __syncthreads();
+  uint64_t i = blockIdx.x * (uint64_t)kOrderBlock + threadIdx.x;
+  uint32_t cls = 0, rank = 0;
+  if (i < n) {
+    cls = ht_size_class(tasks[i]);
+    rank = atomicAdd(&hist[cls], 1u);
+  }
+  __syncthreads();
+  for (uint32_t c = threadIdx.x; c < kHtClasses; c += blockDim.x)
+    if (hist[c]) base[c] = atomicAdd(&counts[kHtClasses + c], (unsigned long long)hist[c]);
+  __syncthreads();
+  if (i < n) order[base[cls] + rank] = (uint32_t)i;
+}
+
+// Solo: one code-block per wave (small batches, where the latency of one
+// block matters): the wave first copies the block's cleanup segment into LDS,
+// then lane 0 decodes it from there, so its stream reads cost LDS latency
+// instead of a global round trip each.  Otherwise lane l of wave w decodes
+// block 64 w + l (in `order`).
+template <bool Solo>
+__global__ __launch_bounds__(64) void ht_decode_kernel(const HtDecTask* __restrict__ tasks,
+                                                      const uint32_t* __restrict__ order, uint64_t n) {
   __shared__ uint8_t lrho[kRowCap * 64];
   __shared__ uint8_t lbl[kRowCap * 64];
   __shared__ uint32_t stage[2 * kStageCols * kStagePitch];
+  __shared__ uint32_t segbuf[Solo ? kSoloSegCap / 4 : 1];
   const uint32_t lane = threadIdx.x;
-  uint64_t i = blockIdx.x * (uint64_t)64 + lane;
+  uint64_t i;
   HtDecTask task{};
-  if (i < n) task = tasks[i];
+  if (Solo) {
+    i = blockIdx.x;
+    const HtDecTask t = tasks[i];
+    const bool staged = (t.flags & kTaskIncluded) && t.len1 <= kSoloSegCap;
+    if (staged) {
+      const uint32_t len = t.len1;
+#pragma unroll 8
+      for (uint32_t off = lane * 4; off < len; off += 256) {
+        uint32_t v;
+        if (off + 4 <= len) {
+          v = *(gwords)(uintptr_t)(t.seg + off);
+        } else {
+          v = 0;
+          for (uint32_t b = 0; off + b < len; ++b) v |= uint32_t(t.seg[off + b]) << (8 * b);
+        }
+        segbuf[off / 4] = v;
+      }
+    }
+    __syncthreads();
+    if (lane == 0) {
+      task = t;
+      if (staged) task.seg = (const uint8_t*)segbuf;
+    } else {
+      i = n;  // the other lanes carry no block
+    }
+  } else {
+    i = blockIdx.x * (uint64_t)64 + lane;
+    if (i < n) {
+      if (order) i = order[i];
+      task = tasks[i];
+    }
+  }
   for (int k = 0; k < kRowCap; ++k) {
     lrho[k * 64 + lane] = 0;
     lbl[k * 64 + lane] = 0;
@@ -784,7 +907,7 @@ struct MelEnc {
     tmp = (tmp << 1) + v;
     remaining -= 1;
     if (remaining == 0) {
-      if (len < kMelCap) buf[len] = (uint8_t)tmp;
+      if (len < kMelCap) st8(buf + len, (uint8_t)tmp);
       len += 1;
       remaining = tmp == 0xFF ? 7 : 8;
       tmp = 0;
@@ -819,7 +942,7 @@ struct VlcEnc {
   bool gt8f;
   __device__ void init(uint8_t* b) {
     buf = b;
-    buf[0] = 0xFF;
+    st8(buf, 0xFF);
     len = 1;
     tmp = 0xF;
     used = 4;
@@ -839,7 +962,7 @@ struct VlcEnc {
           gt8f = false;
           continue;
         }
-        if (len < kVlcCap) buf[len] = (uint8_t)tmp;
+        if (len < kVlcCap) st8(buf + len, (uint8_t)tmp);
         len += 1;
         gt8f = tmp > 0x8F;
         tmp = 0;
@@ -869,7 +992,7 @@ struct MsEnc {
       cwd >>= t;
       cwd_len -= t;
       if (used >= max_bits) {
-        if (len < cap) buf[len] = (uint8_t)tmp;
+        if (len < cap) st8(buf + len, (uint8_t)tmp);
         len += 1;
         max_bits = tmp == 0xFF ? 7 : 8;
         tmp = 0;
@@ -883,7 +1006,7 @@ struct MsEnc {
       tmp |= (0xFFu & ((1u << t) - 1)) << used;
       used += t;
       if (tmp != 0xFF) {
-        if (len < cap) buf[len] = (uint8_t)tmp;
+        if (len < cap) st8(buf + len, (uint8_t)tmp);
         len += 1;
       }
     } else if (max_bits == 7) {
@@ -909,7 +1032,7 @@ __device__ __forceinline__ EQuad load_quad(const HtEncTask& t, uint32_t x, uint3
     if (t.flags & kTaskSignMag) {
       raw = ((const uint32_t*)t.in)[(uint64_t)sy * t.stride + sx];
     } else {
-      int32_t v = t.in[(uint64_t)sy * t.stride + sx];
+      int32_t v = ldi32(t.in + (uint64_t)sy * t.stride + sx);
       raw = (v < 0 ? 0x80000000u : 0u) | ((v < 0 ? 0u - (uint32_t)v : (uint32_t)v) << shift);
     }
     uint32_t val = ((raw + raw) >> p) & ~1u;
@@ -1161,9 +1284,35 @@ void ht_init_tables() {
   done = true;
 }
 
-void launch_ht_decode(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream) {
+void launch_ht_decode(const HtDecTask* d_tasks, const uint32_t* d_order, uint64_t n, hipStream_t stream) {
   if (n == 0) return;
-  hipLaunchKernelGGL(ht_decode_kernel, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, n);
+  hipLaunchKernelGGL(ht_decode_kernel<false>, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, d_order,
+                     n);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_ht_decode_solo(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream) {
+  if (n == 0) return;
+  if (n > 0xFFFFFFFFull) fail(NDIC_INVALID_ARGUMENT, "too many code-blocks for one solo decode launch");
+  hipLaunchKernelGGL(ht_decode_kernel<true>, dim3((uint32_t)n), dim3(64), 0, stream, d_tasks, nullptr, n);
+  NDIC_HIP(hipGetLastError());
+}
+
+// scratch: [class counts and cursors, 2 * kHtClasses u64][order, n u32]; the
+// 64-bit counters come first so that they are 8-byte aligned for the atomics
+size_t ht_order_scratch_bytes(uint64_t n) { return 2 * kHtClasses * 8 + (size_t)n * 4; }
+size_t ht_order_offset() { return 2 * kHtClasses * 8; }
+
+void launch_ht_order(const HtDecTask* d_tasks, uint64_t n, void* scratch, hipStream_t stream) {
+  if (n == 0) return;
+  if ((uintptr_t)scratch % 8) fail(NDIC_INVALID_ARGUMENT, "decode-order scratch must be 8-byte aligned");
+  unsigned long long* counts = (unsigned long long*)scratch;
+  uint32_t* order = (uint32_t*)((uint8_t*)scratch + 2 * kHtClasses * 8);
+  uint32_t grid = (uint32_t)ceil_div(n, kOrderBlock);
+  hipLaunchKernelGGL(ht_zero_counts, dim3(1), dim3(256), 0, stream, counts);
+  hipLaunchKernelGGL(ht_class_count, dim3(grid), dim3(kOrderBlock), 0, stream, d_tasks, n, counts);
+  hipLaunchKernelGGL(ht_class_scan, dim3(1), dim3(256), 0, stream, counts);
+  hipLaunchKernelGGL(ht_class_scatter, dim3(grid), dim3(kOrderBlock), 0, stream, d_tasks, n, counts, order);
   NDIC_HIP(hipGetLastError());
 }
 

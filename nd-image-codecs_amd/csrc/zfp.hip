@@ -19,6 +19,8 @@
 // reversible variant (Lorenzo lifting, precision prefix, float reinterpret).
 #include "zfp.hpp"
 
+#include <mutex>
+
 #include <hipcub/hipcub.hpp>
 
 namespace ndic {
@@ -224,20 +226,26 @@ struct BitWriter {
   }
 };
 
-struct BitReader {
+// Stream words straight from global memory (the parallel block decoders).
+struct GlobalSrc {
   const uint8_t* base;
   uint64_t nbytes;
-  uint64_t next;
-  uint64_t cur;
-  uint32_t left;
-
-  __device__ uint64_t load(uint64_t w) const {
+  __device__ uint64_t load(uint64_t w) {
     uint64_t b = w * 8;
     if (b + 8 <= nbytes) return *(const uint64_t*)(base + b);
     uint64_t v = 0;
     for (uint64_t i = b; i < nbytes && i < b + 8; ++i) v |= (uint64_t)base[i] << (8 * (i - b));
     return v;
   }
+};
+
+template <class Src> struct BitReaderT {
+  Src src;
+  uint64_t next;
+  uint64_t cur;
+  uint32_t left;
+
+  __device__ uint64_t load(uint64_t w) { return src.load(w); }
   __device__ void seek(uint64_t bit) {
     next = bit >> 6;
     uint32_t o = (uint32_t)(bit & 63);
@@ -283,6 +291,7 @@ struct BitReader {
   }
   __device__ void skip(uint64_t n) { seek(pos() + n); }
 };
+using BitReader = BitReaderT<GlobalSrc>;
 
 // ---------------------------------------------------------------------------
 // embedded bit-plane coder over ranked coefficients
@@ -346,8 +355,8 @@ __device__ uint32_t encode_ints(BitWriter& w, uint32_t maxbits, uint32_t maxprec
   return maxbits - bits;
 }
 
-template <int D, class UInt, bool Parse>
-__device__ uint32_t decode_ints(BitReader& r, uint32_t maxbits, uint32_t maxprec, UInt* u) {
+template <int D, class UInt, bool Parse, class Rd>
+__device__ uint32_t decode_ints(Rd& r, uint32_t maxbits, uint32_t maxprec, UInt* u) {
   constexpr int R = D == 4 ? 4 : 1;
   constexpr uint32_t S = 1u << (2 * D);
   constexpr uint32_t intprec = sizeof(UInt) * 8;
@@ -447,8 +456,8 @@ __device__ uint32_t rev_encode_int_block(BitWriter& w, int minbits, int maxbits,
   return (uint32_t)bits;
 }
 
-template <int D, class Int, bool Parse>
-__device__ uint32_t decode_int_block(BitReader& r, int minbits, int maxbits, int maxprec, bool rev, Int* v) {
+template <int D, class Int, bool Parse, class Rd>
+__device__ uint32_t decode_int_block(Rd& r, int minbits, int maxbits, int maxprec, bool rev, Int* v) {
   using UInt = typename std::make_unsigned<Int>::type;
   constexpr int R = D == 4 ? 4 : 1;
   constexpr int pbits = sizeof(Int) == 4 ? 5 : 6;
@@ -608,7 +617,7 @@ template <int D, class S> __device__ uint32_t encode_float_block(BitWriter& w, c
   return 1;
 }
 
-template <int D, class S, bool Parse> __device__ uint32_t decode_float_block(BitReader& r, const ZfpChunk& c, S* f) {
+template <int D, class S, bool Parse, class Rd> __device__ uint32_t decode_float_block(Rd& r, const ZfpChunk& c, S* f) {
   using Int = typename ZT<S>::Int;
   using U = typename ZT<S>::UInt;
   constexpr int R = D == 4 ? 4 : 1;
@@ -673,7 +682,7 @@ template <int D, class S> __device__ uint32_t encode_block(BitWriter& w, const Z
   }
 }
 
-template <int D, class S, bool Parse> __device__ uint32_t decode_block(BitReader& r, const ZfpChunk& c, S* vals) {
+template <int D, class S, bool Parse, class Rd> __device__ uint32_t decode_block(Rd& r, const ZfpChunk& c, S* vals) {
   if constexpr (ZT<S>::fp) {
     return decode_float_block<D, S, Parse>(r, c, vals);
   } else {
@@ -879,8 +888,8 @@ __global__ __launch_bounds__(64) void zfp_index_kernel(const ZfpChunk* __restric
   const ZfpChunk& c = chunks[ci];
   if (c.fixed) return;
   BitReader r;
-  r.base = c.src;
-  r.nbytes = c.src_len;
+  r.src.base = c.src;
+  r.src.nbytes = c.src_len;
   r.seek(c.header_bits);
   S vals[R];
   for (uint64_t k = 0; k < c.nblocks; ++k) {
@@ -908,8 +917,8 @@ __global__ __launch_bounds__(256) void zfp_decode_kernel(const ZfpChunk* __restr
   }
   const ZfpChunk& c = chunks[ci];
   BitReader r;
-  r.base = c.src;
-  r.nbytes = c.src_len;
+  r.src.base = c.src;
+  r.src.nbytes = c.src_len;
   r.seek(c.fixed ? c.header_bits + k * c.maxbits : c.offs[c.block_begin + k]);
   S vals[R];
   decode_block<D, S, false>(r, c, vals);
@@ -1043,7 +1052,9 @@ void launch_zfp_heads(const ZfpHead* d_heads, uint32_t n, uint8_t* d_out, hipStr
 }
 
 void zfp_init_tables() {
+  static std::mutex mu;
   static bool done = false;
+  std::lock_guard<std::mutex> lock(mu);
   if (done) return;
   uint8_t perm[4][256] = {}, iperm[4][256] = {};
   for (int i = 0; i < 4; ++i) perm[0][i] = (uint8_t)i;

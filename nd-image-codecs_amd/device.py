@@ -40,6 +40,7 @@ def _lib():
             ("ndic_dev_synth_chunk", C.c_int, [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32,
                                                C.POINTER(SynthBlobs), C.c_uint64, E]),
             ("ndic_dev_count_mismatch", C.c_int, [P, P, S, C.POINTER(C.c_uint64), E]),
+            ("ndic_dev_count_mismatch_async", C.c_int, [P, P, S, P, E]),
             ("ndic_dev_copy", C.c_int, [P, P, S, E]),
             ("ndic_dev_profile", None, [C.c_int32]),
             ("ndic_dev_stage_times", C.c_size_t, [C.POINTER(C.c_double), C.POINTER(C.c_double), S]),
@@ -131,6 +132,12 @@ def count_mismatch(a: int, b: int, n: int) -> int:
     out = C.c_uint64()
     _call(_lib().ndic_dev_count_mismatch, a, b, int(n), C.byref(out))
     return out.value
+
+
+def count_mismatch_async(a: int, b: int, n: int, d_count: int) -> None:
+    """Adds the mismatch count of a and b to the device counter at d_count
+    on this thread's stream, without waiting."""
+    _call(_lib().ndic_dev_count_mismatch_async, a, b, int(n), d_count)
 
 
 def profile(enable: bool) -> None:

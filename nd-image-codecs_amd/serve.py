@@ -153,12 +153,15 @@ class Stats:
     wall_s: float = 0.0
     call_ms: Dict[str, float] = field(default_factory=dict)   # device-batch call time per kind
     calls: Dict[str, int] = field(default_factory=dict)
+    items: Dict[str, int] = field(default_factory=dict)       # requests per kind over all calls
+    pass_ms: List[float] = field(default_factory=list)        # wall time of each decode pass
     lock: threading.Lock = field(default_factory=threading.Lock)
 
-    def record(self, kind: str, ms: float):
+    def record(self, kind: str, ms: float, n: int = 1):
         with self.lock:
             self.call_ms[kind] = self.call_ms.get(kind, 0.0) + ms
             self.calls[kind] = self.calls.get(kind, 0) + 1
+            self.items[kind] = self.items.get(kind, 0) + n
 
     def summary(self):
         allv = sorted(v for vs in self.latencies_ms.values() for v in vs)
@@ -170,8 +173,10 @@ class Stats:
 
         out = {"requests": len(allv), "passes": self.passes, "p50_ms": pct(allv, 50), "p99_ms": pct(allv, 99),
                "GBps": round(self.bytes_out / self.wall_s / 1e9, 3) if self.wall_s else None,
+               "pass_ms": {"p50": pct(self.pass_ms, 50), "p99": pct(self.pass_ms, 99)},
                "by_kind": {k: {"n": len(v), "p50_ms": pct(sorted(v), 50), "p99_ms": pct(sorted(v), 99),
-                               "mean_call_ms": round(self.call_ms.get(k, 0.0) / max(1, self.calls.get(k, 0)), 3)}
+                               "mean_call_ms": round(self.call_ms.get(k, 0.0) / max(1, self.calls.get(k, 0)), 3),
+                               "mean_batch": round(self.items.get(k, 0) / max(1, self.calls.get(k, 0)), 2)}
                            for k, v in self.latencies_ms.items()}}
         return out
 
@@ -235,14 +240,14 @@ class Server:
             _pkg._check(lib.ndic_htj2k_decode_plane_batch(C.c_void_p(s.ht_arena.ptr), C.c_void_p(self.out.ptr), arr,
                                                           len(thumbs), None, C.byref(err)), err)
             if stats:
-                stats.record(THUMB, (tick() - t0) * 1e3)
+                stats.record(THUMB, (tick() - t0) * 1e3, len(thumbs))
         if decs:
             t0 = tick()
             descs = [_pkg.make_desc(s.ht[r.key].shape, s.ht[r.key].dtype, s.ht[r.key].steps, s.ht[r.key].levels,
                                     s.ht[r.key].offset, s.ht[r.key].length, r.out_offset) for r in decs]
             _pkg.decode_batch(s.ht_arena.ptr, self.out.ptr, descs)
             if stats:
-                stats.record(DECODE, (tick() - t0) * 1e3)
+                stats.record(DECODE, (tick() - t0) * 1e3, len(decs))
         if bricks:
             t0 = tick()
             arr = (BrickDesc * len(bricks))()
@@ -260,10 +265,11 @@ class Server:
             _pkg._check(lib.ndic_zfp_decode_bricks(C.c_void_p(s.zfp_arena.ptr), C.c_void_p(self.out.ptr), arr,
                                                    len(bricks), None, C.byref(err)), err)
             if stats:
-                stats.record(BRICK, (tick() - t0) * 1e3)
+                stats.record(BRICK, (tick() - t0) * 1e3, len(bricks))
         if encs:
             self._run_encodes(encs, stats)
-        dev.synchronize()
+        # every batch call above returns with its work done on this thread's
+        # stream; the encoder thread's stream is not waited for
 
     def _run_encodes(self, encs: List[Request], stats: Optional[Stats] = None):
         if not self.encode_sources:
@@ -282,7 +288,7 @@ class Server:
         for ptr, ds in by_src.items():
             _pkg.encode_batch(ptr, self.enc_out.ptr, self.enc_out.nbytes, ds)
         if stats:
-            stats.record(ENCODE, (time.perf_counter() - t0) * 1e3)
+            stats.record(ENCODE, (time.perf_counter() - t0) * 1e3, len(encs))
 
     def run(self, requests: List[Request], verify=None) -> Stats:
         """Serves requests (sorted by arrival, seconds from start) in real time.
@@ -349,9 +355,11 @@ class Server:
                     batch.append(r)
                     used += b
                 pending = pending[len(batch):]
+                tp = time.perf_counter()
                 self._run_pass(batch, stats)
                 t = time.perf_counter() - t0
                 with stats.lock:
+                    stats.pass_ms.append((t + t0 - tp) * 1e3)
                     stats.passes += 1
                     for r in batch:
                         r.done = t
