@@ -399,6 +399,56 @@ __device__ bool parse_tile(const DevChunk& ch, const uint8_t* cs, uint64_t len, 
   return true;
 }
 
+__device__ __forceinline__ uint32_t find_chunk_task(const DevChunk* c, uint32_t n, uint64_t task) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) / 2;
+    if (c[mid].task_base <= task) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Every code-block task of a decode window starts as "skip" (resolutions
+// past max_res: w = h = 0) or "zero fill" (not included, until the packet
+// header parse marks it), one thread per task.
+__global__ __launch_bounds__(256) void init_tasks_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks,
+                                                         uint64_t ntasks, HtDecTask* tasks,
+                                                         const GeoPacket* __restrict__ gp,
+                                                         const GeoBand* __restrict__ gb) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= ntasks) return;
+  const DevChunk& ch = chunks[find_chunk_task(chunks, nchunks, i)];
+  const uint64_t local = i - ch.task_base;
+  if (!ch.blocks_per_plane || local >= (uint64_t)ch.P * ch.blocks_per_plane) return;
+  const uint64_t p = local / ch.blocks_per_plane;
+  const uint32_t k = (uint32_t)(local % ch.blocks_per_plane);
+  for (uint32_t r = 0; r <= ch.L; ++r) {
+    const GeoPacket& pk = gp[ch.geo_packet + r];
+    for (uint32_t b = 0; b < pk.nbands; ++b) {
+      const GeoBand& bd = gb[ch.geo_band + pk.band_first + b];
+      if (k < bd.block_first || k >= bd.block_first + bd.nx * bd.ny) continue;
+      const uint32_t by = (k - bd.block_first) / bd.nx, bx = (k - bd.block_first) % bd.nx;
+      HtDecTask t;
+      t.seg = nullptr;
+      t.out = ch.coef + p * (uint64_t)ch.W * ch.H + (uint64_t)(bd.y0 + by * 64) * ch.W + bd.x0 + bx * 64;
+      t.err = ch.err;
+      t.stride = ch.W;
+      t.len1 = t.len2 = 0;
+      const bool used = (int32_t)r <= ch.max_res;
+      t.w = used ? (uint16_t)min(64u, bd.w - bx * 64) : 0;
+      t.h = used ? (uint16_t)min(64u, bd.h - by * 64) : 0;
+      t.mmsb = 0;
+      t.passes = 0;
+      t.kmax = 0;
+      t.flags = 0;
+      t.err_offset = 0;
+      tasks[i] = t;
+      return;
+    }
+  }
+}
+
 __global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks,
                                                           uint64_t nplanes, DevPlane* planes, HtDecTask* tasks,
                                                           const GeoPacket* __restrict__ gp,
@@ -409,32 +459,8 @@ __global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __rest
   uint64_t p = gid - ch.plane_base;
   DevPlane& pl = planes[gid];
   pl.ok = 0;
-  // every task of the plane starts as "skip" (w = h = 0) or "zero fill"
+  // the plane's tasks were set to "skip" / "zero fill" by init_tasks_kernel
   HtDecTask* pt = tasks + ch.task_base + p * ch.blocks_per_plane;
-  int32_t* plane = ch.coef + p * (uint64_t)ch.W * ch.H;
-  for (uint32_t r = 0; r <= ch.L; ++r) {
-    const GeoPacket& pk = gp[ch.geo_packet + r];
-    for (uint32_t b = 0; b < pk.nbands; ++b) {
-      const GeoBand& bd = gb[ch.geo_band + pk.band_first + b];
-      for (uint32_t by = 0; by < bd.ny; ++by)
-        for (uint32_t bx = 0; bx < bd.nx; ++bx) {
-          HtDecTask& t = pt[bd.block_first + by * bd.nx + bx];
-          t.seg = nullptr;
-          t.out = plane + (uint64_t)(bd.y0 + by * 64) * ch.W + bd.x0 + bx * 64;
-          t.err = ch.err;
-          t.stride = ch.W;
-          t.len1 = t.len2 = 0;
-          bool used = (int32_t)r <= ch.max_res;
-          t.w = used ? (uint16_t)min(64u, bd.w - bx * 64) : 0;
-          t.h = used ? (uint16_t)min(64u, bd.h - by * 64) : 0;
-          t.mmsb = 0;
-          t.passes = 0;
-          t.kmax = 0;
-          t.flags = 0;
-          t.err_offset = 0;
-        }
-    }
-  }
   if (!locate(ch, p, gid, pl)) return;
   const uint8_t* cs = ch.in + pl.cs_off;
   uint64_t len = pl.cs_len;
@@ -1083,6 +1109,14 @@ void launch_parse_planes(const DevChunk* d_chunks, uint32_t nchunks, uint64_t np
   if (!nplanes) return;
   hipLaunchKernelGGL(parse_planes_kernel, dim3((uint32_t)ceil_div(nplanes, 64)), dim3(64), 0, stream, d_chunks,
                      nchunks, nplanes, planes, tasks, gp, gb);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_init_tasks(const DevChunk* d_chunks, uint32_t nchunks, uint64_t ntasks, HtDecTask* tasks,
+                       const GeoPacket* gp, const GeoBand* gb, hipStream_t stream) {
+  if (!ntasks) return;
+  hipLaunchKernelGGL(init_tasks_kernel, dim3((uint32_t)ceil_div(ntasks, 256)), dim3(256), 0, stream, d_chunks,
+                     nchunks, ntasks, tasks, gp, gb);
   NDIC_HIP(hipGetLastError());
 }
 

@@ -73,6 +73,10 @@ struct DevChunk {
 
 // Tag-tree scratch bytes a plane needs for a band grid of nx x ny blocks.
 uint32_t tag_scratch_bytes(uint32_t nx, uint32_t ny);
+// Sets every code-block task of the window to "skip" / "zero fill" (before
+// launch_parse_planes, which marks the included ones).
+void launch_init_tasks(const DevChunk* d_chunks, uint32_t nchunks, uint64_t ntasks, HtDecTask* tasks,
+                       const GeoPacket* gp, const GeoBand* gb, hipStream_t stream);
 void launch_parse_planes(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, DevPlane* planes,
                          HtDecTask* tasks, const GeoPacket* gp, const GeoBand* gb, hipStream_t stream);
 void launch_walk_chunks(const DevChunk* d_chunks, uint32_t nchunks, DevPlane* planes, hipStream_t stream);

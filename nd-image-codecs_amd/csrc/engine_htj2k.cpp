@@ -167,7 +167,7 @@ uint64_t planes_of(const std::vector<size_t>& shape) {
 
 // Device memory for one window of a decode batch.
 // Up to kSoloTasks code-blocks a window decodes one block per wavefront
-// (latency); above, a block per lane in size-class order (throughput).
+// (latency); above, one block per lane (throughput).
 constexpr uint64_t kSoloTasks = 8192;
 
 void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s) {
@@ -209,11 +209,7 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
            sz_gb = al(geo.bands.size() * sizeof(GeoBand)), sz_planes = al(nplanes * sizeof(DevPlane)),
            sz_tasks = al(ntasks * sizeof(HtDecTask)), sz_tag = al(nplanes * tag_stride), sz_err = al(n * 16),
            sz_ptr = al(n * sizeof(void*)), sz_dims = al(n * 16), sz_flag = al(n * 4);
-  // large windows decode their code-blocks in size-class order
-  const bool ordered = ntasks > kSoloTasks;
-  uint64_t sz_order = ordered ? al(ht_order_scratch_bytes(ntasks)) : 0;
-  uint64_t meta =
-      sz_chunks + sz_gp + sz_gb + sz_planes + sz_tasks + sz_tag + sz_err + sz_ptr + sz_dims + sz_flag + sz_order;
+  uint64_t meta = sz_chunks + sz_gp + sz_gb + sz_planes + sz_tasks + sz_tag + sz_err + sz_ptr + sz_dims + sz_flag;
   uint8_t* m = (uint8_t*)ctx.work.ensure(meta);
   DevChunk* d_chunks = (DevChunk*)m;
   GeoPacket* d_gp = (GeoPacket*)(m + sz_chunks);
@@ -225,7 +221,6 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   int32_t** d_ptr = (int32_t**)((uint8_t*)d_err + sz_err);
   uint64_t* d_dims = (uint64_t*)((uint8_t*)d_ptr + sz_ptr);
   int32_t* d_flag = (int32_t*)((uint8_t*)d_dims + sz_dims);
-  void* d_order = ordered ? (void*)((uint8_t*)d_flag + sz_flag) : nullptr;
   int32_t* A = (int32_t*)ctx.coef.ensure(al(3 * coef * 4 + 64));
   int32_t* T = A + coef;
   int32_t* O = A + 2 * coef;
@@ -245,14 +240,11 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   StageTimer timer(s);
   timer.mark();
   launch_walk_chunks(d_chunks, (uint32_t)n, d_planes, s);
+  launch_init_tasks(d_chunks, (uint32_t)n, ntasks, d_tasks, d_gp, d_gb, s);
   launch_parse_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_gp, d_gb, s);
   timer.mark();
-  if (ordered) {
-    launch_ht_order(d_tasks, ntasks, d_order, s);
-    launch_ht_decode(d_tasks, (const uint32_t*)((uint8_t*)d_order + ht_order_offset()), ntasks, s);
-  } else {
-    launch_ht_decode_solo(d_tasks, ntasks, s);
-  }
+  if (ntasks > kSoloTasks) launch_ht_decode(d_tasks, nullptr, ntasks, s);
+  else launch_ht_decode_solo(d_tasks, ntasks, s);
   launch_ht_refine(d_tasks, ntasks, s);
   timer.mark();
   std::vector<DwtChunk> dw(n);

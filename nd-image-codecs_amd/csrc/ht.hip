@@ -354,10 +354,8 @@ __device__ void report(const HtDecTask& t, int32_t code) {
   if (t.err) atomicCAS(t.err, 0, code);
 }
 
-constexpr int kRowCap = 34;
-constexpr uint32_t kHtClasses = 192;   // size classes of the decode order (170 used)
-constexpr uint32_t kSoloSegCap = 8192; // cleanup-segment bytes a solo decode stages in LDS
-constexpr uint32_t kOrderBlock = 256;  // quads per row (64 columns) + terminators
+constexpr int kRowCap = 34;  // quads per row (64 columns) + terminators
+constexpr uint32_t kSoloSegCap = 8192;  // cleanup-segment bytes a solo decode stages in LDS
 #ifndef NDIC_HT_STAGE_COLS
 #define NDIC_HT_STAGE_COLS 8
 #endif
@@ -566,66 +564,6 @@ __device__ __forceinline__ void flush_stage(const uint32_t* stage_all, uint64_t 
 __device__ void fill_zero(const HtDecTask& t) {
   for (uint32_t y = 0; y < t.h; ++y)
     for (uint32_t x = 0; x < t.w; ++x) st32((uint32_t*)t.out + (uint64_t)y * t.stride + x, 0);
-}
-
-// ---------------------------------------------------------------------------
-// Decode order: a wave runs its row-pair / column-window loop to its largest
-// block, so code-blocks are grouped by size class (rows in steps of 4,
-// columns in steps of 16), largest first, with the blocks that carry no data
-// (all zero) last.  A counting sort: per-class counts, their exclusive sum,
-// then each workgroup reserves a run per class and fills it.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t ht_size_class(const HtDecTask& t) {
-  uint32_t h = t.h > 64 ? 64 : t.h, w = t.w > 64 ? 64 : t.w;
-  uint32_t c = (16 - ((h + 3) >> 2)) * 5 + (4 - ((w + 15) >> 4));
-  return (t.flags & kTaskIncluded) ? c : 85 + c;
-}
-
-__global__ void ht_zero_counts(unsigned long long* counts) {
-  for (uint32_t c = threadIdx.x; c < 2 * kHtClasses; c += blockDim.x) counts[c] = 0;
-}
-
-__global__ __launch_bounds__(kOrderBlock) void ht_class_count(const HtDecTask* __restrict__ tasks, uint64_t n,
-                                                             unsigned long long* counts) {
-  __shared__ uint32_t hist[kHtClasses];
-  for (uint32_t c = threadIdx.x; c < kHtClasses; c += blockDim.x) hist[c] = 0;
-  __syncthreads();
-  uint64_t i = blockIdx.x * (uint64_t)kOrderBlock + threadIdx.x;
-  if (i < n) atomicAdd(&hist[ht_size_class(tasks[i])], 1u);
-  __syncthreads();
-  for (uint32_t c = threadIdx.x; c < kHtClasses; c += blockDim.x)
-    if (hist[c]) atomicAdd(&counts[c], (unsigned long long)hist[c]);
-}
-
-// counts[c] -> start of class c (counts[kHtClasses + c] becomes its cursor)
-__global__ void ht_class_scan(unsigned long long* counts) {
-  if (threadIdx.x == 0) {
-    unsigned long long run = 0;
-    for (uint32_t c = 0; c < kHtClasses; ++c) {
-      unsigned long long k = counts[c];
-      counts[kHtClasses + c] = run;
-      run += k;
-    }
-  }
-}
-
-__global__ __launch_bounds__(kOrderBlock) void ht_class_scatter(const HtDecTask* __restrict__ tasks, uint64_t n,
-                                                               unsigned long long* counts, uint32_t* order) {
-  __shared__ uint32_t hist[kHtClasses];
-  __shared__ unsigned long long base[kHtClasses];
-  for (uint32_t c = threadIdx.x; c < kHtClasses; c += blockDim.x) hist[c] = 0;
-  __syncthreads();
-  uint64_t i = blockIdx.x * (uint64_t)kOrderBlock + threadIdx.x;
-  uint32_t cls = 0, rank = 0;
-  if (i < n) {
-    cls = ht_size_class(tasks[i]);
-    rank = atomicAdd(&hist[cls], 1u);
-  }
-  __syncthreads();
-  for (uint32_t c = threadIdx.x; c < kHtClasses; c += blockDim.x)
-    if (hist[c]) base[c] = atomicAdd(&counts[kHtClasses + c], (unsigned long long)hist[c]);
-  __syncthreads();
-  if (i < n) order[base[cls] + rank] = (uint32_t)i;
 }
 
 // Solo: one code-block per wave (small batches, where the latency of one
@@ -1295,24 +1233,6 @@ void launch_ht_decode_solo(const HtDecTask* d_tasks, uint64_t n, hipStream_t str
   if (n == 0) return;
   if (n > 0xFFFFFFFFull) fail(NDIC_INVALID_ARGUMENT, "too many code-blocks for one solo decode launch");
   hipLaunchKernelGGL(ht_decode_kernel<true>, dim3((uint32_t)n), dim3(64), 0, stream, d_tasks, nullptr, n);
-  NDIC_HIP(hipGetLastError());
-}
-
-// scratch: [class counts and cursors, 2 * kHtClasses u64][order, n u32]; the
-// 64-bit counters come first so that they are 8-byte aligned for the atomics
-size_t ht_order_scratch_bytes(uint64_t n) { return 2 * kHtClasses * 8 + (size_t)n * 4; }
-size_t ht_order_offset() { return 2 * kHtClasses * 8; }
-
-void launch_ht_order(const HtDecTask* d_tasks, uint64_t n, void* scratch, hipStream_t stream) {
-  if (n == 0) return;
-  if ((uintptr_t)scratch % 8) fail(NDIC_INVALID_ARGUMENT, "decode-order scratch must be 8-byte aligned");
-  unsigned long long* counts = (unsigned long long*)scratch;
-  uint32_t* order = (uint32_t*)((uint8_t*)scratch + 2 * kHtClasses * 8);
-  uint32_t grid = (uint32_t)ceil_div(n, kOrderBlock);
-  hipLaunchKernelGGL(ht_zero_counts, dim3(1), dim3(256), 0, stream, counts);
-  hipLaunchKernelGGL(ht_class_count, dim3(grid), dim3(kOrderBlock), 0, stream, d_tasks, n, counts);
-  hipLaunchKernelGGL(ht_class_scan, dim3(1), dim3(256), 0, stream, counts);
-  hipLaunchKernelGGL(ht_class_scatter, dim3(grid), dim3(kOrderBlock), 0, stream, d_tasks, n, counts, order);
   NDIC_HIP(hipGetLastError());
 }
 

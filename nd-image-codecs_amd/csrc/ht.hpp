@@ -47,11 +47,6 @@ void launch_ht_decode(const HtDecTask* d_tasks, const uint32_t* d_order, uint64_
 // The same decode with one code-block per wavefront (segment staged in LDS):
 // a lower latency per block, for batches too small to fill the GPU.
 void launch_ht_decode_solo(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
-// Writes the size-class decode order of n tasks to the start of scratch
-// (ht_order_scratch_bytes(n) bytes, 8-byte aligned) for launch_ht_decode.
-size_t ht_order_scratch_bytes(uint64_t n);
-size_t ht_order_offset();  // where the order array starts in that scratch
-void launch_ht_order(const HtDecTask* d_tasks, uint64_t n, void* scratch, hipStream_t stream);
 // SigProp/MagRef of the tasks with more than one coding pass (after launch_ht_decode)
 void launch_ht_refine(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
 void launch_ht_encode(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream);
