@@ -6,6 +6,7 @@
 #include <cstring>
 #include <functional>
 #include <sstream>
+#include <thread>
 
 #include "engine.hpp"
 
@@ -67,20 +68,40 @@ void* Context::down(const void* d, size_t n, hipStream_t s) {
 
 // One context (stream + reusable device buffers) per host thread: calls
 // from different threads run on their own streams and overlap on the GPU;
-// calls from one thread are ordered on its stream.  Contexts live until the
-// process ends (the buffers are reused by the thread's later calls).
+// calls from one thread are ordered on its stream.  A thread's context is
+// released when the thread exits (its buffers are reused by the thread's
+// later calls until then); the thread that loaded the library keeps its
+// context until the process ends, when the runtime may already be gone.
+namespace {
+const std::thread::id g_loader_thread = std::this_thread::get_id();
+
+struct ContextOwner {
+  Context* c = nullptr;
+  ~ContextOwner() {
+    if (!c || std::this_thread::get_id() == g_loader_thread) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (DevBuf* b : {&c->in, &c->out, &c->plane, &c->tmp, &c->coef, &c->work, &c->jobs, &c->dwtjobs, &c->small})
+      b->release();
+    c->pin.reset();
+    if (c->pin.base) (void)hipHostFree(c->pin.base);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+  }
+};
+}  // namespace
+
 Context& Context::get() {
-  static thread_local Context* ctx = nullptr;
-  if (!ctx) {
+  static thread_local ContextOwner owner;
+  if (!owner.c) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
       fail(NDIC_IO, "no HIP device is visible: the nd-image-codecs MI355X path needs a GPU");
     auto* c = new Context();
     NDIC_HIP(hipGetDevice(&c->device));
     NDIC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    ctx = c;
+    owner.c = c;
   }
-  return *ctx;
+  return *owner.c;
 }
 
 bool lift_wide(int dtype) { return dtype == NDIC_U64 || dtype == NDIC_I64; }

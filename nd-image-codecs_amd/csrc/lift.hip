@@ -224,7 +224,8 @@ __global__ __launch_bounds__(256) void lift_tile_kernel(const LiftJob* __restric
   const uint64_t elem0 = o * job.len * job.inner + g0 * job.inner + j;
   T* plane = (T*)job.plane;
   const uint64_t inner = job.inner;
-  // load
+  // load (unrolled: four rows' loads in flight per thread)
+#pragma unroll 4
   for (uint32_t k = grp; k < n; k += 4) {
     T v = 0;
     if (valid) {
@@ -335,13 +336,25 @@ __global__ void convert_kernel(const uint8_t* __restrict__ src, uint32_t src_dty
     if (dst) ok &= store_dtype<T>(dst + i * ds, dst_dtype, v);
   }
   if (minmax) {
-    // wave-level reduction then one atomic per wave
+    // wave, then workgroup reduction: one atomic pair per workgroup (atomics
+    // on one address serialise, so one per wave costs more than the pass)
     for (int off = 32; off > 0; off >>= 1) {
       int64_t a = __shfl_xor(lo, off), b = __shfl_xor(hi, off);
       lo = a < lo ? a : lo;
       hi = b > hi ? b : hi;
     }
+    __shared__ int64_t wlo[4], whi[4];  // blockDim.x == 256: four waves
+    const uint32_t wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+      wlo[wv] = lo;
+      whi[wv] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (uint32_t k = 1; k < blockDim.x / 64; ++k) {
+        lo = wlo[k] < lo ? wlo[k] : lo;
+        hi = whi[k] > hi ? whi[k] : hi;
+      }
       atomicMin((long long*)&minmax[0], (long long)lo);
       atomicMax((long long*)&minmax[1], (long long)hi);
     }
@@ -378,7 +391,7 @@ void launch_lift_step(const LiftJob* d_jobs, uint32_t njobs, uint64_t items, boo
 void launch_convert(const uint8_t* src, uint32_t src_dtype, void* plane, bool wide, uint8_t* dst, uint32_t dst_dtype,
                     uint64_t n, int64_t* minmax, int32_t* flag, hipStream_t stream) {
   if (n == 0) return;
-  uint32_t blocks = (uint32_t)(ceil_div(n, 256) < 4096 ? ceil_div(n, 256) : 4096);
+  uint32_t blocks = (uint32_t)(ceil_div(n, 256) < 1024 ? ceil_div(n, 256) : 1024);
   if (wide)
     hipLaunchKernelGGL(convert_kernel<int64_t>, dim3(blocks), dim3(256), 0, stream, src, src_dtype, (int64_t*)plane,
                        dst, dst_dtype, n, minmax, flag);
