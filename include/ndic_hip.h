@@ -236,6 +236,20 @@ typedef struct {
 ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp_brick_desc* descs, size_t n,
                                    void* stream, ndic_error* err);
 
+/* ---- launch-parameter knobs (no counterpart in the reference) ----------
+ * The autotuner (nd-image-codecs_amd/autotune.py) measures the kernel
+ * variants per shape class and stores its choices here; tests pin them.
+ * Knobs: "ht_decode_wave_max_blocks" -- batch windows of at most this many
+ * code-blocks decode one block per wavefront, larger ones one block per lane.
+ * ndic_tune_set returns 0, -1 for an unknown key, -2 for a value out of
+ * range; ndic_tune_get returns the value in force (-1: unknown key);
+ * ndic_tune_reset restores the built-in default; ndic_tune_name lists the
+ * keys (NULL past the last). */
+int32_t ndic_tune_set(const char* key, int64_t value);
+int64_t ndic_tune_get(const char* key);
+int32_t ndic_tune_reset(const char* key);
+const char* ndic_tune_name(int32_t index);
+
 #ifdef __cplusplus
 }
 #endif

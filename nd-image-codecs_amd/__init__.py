@@ -111,6 +111,10 @@ def _load() -> C.CDLL:
                                             P, E]),
         "ndic_zfp_encode_batch": (C.c_int, [P, P, S, P, S, P, E]),
         "ndic_zfp_decode_batch": (C.c_int, [P, P, P, S, P, E]),
+        "ndic_tune_set": (C.c_int32, [C.c_char_p, C.c_int64]),
+        "ndic_tune_get": (C.c_int64, [C.c_char_p]),
+        "ndic_tune_reset": (C.c_int32, [C.c_char_p]),
+        "ndic_tune_name": (C.c_char_p, [C.c_int32]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name, None)
@@ -426,10 +430,41 @@ def version() -> str:
     return _lib.ndic_version().decode()
 
 
+def tune_set(key: str, value: int) -> None:
+    """Sets a launch-parameter knob of the library (see ndic_tune_set)."""
+    rc = _lib.ndic_tune_set(key.encode(), int(value))
+    if rc == -1:
+        raise KeyError(key)
+    if rc != 0:
+        raise ValueError(f"{key}: value {value} out of range")
+
+
+def tune_get(key: str) -> int:
+    v = _lib.ndic_tune_get(key.encode())
+    if v == -1 and key not in tune_keys():
+        raise KeyError(key)
+    return int(v)
+
+
+def tune_reset(key: str) -> None:
+    if _lib.ndic_tune_reset(key.encode()) != 0:
+        raise KeyError(key)
+
+
+def tune_keys() -> list:
+    out, i = [], 0
+    while True:
+        n = _lib.ndic_tune_name(i)
+        if not n:
+            return out
+        out.append(n.decode())
+        i += 1
+
+
 __all__ = ["htj2k_encode", "htj2k_decode", "htj2k_decode_plane", "forward", "inverse", "lift_forward_bytes",
            "lift_inverse_bytes", "decode_batch", "encode_batch", "make_desc", "nd_zfp_encode", "nd_zfp_decode",
            "nd_zfp_decode_brick", "zfp_config", "make_zfp_desc", "zfp_encode_batch", "zfp_decode_batch",
-           "NdicError", "lib", "version", "LIB_PATH"]
+           "NdicError", "lib", "version", "LIB_PATH", "tune_set", "tune_get", "tune_reset", "tune_keys"]
 
 if os.environ.get("NDIC_VERBOSE"):
     print(f"nd-image-codecs_amd: loaded {LIB_PATH}")

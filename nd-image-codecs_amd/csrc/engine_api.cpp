@@ -12,6 +12,7 @@
 #include "ht.hpp"
 #include "../../include/ndic_hip_dev.h"
 #include "items.hpp"
+#include "tune.hpp"
 
 namespace ndic {
 
@@ -399,7 +400,8 @@ ndic_status ndic_dev_ht_decode_block(const uint8_t* coded, size_t len, uint32_t*
     t.kmax = 31;
     t.flags = kTaskIncluded | kTaskSignMag | (stripe_causal ? kTaskCausal : 0);
     NDIC_HIP(hipMemcpyAsync(d_task, &t, sizeof(t), hipMemcpyHostToDevice, s));
-    launch_ht_decode(d_task, nullptr, 1, s);
+    if (tune(kTuneHtDecodeWaveMaxBlocks) >= 1) launch_ht_decode_wave(d_task, 1, s);
+    else launch_ht_decode(d_task, nullptr, 1, s);
     launch_ht_refine(d_task, 1, s);
     int32_t e = 0;
     NDIC_HIP(hipMemcpyAsync(&e, d_err, 4, hipMemcpyDeviceToHost, s));

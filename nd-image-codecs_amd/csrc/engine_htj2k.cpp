@@ -19,6 +19,7 @@
 #include "engine.hpp"
 #include "ht.hpp"
 #include "items.hpp"
+#include "tune.hpp"
 
 namespace ndic {
 
@@ -165,10 +166,9 @@ uint64_t planes_of(const std::vector<size_t>& shape) {
   return p;
 }
 
-// Device memory for one window of a decode batch.
-// Up to kSoloTasks code-blocks a window decodes one block per wavefront
-// (latency); above, one block per lane (throughput).
-constexpr uint64_t kSoloTasks = 8192;
+// Device memory for one window of a decode batch.  Up to
+// ht_decode_wave_max_blocks code-blocks a window decodes one block per
+// wavefront (latency); above, one block per lane (throughput).
 
 void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s) {
   ctx.release_kept();  // every earlier window of this context ended synchronised
@@ -243,8 +243,8 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   launch_init_tasks(d_chunks, (uint32_t)n, ntasks, d_tasks, d_gp, d_gb, s);
   launch_parse_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_gp, d_gb, s);
   timer.mark();
-  if (ntasks > kSoloTasks) launch_ht_decode(d_tasks, nullptr, ntasks, s);
-  else launch_ht_decode_solo(d_tasks, ntasks, s);
+  if ((int64_t)ntasks > tune(kTuneHtDecodeWaveMaxBlocks)) launch_ht_decode(d_tasks, nullptr, ntasks, s);
+  else launch_ht_decode_wave(d_tasks, ntasks, s);
   launch_ht_refine(d_tasks, ntasks, s);
   timer.mark();
   std::vector<DwtChunk> dw(n);

@@ -655,6 +655,391 @@ __global__ __launch_bounds__(64) void ht_decode_kernel(const HtDecTask* __restri
   if (i < n && task.passes > 1 && (!d.active || d.bad)) ((HtDecTask*)tasks)[i].passes = 1;
 }
 
+// ---------------------------------------------------------------------------
+// Wave-parallel cleanup decode: one wavefront per code-block.
+//
+// 1. The three byte streams are unstuffed at once into LDS bit arrays (word w
+//    holds stream bits 32w..32w+31, first bit lowest).  A byte's position is
+//    a wave prefix sum of the widths its stuffing rule gives it, a function
+//    of the byte and its predecessor only: MagSgn 8 - [b(k-1) == 0xFF]; VLC
+//    (read backward) 8 - [b(k-1) > 0x8F and b(k) & 0x7F == 0x7F]; MEL (MSB
+//    first, so stored bit-reversed) 8 - [b(k) == 0xFF].  Bytes are OR-ed into
+//    place, which is what the serial readers do with a stuffed byte's spare
+//    top bit.  Past its end MagSgn and MEL read ones, VLC zeros, as there.
+// 2. Per quad row, the VLC/MEL decode, the one serial dependency (a
+//    codeword's context is the significance of the codewords before it),
+//    runs wave-uniform and leaves each quad's CxtVLC entry and residual
+//    exponent offset u in the two lanes of its columns.
+// 3. The row's MagSgn samples are decoded by all lanes at once, lane c owning
+//    column c: the exponent bound from the row above comes from neighbouring
+//    lanes, the bit offsets from a wave prefix sum of the bit counts, each
+//    sample from one 64-bit LDS read; rows are stored 64 columns wide.
+// The result equals the lane-per-block decoder's sample for sample, errors
+// included.  Segments whose MagSgn part does not fit the LDS array are
+// decoded by lane 0 with that serial decoder.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kWaveMsWords = 3072;  // MagSgn bits: segments up to 12,280 MagSgn bytes
+constexpr uint32_t kWaveSWords = 1024;   // MEL / VLC bits: Scup <= 4079 bytes
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(x, o);
+    if (lane >= (uint32_t)o) x += t;
+  }
+  *total = __shfl(x, 63);
+  return x - v;
+}
+
+__device__ __forceinline__ uint32_t uread(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint32_t lane_of(uint32_t v, uint32_t l) {
+  return l < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l) : 0u;
+}
+
+enum { kUnMs = 0, kUnVlc = 1, kUnMel = 2 };
+
+// Unstuffs n bytes (byte k = at(k)) into buf starting at bit p0; prev0 stands
+// for the byte before the first.  Returns the end position.
+template <int Kind, class At>
+__device__ uint32_t unstuff(uint32_t* buf, uint32_t n, uint32_t p0, uint32_t prev0, At at) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t base = p0;
+  for (uint32_t c0 = 0; c0 < n; c0 += 256) {
+    const uint32_t k0 = c0 + 4 * lane;
+    uint32_t prev = k0 == 0 ? prev0 : (k0 - 1 < n ? at(k0 - 1) : 0u);
+    uint32_t b[4], w[4], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t k = k0 + j;
+      b[j] = k < n ? at(k) : 0u;
+      uint32_t wd;
+      if (Kind == kUnMs) wd = 8 - uint32_t(prev == 0xFF);
+      else if (Kind == kUnVlc) wd = 8 - uint32_t(prev > 0x8F && (b[j] & 0x7F) == 0x7F);
+      else wd = 8 - uint32_t(b[j] == 0xFF);
+      w[j] = k < n ? wd : 0u;
+      sum += w[j];
+      prev = b[j];
+    }
+    uint32_t total;
+    uint32_t pos = base + wave_excl_scan(sum, &total);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (k0 + j < n) {
+        const uint32_t v = Kind == kUnMel ? (__builtin_bitreverse32(b[j]) >> 24) : b[j];
+        const uint32_t s = pos & 31, wi = pos >> 5;
+        atomicOr(&buf[wi], v << s);
+        if (s > 24) atomicOr(&buf[wi + 1], v >> (32 - s));
+      }
+      pos += w[j];
+    }
+    base += total;
+  }
+  return base;
+}
+
+// Wave-uniform reader of an unstuffed LDS bit array (words past nvalid read
+// as `fill`).
+struct WaveBits {
+  const uint32_t* buf;
+  uint32_t nvalid, fill, next, have;
+  uint64_t win;  // bits [pos, pos + have)
+  __device__ void init(const uint32_t* b, uint32_t nv, uint32_t f) {
+    buf = b;
+    nvalid = nv;
+    fill = f;
+    next = 0;
+    have = 0;
+    win = 0;
+  }
+  __device__ uint32_t peek() {  // at least 33 bits valid afterwards
+    while (have <= 32) {
+      const uint32_t v = next < nvalid ? uread(buf[next]) : fill;
+      win |= (uint64_t)v << have;
+      have += 32;
+      next += 1;
+    }
+    return (uint32_t)win;
+  }
+  __device__ uint32_t skip(uint32_t n) {  // n <= have
+    win >>= n;
+    have -= n;
+    return (uint32_t)win;
+  }
+};
+
+// The MEL symbol decoder of mel.rs:14-173 over the unstuffed MSB-first bits.
+struct WaveMel {
+  WaveBits b;
+  int k;
+  __device__ int next_run() {
+    const uint32_t v = b.peek();
+    const uint32_t e = c_mel_exp[k];
+    if (v & 1) {
+      b.skip(1);
+      k = k + 1 < 12 ? k + 1 : 12;
+      return (int)(((1u << e) - 1) << 1);
+    }
+    const uint32_t cnt = e ? (__builtin_bitreverse32(v >> 1) >> (32 - e)) : 0u;
+    b.skip(e + 1);
+    k = k > 0 ? k - 1 : 0;
+    return (int)((cnt << 1) | 1);
+  }
+};
+
+__device__ __forceinline__ uint32_t lds_bits32(const uint32_t* buf, uint32_t pos, uint32_t nvalid) {
+  const uint32_t wi = pos >> 5;
+  const uint32_t lo = wi < nvalid ? buf[wi] : ~0u;
+  const uint32_t hi = wi + 1 < nvalid ? buf[wi + 1] : ~0u;
+  return __builtin_amdgcn_alignbit(hi, lo, pos & 31);
+}
+
+// Lane 0 alone: the serial decoder over the segment in global memory, for
+// segments too large for the LDS bit arrays.  lds holds rho / bit-length
+// rows and a one-lane stage (laid out as in ht_decode_kernel).
+__device__ void decode_block_serial(const HtDecTask& task, uint8_t* lds, HtDecTask* slot) {
+  uint8_t* lrho = lds;
+  uint8_t* lbl = lds + kRowCap * 64;
+  uint32_t* stage = (uint32_t*)(lds + 2 * kRowCap * 64);
+  for (int k = 0; k < kRowCap; ++k) lrho[k * 64] = lbl[k * 64] = 0;
+  BlockDec d;
+  d.start(task);
+  const uint32_t w = task.w, h = task.h;
+  uint32_t* out = (uint32_t*)task.out;
+  for (uint32_t y = 0; y < 64 && y < h; y += 2) {
+    if (d.active) d.row_begin();
+    for (uint32_t xb = 0; xb < 64 && xb < w; xb += kStageCols) {
+      for (uint32_t x = xb; x < xb + kStageCols; x += 4) {
+        if (x >= w) break;
+        if (d.active) {
+          d.pair(x, y, lrho, lbl, stage, xb);
+        } else {
+          for (uint32_t s = 0; s < 8; ++s) stage[((s & 1) * kStageCols + (x - xb + (s >> 1))) * kStagePitch] = 0;
+        }
+      }
+      for (uint32_t r = 0; r < 2; ++r)
+        for (uint32_t c = 0; c < kStageCols; ++c)
+          if (xb + c < w && y + r < h)
+            st32(out + (uint64_t)(y + r) * task.stride + xb + c, stage[(r * kStageCols + c) * kStagePitch]);
+    }
+    if (d.active) d.row_end(lrho, lbl);
+  }
+  if (d.active && d.bad) {
+    report(task, kErrBlock);
+    fill_zero(task);
+  }
+  if (task.passes > 1 && (!d.active || d.bad)) slot->passes = 1;
+}
+
+__global__ __launch_bounds__(64) void ht_decode_wave_kernel(const HtDecTask* __restrict__ tasks, uint64_t n) {
+  __shared__ uint32_t msbuf[kWaveMsWords];
+  __shared__ uint32_t melbuf[kWaveSWords];
+  __shared__ uint32_t vlcbuf[kWaveSWords];
+  __shared__ uint32_t blrow[68];
+  const uint32_t lane = threadIdx.x;
+  const uint64_t i = blockIdx.x;
+  if (i >= n) return;
+  const HtDecTask t = tasks[i];
+  HtDecTask* slot = (HtDecTask*)tasks + i;
+  const uint32_t width = t.w, height = t.h;
+  uint32_t* out = (uint32_t*)t.out;
+  const uint64_t stride = t.stride;
+  // decode.rs:54-110, as BlockDec::start
+  bool active = false, refine = false;
+  uint32_t p = 0, shift = 0, mmsbp2 = 0, lcup = 0, scup = 0;
+  if ((t.flags & kTaskIncluded) && width && height) {
+    uint32_t num_passes = t.passes;
+    const uint32_t missing_msbs = t.mmsb;
+    if (num_passes > 1 && t.len2 == 0) num_passes = 1;
+    if (num_passes > 3 || missing_msbs >= 30 || t.len1 < 2 || width > 64 || height > 64) {
+      if (lane == 0) report(t, kErrBlock);
+    } else {
+      if (missing_msbs == 29) num_passes = 1;
+      refine = num_passes > 1;
+      lcup = t.len1;
+      scup = (ld8(t.seg + lcup - 1) << 4) + (ld8(t.seg + lcup - 2) & 0xF);
+      if (scup < 2 || scup > lcup || scup > 4079) {
+        if (lane == 0) report(t, kErrBlock);
+      } else {
+        active = true;
+        p = 30 - missing_msbs;
+        shift = 31 - t.kmax;
+        mmsbp2 = missing_msbs + 2;
+      }
+    }
+  }
+  const uint32_t wcap = width < 64 ? width : 64, hcap = height < 64 ? height : 64;
+  if (!active) {
+    if (out)
+      for (uint32_t y = 0; y < hcap; ++y)
+        if (lane < wcap) st32(out + (uint64_t)y * stride + lane, 0);
+    if (lane == 0 && t.passes > 1) slot->passes = 1;
+    return;
+  }
+  const uint32_t n_ms = lcup - scup;
+  if (n_ms > 4 * (kWaveMsWords - 2)) {
+    if (lane == 0) decode_block_serial(t, (uint8_t*)msbuf, slot);
+    return;
+  }
+  const uint8_t* seg = t.seg;
+  // ---- 1. unstuff
+  const uint32_t ms_words = ((8 * n_ms) >> 5) + 2, s_words = ((8 * scup + 4) >> 5) + 2;
+  for (uint32_t w = lane; w < ms_words; w += 64) msbuf[w] = 0;
+  for (uint32_t w = lane; w < s_words; w += 64) {
+    melbuf[w] = 0;
+    vlcbuf[w] = 0;
+  }
+  __syncthreads();
+  const uint32_t t_ms = unstuff<kUnMs>(msbuf, n_ms, 0, 0, [&](uint32_t k) { return ld8(seg + k); });
+  const uint32_t mel0 = lcup - scup, nmel = scup - 1;
+  const uint32_t t_mel = unstuff<kUnMel>(melbuf, nmel, 0, 0, [&](uint32_t k) {
+    const uint32_t v = ld8(seg + mel0 + k);
+    return k + 1 == nmel ? (v | 0xF) : v;
+  });
+  const uint32_t d0 = ld8(seg + lcup - 2);
+  if (lane == 0) atomicOr(&vlcbuf[0], d0 >> 4);
+  const uint32_t t_vlc = unstuff<kUnVlc>(vlcbuf, scup - 2, 4 - uint32_t(((d0 >> 4) & 7) == 7), d0 | 0xF,
+                                         [&](uint32_t k) { return ld8(seg + lcup - 3 - k); });
+  if (lane == 0) {
+    msbuf[t_ms >> 5] |= ~0u << (t_ms & 31);
+    melbuf[t_mel >> 5] |= ~0u << (t_mel & 31);
+  }
+  if (lane < 68) blrow[lane] = 0;
+  if (lane < 4) blrow[64 + lane] = 0;
+  __syncthreads();
+  const uint32_t ms_nvalid = (t_ms >> 5) + 1;
+  WaveBits vlc;
+  vlc.init(vlcbuf, (t_vlc >> 5) + 1, 0u);
+  WaveMel mel;
+  mel.b.init(melbuf, (t_mel >> 5) + 1, ~0u);
+  mel.k = 0;
+  int run = mel.next_run();
+  const bool raw = (t.flags & kTaskSignMag) || refine;
+  uint32_t prev_q = 0;  // lane c: the row above's entry of quad c / 2
+  uint32_t msbase = 0;
+  bool bad = false;
+  for (uint32_t y = 0; y < height; y += 2) {
+    const bool initial = y == 0;
+    // ---- 2. VLC / MEL, wave-uniform (decode.rs:115-300)
+    uint32_t qinfo = 0, c_q = 0;
+    for (uint32_t x = 0, l4 = 0; x < width; x += 4, l4 += 4) {
+      uint32_t a0 = 0, a1 = 0, a2 = 0;
+      if (!initial) {
+        a0 = lane_of(prev_q, l4) & 0xF0;
+        a1 = lane_of(prev_q, l4 + 2) & 0xF0;
+        a2 = lane_of(prev_q, l4 + 4) & 0xF0;
+        c_q |= (a0 & 0xA0) << 2;
+        c_q |= (a1 & 0x20) << 4;
+      }
+      uint32_t vlc_val = vlc.peek();
+      uint32_t t0 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
+      if (c_q == 0) {
+        run -= 2;
+        t0 = run == -1 ? t0 : 0;
+        if (run < 0) run = mel.next_run();
+      }
+      if (initial) {
+        c_q = ((t0 & 0x10) << 3) | ((t0 & 0xE0) << 2);
+      } else {
+        c_q = ((t0 & 0x40) << 2) | ((t0 & 0x80) << 1);
+        c_q |= a0 & 0x80;
+        c_q |= (a1 & 0xA0) << 2;
+        c_q |= (a2 & 0x20) << 4;
+      }
+      vlc_val = vlc.skip(t0 & 0x7);
+      uint32_t t1 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
+      if (c_q == 0 && x + 2 < width) {
+        run -= 2;
+        t1 = run == -1 ? t1 : 0;
+        if (run < 0) run = mel.next_run();
+      }
+      if (x + 2 >= width) t1 = 0;
+      if (initial) {
+        c_q = ((t1 & 0x10) << 3) | ((t1 & 0xE0) << 2);
+      } else {
+        c_q = ((t1 & 0x40) << 2) | ((t1 & 0x80) << 1);
+        c_q |= a1 & 0x80;
+      }
+      vlc_val = vlc.skip(t1 & 0x7);
+      uint32_t uvlc_mode = ((t0 & 0x8) << 3) | ((t1 & 0x8) << 4);
+      if (initial && uvlc_mode == 0xC0) {
+        run -= 2;
+        uvlc_mode += run == -1 ? 0x40 : 0;
+        if (run < 0) run = mel.next_run();
+      }
+      uint32_t e = initial ? c_uvlc0[uvlc_mode + (vlc_val & 0x3F)] : c_uvlc1[uvlc_mode + (vlc_val & 0x3F)];
+      vlc_val = vlc.skip(e & 0x7);
+      e >>= 3;
+      const uint32_t len = e & 0xF;
+      const uint32_t tmp = vlc_val & ((1u << len) - 1);
+      vlc.skip(len);
+      e >>= 4;
+      const uint32_t len0 = e & 0x7;
+      e >>= 3;
+      const uint32_t base = initial ? 1 : 0;
+      const uint32_t q0 = t0 | ((base + (e & 7) + (tmp & ~(0xFFu << len0))) << 16);
+      const uint32_t q1 = t1 | ((base + (e >> 3) + (tmp >> len0)) << 16);
+      if ((lane & ~3u) == l4) qinfo = (lane & 2) ? q1 : q0;
+    }
+    // ---- 3. MagSgn of rows y, y+1, lane = column (decode.rs:302-396)
+    const bool col = lane < width;
+    const uint32_t tq = qinfo & 0xFFFF;
+    uint32_t U = qinfo >> 16;
+    if (!initial) {
+      uint32_t gamma = tq & 0xF0;
+      gamma &= gamma - 0x10;
+      const uint32_t q2 = lane & ~1u;  // columns 2q-1 .. 2q+2 of the row above: blrow[2q .. 2q+3]
+      const uint32_t e4 = max(max(blrow[q2], blrow[q2 + 1]), max(blrow[q2 + 2], blrow[q2 + 3]));
+      U += gamma != 0 ? max(e4, 2u) - 1 : 1;
+    }
+    if (col && !(lane & 1) && U > mmsbp2) bad = true;
+    const uint32_t nb0 = 2 * (lane & 1);
+    uint32_t m[2], sig[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const uint32_t nbit = nb0 + r;
+      sig[r] = col ? (tq >> (4 + nbit)) & 1 : 0u;
+      m[r] = U - ((tq >> (12 + nbit)) & 1);
+    }
+    const uint32_t c0 = sig[0] ? m[0] : 0u, c1 = sig[1] ? m[1] : 0u;
+    uint32_t total;
+    const uint32_t off = msbase + wave_excl_scan(c0 + c1, &total);
+    uint32_t blnew = 0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const uint32_t nbit = nb0 + r;
+      uint32_t val = 0, vv = 0;
+      if (sig[r]) {
+        const uint32_t ms_val = lds_bits32(msbuf, off + (r ? c0 : 0u), ms_nvalid);
+        vv = ms_val & ((1u << (m[r] & 31)) - 1);
+        vv |= ((tq >> (8 + nbit)) & 1) << (m[r] & 31);
+        vv |= 1;
+        val = (ms_val << 31) | ((vv + 2) << ((p - 1) & 31));
+      }
+      if (r == 1) blnew = bitlen(vv);
+      if (col && y + r < height) {
+        uint32_t o = val;
+        if (!raw) {
+          const int32_t mag = (int32_t)((val & 0x7FFFFFFFu) >> shift);
+          o = (uint32_t)((val & 0x80000000u) ? -mag : mag);
+        }
+        st32(out + (uint64_t)(y + r) * stride + lane, o);
+      }
+    }
+    msbase += total;
+    blrow[lane + 1] = blnew;
+    prev_q = qinfo;
+  }
+  // a magnitude bound violated mid-block: the whole block reads as zero
+  if (__ballot(bad)) {
+    if (lane == 0) report(t, kErrBlock);
+    for (uint32_t y = 0; y < hcap; ++y)
+      if (lane < wcap) st32(out + (uint64_t)y * stride + lane, 0);
+    if (lane == 0 && t.passes > 1) slot->passes = 1;
+  }
+}
+
 // SigProp and MagRef passes (decode.rs:398-600) of the blocks that carry
 // them, one lane per block, in place on the raw sign-magnitude samples the
 // cleanup decode left; then the samples become coefficients.  The cleanup
@@ -1233,6 +1618,13 @@ void launch_ht_decode_solo(const HtDecTask* d_tasks, uint64_t n, hipStream_t str
   if (n == 0) return;
   if (n > 0xFFFFFFFFull) fail(NDIC_INVALID_ARGUMENT, "too many code-blocks for one solo decode launch");
   hipLaunchKernelGGL(ht_decode_kernel<true>, dim3((uint32_t)n), dim3(64), 0, stream, d_tasks, nullptr, n);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream) {
+  if (n == 0) return;
+  if (n > 0xFFFFFFFFull) fail(NDIC_INVALID_ARGUMENT, "too many code-blocks for one wavefront-per-block decode launch");
+  hipLaunchKernelGGL(ht_decode_wave_kernel, dim3((uint32_t)n), dim3(64), 0, stream, d_tasks, n);
   NDIC_HIP(hipGetLastError());
 }
 

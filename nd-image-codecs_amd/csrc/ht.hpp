@@ -47,6 +47,10 @@ void launch_ht_decode(const HtDecTask* d_tasks, const uint32_t* d_order, uint64_
 // The same decode with one code-block per wavefront (segment staged in LDS):
 // a lower latency per block, for batches too small to fill the GPU.
 void launch_ht_decode_solo(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
+// One code-block per wavefront with the lanes working on its columns
+// together (wave-parallel stream unstuffing and MagSgn decode): the lowest
+// latency per block.
+void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
 // SigProp/MagRef of the tasks with more than one coding pass (after launch_ht_decode)
 void launch_ht_refine(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
 void launch_ht_encode(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream);

@@ -1,0 +1,18 @@
+// Launch parameters chosen per shape class by the autotuner (and pinned by
+// tests): named integer knobs with defaults, read by the orchestration code.
+#pragma once
+
+#include <cstdint>
+
+namespace ndic {
+
+enum TuneKey : int {
+  // batch windows with at most this many code-blocks decode one block per
+  // wavefront (ht_decode_wave_kernel); larger ones one block per lane
+  kTuneHtDecodeWaveMaxBlocks = 0,
+  kTuneCount
+};
+
+int64_t tune(TuneKey k);
+
+}  // namespace ndic

@@ -234,33 +234,3 @@ def test_lift_ht_batch_matches_oracle(ndic, oracle):
     enc = d_enc.download()
     for d, s in zip(arr, specs):
         assert enc[d.out_offset:d.out_offset + d.out_len] == s[4]
-
-
-def test_large_window_decodes_in_size_class_order(ndic):
-    """A window of more than 8,192 code-blocks (an odd number, 9,217) decodes
-    its blocks in size-class order; the round trip stays bit-exact."""
-    import importlib
-
-    dev = importlib.import_module("nd-image-codecs_amd.device")
-    shapes = [(64, 200, 200)] * 9 + [(1, 64, 64)]
-    al = lambda n: (n + 15) // 16 * 16
-    sizes = [int(np.prod(s)) * 2 for s in shapes]
-    d_raw = dev.DeviceBuffer(sum(al(n) for n in sizes), zero=True)
-    offs, off = [], 0
-    for k, (s, n) in enumerate(zip(shapes, sizes)):
-        dev.synth_chunk(d_raw.ptr + off, s, "uint16", dev.SYNTH_BLOBS,
-                        [(s[0] / 2, s[1] / 3, s[2] / 2, 3000.0, 9.0), (s[0] / 4, s[1] / 2, s[2] / 5, 900.0, 20.0)],
-                        100 + k)
-        offs.append(off)
-        off += al(n)
-    steps = [{"dimension": 0, "kind": "lift53", "levels": 2, "group": 0}]
-    cap = 3 * off + (1 << 20)
-    d_enc = dev.DeviceBuffer(cap, zero=True)
-    arr = (ndic.ChunkDesc * len(shapes))(*[ndic.make_desc(s, "uint16", steps, 0, offs[i], 0, 0)
-                                             for i, s in enumerate(shapes)])
-    ndic.encode_batch(d_raw.ptr, d_enc.ptr, cap, arr)
-    d_out = dev.DeviceBuffer(off, zero=True)
-    descs = [ndic.make_desc(s, "uint16", steps, 0, arr[i].out_offset, arr[i].out_len, offs[i])
-             for i, s in enumerate(shapes)]
-    ndic.decode_batch(d_enc.ptr, d_out.ptr, descs)
-    assert dev.count_mismatch(d_out.ptr, d_raw.ptr, off) == 0
