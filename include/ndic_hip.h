@@ -240,7 +240,8 @@ ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp
  * The autotuner (nd-image-codecs_amd/autotune.py) measures the kernel
  * variants per shape class and stores its choices here; tests pin them.
  * Knobs: "ht_decode_wave_max_blocks" -- batch windows of at most this many
- * code-blocks decode one block per wavefront, larger ones one block per lane.
+ * code-blocks decode one block per wavefront, larger ones one block per lane;
+ * "ht_encode_wave_max_blocks" -- the same choice for the encoder.
  * ndic_tune_set returns 0, -1 for an unknown key, -2 for a value out of
  * range; ndic_tune_get returns the value in force (-1: unknown key);
  * ndic_tune_reset restores the built-in default; ndic_tune_name lists the

@@ -354,7 +354,8 @@ ndic_status ndic_dev_ht_encode_block(const uint32_t* samples, size_t width, size
     t.flags = kTaskSignMag;
     NDIC_HIP(hipMemsetAsync(d_lens, 0, 64, s));
     NDIC_HIP(hipMemcpyAsync(d_task, &t, sizeof(t), hipMemcpyHostToDevice, s));
-    launch_ht_encode(d_task, 1, s);
+    if (tune(kTuneHtEncodeWaveMaxBlocks) >= 1) launch_ht_encode_wave(d_task, 1, s);
+    else launch_ht_encode(d_task, 1, s);
     uint32_t lens[16];
     std::vector<uint8_t> host(slot);
     NDIC_HIP(hipMemcpyAsync(lens, d_lens, 64, hipMemcpyDeviceToHost, s));

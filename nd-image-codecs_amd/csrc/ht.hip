@@ -1584,6 +1584,432 @@ __global__ __launch_bounds__(64) void ht_encode_kernel(const HtEncTask* __restri
   encode_one(t, le + threadIdx.x, lc + threadIdx.x);
 }
 
+
+// ---------------------------------------------------------------------------
+// Wave-parallel cleanup encode: one wavefront per code-block, lane c owning
+// column c of each quad row (rows y, y+1).
+//
+// Every quad's significance pattern rho, exponents, exponent bound from the
+// row above (neighbouring lanes), context, CxtVLC codeword, u and embedded-MSB
+// mask are computed by its two lanes at once.  The MagSgn bits of a row are
+// placed by a wave prefix sum of the lanes' bit counts and OR-ed into an LDS
+// bit array; so are each quad pair's VLC bits (codewords, then the UVLC
+// prefixes and suffixes, assembled by the pair's last lane).  MEL events are
+// kept as two ballots per row and coded after the last row by a wave-uniform
+// loop that takes runs of zero events in bulk.  The raw MagSgn and VLC bits
+// then become stuffed bytes up to 256 at a time: each step assumes 8-bit
+// bytes, finds by a ballot the first byte that the stuffing rule shortens
+// (after 0xFF for MagSgn; seven ones after a byte above 0x8F for VLC),
+// commits the bytes before it and restarts there.  Slot, lengths and errors
+// equal ht_encode_kernel's.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kEncMsWords = 3976;  // 4096 samples x 31 bits + slack
+constexpr uint32_t kEncVlcWords = 488;  // 4 + 512 pairs x 30 bits + slack
+
+// ORs the low n (<= 64) bits of v into the LDS bit array at bit pos.
+__device__ __forceinline__ void lds_or_bits(uint32_t* buf, uint32_t pos, uint64_t v, uint32_t n) {
+  if (n == 0) return;
+  const uint32_t s = pos & 31, w = pos >> 5;
+  atomicOr(&buf[w], (uint32_t)(v << s));
+  if (s + n > 32) atomicOr(&buf[w + 1], (uint32_t)(s ? v >> (32 - s) : v >> 32));
+  if (s + n > 64) atomicOr(&buf[w + 2], (uint32_t)(v >> (64 - s)));
+}
+
+// Bits [pos, pos + 32) of an LDS bit array of `words` words (reads past the
+// end are clamped: callers never use such bits).
+__device__ __forceinline__ uint32_t lds_window(const uint32_t* buf, uint32_t pos, uint32_t words) {
+  uint32_t w = pos >> 5;
+  w = w < words - 1 ? w : words - 2;
+  return __builtin_amdgcn_alignbit(buf[w + 1], buf[w], pos & 31);
+}
+
+// Emits the full bytes of raw bits [0, T) of R under the MagSgn (Vlc =
+// false: a byte after 0xFF carries 7 bits) or VLC (a byte after one above
+// 0x8F carries 7 bits when they are all ones) stuffing rule: byte k goes to
+// dst[k] for k < cap, except byte 0 when hold_first (its value is returned
+// in *first).  prev0 stands for the byte before the first.  Returns the byte
+// count; *endpos is the first raw bit not in a full byte, *last the last
+// byte (prev0 if none).
+template <bool Vlc>
+__device__ __forceinline__ uint32_t stuff_bytes(const uint32_t* R, uint32_t words, uint32_t T, uint32_t prev0, uint8_t* dst,
+                                uint32_t cap, bool hold_first, uint32_t* endpos, uint32_t* last, uint32_t* first) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t pos = 0, ob = 0, prev = prev0;
+  *first = 0;
+  while (true) {
+    uint32_t w0 = 8;
+    if (Vlc) {
+      if (prev > 0x8F && pos + 7 <= T && (lds_window(R, pos, words) & 0x7F) == 0x7F) w0 = 7;
+    } else if (prev == 0xFF) {
+      w0 = 7;
+    }
+    if (pos + w0 > T) break;  // the next byte is partial
+    const uint32_t nfull = min(256u, 1u + (T - pos - w0) / 8);
+    // byte 0 at pos (w0 bits), byte j >= 1 at pos + w0 + 8 (j - 1)
+    const uint32_t j0 = 4 * lane;
+    const uint32_t s0 = j0 == 0 ? pos : pos + w0 + 8 * (j0 - 1);
+    const uint32_t win = lds_window(R, s0, words);
+    const uint32_t sh = j0 == 0 ? w0 : 8;
+    uint32_t v[4];
+    v[0] = win & (sh == 8 ? 0xFFu : 0x7Fu);
+    v[1] = (win >> sh) & 0xFF;
+    v[2] = (win >> (sh + 8)) & 0xFF;
+    v[3] = (win >> (sh + 16)) & 0xFF;
+    const uint32_t before = __shfl_up(v[3], 1);
+    // the first byte j in 1..min(nfull, 255) of this lane that its
+    // predecessor shortens
+    uint32_t ev = 4;
+#pragma unroll
+    for (int k = 3; k >= 0; --k) {
+      const uint32_t j = j0 + k;
+      if (j == 0 || j > nfull) continue;
+      const uint32_t pv = k == 0 ? before : v[k - 1];
+      const bool e = Vlc ? (pv > 0x8F && (v[k] & 0x7F) == 0x7F) : (pv == 0xFF);
+      if (e) ev = (uint32_t)k;
+    }
+    const uint64_t evm = __ballot(ev < 4);
+    uint32_t commit = nfull;
+    if (evm) {
+      const uint32_t fl = (uint32_t)__builtin_ctzll(evm);
+      commit = 4 * fl + (uint32_t)__builtin_amdgcn_readlane((int)ev, (int)fl);
+    }
+    if (ob == 0 && hold_first) *first = (uint32_t)__builtin_amdgcn_readlane((int)v[0], 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t j = j0 + k, o = ob + j;
+      if (j < commit && o < cap && !(hold_first && o == 0)) st8(dst + o, (uint8_t)v[k]);
+    }
+    const uint32_t lj = commit - 1;
+    uint32_t lv = v[0];
+    if ((lj & 3) == 1) lv = v[1];
+    else if ((lj & 3) == 2) lv = v[2];
+    else if ((lj & 3) == 3) lv = v[3];
+    prev = (uint32_t)__builtin_amdgcn_readlane((int)lv, (int)(lj >> 2));
+    pos = pos + w0 + 8 * lj;
+    ob += commit;
+  }
+  *endpos = pos;
+  *last = prev;
+  return ob;
+}
+
+// MEL coder of mel.rs:175-262, wave-uniform; lane 0 stores the bytes.
+struct WaveMelEnc {
+  uint8_t* buf;
+  uint32_t len, tmp, remaining, run, threshold, k, lastb;
+  __device__ void init(uint8_t* b) {
+    buf = b;
+    len = 0;
+    lastb = 0;
+    tmp = 0;
+    remaining = 8;
+    run = 0;
+    k = 0;
+    threshold = 1;
+  }
+  __device__ void bit(uint32_t v) {
+    tmp = (tmp << 1) + v;
+    remaining -= 1;
+    if (remaining == 0) {
+      if (len < kMelCap && (threadIdx.x & 63) == 0) st8(buf + len, (uint8_t)tmp);
+      lastb = tmp;
+      len += 1;
+      remaining = tmp == 0xFF ? 7 : 8;
+      tmp = 0;
+    }
+  }
+  // z zero events in a row
+  __device__ void zeros(uint32_t z) {
+    while (z) {
+      const uint32_t need = threshold - run;
+      if (z < need) {
+        run += z;
+        return;
+      }
+      z -= need;
+      bit(1);
+      run = 0;
+      k = k + 1 < 12 ? k + 1 : 12;
+      threshold = 1u << c_mel_exp[k];
+    }
+  }
+  __device__ void one() {
+    bit(0);
+    for (int t = (int)c_mel_exp[k]; t > 0;) {
+      t -= 1;
+      bit((run >> t) & 1);
+    }
+    run = 0;
+    k = k > 0 ? k - 1 : 0;
+    threshold = 1u << c_mel_exp[k];
+  }
+};
+
+__global__ __launch_bounds__(64) void ht_encode_wave_kernel(const HtEncTask* __restrict__ tasks, uint64_t n) {
+  __shared__ uint32_t msraw[kEncMsWords];
+  __shared__ uint32_t vlcraw[kEncVlcWords];
+  __shared__ uint32_t erow[68];  // 1 + column: the row above's bottom-sample exponent
+  __shared__ uint32_t srow[68];  // 1 + column: the row above's bottom-sample significance
+  __shared__ uint64_t melhas[32], melbit[32];
+  const uint32_t lane = threadIdx.x;
+  const uint64_t i = blockIdx.x;
+  if (i >= n) return;
+  const HtEncTask t = tasks[i];
+  const uint32_t width = t.w, height = t.h;
+  const uint32_t p = 31 - (uint32_t)t.kmax;  // 30 - missing_msbs
+  const uint32_t shift = 31 - (uint32_t)t.kmax;
+  const uint32_t ms_words = min(kEncMsWords, (width * height * ((uint32_t)t.kmax + 1) + 31) / 32 + 3);
+  for (uint32_t w = lane; w < ms_words; w += 64) msraw[w] = 0;
+  for (uint32_t w = lane; w < kEncVlcWords; w += 64) vlcraw[w] = 0;
+  if (lane < 32) melhas[lane] = melbit[lane] = 0;
+  erow[lane] = 0;
+  srow[lane] = 0;
+  if (lane < 4) erow[64 + lane] = srow[64 + lane] = 0;
+  __syncthreads();
+  if (lane == 0) vlcraw[0] = 0xF;  // VlcEnc starts with four ones (streams.rs:267-300)
+  const uint32_t c = lane, j = lane & 1, q = lane >> 1;
+  const uint32_t lq = (lane & ~1u) - 2;  // first lane of the left quad
+  uint32_t mspos = 0, vlcpos = 4;
+  uint64_t anysig = 0;
+  for (uint32_t y = 0, r = 0; y < height; y += 2, ++r) {
+    const bool initial = y == 0;
+    // ---- the two samples of this column (encode.rs:24-60)
+    uint32_t sig[2], e[2], s[2];
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+      sig[rr] = e[rr] = s[rr] = 0;
+      if (c < width && y + rr < height) {
+        uint32_t raw;
+        if (t.flags & kTaskSignMag) {
+          raw = ((const uint32_t*)t.in)[(uint64_t)(y + rr) * t.stride + c];
+        } else {
+          const int32_t v = ldi32(t.in + (uint64_t)(y + rr) * t.stride + c);
+          raw = (v < 0 ? 0x80000000u : 0u) | ((v < 0 ? 0u - (uint32_t)v : (uint32_t)v) << shift);
+        }
+        uint32_t val = ((raw + raw) >> p) & ~1u;
+        if (val != 0) {
+          sig[rr] = 1;
+          val -= 1;
+          e[rr] = 32 - __clz(val);
+          s[rr] = (val - 1) + (raw >> 31);
+        }
+      }
+    }
+    // ---- the quad (lanes 2q, 2q+1): sample n = 2 j + row
+    const uint32_t mine = (sig[0] | (sig[1] << 1)) << (2 * j);
+    const uint32_t rho = mine | __shfl_xor(mine, 1);
+    const uint32_t emine = max(e[0], e[1]);
+    const uint32_t eqmax = max(emine, (uint32_t)__shfl_xor((int)emine, 1));
+    anysig |= __ballot(rho != 0);
+    const bool exists = 2 * q < width;
+    const uint32_t rl = (uint32_t)__shfl((int)rho, (int)(lq & 63));
+    const uint32_t rho_left = q > 0 ? rl : 0u;
+    uint32_t cq, uq, kappa = 1;
+    if (initial) {
+      cq = q > 0 ? ((rho_left >> 1) | (rho_left & 1)) : 0u;
+    } else {
+      // context: the row above's bottom significance in columns 2q-1 .. 2q+2
+      // and the left quad's right column (encode.rs:241-309)
+      const uint32_t b = 2 * q;  // srow[col + 1]
+      cq = (srow[b] | srow[b + 1]) | ((srow[b + 2] | srow[b + 3]) << 2) | ((((rho_left >> 2) | (rho_left >> 3)) & 1) << 1);
+      const int max_e = (int)max(max(erow[b], erow[b + 1]), max(erow[b + 2], erow[b + 3])) - 1;
+      if (rho & (rho - 1)) kappa = (uint32_t)(max_e > 1 ? max_e : 1);
+    }
+    uq = max(eqmax, kappa);
+    const uint32_t u = uq - kappa;
+    const uint32_t em = (uint32_t(sig[0] && e[0] == eqmax) | (uint32_t(sig[1] && e[1] == eqmax) << 1)) << (2 * j);
+    const uint32_t emq = em | __shfl_xor(em, 1);
+    const uint32_t eps = u > 0 ? emq : 0u;
+    const uint32_t tuple = initial ? c_enc0[(cq << 8) + (rho << 4) + eps] : c_enc1[(cq << 8) + (rho << 4) + eps];
+    // ---- MagSgn bits of this column's samples
+    uint32_t m[2], nb[2];
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+      m[rr] = uq - ((tuple >> (2 * j + rr)) & 1);
+      nb[rr] = (exists && sig[rr]) ? m[rr] : 0u;
+    }
+    const uint64_t p0 = nb[0] ? (uint64_t)(s[0] & (m[0] >= 32 ? ~0u : (1u << m[0]) - 1)) : 0ull;
+    const uint64_t p1 = nb[1] ? (uint64_t)(s[1] & (m[1] >= 32 ? ~0u : (1u << m[1]) - 1)) : 0ull;
+    uint32_t mstot;
+    const uint32_t moff = mspos + wave_excl_scan(nb[0] + nb[1], &mstot);
+    lds_or_bits(msraw, moff, p0 | (p1 << nb[0]), nb[0] + nb[1]);
+    mspos += mstot;
+    // ---- MEL events: quad 2p at lane 4p, quad 2p+1 at lane 4p+2, the
+    // initial row's UVLC event of the pair at lane 4p+3
+    const uint32_t l4 = lane & 3;
+    const uint32_t u_other = (uint32_t)__shfl((int)u, (int)(lane & ~3u));  // u of the pair's first quad
+    const bool q1ex = (lane & ~3u) + 2 < width;
+    const uint32_t u_second = (uint32_t)__shfl((int)u, (int)((lane & ~3u) + 2));
+    const uint32_t u0 = u_other, u1 = q1ex ? u_second : 0u;
+    bool has = false, bitv = false;
+    if (exists && (l4 == 0 || l4 == 2) && cq == 0) {
+      has = true;
+      bitv = rho != 0;
+    }
+    if (initial && l4 == 3 && (lane & ~3u) < width && u0 > 0 && u1 > 0) {
+      has = true;
+      bitv = (u0 < u1 ? u0 : u1) > 2;
+    }
+    const uint64_t hm = __ballot(has), bm = __ballot(bitv);
+    if (lane == 0) {
+      melhas[r] = hm;
+      melbit[r] = bm;
+    }
+    // ---- VLC bits of the pair, assembled at its last lane
+    const uint32_t cw = tuple >> 8, cl = (tuple >> 4) & 7;
+    const uint32_t cw0 = (uint32_t)__shfl((int)cw, (int)(lane & ~3u)), cl0 = (uint32_t)__shfl((int)cl, (int)(lane & ~3u));
+    const uint32_t cw1 = (uint32_t)__shfl((int)cw, (int)((lane & ~3u) + 2)),
+                   cl1 = (uint32_t)__shfl((int)cl, (int)((lane & ~3u) + 2));
+    uint64_t vb = 0;
+    uint32_t vn = 0;
+    if (l4 == 3 && (lane & ~3u) < width) {
+      auto put = [&](uint32_t bits, uint32_t len) {
+        vb |= (uint64_t)(bits & ((1u << len) - 1)) << vn;
+        vn += len;
+      };
+      put(cw0, cl0);
+      if (q1ex) put(cw1, cl1);
+      auto code = [](uint32_t uu, uint32_t* pf, uint32_t* pl, uint32_t* sf, uint32_t* sl) {
+        if (uu == 0) { *pf = 0; *pl = 0; *sf = 0; *sl = 0; }
+        else if (uu == 1) { *pf = 1; *pl = 1; *sf = 0; *sl = 0; }
+        else if (uu == 2) { *pf = 2; *pl = 2; *sf = 0; *sl = 0; }
+        else if (uu <= 4) { *pf = 4; *pl = 3; *sf = uu - 3; *sl = 1; }
+        else { *pf = 0; *pl = 3; *sf = uu - 5; *sl = 5; }
+      };
+      uint32_t pf0, pl0, sf0, sl0, pf1, pl1, sf1, sl1;
+      if (initial && u0 > 2 && u1 > 2) {
+        code(u0 - 2, &pf0, &pl0, &sf0, &sl0);
+        code(u1 - 2, &pf1, &pl1, &sf1, &sl1);
+        put(pf0, pl0);
+        put(pf1, pl1);
+        put(sf0, sl0);
+        put(sf1, sl1);
+      } else if (initial && u0 > 2 && u1 > 0) {
+        code(u0, &pf0, &pl0, &sf0, &sl0);
+        put(pf0, pl0);
+        put(u1 - 1, 1);
+        put(sf0, sl0);
+      } else {
+        code(u0, &pf0, &pl0, &sf0, &sl0);
+        code(u1, &pf1, &pl1, &sf1, &sl1);
+        put(pf0, pl0);
+        put(pf1, pl1);
+        put(sf0, sl0);
+        put(sf1, sl1);
+      }
+    }
+    uint32_t vtot;
+    const uint32_t voff = vlcpos + wave_excl_scan(vn, &vtot);
+    lds_or_bits(vlcraw, voff, vb, vn);
+    vlcpos += vtot;
+    // ---- this row's bottom samples are the next row's context
+    erow[c + 1] = (c < width) ? e[1] : 0u;
+    srow[c + 1] = (c < width) ? sig[1] : 0u;
+    __syncthreads();
+  }
+  if (!anysig) {
+    if (lane == 0) t.lens[0] = t.lens[1] = t.lens[2] = t.lens[3] = 0;
+    return;
+  }
+  // ---- MEL (wave-uniform)
+  uint8_t* melbuf = t.slot + t.ms_cap;
+  uint8_t* vlcbuf = t.slot + t.ms_cap + kMelCap;
+  WaveMelEnc mel;
+  mel.init(melbuf);
+  const uint32_t rows = (height + 1) / 2;
+  for (uint32_t r = 0; r < rows; ++r) {
+    uint64_t has = melhas[r], bits = melbit[r];
+    has = ((uint64_t)__builtin_amdgcn_readfirstlane((int)(has >> 32)) << 32) |
+          (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)has);
+    bits = ((uint64_t)__builtin_amdgcn_readfirstlane((int)(bits >> 32)) << 32) |
+           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)bits);
+    bits &= has;
+    while (has) {
+      const uint64_t ones = bits;
+      const uint32_t first1 = ones ? (uint32_t)__builtin_ctzll(ones) : 64u;
+      const uint64_t below = first1 >= 64 ? ~0ull : ((1ull << first1) - 1);
+      const uint32_t z = (uint32_t)__builtin_popcountll(has & below);
+      mel.zeros(z);
+      has &= ~below;
+      if (first1 < 64) {
+        mel.one();
+        has &= ~(1ull << first1);
+        bits &= ~(1ull << first1);
+      }
+    }
+  }
+  __syncthreads();
+  // ---- MagSgn bytes (streams.rs:375-441)
+  uint32_t ms_end, ms_last, unused;
+  uint32_t ms_len = stuff_bytes<false>(msraw, kEncMsWords, mspos, 0u, t.slot, t.ms_cap, false, &ms_end, &ms_last,
+                                       &unused);
+  {
+    const uint32_t rem = mspos - ms_end;
+    const uint32_t maxb = ms_last == 0xFF ? 7u : 8u;
+    if (rem > 0) {
+      uint32_t tmp = lds_window(msraw, ms_end, kEncMsWords) & ((1u << rem) - 1);
+      tmp |= (0xFFu & ((1u << (maxb - rem)) - 1)) << rem;
+      if (tmp != 0xFF) {
+        if (lane == 0 && ms_len < t.ms_cap) st8(t.slot + ms_len, (uint8_t)tmp);
+        ms_len += 1;
+      }
+    } else if (maxb == 7) {
+      ms_len -= 1;
+    }
+  }
+  // ---- VLC bytes (streams.rs:267-372): production order after the 0xFF
+  // placeholder; the first byte is held for the Scup patch
+  uint32_t v_end, v_last, v_first;
+  const uint32_t nv = stuff_bytes<true>(vlcraw, kEncVlcWords, vlcpos, 0xFFu, vlcbuf + 1, kVlcCap - 1, true, &v_end,
+                                        &v_last, &v_first);
+  uint32_t vlc_len = 1 + nv;
+  const uint32_t vlc_used = vlcpos - v_end;
+  const uint32_t vlc_tmp = vlc_used ? (lds_window(vlcraw, v_end, kEncVlcWords) & ((1u << vlc_used) - 1)) : 0u;
+  // ---- terminate MEL and fuse its last byte with VLC's (mel.rs:247-261)
+  if (mel.run > 0) mel.bit(1);
+  const uint32_t mel_tmp = (mel.tmp << mel.remaining) & 0xFF;
+  const uint32_t mel_mask = (0xFFu << mel.remaining) & 0xFF;
+  const uint32_t vlc_mask = vlc_used == 0 ? 0u : (0xFFu >> (8 - vlc_used));
+  if ((mel_mask | vlc_mask) != 0) {
+    const uint32_t fuse = mel_tmp | vlc_tmp;
+    if ((((fuse ^ mel_tmp) & mel_mask) | ((fuse ^ vlc_tmp) & vlc_mask)) == 0 && fuse != 0xFF && vlc_len > 1) {
+      if (lane == 0 && mel.len < kMelCap) st8(melbuf + mel.len, (uint8_t)fuse);
+      mel.lastb = fuse;
+      mel.len += 1;
+    } else {
+      if (lane == 0 && mel.len < kMelCap) st8(melbuf + mel.len, (uint8_t)mel_tmp);
+      mel.lastb = mel_tmp;
+      mel.len += 1;
+      if (vlc_len == 1) v_first = vlc_tmp;
+      else if (lane == 0 && vlc_len < kVlcCap) st8(vlcbuf + vlc_len, (uint8_t)vlc_tmp);
+      vlc_len += 1;
+    }
+  }
+  const uint32_t scup = mel.len + vlc_len;
+  if (scup < 2 || scup > 4079 || mel.len > kMelCap || vlc_len > kVlcCap || ms_len > t.ms_cap) {
+    if (lane == 0) {
+      if (t.err) atomicCAS(t.err, 0, kErrEncodeScup);
+      t.lens[0] = t.lens[1] = t.lens[2] = t.lens[3] = 0;
+    }
+    return;
+  }
+  if (lane == 0) {
+    // Scup in the last two bytes of [MagSgn][MEL][VLC reversed]: the last is
+    // the placeholder (production index 0), the one before it production
+    // index 1, or the last MEL byte when VLC has no other byte
+    st8(vlcbuf, (uint8_t)(scup >> 4));
+    if (vlc_len >= 2) {
+      st8(vlcbuf + 1, (uint8_t)((v_first & 0xF0) | (scup & 0xF)));
+    } else {
+      st8(melbuf + mel.len - 1, (uint8_t)((mel.lastb & 0xF0) | (scup & 0xF)));
+    }
+    t.lens[0] = ms_len;
+    t.lens[1] = mel.len;
+    t.lens[2] = vlc_len;
+    t.lens[3] = 1;
+  }
+}
+
 }  // namespace
 
 void ht_init_tables() {
@@ -1631,6 +2057,13 @@ void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, hipStream_t str
 void launch_ht_refine(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream) {
   if (n == 0) return;
   hipLaunchKernelGGL(ht_refine_kernel, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, n);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_ht_encode_wave(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream) {
+  if (n == 0) return;
+  if (n > 0xFFFFFFFFull) fail(NDIC_INVALID_ARGUMENT, "too many code-blocks for one wavefront-per-block encode launch");
+  hipLaunchKernelGGL(ht_encode_wave_kernel, dim3((uint32_t)n), dim3(64), 0, stream, d_tasks, n);
   NDIC_HIP(hipGetLastError());
 }
 

@@ -54,6 +54,9 @@ void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, hipStream_t str
 // SigProp/MagRef of the tasks with more than one coding pass (after launch_ht_decode)
 void launch_ht_refine(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
 void launch_ht_encode(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream);
+// The same encode with one code-block per wavefront (lanes over columns, bit
+// offsets from wave prefix sums): the lowest latency per block.
+void launch_ht_encode_wave(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream);
 // Byte k of an encoded segment from its three pieces (see HtEncTask::slot).
 __host__ __device__ inline uint8_t ht_seg_byte(const uint8_t* slot, uint32_t ms_cap, const uint32_t* lens, uint32_t k) {
   if (k < lens[0]) return slot[k];

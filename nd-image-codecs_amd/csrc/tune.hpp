@@ -10,6 +10,8 @@ enum TuneKey : int {
   // batch windows with at most this many code-blocks decode one block per
   // wavefront (ht_decode_wave_kernel); larger ones one block per lane
   kTuneHtDecodeWaveMaxBlocks = 0,
+  // the same choice for the encoder (ht_encode_wave_kernel / ht_encode_kernel)
+  kTuneHtEncodeWaveMaxBlocks,
   kTuneCount
 };
 

@@ -1,6 +1,7 @@
-"""The two HT cleanup decoders on the device against each other and the
+"""The two HT cleanup coders on the device against each other and the
 oracle: one code-block per wavefront (lanes over columns, wave-parallel
-stream unstuffing and MagSgn offsets) and one code-block per lane.
+stream unstuffing / bit packing and MagSgn offsets) and one code-block per
+lane.
 
 Covers what the reference's block tests cover (crates/ndic-htj2k/tests/
 block_roundtrip.rs:110-140: every block size 1..64, K_max 2..28, densities
@@ -8,7 +9,8 @@ from empty to full), the refinement passes after the cleanup pass
 (decode.rs:398-600), segments whose MagSgn part is too large for the wave
 decoder's LDS bit array (decoded serially by lane 0), corrupted segments
 (both decoders give the same samples and the same error), and whole chunk
-batches decoded with each decoder forced.
+batches decoded with each decoder forced; and the encoders' byte output
+against the oracle over the same range of blocks and chunks.
 """
 
 from __future__ import annotations
@@ -24,6 +26,7 @@ import devhooks
 
 pytestmark = pytest.mark.gpu
 KNOB = "ht_decode_wave_max_blocks"
+ENC_KNOB = "ht_encode_wave_max_blocks"
 
 
 @pytest.fixture
@@ -33,6 +36,15 @@ def force(ndic):
 
     yield set_
     ndic.tune_reset(KNOB)
+
+
+@pytest.fixture
+def force_enc(ndic):
+    def set_(mode: str) -> None:
+        ndic.tune_set(ENC_KNOB, 1 << 40 if mode == "wave" else 0)
+
+    yield set_
+    ndic.tune_reset(ENC_KNOB)
 
 
 def decode_rc(ndic, coded: bytes, w: int, h: int, mmsb: int, passes: int = 1, len1=None, len2: int = 0,
@@ -191,3 +203,55 @@ def test_window_of_9217_blocks_with_either_decoder(ndic, force, mode):
              for i, s in enumerate(shapes)]
     ndic.decode_batch(d_enc.ptr, d_out.ptr, descs)
     assert dev.count_mismatch(d_out.ptr, d_raw.ptr, off) == 0
+
+
+@pytest.mark.parametrize("mode", ["wave", "lane"])
+def test_encoders_match_oracle(ndic, oracle, force_enc, mode):
+    """Every block size, K_max 2..30, densities from one sample to full,
+    plus blocks of all-ones magnitudes (0xFF runs in MagSgn: stuffing after
+    every byte) and checkerboards."""
+    force_enc(mode)
+    rng = np.random.default_rng(41)
+    cases = []
+    for i in range(260):
+        w, h = int(rng.integers(1, 65)), int(rng.integers(1, 65))
+        if i % 9 == 0:
+            w, h = 64, 64
+        kmax = int(rng.integers(2, 31))
+        dens = [0.0, 0.01, 0.3, 0.8, 1.0][i % 5]
+        cases.append((w, h, kmax, datagen.sm_block(rng, w, h, kmax, dens)))
+    for kmax in (8, 16, 30):
+        ones = np.full((64, 64), ((1 << kmax) - 1) << (31 - kmax), np.uint32)
+        ones[::3, ::2] |= np.uint32(1 << 31)
+        cases.append((64, 64, kmax, ones))
+        chk = np.zeros((64, 64), np.uint32)
+        chk[::2, ::2] = np.uint32(5 << (31 - kmax))
+        cases.append((64, 64, kmax, chk))
+    for w, h, kmax, sm in cases:
+        exp = oracle.ht_encode_block(sm, kmax - 1)
+        got = devhooks.ht_encode_block(ndic, sm, kmax - 1)
+        assert got == exp, (mode, w, h, kmax)
+    # encode_block's callers skip all-zero blocks (crates/ndic-htj2k/src/block/
+    # encode.rs:103-105; the packet header signals non-inclusion): the device
+    # coders mark such a block not included
+    assert devhooks.ht_encode_block(ndic, np.zeros((5, 17), np.uint32), 3) == b""
+
+
+@pytest.mark.parametrize("mode", ["wave", "lane"])
+def test_chunks_encode_with_either_encoder(ndic, oracle, force_enc, mode):
+    force_enc(mode)
+    rng = np.random.default_rng(42)
+    for k, (shape, dt, lv) in enumerate([((3, 33, 61), "u16", 3), ((1, 64, 64), "i16", 0), ((2, 97, 131), "u8", 5),
+                                         ((5, 1, 9), "u16", 2), ((2, 200, 70), "i32", 4), ((1, 129, 257), "u16", 5)]):
+        npt = {"u8": np.uint8, "u16": np.uint16, "i16": np.int16, "i32": np.int32}[dt]
+        info = np.iinfo(npt)
+        lo, hi = max(info.min, -(2 ** 20)), min(info.max, 2 ** 20)
+        arr = rng.integers(lo, hi, size=shape, endpoint=True).astype(npt)
+        if k == 1:
+            arr[:] = 0
+        name = {"u8": "uint8", "u16": "uint16", "i16": "int16", "i32": "int32"}[dt]
+        exp = oracle.htj2k_encode_chunk(arr.tobytes(), shape, dt, xy_levels=lv)
+        assert ndic.htj2k_encode(arr, shape, name, xy_levels=lv) == exp, (mode, shape, dt, lv)
+    vol = datagen.c0_volume((4, 256, 256), seed=3)
+    exp = oracle.htj2k_encode_chunk(vol.tobytes(), vol.shape, "u16", xy_levels=5)
+    assert ndic.htj2k_encode(vol, vol.shape, "uint16", xy_levels=5) == exp
