@@ -139,11 +139,14 @@ def serve_bench(ndic, dev, d_raw, d_enc, ddescs, specs, args):
             if r.kind == serve.DECODE:
                 c = store.ht[r.key]
                 n = int(np.prod(c.shape)) * np.dtype(c.dtype).itemsize
-                dev.count_mismatch_async(srv.out.ptr + r.out_offset, d_raw.ptr + c.raw_offset, n, d_bad.ptr)
+                dev.count_mismatch_async(srv.result_ptr(r), d_raw.ptr + c.raw_offset, n, d_bad.ptr)
 
     out = {"mix": "60% thumbnail (plane 0, resolution 3, index prefix) / 25% full decode / 10% ZFP 4^3 brick "
                   "(f32 64x256x256, rate 8|16) / 5% encode (u16 8x1x32x256x256)",
            "store_bytes": d_enc.nbytes + d_z.nbytes, "rates": {}}
+    # warm-up: every lane's library context and device buffers exist before timing
+    server.run(serve.poisson_requests(np.random.default_rng(9), 400.0, 0.25, store, n_encode_sources=1),
+               verify=verify)
     for rate in [float(v) for v in args.serve_rates.split(",") if v]:
         reqs = serve.poisson_requests(np.random.default_rng(int(rate)), rate, args.serve_seconds, store,
                                       n_encode_sources=1)

@@ -1,16 +1,17 @@
 """Serving loop over an HBM-resident store of compressed chunks.
 
-Requests arrive over time (thumbnail, full-chunk decode, ZFP brick, encode).
-Each pass of the loop admits every request that has arrived, as far as the
-output memory budget allows, and runs them as device batches: thumbnails
+Requests arrive over time (thumbnail, full-chunk decode, ZFP brick, encode)
+and go to one of three lanes by kind: small reads (thumbnails and bricks),
+full-chunk decodes, encodes.  Each lane is a host thread, which the library
+gives its own HIP stream, so the lanes' work overlaps on the device.  Each
+pass of a lane admits every request of its kind that has arrived, as far as
+the lane's output memory allows, and runs them as device batches: thumbnails
 through ``ndic_htj2k_decode_plane_batch`` (plane 0 cut at the ndht index's
-``prefix[r]``, resolutions 0..r), full decodes through
-``ndic_lift_ht_decode_batch``, bricks through ``ndic_zfp_decode_bricks`` and
-encodes through ``ndic_lift_ht_encode_batch``.  Requests that arrive while a
-pass is on the GPU join the next pass (continuous batching); latency is the
-time from a request's arrival to the end of the pass that served it.  Encodes
-run on a second host thread, which the library gives its own HIP stream, so
-their work overlaps the decode passes on the device.
+``prefix[r]``, resolutions 0..r), bricks through ``ndic_zfp_decode_bricks``,
+full decodes through ``ndic_lift_ht_decode_batch``, encodes through
+``ndic_lift_ht_encode_batch``.  Requests that arrive while a pass is on the
+GPU join the lane's next pass (continuous batching); latency is the time
+from a request's arrival to the end of the pass that served it.
 
 The thumbnail byte ranges follow ``RangeIndex::thumbnail``
 (crates/ndic-codestream/src/range.rs:128-200): plane 0's codestream up to
@@ -154,7 +155,8 @@ class Stats:
     call_ms: Dict[str, float] = field(default_factory=dict)   # device-batch call time per kind
     calls: Dict[str, int] = field(default_factory=dict)
     items: Dict[str, int] = field(default_factory=dict)       # requests per kind over all calls
-    pass_ms: List[float] = field(default_factory=list)        # wall time of each decode pass
+    pass_ms: List[float] = field(default_factory=list)        # wall time of each pass
+    lane_passes: Dict[str, int] = field(default_factory=dict)
     lock: threading.Lock = field(default_factory=threading.Lock)
 
     def record(self, kind: str, ms: float, n: int = 1):
@@ -171,7 +173,8 @@ class Stats:
                 return None
             return round(float(np.percentile(np.array(v), q)), 3)
 
-        out = {"requests": len(allv), "passes": self.passes, "p50_ms": pct(allv, 50), "p99_ms": pct(allv, 99),
+        out = {"requests": len(allv), "passes": self.passes, "lane_passes": dict(self.lane_passes),
+               "p50_ms": pct(allv, 50), "p99_ms": pct(allv, 99),
                "GBps": round(self.bytes_out / self.wall_s / 1e9, 3) if self.wall_s else None,
                "pass_ms": {"p50": pct(self.pass_ms, 50), "p99": pct(self.pass_ms, 99)},
                "by_kind": {k: {"n": len(v), "p50_ms": pct(sorted(v), 50), "p99_ms": pct(sorted(v), 99),
@@ -182,12 +185,23 @@ class Stats:
 
 
 class Server:
-    """Continuous-batching request loop on one GPU."""
+    """Continuous-batching request loop on one GPU.
 
-    def __init__(self, store: Store, out_budget: int = 4 << 30, encode_sources=None):
+    Requests are routed by kind to three lanes, each a host thread with its
+    own library context and so its own HIP stream: small reads (thumbnails
+    and ZFP bricks), full-chunk decodes, and encodes.  A lane runs passes:
+    each admits every request of its kind that has arrived, as far as the
+    lane's output memory allows, and runs them as one device batch per kind.
+    The lanes' work overlaps on the device, so a thumbnail never waits behind
+    a large chunk decode or an encode.
+    """
+
+    def __init__(self, store: Store, out_budget: int = 4 << 30, encode_sources=None, small_budget: int = 64 << 20):
         self.store = store
-        self.out_budget = int(out_budget)
+        self.out_budget = int(out_budget)      # full-chunk decodes
+        self.small_budget = int(small_budget)  # thumbnails and bricks
         self.out = dev.DeviceBuffer(self.out_budget)
+        self.small_out = dev.DeviceBuffer(self.small_budget)
         # encode requests take samples from these (device buffer, shape, dtype, steps, levels) sources
         self.encode_sources = encode_sources or []
         self.enc_out: Optional[dev.DeviceBuffer] = None
@@ -211,65 +225,64 @@ class Server:
             return n * np.dtype(f.dtype).itemsize
         return 0
 
-    def _run_pass(self, reqs: List[Request], stats: Optional[Stats] = None):
+    def buffer_of(self, r: Request) -> "dev.DeviceBuffer":
+        return self.out if r.kind == DECODE else self.small_out
+
+    def result_ptr(self, r: Request) -> int:
+        """Device address of a served request's output (valid until its lane's next pass)."""
+        return self.buffer_of(r).ptr + r.out_offset
+
+    def result(self, r: Request) -> bytes:
+        return self.buffer_of(r).download(r.out_offset, self.out_bytes(r))
+
+    def _thumbs(self, thumbs: List[Request], stats: Optional[Stats]):
+        s, lib = self.store, _lib()
+        t0 = time.perf_counter()
+        arr = (PlaneDesc * len(thumbs))()
+        for i, r in enumerate(thumbs):
+            c = s.ht[r.key]
+            off, ln, pre = c.plane0
+            res = min(r.res, c.levels)
+            d = arr[i]
+            d.in_offset = c.offset + off
+            d.in_len = pre[res] if pre else ln
+            d.out_offset = r.out_offset
+            d.width, d.height, d.levels = c.shape[-1], c.shape[-2], c.levels
+            d.max_res, d.prefix = res, 1
+        err = _pkg.Error()
+        _pkg._check(lib.ndic_htj2k_decode_plane_batch(C.c_void_p(s.ht_arena.ptr), C.c_void_p(self.small_out.ptr), arr,
+                                                      len(thumbs), None, C.byref(err)), err)
+        if stats:
+            stats.record(THUMB, (time.perf_counter() - t0) * 1e3, len(thumbs))
+
+    def _bricks(self, bricks: List[Request], stats: Optional[Stats]):
+        s, lib = self.store, _lib()
+        t0 = time.perf_counter()
+        arr = (BrickDesc * len(bricks))()
+        for i, r in enumerate(bricks):
+            f = s.zfp[r.key]
+            d = arr[i]
+            d.in_offset, d.in_len, d.out_offset = f.offset, f.length, r.out_offset
+            d.ndim = len(f.shape)
+            for a in range(len(f.shape)):
+                d.shape[a] = f.shape[a]
+                d.brick[a] = r.brick[a]
+            d.dtype = _pkg._zfp_dtype(f.dtype)
+            d.rate = f.rate
+        err = _pkg.Error()
+        _pkg._check(lib.ndic_zfp_decode_bricks(C.c_void_p(s.zfp_arena.ptr), C.c_void_p(self.small_out.ptr), arr,
+                                               len(bricks), None, C.byref(err)), err)
+        if stats:
+            stats.record(BRICK, (time.perf_counter() - t0) * 1e3, len(bricks))
+
+    def _decodes(self, decs: List[Request], stats: Optional[Stats]):
         s = self.store
-        tick = time.perf_counter
-        lib = _lib()
-        cursor = 0
-        for r in reqs:
-            r.out_offset = cursor
-            cursor += (self.out_bytes(r) + 255) // 256 * 256
-        thumbs = [r for r in reqs if r.kind == THUMB]
-        decs = [r for r in reqs if r.kind == DECODE]
-        bricks = [r for r in reqs if r.kind == BRICK]
-        encs = [r for r in reqs if r.kind == ENCODE]
-        if thumbs:
-            t0 = tick()
-            arr = (PlaneDesc * len(thumbs))()
-            for i, r in enumerate(thumbs):
-                c = s.ht[r.key]
-                off, ln, pre = c.plane0
-                res = min(r.res, c.levels)
-                d = arr[i]
-                d.in_offset = c.offset + off
-                d.in_len = pre[res] if pre else ln
-                d.out_offset = r.out_offset
-                d.width, d.height, d.levels = c.shape[-1], c.shape[-2], c.levels
-                d.max_res, d.prefix = res, 1
-            err = _pkg.Error()
-            _pkg._check(lib.ndic_htj2k_decode_plane_batch(C.c_void_p(s.ht_arena.ptr), C.c_void_p(self.out.ptr), arr,
-                                                          len(thumbs), None, C.byref(err)), err)
-            if stats:
-                stats.record(THUMB, (tick() - t0) * 1e3, len(thumbs))
-        if decs:
-            t0 = tick()
-            descs = [_pkg.make_desc(s.ht[r.key].shape, s.ht[r.key].dtype, s.ht[r.key].steps, s.ht[r.key].levels,
-                                    s.ht[r.key].offset, s.ht[r.key].length, r.out_offset) for r in decs]
-            _pkg.decode_batch(s.ht_arena.ptr, self.out.ptr, descs)
-            if stats:
-                stats.record(DECODE, (tick() - t0) * 1e3, len(decs))
-        if bricks:
-            t0 = tick()
-            arr = (BrickDesc * len(bricks))()
-            for i, r in enumerate(bricks):
-                f = s.zfp[r.key]
-                d = arr[i]
-                d.in_offset, d.in_len, d.out_offset = f.offset, f.length, r.out_offset
-                d.ndim = len(f.shape)
-                for a in range(len(f.shape)):
-                    d.shape[a] = f.shape[a]
-                    d.brick[a] = r.brick[a]
-                d.dtype = _pkg._zfp_dtype(f.dtype)
-                d.rate = f.rate
-            err = _pkg.Error()
-            _pkg._check(lib.ndic_zfp_decode_bricks(C.c_void_p(s.zfp_arena.ptr), C.c_void_p(self.out.ptr), arr,
-                                                   len(bricks), None, C.byref(err)), err)
-            if stats:
-                stats.record(BRICK, (tick() - t0) * 1e3, len(bricks))
-        if encs:
-            self._run_encodes(encs, stats)
-        # every batch call above returns with its work done on this thread's
-        # stream; the encoder thread's stream is not waited for
+        t0 = time.perf_counter()
+        descs = [_pkg.make_desc(s.ht[r.key].shape, s.ht[r.key].dtype, s.ht[r.key].steps, s.ht[r.key].levels,
+                                s.ht[r.key].offset, s.ht[r.key].length, r.out_offset) for r in decs]
+        _pkg.decode_batch(s.ht_arena.ptr, self.out.ptr, descs)
+        if stats:
+            stats.record(DECODE, (time.perf_counter() - t0) * 1e3, len(decs))
 
     def _run_encodes(self, encs: List[Request], stats: Optional[Stats] = None):
         if not self.encode_sources:
@@ -290,92 +303,139 @@ class Server:
         if stats:
             stats.record(ENCODE, (time.perf_counter() - t0) * 1e3, len(encs))
 
-    def run(self, requests: List[Request], verify=None) -> Stats:
-        """Serves requests (sorted by arrival, seconds from start) in real time.
-        Encodes go to an encoder thread (its own HIP stream); everything else
-        is served by passes of this thread."""
-        stats = Stats()
+    def _run_pass(self, lane: str, batch: List[Request], stats: Optional[Stats]):
+        """One pass of a lane; every call returns with its work done on the
+        calling thread's stream."""
+        cursor = 0
+        for r in batch:
+            r.out_offset = cursor
+            cursor += (self.out_bytes(r) + 255) // 256 * 256
+        if lane == "small":
+            thumbs = [r for r in batch if r.kind == THUMB]
+            bricks = [r for r in batch if r.kind == BRICK]
+            if thumbs:
+                self._thumbs(thumbs, stats)
+            if bricks:
+                self._bricks(bricks, stats)
+        elif lane == "bulk":
+            self._decodes(batch, stats)
+        else:
+            self._run_encodes(batch, stats)
+
+    def _admit(self, lane: str, pending: List[Request]) -> List[Request]:
+        if lane == "encode":
+            return list(pending)
+        budget = self.out_budget if lane == "bulk" else self.small_budget
+        batch, used = [], 0
+        for r in pending:
+            b = (self.out_bytes(r) + 255) // 256 * 256
+            if b > budget:
+                raise ValueError("a request's output exceeds its lane's output budget")
+            if batch and used + b > budget:
+                break
+            batch.append(r)
+            used += b
+        return batch
+
+    # -- lanes: persistent host threads (each keeps its library context warm)
+    _ROUTE = {THUMB: "small", BRICK: "small", DECODE: "bulk", ENCODE: "encode"}
+
+    def _lanes(self):
+        if getattr(self, "_lane_q", None) is None:
+            self._lane_q = {k: queue.Queue() for k in ("small", "bulk", "encode")}
+            self._lane_th = [threading.Thread(target=self._lane_loop, args=(k, q), daemon=True)
+                             for k, q in self._lane_q.items()]
+            for th in self._lane_th:
+                th.start()
+        return self._lane_q
+
+    def close(self):
+        """Stops the lane threads (their streams drain first)."""
+        if getattr(self, "_lane_q", None) is not None:
+            for q in self._lane_q.values():
+                q.put(("stop", None))
+            for th in self._lane_th:
+                th.join()
+            self._lane_q = None
+
+    def _lane_loop(self, name: str, q: "queue.Queue"):
         pending: List[Request] = []
-        i, n = 0, len(requests)
-        encq: "queue.Queue[Optional[Request]]" = queue.Queue()
-        finished = [0]
-        t0 = time.perf_counter()
-
-        def encoder():
+        ends: List[threading.Event] = []
+        while True:
+            block = not pending
             while True:
-                r = encq.get()
-                if r is None:
+                try:
+                    kind, val = q.get(block=block)
+                except queue.Empty:
+                    break
+                block = False
+                if kind == "req":
+                    pending.append(val)
+                elif kind == "end":
+                    ends.append(val)
+                else:
                     return
-                batch = [r]
-                while True:
+            if pending:
+                run = self._run_ctx
+                stats = run["stats"]
+                try:
+                    batch = self._admit(name, pending)
+                except BaseException as e:
+                    run["errors"].append(e)
+                    batch = list(pending)
+                    pending = []
+                else:
+                    pending = pending[len(batch):]
                     try:
-                        nxt = encq.get_nowait()
-                    except queue.Empty:
-                        break
-                    if nxt is None:
-                        encq.put(None)
-                        break
-                    batch.append(nxt)
-                self._run_encodes(batch, stats)
-                t = time.perf_counter() - t0
-                with stats.lock:
-                    for q in batch:
-                        q.done = t
-                        stats.latencies_ms.setdefault(q.kind, []).append((t - q.arrival) * 1e3)
-                    finished[0] += len(batch)
+                        tp = time.perf_counter()
+                        self._run_pass(name, batch, stats)
+                        t = time.perf_counter() - run["t0"]
+                        with stats.lock:
+                            stats.pass_ms.append((time.perf_counter() - tp) * 1e3)
+                            stats.passes += 1
+                            stats.lane_passes[name] = stats.lane_passes.get(name, 0) + 1
+                            for r in batch:
+                                r.done = t
+                                stats.latencies_ms.setdefault(r.kind, []).append((t - r.arrival) * 1e3)
+                                stats.bytes_out += self.out_bytes(r)
+                        if run["verify"] is not None and name != "encode":
+                            run["verify"](self, batch)
+                    except BaseException as e:
+                        run["errors"].append(e)
+            if not pending and ends:
+                try:
+                    dev.synchronize()  # work the verify hook queued on this lane's stream
+                except BaseException as e:
+                    self._run_ctx["errors"].append(e)
+                for ev in ends:
+                    ev.set()
+                ends = []
 
-        th = threading.Thread(target=encoder, daemon=True)
-        th.start()
+    def run(self, requests: List[Request], verify=None) -> Stats:
+        """Serves requests (sorted by arrival, seconds from start) in real time
+        and returns when every one is done and the lanes' streams have
+        drained.  verify(server, batch) runs on the lane's thread after each
+        decode pass, before the lane reuses its output memory."""
+        lanes = self._lanes()
+        stats = Stats()
+        self._run_ctx = {"stats": stats, "verify": verify, "t0": time.perf_counter(), "errors": []}
+        t0 = self._run_ctx["t0"]
+        done = {k: threading.Event() for k in lanes}
         try:
-            while True:
-                with stats.lock:
-                    if finished[0] >= n:
-                        break
-                now = time.perf_counter() - t0
-                while i < n and requests[i].arrival <= now:
-                    r = requests[i]
-                    i += 1
-                    if r.kind == ENCODE:
-                        encq.put(r)
-                    else:
-                        pending.append(r)
-                if not pending:
-                    wait = requests[i].arrival - now if i < n else 0.0005
-                    if wait > 0:
-                        time.sleep(min(wait, 0.0005))
-                    continue
-                # admission by output memory
-                batch, used = [], 0
-                for r in pending:
-                    b = (self.out_bytes(r) + 255) // 256 * 256
-                    if batch and used + b > self.out_budget:
-                        break
-                    if b > self.out_budget:
-                        raise ValueError("a request's output exceeds the server's output budget")
-                    batch.append(r)
-                    used += b
-                pending = pending[len(batch):]
-                tp = time.perf_counter()
-                self._run_pass(batch, stats)
-                t = time.perf_counter() - t0
-                with stats.lock:
-                    stats.pass_ms.append((t + t0 - tp) * 1e3)
-                    stats.passes += 1
-                    for r in batch:
-                        r.done = t
-                        stats.latencies_ms.setdefault(r.kind, []).append((t - r.arrival) * 1e3)
-                        stats.bytes_out += self.out_bytes(r)
-                    finished[0] += len(batch)
-                if verify is not None:
-                    verify(self, batch)
+            for r in requests:
+                wait = r.arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+                lanes[self._ROUTE[r.kind]].put(("req", r))
         finally:
-            encq.put(None)
-            th.join()
+            for k, q in lanes.items():
+                q.put(("end", done[k]))
+            for ev in done.values():
+                ev.wait()
         stats.wall_s = time.perf_counter() - t0
+        if self._run_ctx["errors"]:
+            raise self._run_ctx["errors"][0]
         return stats
-
-    def result(self, r: Request) -> bytes:
-        return self.out.download(r.out_offset, self.out_bytes(r))
 
 
 def poisson_requests(rng: np.random.Generator, rate: float, duration: float, store: Store,
