@@ -241,7 +241,10 @@ ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp
  * variants per shape class and stores its choices here; tests pin them.
  * Knobs: "ht_decode_wave_max_blocks" -- batch windows of at most this many
  * code-blocks decode one block per wavefront, larger ones one block per lane;
- * "ht_encode_wave_max_blocks" -- the same choice for the encoder.
+ * "ht_encode_wave_max_blocks" -- the same choice for the encoder;
+ * "parse_wave_max_planes" -- windows of at most this many planes parse one
+ * plane per wavefront (bytes and tag trees staged in LDS), larger ones one
+ * plane per thread.
  * ndic_tune_set returns 0, -1 for an unknown key, -2 for a value out of
  * range; ndic_tune_get returns the value in force (-1: unknown key);
  * ndic_tune_reset restores the built-in default; ndic_tune_name lists the

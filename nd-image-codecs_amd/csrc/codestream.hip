@@ -1,12 +1,14 @@
 // Part 1/15 codestreams and the ndht container, parsed and written on the
-// device, one thread (or workgroup) per plane.
+// device, one wavefront, thread or workgroup per plane.
 //
-// Decode: each plane's thread locates its codestream (container index
+// Decode: each plane's parser locates its codestream (container index
 // entry, crates/ndic-codestream/src/container.rs:287-340), parses the main
 // and tile-part headers (crates/ndic-codestream/src/reader.rs:120-291,
 // markers.rs:130-521, quant.rs:159-202), then every packet header in
 // resolution order with its tag trees (packet.rs:199-362, tagtree.rs,
-// bitio.rs:104-161), and writes one HT decode task per code-block.
+// bitio.rs:104-161), and writes one HT decode task per code-block.  The
+// wavefront-per-plane form stages the bytes it parses and the tag trees in
+// LDS first; the thread-per-plane form reads them from global memory.
 //
 // Encode: per-plane depth and reversible ranging (crates/ndic-zarr/src/
 // htj2k.rs:261-276, quant.rs:51-85), packet headers with tag trees
@@ -14,11 +16,13 @@
 // header and coefficient-plane index (container.rs:80-100, 249-280), then a
 // workgroup per plane copies every piece into the packed output.
 #include "codestream.hpp"
+#include "tune.hpp"
 
 namespace ndic {
 
 namespace {
 
+constexpr uint32_t kMaxPackets = 33;  // one per resolution, 32 levels at most
 constexpr uint16_t kSOC = 0xFF4F, kSIZ = 0xFF51, kCOD = 0xFF52, kCOC = 0xFF53, kQCD = 0xFF5C, kQCC = 0xFF5D,
                    kCAP = 0xFF50, kCOM = 0xFF64, kTLM = 0xFF55, kPLT = 0xFF58, kSOT = 0xFF90, kSOD = 0xFF93,
                    kEOC = 0xFFD9;
@@ -115,14 +119,33 @@ uint32_t tree_bytes_host(uint32_t w, uint32_t h) {
   return 2 * t;
 }
 
-// bitio.rs:104-161
+// Bytes of one plane codestream: a window [base, base + n) copied to LDS,
+// everything else read from global memory (the window is only a cache of
+// the same bytes).
+struct Bytes {
+  const uint8_t* g;
+  const uint8_t* s;
+  uint64_t base;
+  uint32_t n;
+  __device__ __forceinline__ uint32_t operator[](uint64_t i) const {
+    const uint64_t k = i - base;
+    return k < n ? s[k] : g[i];
+  }
+};
+__device__ __forceinline__ uint32_t rd16(const Bytes& b, uint64_t i) { return (b[i] << 8) | b[i + 1]; }
+__device__ __forceinline__ uint32_t rd32(const Bytes& b, uint64_t i) {
+  return (b[i] << 24) | (b[i + 1] << 16) | (b[i + 2] << 8) | b[i + 3];
+}
+
+// bitio.rs:104-161, over bytes [start, start + len) of a codestream
 struct BitReader {
-  const uint8_t* data;
-  uint64_t len, pos;
+  Bytes src;
+  uint64_t start, len, pos;
   uint32_t tmp, avail;
   bool unstuff, bad;
-  __device__ void init(const uint8_t* d, uint64_t n) {
-    data = d;
+  __device__ void init(const Bytes& b, uint64_t st, uint64_t n) {
+    src = b;
+    start = st;
     len = n;
     pos = 0;
     tmp = 0;
@@ -136,7 +159,8 @@ struct BitReader {
         bad = true;
         return 0;
       }
-      uint8_t t = data[pos++];
+      uint32_t t = src[start + pos];
+      pos += 1;
       tmp = t;
       avail = 8 - (unstuff ? 1 : 0);
       unstuff = t == 0xFF;
@@ -252,63 +276,63 @@ __device__ bool locate(const DevChunk& ch, uint64_t p, uint64_t gid, DevPlane& p
 
 // Main header (reader.rs:120-176, markers.rs Siz/Cod/Cap/TLM parsers,
 // quant.rs:159-202) and the device path's scope checks.
-__device__ bool parse_main(const DevChunk& ch, const uint8_t* cs, uint64_t len, uint64_t base, uint64_t gid, Hdr& h,
+__device__ bool parse_main(const DevChunk& ch, const Bytes& cs, uint64_t len, uint64_t base, uint64_t gid, Hdr& h,
                            uint64_t* pos_out) {
-  if (len < 2 || rd16(cs) != kSOC) return raise(ch, kErrCsMarker, base, gid), false;
+  if (len < 2 || rd16(cs, 0) != kSOC) return raise(ch, kErrCsMarker, base, gid), false;
   uint64_t pos = 2;
   bool has_siz = false, has_cod = false, has_qcd = false;
   for (;;) {
     if (pos + 2 > len) return raise(ch, kErrCsMarker, base + pos, gid), false;
-    uint32_t m = rd16(cs + pos);
+    uint32_t m = rd16(cs, pos);
     if (m == kSOT) break;
     if (m < 0xFF00 || pos + 4 > len) return raise(ch, kErrCsMarker, base + pos, gid), false;
-    uint32_t L = rd16(cs + pos + 2);
+    uint32_t L = rd16(cs, pos + 2);
     if (L < 2 || pos + 2 + L > len) return raise(ch, kErrCsMarker, base + pos, gid), false;
-    const uint8_t* q = cs + pos + 4;
+    const uint64_t q = pos + 4;
     uint32_t n = L - 2;
     if (m == kSIZ) {
       if (n < 36) return raise(ch, kErrCsMarker, base + pos, gid), false;
-      uint32_t csiz = rd16(q + 34);
+      uint32_t csiz = rd16(cs, q + 34);
       if (csiz == 0 || n < 36 + 3 * csiz) return raise(ch, kErrCsMarker, base + pos, gid), false;
-      uint32_t xs = rd32(q + 2), ys = rd32(q + 6), xo = rd32(q + 10), yo = rd32(q + 14);
-      uint32_t xt = rd32(q + 18), yt = rd32(q + 22), xto = rd32(q + 26), yto = rd32(q + 30);
+      uint32_t xs = rd32(cs, q + 2), ys = rd32(cs, q + 6), xo = rd32(cs, q + 10), yo = rd32(cs, q + 14);
+      uint32_t xt = rd32(cs, q + 18), yt = rd32(cs, q + 22), xto = rd32(cs, q + 26), yto = rd32(cs, q + 30);
       if (xt == 0 || yt == 0 || xs <= xo || ys <= yo || xto > xo || yto > yo)
         return raise(ch, kErrCsMarker, base + pos, gid), false;
       uint64_t tx = (uint64_t(xs - xto) + xt - 1) / xt, ty = (uint64_t(ys - yto) + yt - 1) / yt;
       if (tx != 1 || ty != 1 || xo || yo || xto || yto) return raise(ch, kErrCsUnsupported, base + pos, gid), false;
       if (csiz != 1) return raise(ch, kErrCsGeometry, base + pos, gid), false;
-      if (q[37] != 1 || q[38] != 1) return raise(ch, kErrCsUnsupported, base + pos, gid), false;
+      if (cs[q + 37] != 1 || cs[q + 38] != 1) return raise(ch, kErrCsUnsupported, base + pos, gid), false;
       h.xsiz = xs;
       h.ysiz = ys;
-      h.depth = (q[36] & 0x7F) + 1;
-      h.is_signed = (q[36] & 0x80) != 0;
+      h.depth = (cs[q + 36] & 0x7F) + 1;
+      h.is_signed = (cs[q + 36] & 0x80) != 0;
       if (h.depth > 38) return raise(ch, kErrCsMarker, base + pos, gid), false;
       has_siz = true;
     } else if (m == kCOD) {
       if (n < 10) return raise(ch, kErrCsMarker, base + pos, gid), false;
-      h.scod = q[0];
-      h.decomps = q[5];
+      h.scod = cs[q + 0];
+      h.decomps = cs[q + 5];
       if (h.decomps > 32) return raise(ch, kErrCsMarker, base + pos, gid), false;
-      if (q[6] > 8 || q[7] > 8 || q[6] + q[7] > 8) return raise(ch, kErrCsMarker, base + pos, gid), false;
+      if (cs[q + 6] > 8 || cs[q + 7] > 8 || cs[q + 6] + cs[q + 7] > 8) return raise(ch, kErrCsMarker, base + pos, gid), false;
       if (h.scod & 1) {
         if (n < 10 + h.decomps + 1) return raise(ch, kErrCsMarker, base + pos, gid), false;
         for (uint32_t i = 1; i <= h.decomps; ++i)
-          if ((q[10 + i] & 0xF) == 0 || (q[10 + i] >> 4) == 0) return raise(ch, kErrCsMarker, base + pos, gid), false;
+          if ((cs[q + 10 + i] & 0xF) == 0 || (cs[q + 10 + i] >> 4) == 0) return raise(ch, kErrCsMarker, base + pos, gid), false;
       }
-      h.layers = rd16(q + 2);
-      h.cbw = q[6];
-      h.cbh = q[7];
-      h.cb_style = q[8];
-      h.wavelet = q[9];
+      h.layers = rd16(cs, q + 2);
+      h.cbw = cs[q + 6];
+      h.cbh = cs[q + 7];
+      h.cb_style = cs[q + 8];
+      h.wavelet = cs[q + 9];
       has_cod = true;
     } else if (m == kQCD) {
       if (n == 0) return raise(ch, kErrCsMarker, base + pos, gid), false;
-      h.style = q[0] & 0x1F;
-      h.guard = q[0] >> 5;
+      h.style = cs[q + 0] & 0x1F;
+      h.guard = cs[q + 0] >> 5;
       if (h.style == 0) {
         h.nvals = n - 1;
         if (h.nvals > 100) return raise(ch, kErrCsUnsupported, base + pos, gid), false;
-        for (uint32_t i = 0; i < h.nvals; ++i) h.vals[i] = q[1 + i] >> 3;
+        for (uint32_t i = 0; i < h.nvals; ++i) h.vals[i] = cs[q + 1 + i] >> 3;
       } else if (h.style == 2) {
         if ((n - 1) % 2) return raise(ch, kErrCsMarker, base + pos, gid), false;
         h.nvals = (n - 1) / 2;  // irreversible: rejected by the scope check below
@@ -323,10 +347,10 @@ __device__ bool parse_main(const DevChunk& ch, const uint8_t* cs, uint64_t len, 
         }
       has_qcd = true;
     } else if (m == kCAP) {
-      if (n < 6 || !(rd32(q) & 0x00020000u)) return raise(ch, kErrCsMarker, base + pos, gid), false;
+      if (n < 6 || !(rd32(cs, q) & 0x00020000u)) return raise(ch, kErrCsMarker, base + pos, gid), false;
     } else if (m == kTLM) {
       if (n < 2) return raise(ch, kErrCsMarker, base + pos, gid), false;
-      uint32_t st = (q[1] >> 4) & 3, sp = (q[1] & 0x40) ? 4 : 2;
+      uint32_t st = (cs[q + 1] >> 4) & 3, sp = (cs[q + 1] & 0x40) ? 4 : 2;
       if (st == 3 || (n - 2) % (st + sp)) return raise(ch, kErrCsMarker, base + pos, gid), false;
     } else if (m == kCOC || m == kQCC) {
       return raise(ch, kErrCsUnsupported, base + pos, gid), false;
@@ -353,26 +377,38 @@ __device__ uint32_t kmax_of(const Hdr& h, uint32_t kidx) {
 }
 
 // Tile-part (reader.rs:178-291): SOT, PLT..., SOD; returns the body range.
-__device__ bool parse_tile(const DevChunk& ch, const uint8_t* cs, uint64_t len, uint64_t pos, uint64_t base,
-                           uint64_t gid, bool strict, uint64_t* body_start, uint64_t* body_end) {
+__device__ bool parse_tile(const DevChunk& ch, const Bytes& cs, uint64_t len, uint64_t pos, uint64_t base,
+                           uint64_t gid, bool strict, uint64_t* body_start, uint64_t* body_end, uint32_t* plt = nullptr,
+                           uint32_t* nplt = nullptr) {
   uint64_t sot = pos;
-  if (pos + 12 > len || rd16(cs + pos + 2) != 10) return raise(ch, kErrCsMarker, base + pos, gid), false;
-  uint32_t psot = rd32(cs + pos + 6);
+  if (pos + 12 > len || rd16(cs, pos + 2) != 10) return raise(ch, kErrCsMarker, base + pos, gid), false;
+  uint32_t psot = rd32(cs, pos + 6);
   pos += 12;
   for (;;) {
     if (pos + 2 > len) return raise(ch, kErrCsMarker, base + pos, gid), false;
-    uint32_t m = rd16(cs + pos);
+    uint32_t m = rd16(cs, pos);
     if (m == kSOD) {
       pos += 2;
       break;
     }
     if (pos + 4 > len) return raise(ch, kErrCsMarker, base + pos, gid), false;
-    uint32_t L = rd16(cs + pos + 2);
+    uint32_t L = rd16(cs, pos + 2);
     if (L < 2 || pos + 2 + L > len) return raise(ch, kErrCsMarker, base + pos, gid), false;
     if (m == kPLT) {
-      const uint8_t* q = cs + pos + 4;
+      const uint64_t q = pos + 4;
       bool pending = false;
-      for (uint32_t i = 1; i < L - 2; ++i) pending = (q[i] & 0x80) != 0;
+      uint32_t acc = 0;
+      for (uint32_t i = 1; i < L - 2; ++i) {
+        const uint32_t v = cs[q + i];
+        pending = (v & 0x80) != 0;
+        acc = (acc << 7) | (v & 0x7F);
+        if (!pending && plt) {
+          // packet lengths (markers.rs plt_payload): a cache hint for staging
+          if (*nplt < kMaxPackets) plt[*nplt] = acc;
+          *nplt += 1;
+          acc = 0;
+        }
+      }
       if (L - 2 == 0 || pending) return raise(ch, kErrCsMarker, base + pos, gid), false;
     } else if (m == kCOD || m == kQCD || m == kCOC || m == kQCC) {
       return raise(ch, kErrCsUnsupported, base + pos, gid), false;
@@ -389,8 +425,8 @@ __device__ bool parse_tile(const DevChunk& ch, const uint8_t* cs, uint64_t len, 
   if (end < pos || (strict && end > len)) return raise(ch, kErrCsMarker, base + sot, gid), false;
   if (strict) {
     // one tile-part, then EOC
-    if (end + 2 > len || rd16(cs + end) != kEOC) {
-      if (end + 2 <= len && rd16(cs + end) == kSOT) return raise(ch, kErrCsUnsupported, base + end, gid), false;
+    if (end + 2 > len || rd16(cs, end) != kEOC) {
+      if (end + 2 <= len && rd16(cs, end) == kSOT) return raise(ch, kErrCsUnsupported, base + end, gid), false;
       return raise(ch, kErrCsMarker, base + end, gid), false;
     }
   }
@@ -449,43 +485,27 @@ __global__ __launch_bounds__(256) void init_tasks_kernel(const DevChunk* __restr
   }
 }
 
-__global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks,
-                                                          uint64_t nplanes, DevPlane* planes, HtDecTask* tasks,
-                                                          const GeoPacket* __restrict__ gp,
-                                                          const GeoBand* __restrict__ gb) {
-  uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (gid >= nplanes) return;
-  const DevChunk& ch = chunks[find_chunk(chunks, nchunks, gid)];
-  uint64_t p = gid - ch.plane_base;
-  DevPlane& pl = planes[gid];
-  pl.ok = 0;
-  // the plane's tasks were set to "skip" / "zero fill" by init_tasks_kernel
-  HtDecTask* pt = tasks + ch.task_base + p * ch.blocks_per_plane;
-  if (!locate(ch, p, gid, pl)) return;
-  const uint8_t* cs = ch.in + pl.cs_off;
-  uint64_t len = pl.cs_len;
-  Hdr h;
-  uint64_t pos;
-  if (!parse_main(ch, cs, len, pl.cs_off, gid, h, &pos)) return;
-  pl.depth = h.depth;
-  pl.is_signed = h.is_signed;
-  for (uint32_t i = 0; i < 1 + 3 * ch.L && i < 100; ++i) pl.kmax[i] = (uint8_t)kmax_of(h, i);
-  uint64_t bs, be;
-  if (!parse_tile(ch, cs, len, pos, pl.cs_off, gid, !(ch.no_container & 2), &bs, &be)) return;
-  const uint8_t* body = cs + bs;
+// Parses one plane: locate, main and tile-part headers, then every packet
+// header in resolution order with its tag trees (tag: scratch of
+// ch.tag_stride bytes), writing the plane's HT tasks.  win(r, off) gives the
+// byte source for packet r starting at codestream offset off.
+template <class Win>
+__device__ void parse_plane(const DevChunk& ch, uint64_t p, uint64_t gid, DevPlane& pl, HtDecTask* pt,
+                            const Bytes& cs, const Hdr& h, uint64_t bs, uint64_t be, uint8_t* tag,
+                            const GeoPacket* __restrict__ gp, const GeoBand* __restrict__ gb, Win win) {
   uint64_t blen = be - bs;
   bool sop = (h.scod & 2) != 0, eph = (h.scod & 4) != 0;
-  uint8_t* tag = ch.tag + gid * ch.tag_stride;
   uint64_t cursor = 0;
   for (uint32_t r = 0; r <= ch.L && (int32_t)r <= ch.max_res; ++r) {
     const GeoPacket& pk = gp[ch.geo_packet + r];
     if (cursor > blen) return raise(ch, kErrPacket, pl.cs_off + bs + cursor, gid);
     uint64_t start = 0;
-    const uint8_t* d = body + cursor;
+    const uint64_t d = bs + cursor;  // packet start (codestream offset)
     uint64_t dn = blen - cursor;
-    if (sop && dn >= 6 && d[0] == 0xFF && d[1] == 0x91) start = 6;
+    const Bytes pw = win(r, d);
+    if (sop && dn >= 6 && pw[d] == 0xFF && pw[d + 1] == 0x91) start = 6;
     BitReader bb;
-    bb.init(d + start, dn - start);
+    bb.init(pw, d + start, dn - start);
     bool empty = true, any_band = false;
     // first pass over the header: inclusion, missing MSBs, passes, lengths
     uint32_t nincl = 0;
@@ -579,7 +599,7 @@ __global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __rest
     if (bb.bad) return raise(ch, kErrPacket, pl.cs_off + bs + cursor, gid);
     uint64_t hl = start + bb.terminate();
     if (eph) {
-      if (dn >= hl + 2 && d[hl] == 0xFF && d[hl + 1] == 0x92) hl += 2;
+      if (dn >= hl + 2 && pw[d + hl] == 0xFF && pw[d + hl + 1] == 0x92) hl += 2;
       else return raise(ch, kErrPacket, pl.cs_off + bs + cursor + hl, gid);
     }
     // bodies follow in band / raster order
@@ -591,7 +611,7 @@ __global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __rest
         if (!(t.flags & kTaskIncluded)) continue;
         uint64_t sl = (uint64_t)t.len1 + t.len2;
         if (q + sl > blen) return raise(ch, kErrPacket, pl.cs_off + bs + q, gid);
-        t.seg = body + q;
+        t.seg = cs.g + bs + q;
         t.err_offset = (uint32_t)(pl.cs_off + bs + q);
         q += sl;
       }
@@ -599,6 +619,140 @@ __global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __rest
     cursor = q;
   }
   pl.ok = 1;
+}
+
+__device__ bool plane_headers(const DevChunk& ch, uint64_t p, uint64_t gid, DevPlane& pl, const Bytes& cs,
+                              Hdr& h, uint64_t* bs, uint64_t* be, uint32_t* plt, uint32_t* nplt) {
+  const uint64_t len = pl.cs_len;
+  uint64_t pos;
+  if (!parse_main(ch, cs, len, pl.cs_off, gid, h, &pos)) return false;
+  pl.depth = h.depth;
+  pl.is_signed = h.is_signed;
+  for (uint32_t i = 0; i < 1 + 3 * ch.L && i < 100; ++i) pl.kmax[i] = (uint8_t)kmax_of(h, i);
+  return parse_tile(ch, cs, len, pos, pl.cs_off, gid, !(ch.no_container & 2), bs, be, plt, nplt);
+}
+
+// Thread per plane, everything read from global memory (tag trees in the
+// plane's global scratch).
+__global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks,
+                                                          uint64_t nplanes, DevPlane* planes, HtDecTask* tasks,
+                                                          const GeoPacket* __restrict__ gp,
+                                                          const GeoBand* __restrict__ gb) {
+  uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (gid >= nplanes) return;
+  const DevChunk& ch = chunks[find_chunk(chunks, nchunks, gid)];
+  uint64_t p = gid - ch.plane_base;
+  DevPlane& pl = planes[gid];
+  pl.ok = 0;
+  // the plane's tasks were set to "skip" / "zero fill" by init_tasks_kernel
+  HtDecTask* pt = tasks + ch.task_base + p * ch.blocks_per_plane;
+  if (!locate(ch, p, gid, pl)) return;
+  const Bytes cs{ch.in + pl.cs_off, nullptr, 0, 0};
+  Hdr h;
+  uint64_t bs, be;
+  if (!plane_headers(ch, p, gid, pl, cs, h, &bs, &be, nullptr, nullptr)) return;
+  parse_plane(ch, p, gid, pl, pt, cs, h, bs, be, ch.tag + gid * ch.tag_stride, gp, gb,
+              [&](uint32_t, uint64_t) { return cs; });
+}
+
+// The wave copies n bytes g[from, from + n) to LDS dst with aligned dword
+// loads (a dword holding a byte of the codestream lies in its allocation).
+__device__ void stage_bytes(uint8_t* dst, const uint8_t* g, uint64_t from, uint32_t n) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uintptr_t a0 = (uintptr_t)(g + from) & ~(uintptr_t)3;
+  const uint32_t skew = (uint32_t)((uintptr_t)(g + from) - a0);
+  const uint32_t words = (skew + n + 3) / 4;
+  for (uint32_t w0 = 0; w0 < words; w0 += 256) {
+    uint32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t w = w0 + lane + 64 * k;
+      v[k] = w < words ? *(const __attribute__((address_space(1))) uint32_t*)(a0 + 4 * (uintptr_t)w) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t w = w0 + lane + 64 * k;
+      for (uint32_t b = 0; b < 4; ++b) {
+        const int64_t i = (int64_t)(4 * w + b) - skew;
+        if (w < words && i >= 0 && i < (int64_t)n) dst[i] = (uint8_t)(v[k] >> (8 * b));
+      }
+    }
+  }
+}
+
+// Wavefront per plane: the wave copies the codestream's first kStageBytes
+// (or its headers up to SOD and the start of every packet that PLT locates)
+// into LDS, then lane 0 parses from there with the tag trees in LDS.
+constexpr uint32_t kStageBytes = 8192, kTagLds = 4096;
+__global__ __launch_bounds__(64) void parse_planes_wave_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks,
+                                                               uint64_t nplanes, DevPlane* planes, HtDecTask* tasks,
+                                                               const GeoPacket* __restrict__ gp,
+                                                               const GeoBand* __restrict__ gb) {
+  __shared__ __align__(16) uint8_t stage[kStageBytes];
+  __shared__ __align__(16) uint8_t tagl[kTagLds];
+  __shared__ uint32_t plt[kMaxPackets];
+  __shared__ uint64_t wbase[kMaxPackets];
+  __shared__ uint32_t woff[kMaxPackets], wlen[kMaxPackets];
+  __shared__ uint32_t s_ok, s_nplt;
+  __shared__ uint64_t s_bs, s_be, s_len, s_off;
+  const uint64_t gid = blockIdx.x;
+  const uint32_t lane = threadIdx.x;
+  if (gid >= nplanes) return;
+  const DevChunk& ch = chunks[find_chunk(chunks, nchunks, gid)];
+  const uint64_t p = gid - ch.plane_base;
+  DevPlane& pl = planes[gid];
+  HtDecTask* pt = tasks + ch.task_base + p * ch.blocks_per_plane;
+  Hdr h;
+  if (lane == 0) {
+    pl.ok = 0;
+    s_ok = locate(ch, p, gid, pl) ? 1u : 0u;
+    s_len = pl.cs_len;
+    s_off = pl.cs_off;
+  }
+  __syncthreads();
+  if (!s_ok) return;
+  const uint8_t* g = ch.in + s_off;
+  const uint64_t len = s_len;
+  // the first bytes: every header up to SOD, and all of a small codestream
+  const uint32_t n0 = (uint32_t)min<uint64_t>(len, kStageBytes);
+  stage_bytes(stage, g, 0, n0);
+  __syncthreads();
+  if (lane == 0) {
+    const Bytes cs{g, stage, 0, n0};
+    s_nplt = 0;
+    s_ok = plane_headers(ch, p, gid, pl, cs, h, &s_bs, &s_be, plt, &s_nplt) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!s_ok) return;
+  const uint64_t bs = s_bs;
+  // packet windows from the PLT lengths, when the first bytes do not cover
+  // the packets: the staging after the headers is shared out among them
+  const uint32_t npk = min<uint32_t>((uint32_t)ch.L + 1, (uint32_t)max(ch.max_res, 0) + 1);
+  const bool use_plt = len > kStageBytes && s_nplt == ch.L + 1 && bs < kStageBytes;
+  if (lane == 0) {
+    uint64_t at = bs;
+    const uint32_t room = use_plt ? (uint32_t)(kStageBytes - ((bs + 15) & ~15ull)) : 0u;
+    const uint32_t each = npk ? (room / npk) & ~15u : 0u;
+    for (uint32_t r = 0; r < npk && r < kMaxPackets; ++r) {
+      wbase[r] = at;
+      woff[r] = (uint32_t)((bs + 15) & ~15ull) + r * each;
+      wlen[r] = use_plt ? (uint32_t)min<uint64_t>(each, plt[r]) : 0u;
+      if (use_plt) at += plt[r];
+    }
+  }
+  __syncthreads();
+  if (use_plt)
+    for (uint32_t r = 0; r < npk; ++r) stage_bytes(stage + woff[r], g, wbase[r], (uint32_t)min<uint64_t>(wlen[r], len - min(len, wbase[r])));
+  __syncthreads();
+  if (lane != 0) return;
+  // with packet windows the staging after the headers holds them, so the
+  // first window shrinks to the headers
+  const Bytes cs{g, stage, 0, use_plt ? (uint32_t)((bs + 15) & ~15ull) : n0};
+  uint8_t* tag = ch.tag_stride <= kTagLds ? tagl : ch.tag + gid * ch.tag_stride;
+  parse_plane(ch, p, gid, pl, pt, cs, h, bs, s_be, tag, gp, gb, [&](uint32_t r, uint64_t off) {
+    if (use_plt && r < npk && off == wbase[r]) return Bytes{g, stage + woff[r], wbase[r], wlen[r]};
+    return cs;
+  });
 }
 
 // Walks concatenated codestreams of index-less chunks (htj2k.rs:367-381).
@@ -1107,8 +1261,12 @@ uint32_t tag_scratch_bytes(uint32_t nx, uint32_t ny) { return tree_bytes_host(nx
 void launch_parse_planes(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, DevPlane* planes,
                          HtDecTask* tasks, const GeoPacket* gp, const GeoBand* gb, hipStream_t stream) {
   if (!nplanes) return;
-  hipLaunchKernelGGL(parse_planes_kernel, dim3((uint32_t)ceil_div(nplanes, 64)), dim3(64), 0, stream, d_chunks,
-                     nchunks, nplanes, planes, tasks, gp, gb);
+  if ((int64_t)nplanes <= tune(kTuneParseWaveMaxPlanes) && nplanes <= 0xFFFFFFFFull)
+    hipLaunchKernelGGL(parse_planes_wave_kernel, dim3((uint32_t)nplanes), dim3(64), 0, stream, d_chunks, nchunks,
+                       nplanes, planes, tasks, gp, gb);
+  else
+    hipLaunchKernelGGL(parse_planes_kernel, dim3((uint32_t)ceil_div(nplanes, 64)), dim3(64), 0, stream, d_chunks,
+                       nchunks, nplanes, planes, tasks, gp, gb);
   NDIC_HIP(hipGetLastError());
 }
 

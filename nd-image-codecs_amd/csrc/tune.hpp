@@ -12,6 +12,9 @@ enum TuneKey : int {
   kTuneHtDecodeWaveMaxBlocks = 0,
   // the same choice for the encoder (ht_encode_wave_kernel / ht_encode_kernel)
   kTuneHtEncodeWaveMaxBlocks,
+  // windows with at most this many planes parse one plane per wavefront
+  // (bytes and tag trees staged in LDS), larger ones one plane per thread
+  kTuneParseWaveMaxPlanes,
   kTuneCount
 };
 
