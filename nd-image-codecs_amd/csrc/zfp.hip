@@ -227,15 +227,52 @@ struct BitWriter {
 };
 
 // Stream words straight from global memory (the parallel block decoders).
+// (global-space loads: a flat load would also hold up every later wait for
+// scalar loads)
 struct GlobalSrc {
   const uint8_t* base;
   uint64_t nbytes;
   __device__ uint64_t load(uint64_t w) {
+    typedef const __attribute__((address_space(1))) uint64_t* g64;
+    typedef const __attribute__((address_space(1))) uint8_t* g8;
     uint64_t b = w * 8;
-    if (b + 8 <= nbytes) return *(const uint64_t*)(base + b);
+    if (b + 8 <= nbytes) return *(g64)(uintptr_t)(base + b);
     uint64_t v = 0;
-    for (uint64_t i = b; i < nbytes && i < b + 8; ++i) v |= (uint64_t)base[i] << (8 * (i - b));
+    for (uint64_t i = b; i < nbytes && i < b + 8; ++i) v |= (uint64_t)*(g8)(uintptr_t)(base + i) << (8 * (i - b));
     return v;
+  }
+};
+
+// The stream seen through a window of 64 words, one per lane, read with
+// readlane; the next 64 words are loaded as the window moves, so the serial
+// block walk of the variable-size modes waits on memory once per 512 bytes
+// instead of once per word.
+struct WindowSrc {
+  GlobalSrc g;
+  uint64_t first;      // first word of the window (wave-uniform)
+  uint64_t cur, nxt;   // lane l: words first + l and first + 64 + l
+  __device__ void init(const uint8_t* base, uint64_t nbytes, uint64_t w) {
+    g.base = base;
+    g.nbytes = nbytes;
+    first = w;
+    cur = g.load(w + lane());
+    nxt = g.load(w + 64 + lane());
+  }
+  __device__ uint64_t load(uint64_t w) {
+    if (w - first >= 64) {
+      if (w - first < 128) {
+        cur = nxt;
+        first += 64;
+      } else {
+        first = w;
+        cur = g.load(w + lane());
+      }
+      nxt = g.load(first + 64 + lane());
+    }
+    const int i = (int)(w - first);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)cur, i);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(cur >> 32), i);
+    return ((uint64_t)hi << 32) | lo;
   }
 };
 
@@ -366,6 +403,15 @@ __device__ uint32_t decode_ints(Rd& r, uint32_t maxbits, uint32_t maxprec, UInt*
   for (int q = 0; q < R; ++q) u[q] = 0;
   uint32_t l = lane();
   for (uint32_t k = intprec; bits && k-- > kmin;) {
+    if (Parse && n == S) {
+      // every coefficient is significant: each plane left (k .. kmin) is S
+      // verbatim bits and nothing else, up to the bit budget
+      const uint64_t need = (uint64_t)(k + 1 - kmin) * S;
+      const uint32_t take = need < bits ? (uint32_t)need : bits;
+      r.skip(take);
+      bits -= take;
+      break;
+    }
     uint64_t x[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) x[q] = 0;
@@ -887,9 +933,8 @@ __global__ __launch_bounds__(64) void zfp_index_kernel(const ZfpChunk* __restric
   if (ci >= nchunks) return;
   const ZfpChunk& c = chunks[ci];
   if (c.fixed) return;
-  BitReader r;
-  r.src.base = c.src;
-  r.src.nbytes = c.src_len;
+  BitReaderT<WindowSrc> r;
+  r.src.init(c.src, c.src_len, c.header_bits >> 6);
   r.seek(c.header_bits);
   S vals[R];
   for (uint64_t k = 0; k < c.nblocks; ++k) {
