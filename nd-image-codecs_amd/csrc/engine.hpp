@@ -41,6 +41,9 @@ struct Context {
   hipStream_t stream = nullptr;
   int device = 0;
   DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, small;
+  uint64_t* queues = nullptr;  // work-queue counter pairs of the persistent kernels (zeroed)
+  // Counter pair k (0: HT decode, 1: HT encode) of this context's queues.
+  uint64_t* queue(int k);
   PinnedArena pin;
   // A pinned copy of p[0..n) for an upload queued on the stream; valid until
   // release_kept().

@@ -354,7 +354,7 @@ ndic_status ndic_dev_ht_encode_block(const uint32_t* samples, size_t width, size
     t.flags = kTaskSignMag;
     NDIC_HIP(hipMemsetAsync(d_lens, 0, 64, s));
     NDIC_HIP(hipMemcpyAsync(d_task, &t, sizeof(t), hipMemcpyHostToDevice, s));
-    if (tune(kTuneHtEncodeWaveMaxBlocks) >= 1) launch_ht_encode_wave(d_task, 1, s);
+    if (tune(kTuneHtEncodeWaveMaxBlocks) >= 1) launch_ht_encode_wave(d_task, 1, ctx.queue(1), s);
     else launch_ht_encode(d_task, 1, s);
     uint32_t lens[16];
     std::vector<uint8_t> host(slot);
@@ -401,7 +401,7 @@ ndic_status ndic_dev_ht_decode_block(const uint8_t* coded, size_t len, uint32_t*
     t.kmax = 31;
     t.flags = kTaskIncluded | kTaskSignMag | (stripe_causal ? kTaskCausal : 0);
     NDIC_HIP(hipMemcpyAsync(d_task, &t, sizeof(t), hipMemcpyHostToDevice, s));
-    if (tune(kTuneHtDecodeWaveMaxBlocks) >= 1) launch_ht_decode_wave(d_task, 1, s);
+    if (tune(kTuneHtDecodeWaveMaxBlocks) >= 1) launch_ht_decode_wave(d_task, 1, ctx.queue(0), s);
     else launch_ht_decode(d_task, nullptr, 1, s);
     launch_ht_refine(d_task, 1, s);
     int32_t e = 0;

@@ -244,7 +244,7 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   launch_parse_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_gp, d_gb, s);
   timer.mark();
   if ((int64_t)ntasks > tune(kTuneHtDecodeWaveMaxBlocks)) launch_ht_decode(d_tasks, nullptr, ntasks, s);
-  else launch_ht_decode_wave(d_tasks, ntasks, s);
+  else launch_ht_decode_wave(d_tasks, ntasks, ctx.queue(0), s);
   launch_ht_refine(d_tasks, ntasks, s);
   timer.mark();
   std::vector<DwtChunk> dw(n);
@@ -564,7 +564,7 @@ void encode_window(Context& ctx, std::vector<EncodeItem*>& items, uint8_t* d_out
   if (reupload) ctx.up(d_chunks, dc.data(), n * sizeof(DevChunk), s);
   launch_build_enc_tasks(d_chunks, (uint32_t)n, nplanes, d_planes, d_gp, d_gb, d_tasks, d_lens, slot_stride, s);
   if ((int64_t)ntasks > tune(kTuneHtEncodeWaveMaxBlocks)) launch_ht_encode(d_tasks, ntasks, s);
-  else launch_ht_encode_wave(d_tasks, ntasks, s);
+  else launch_ht_encode_wave(d_tasks, ntasks, ctx.queue(1), s);
   launch_write_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_lens, d_gp, d_gb, s);
   launch_write_chunks(d_chunks, (uint32_t)n, d_planes, d_clen, s);
   const uint64_t* p_clen = (const uint64_t*)ctx.down(d_clen, n * 8, s);

@@ -82,6 +82,7 @@ struct ContextOwner {
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->in, &c->out, &c->plane, &c->tmp, &c->coef, &c->work, &c->jobs, &c->dwtjobs, &c->small})
       b->release();
+    if (c->queues) (void)hipFree(c->queues);
     c->pin.reset();
     if (c->pin.base) (void)hipHostFree(c->pin.base);
     (void)hipStreamDestroy(c->stream);
@@ -89,6 +90,14 @@ struct ContextOwner {
   }
 };
 }  // namespace
+
+uint64_t* Context::queue(int k) {
+  if (!queues) {
+    NDIC_HIP(hipMalloc((void**)&queues, 256));
+    NDIC_HIP(hipMemsetAsync(queues, 0, 256, stream));
+  }
+  return queues + 2 * k;
+}
 
 Context& Context::get() {
   static thread_local ContextOwner owner;

@@ -18,10 +18,12 @@
 //
 // The CxtVLC/UVLC tables are derived on the host from the T.814 rows and
 // live in constant memory.
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
 #include "ht.hpp"
+#include "tune.hpp"
 
 namespace ndic {
 
@@ -104,23 +106,9 @@ void derive_uvlc(uint16_t* tbl, int modes, bool initial) {
 // ---------------------------------------------------------------------------
 // device: stream readers (mel.rs:14-173, streams.rs:20-265)
 // ---------------------------------------------------------------------------
-// Segment reads: the segment is in global memory, or (one-block-per-wave
-// decode) staged in LDS, where a word is read as two aligned words.
+// Segment reads (global memory; a 32-bit read need not be aligned).
 typedef const __attribute__((address_space(1))) uint32_t* gwords;
-__device__ __forceinline__ uint32_t ld32(const uint8_t* p) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const bool in_lds = __builtin_amdgcn_is_shared((const void*)p);
-#else
-  const bool in_lds = false;  // host pass: never executed
-#endif
-  if (in_lds) {
-    const uint32_t* a = (const uint32_t*)((uintptr_t)p & ~(uintptr_t)3);
-    const uint32_t o = (uint32_t)((uintptr_t)p & 3);
-    const uint32_t lo = a[0];
-    return o ? __builtin_amdgcn_alignbyte(a[1], lo, o) : lo;
-  }
-  return *(gwords)(uintptr_t)p;
-}
+__device__ __forceinline__ uint32_t ld32(const uint8_t* p) { return *(gwords)(uintptr_t)p; }
 __device__ __forceinline__ uint32_t ld8(const uint8_t* p) { return *p; }
 __device__ __forceinline__ void st8(uint8_t* p, uint8_t v) { *(__attribute__((address_space(1))) uint8_t*)(uintptr_t)p = v; }
 __device__ __forceinline__ void st32(void* p, uint32_t v) { *(__attribute__((address_space(1))) uint32_t*)(uintptr_t)p = v; }
@@ -355,7 +343,6 @@ __device__ void report(const HtDecTask& t, int32_t code) {
 }
 
 constexpr int kRowCap = 34;  // quads per row (64 columns) + terminators
-constexpr uint32_t kSoloSegCap = 8192;  // cleanup-segment bytes a solo decode stages in LDS
 #ifndef NDIC_HT_STAGE_COLS
 #define NDIC_HT_STAGE_COLS 8
 #endif
@@ -566,52 +553,18 @@ __device__ void fill_zero(const HtDecTask& t) {
     for (uint32_t x = 0; x < t.w; ++x) st32((uint32_t*)t.out + (uint64_t)y * t.stride + x, 0);
 }
 
-// Solo: one code-block per wave (small batches, where the latency of one
-// block matters): the wave first copies the block's cleanup segment into LDS,
-// then lane 0 decodes it from there, so its stream reads cost LDS latency
-// instead of a global round trip each.  Otherwise lane l of wave w decodes
-// block 64 w + l (in `order`).
-template <bool Solo>
+// Lane l of wave w decodes block 64 w + l (in `order`).
 __global__ __launch_bounds__(64) void ht_decode_kernel(const HtDecTask* __restrict__ tasks,
                                                       const uint32_t* __restrict__ order, uint64_t n) {
   __shared__ uint8_t lrho[kRowCap * 64];
   __shared__ uint8_t lbl[kRowCap * 64];
   __shared__ uint32_t stage[2 * kStageCols * kStagePitch];
-  __shared__ uint32_t segbuf[Solo ? kSoloSegCap / 4 : 1];
   const uint32_t lane = threadIdx.x;
-  uint64_t i;
+  uint64_t i = blockIdx.x * (uint64_t)64 + lane;
   HtDecTask task{};
-  if (Solo) {
-    i = blockIdx.x;
-    const HtDecTask t = tasks[i];
-    const bool staged = (t.flags & kTaskIncluded) && t.len1 <= kSoloSegCap;
-    if (staged) {
-      const uint32_t len = t.len1;
-#pragma unroll 8
-      for (uint32_t off = lane * 4; off < len; off += 256) {
-        uint32_t v;
-        if (off + 4 <= len) {
-          v = *(gwords)(uintptr_t)(t.seg + off);
-        } else {
-          v = 0;
-          for (uint32_t b = 0; off + b < len; ++b) v |= uint32_t(t.seg[off + b]) << (8 * b);
-        }
-        segbuf[off / 4] = v;
-      }
-    }
-    __syncthreads();
-    if (lane == 0) {
-      task = t;
-      if (staged) task.seg = (const uint8_t*)segbuf;
-    } else {
-      i = n;  // the other lanes carry no block
-    }
-  } else {
-    i = blockIdx.x * (uint64_t)64 + lane;
-    if (i < n) {
-      if (order) i = order[i];
-      task = tasks[i];
-    }
+  if (i < n) {
+    if (order) i = order[i];
+    task = tasks[i];
   }
   for (int k = 0; k < kRowCap; ++k) {
     lrho[k * 64 + lane] = 0;
@@ -832,14 +785,10 @@ __device__ void decode_block_serial(const HtDecTask& task, uint8_t* lds, HtDecTa
   if (task.passes > 1 && (!d.active || d.bad)) slot->passes = 1;
 }
 
-__global__ __launch_bounds__(64) void ht_decode_wave_kernel(const HtDecTask* __restrict__ tasks, uint64_t n) {
-  __shared__ uint32_t msbuf[kWaveMsWords];
-  __shared__ uint32_t melbuf[kWaveSWords];
-  __shared__ uint32_t vlcbuf[kWaveSWords];
-  __shared__ uint32_t blrow[68];
+// Decodes task i (the whole wave; returns together).
+__device__ void ht_decode_wave_block(const HtDecTask* __restrict__ tasks, uint64_t i, uint32_t* msbuf,
+                                     uint32_t* melbuf, uint32_t* vlcbuf, uint32_t* blrow) {
   const uint32_t lane = threadIdx.x;
-  const uint64_t i = blockIdx.x;
-  if (i >= n) return;
   const HtDecTask t = tasks[i];
   HtDecTask* slot = (HtDecTask*)tasks + i;
   const uint32_t width = t.w, height = t.h;
@@ -1038,6 +987,46 @@ __global__ __launch_bounds__(64) void ht_decode_wave_kernel(const HtDecTask* __r
       if (lane < wcap) st32(out + (uint64_t)y * stride + lane, 0);
     if (lane == 0 && t.passes > 1) slot->passes = 1;
   }
+}
+
+// Work queue of a persistent launch: q[0] hands out task indices `grain` at
+// a time, q[1] counts the waves that found it drained; the last of them
+// resets both, so the next launch on the stream starts from zero.
+__device__ __forceinline__ uint64_t queue_pop(uint64_t* q, uint32_t grain, uint64_t* s_base) {
+  __syncthreads();
+  if (threadIdx.x == 0) *s_base = atomicAdd((unsigned long long*)q, (unsigned long long)grain);
+  __syncthreads();
+  return *s_base;
+}
+__device__ __forceinline__ void queue_done(uint64_t* q) {
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd((unsigned long long*)(q + 1), 1ull) == gridDim.x - 1) {
+      atomicExch((unsigned long long*)q, 0ull);
+      atomicExch((unsigned long long*)(q + 1), 0ull);
+    }
+  }
+}
+
+// Persistent: each wave takes `grain` code-blocks at a time from the queue
+// until it is drained.
+__global__ __launch_bounds__(64) void ht_decode_wave_kernel(const HtDecTask* __restrict__ tasks, uint64_t n,
+                                                           uint64_t* queue, uint32_t grain) {
+  __shared__ uint32_t msbuf[kWaveMsWords];
+  __shared__ uint32_t melbuf[kWaveSWords];
+  __shared__ uint32_t vlcbuf[kWaveSWords];
+  __shared__ uint32_t blrow[68];
+  __shared__ uint64_t s_base;
+  for (;;) {
+    const uint64_t base = queue_pop(queue, grain, &s_base);
+    if (base >= n) break;
+    const uint64_t end = base + grain < n ? base + grain : n;
+    for (uint64_t i = base; i < end; ++i) {
+      ht_decode_wave_block(tasks, i, msbuf, melbuf, vlcbuf, blrow);
+      __syncthreads();
+    }
+  }
+  queue_done(queue);
 }
 
 // SigProp and MagRef passes (decode.rs:398-600) of the blocks that carry
@@ -1745,15 +1734,12 @@ struct WaveMelEnc {
   }
 };
 
-__global__ __launch_bounds__(64) void ht_encode_wave_kernel(const HtEncTask* __restrict__ tasks, uint64_t n) {
-  __shared__ uint32_t msraw[kEncMsWords];
-  __shared__ uint32_t vlcraw[kEncVlcWords];
-  __shared__ uint32_t erow[68];  // 1 + column: the row above's bottom-sample exponent
-  __shared__ uint32_t srow[68];  // 1 + column: the row above's bottom-sample significance
-  __shared__ uint64_t melhas[32], melbit[32];
+// Encodes task i (the whole wave; returns together).  erow / srow: 1 +
+// column, the row above's bottom-sample exponent / significance.
+__device__ void ht_encode_wave_block(const HtEncTask* __restrict__ tasks, uint64_t i, uint32_t* msraw,
+                                     uint32_t* vlcraw, uint32_t* erow, uint32_t* srow, uint64_t* melhas,
+                                     uint64_t* melbit) {
   const uint32_t lane = threadIdx.x;
-  const uint64_t i = blockIdx.x;
-  if (i >= n) return;
   const HtEncTask t = tasks[i];
   const uint32_t width = t.w, height = t.h;
   const uint32_t p = 31 - (uint32_t)t.kmax;  // 30 - missing_msbs
@@ -2010,6 +1996,27 @@ __global__ __launch_bounds__(64) void ht_encode_wave_kernel(const HtEncTask* __r
   }
 }
 
+// Persistent, fed by the same kind of queue as ht_decode_wave_kernel.
+__global__ __launch_bounds__(64) void ht_encode_wave_kernel(const HtEncTask* __restrict__ tasks, uint64_t n,
+                                                           uint64_t* queue, uint32_t grain) {
+  __shared__ uint32_t msraw[kEncMsWords];
+  __shared__ uint32_t vlcraw[kEncVlcWords];
+  __shared__ uint32_t erow[68];
+  __shared__ uint32_t srow[68];
+  __shared__ uint64_t melhas[32], melbit[32];
+  __shared__ uint64_t s_base;
+  for (;;) {
+    const uint64_t base = queue_pop(queue, grain, &s_base);
+    if (base >= n) break;
+    const uint64_t end = base + grain < n ? base + grain : n;
+    for (uint64_t i = base; i < end; ++i) {
+      ht_encode_wave_block(tasks, i, msraw, vlcraw, erow, srow, melhas, melbit);
+      __syncthreads();
+    }
+  }
+  queue_done(queue);
+}
+
 }  // namespace
 
 void ht_init_tables() {
@@ -2035,22 +2042,32 @@ void ht_init_tables() {
 
 void launch_ht_decode(const HtDecTask* d_tasks, const uint32_t* d_order, uint64_t n, hipStream_t stream) {
   if (n == 0) return;
-  hipLaunchKernelGGL(ht_decode_kernel<false>, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, d_order,
-                     n);
+  hipLaunchKernelGGL(ht_decode_kernel, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, d_order, n);
   NDIC_HIP(hipGetLastError());
 }
 
-void launch_ht_decode_solo(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream) {
-  if (n == 0) return;
-  if (n > 0xFFFFFFFFull) fail(NDIC_INVALID_ARGUMENT, "too many code-blocks for one solo decode launch");
-  hipLaunchKernelGGL(ht_decode_kernel<true>, dim3((uint32_t)n), dim3(64), 0, stream, d_tasks, nullptr, n);
-  NDIC_HIP(hipGetLastError());
+namespace {
+// Workgroups of a persistent launch: enough to fill every CU at the
+// kernel's occupancy, fewer when the queue holds fewer grains.
+template <class K> uint32_t persistent_grid(K kernel, uint64_t grains) {
+  static int cap = 0;
+  static std::once_flag once;
+  std::call_once(once, [&] {
+    int dev = 0, cus = 0, per = 0;
+    NDIC_HIP(hipGetDevice(&dev));
+    NDIC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    NDIC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64, 0));
+    cap = std::max(1, cus * std::max(1, per));
+  });
+  return (uint32_t)std::min<uint64_t>(grains, (uint64_t)cap);
 }
+}  // namespace
 
-void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream) {
+void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, uint64_t* queue, hipStream_t stream) {
   if (n == 0) return;
-  if (n > 0xFFFFFFFFull) fail(NDIC_INVALID_ARGUMENT, "too many code-blocks for one wavefront-per-block decode launch");
-  hipLaunchKernelGGL(ht_decode_wave_kernel, dim3((uint32_t)n), dim3(64), 0, stream, d_tasks, n);
+  const uint32_t grain = (uint32_t)std::max<int64_t>(1, tune(kTuneHtWaveGrain));
+  hipLaunchKernelGGL(ht_decode_wave_kernel, dim3(persistent_grid(ht_decode_wave_kernel, ceil_div(n, grain))), dim3(64),
+                     0, stream, d_tasks, n, queue, grain);
   NDIC_HIP(hipGetLastError());
 }
 
@@ -2060,10 +2077,11 @@ void launch_ht_refine(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream) 
   NDIC_HIP(hipGetLastError());
 }
 
-void launch_ht_encode_wave(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream) {
+void launch_ht_encode_wave(const HtEncTask* d_tasks, uint64_t n, uint64_t* queue, hipStream_t stream) {
   if (n == 0) return;
-  if (n > 0xFFFFFFFFull) fail(NDIC_INVALID_ARGUMENT, "too many code-blocks for one wavefront-per-block encode launch");
-  hipLaunchKernelGGL(ht_encode_wave_kernel, dim3((uint32_t)n), dim3(64), 0, stream, d_tasks, n);
+  const uint32_t grain = (uint32_t)std::max<int64_t>(1, tune(kTuneHtWaveGrain));
+  hipLaunchKernelGGL(ht_encode_wave_kernel, dim3(persistent_grid(ht_encode_wave_kernel, ceil_div(n, grain))), dim3(64),
+                     0, stream, d_tasks, n, queue, grain);
   NDIC_HIP(hipGetLastError());
 }
 

@@ -44,19 +44,17 @@ __host__ __device__ inline uint32_t ht_ms_cap(uint32_t w, uint32_t h, uint32_t k
 void ht_init_tables();
 // d_order: the task indices to decode, in order (nullptr: 0..n-1)
 void launch_ht_decode(const HtDecTask* d_tasks, const uint32_t* d_order, uint64_t n, hipStream_t stream);
-// The same decode with one code-block per wavefront (segment staged in LDS):
-// a lower latency per block, for batches too small to fill the GPU.
-void launch_ht_decode_solo(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
 // One code-block per wavefront with the lanes working on its columns
 // together (wave-parallel stream unstuffing and MagSgn decode): the lowest
-// latency per block.
-void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
+// latency per block.  A persistent launch fed by the work queue at `queue`
+// (two zeroed counters the calling context owns; left zeroed).
+void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, uint64_t* queue, hipStream_t stream);
 // SigProp/MagRef of the tasks with more than one coding pass (after launch_ht_decode)
 void launch_ht_refine(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
 void launch_ht_encode(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream);
 // The same encode with one code-block per wavefront (lanes over columns, bit
 // offsets from wave prefix sums): the lowest latency per block.
-void launch_ht_encode_wave(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream);
+void launch_ht_encode_wave(const HtEncTask* d_tasks, uint64_t n, uint64_t* queue, hipStream_t stream);
 // Byte k of an encoded segment from its three pieces (see HtEncTask::slot).
 __host__ __device__ inline uint8_t ht_seg_byte(const uint8_t* slot, uint32_t ms_cap, const uint32_t* lens, uint32_t k) {
   if (k < lens[0]) return slot[k];

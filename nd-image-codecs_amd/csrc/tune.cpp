@@ -19,6 +19,7 @@ const Knob kKnobs[kTuneCount] = {
     {"ht_decode_wave_max_blocks", 8192, 0, INT64_MAX},
     {"ht_encode_wave_max_blocks", INT64_MAX, 0, INT64_MAX},
     {"parse_wave_max_planes", 4096, 0, INT64_MAX},
+    {"ht_wave_grain", 1, 1, 64},
 };
 
 std::atomic<int64_t> g_vals[kTuneCount] = {};

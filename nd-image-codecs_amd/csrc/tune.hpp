@@ -15,6 +15,9 @@ enum TuneKey : int {
   // windows with at most this many planes parse one plane per wavefront
   // (bytes and tag trees staged in LDS), larger ones one plane per thread
   kTuneParseWaveMaxPlanes,
+  // code-blocks a wave of the persistent wave-per-block coders takes from
+  // the work queue at a time
+  kTuneHtWaveGrain,
   kTuneCount
 };
 
