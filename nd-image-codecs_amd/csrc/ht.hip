@@ -32,10 +32,14 @@ namespace {
 }
 
 __constant__ uint16_t c_dec0[1024], c_dec1[1024], c_enc0[2048], c_enc1[2048], c_uvlc0[320], c_uvlc1[256];
+// 32-bit copies of the decode tables for the wave-uniform decoder: a scalar
+// load reads whole dwords, so its lookups stay on the scalar unit
+__constant__ uint32_t c_dec0w[1024], c_dec1w[1024], c_uvlc0w[320], c_uvlc1w[256];
 
 namespace {
 
 __constant__ uint8_t c_mel_exp[13] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 4, 5};
+__constant__ uint32_t c_mel_expw[13] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 4, 5};
 
 // ---------------------------------------------------------------------------
 // host: table derivation (T.814 Annex C rows -> lookup tables)
@@ -109,7 +113,9 @@ void derive_uvlc(uint16_t* tbl, int modes, bool initial) {
 // Segment reads (global memory; a 32-bit read need not be aligned).
 typedef const __attribute__((address_space(1))) uint32_t* gwords;
 __device__ __forceinline__ uint32_t ld32(const uint8_t* p) { return *(gwords)(uintptr_t)p; }
-__device__ __forceinline__ uint32_t ld8(const uint8_t* p) { return *p; }
+__device__ __forceinline__ uint32_t ld8(const uint8_t* p) {
+  return *(const __attribute__((address_space(1))) uint8_t*)(uintptr_t)p;
+}
 __device__ __forceinline__ void st8(uint8_t* p, uint8_t v) { *(__attribute__((address_space(1))) uint8_t*)(uintptr_t)p = v; }
 __device__ __forceinline__ void st32(void* p, uint32_t v) { *(__attribute__((address_space(1))) uint32_t*)(uintptr_t)p = v; }
 __device__ __forceinline__ int32_t ldi32(const int32_t* p) { return *(const __attribute__((address_space(1))) int32_t*)(uintptr_t)p; }
@@ -657,33 +663,36 @@ enum { kUnMs = 0, kUnVlc = 1, kUnMel = 2 };
 // for the byte before the first.  Returns the end position.
 template <int Kind, class At>
 __device__ uint32_t unstuff(uint32_t* buf, uint32_t n, uint32_t p0, uint32_t prev0, At at) {
+  constexpr int kPer = 16;  // bytes per lane per step: 16 loads in flight
   const uint32_t lane = threadIdx.x & 63;
   uint32_t base = p0;
-  for (uint32_t c0 = 0; c0 < n; c0 += 256) {
-    const uint32_t k0 = c0 + 4 * lane;
-    uint32_t prev = k0 == 0 ? prev0 : (k0 - 1 < n ? at(k0 - 1) : 0u);
-    uint32_t b[4], w[4], sum = 0;
+  for (uint32_t c0 = 0; c0 < n; c0 += 64 * kPer) {
+    const uint32_t k0 = c0 + kPer * lane;
+    uint32_t b[kPer];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const uint32_t k = k0 + j;
-      b[j] = k < n ? at(k) : 0u;
+    for (int j = 0; j < kPer; ++j) b[j] = k0 + j < n ? at(k0 + j) : 0u;
+    uint32_t prev = k0 == 0 ? prev0 : (k0 - 1 < n ? at(k0 - 1) : 0u);
+    uint32_t sum = 0;
+    uint32_t w[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
       uint32_t wd;
       if (Kind == kUnMs) wd = 8 - uint32_t(prev == 0xFF);
       else if (Kind == kUnVlc) wd = 8 - uint32_t(prev > 0x8F && (b[j] & 0x7F) == 0x7F);
       else wd = 8 - uint32_t(b[j] == 0xFF);
-      w[j] = k < n ? wd : 0u;
+      w[j] = k0 + j < n ? wd : 0u;
       sum += w[j];
       prev = b[j];
     }
     uint32_t total;
     uint32_t pos = base + wave_excl_scan(sum, &total);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < kPer; ++j) {
       if (k0 + j < n) {
         const uint32_t v = Kind == kUnMel ? (__builtin_bitreverse32(b[j]) >> 24) : b[j];
-        const uint32_t s = pos & 31, wi = pos >> 5;
-        atomicOr(&buf[wi], v << s);
-        if (s > 24) atomicOr(&buf[wi + 1], v >> (32 - s));
+        const uint32_t sh = pos & 31, wi = pos >> 5;
+        atomicOr(&buf[wi], v << sh);
+        if (sh > 24) atomicOr(&buf[wi + 1], v >> (32 - sh));
       }
       pos += w[j];
     }
@@ -728,7 +737,7 @@ struct WaveMel {
   int k;
   __device__ int next_run() {
     const uint32_t v = b.peek();
-    const uint32_t e = c_mel_exp[k];
+    const uint32_t e = uread(c_mel_expw[k]);
     if (v & 1) {
       b.skip(1);
       k = k + 1 < 12 ? k + 1 : 12;
@@ -882,7 +891,7 @@ __device__ void ht_decode_wave_block(const HtDecTask* __restrict__ tasks, uint64
         c_q |= (a1 & 0x20) << 4;
       }
       uint32_t vlc_val = vlc.peek();
-      uint32_t t0 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
+      uint32_t t0 = uread(initial ? c_dec0w[c_q | (vlc_val & 0x7F)] : c_dec1w[c_q | (vlc_val & 0x7F)]);
       if (c_q == 0) {
         run -= 2;
         t0 = run == -1 ? t0 : 0;
@@ -897,7 +906,7 @@ __device__ void ht_decode_wave_block(const HtDecTask* __restrict__ tasks, uint64
         c_q |= (a2 & 0x20) << 4;
       }
       vlc_val = vlc.skip(t0 & 0x7);
-      uint32_t t1 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
+      uint32_t t1 = uread(initial ? c_dec0w[c_q | (vlc_val & 0x7F)] : c_dec1w[c_q | (vlc_val & 0x7F)]);
       if (c_q == 0 && x + 2 < width) {
         run -= 2;
         t1 = run == -1 ? t1 : 0;
@@ -917,7 +926,7 @@ __device__ void ht_decode_wave_block(const HtDecTask* __restrict__ tasks, uint64
         uvlc_mode += run == -1 ? 0x40 : 0;
         if (run < 0) run = mel.next_run();
       }
-      uint32_t e = initial ? c_uvlc0[uvlc_mode + (vlc_val & 0x3F)] : c_uvlc1[uvlc_mode + (vlc_val & 0x3F)];
+      uint32_t e = uread(initial ? c_uvlc0w[uvlc_mode + (vlc_val & 0x3F)] : c_uvlc1w[uvlc_mode + (vlc_val & 0x3F)]);
       vlc_val = vlc.skip(e & 0x7);
       e >>= 3;
       const uint32_t len = e & 0xF;
@@ -2037,6 +2046,12 @@ void ht_init_tables() {
   NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_enc1), e1.data(), 4096));
   NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_uvlc0), u0.data(), 640));
   NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_uvlc1), u1.data(), 512));
+  std::vector<uint32_t> w0(d0.begin(), d0.end()), w1(d1.begin(), d1.end()), wu0(u0.begin(), u0.end()),
+      wu1(u1.begin(), u1.end());
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dec0w), w0.data(), 4096));
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dec1w), w1.data(), 4096));
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_uvlc0w), wu0.data(), 1280));
+  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_uvlc1w), wu1.data(), 1024));
   done = true;
 }
 
