@@ -963,9 +963,13 @@ __device__ __forceinline__ void put32(uint8_t* o, uint32_t& n, uint32_t v) {
 
 // Packet headers (packet.rs:69-168) and the plane's main + tile-part header
 // (writer.rs:198-322).  Sizes are bounded by the planner's strides.
-__global__ void write_planes_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks, uint64_t nplanes,
-                                    DevPlane* planes, const HtEncTask* __restrict__ tasks, const uint32_t* lens,
-                                    const GeoPacket* __restrict__ gp, const GeoBand* __restrict__ gb) {
+constexpr uint32_t kWriteTagLds = 32768;  // LDS tag-tree scratch of a 64-thread workgroup
+__global__ __launch_bounds__(64) void write_planes_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks,
+                                                          uint64_t nplanes, DevPlane* planes,
+                                                          const HtEncTask* __restrict__ tasks, const uint32_t* lens,
+                                                          const GeoPacket* __restrict__ gp,
+                                                          const GeoBand* __restrict__ gb) {
+  __shared__ __align__(16) uint8_t ltag[kWriteTagLds];
   uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   if (gid >= nplanes) return;
   const DevChunk& ch = chunks[find_chunk(chunks, nchunks, gid)];
@@ -973,7 +977,9 @@ __global__ void write_planes_kernel(const DevChunk* __restrict__ chunks, uint32_
   if (!pl.ok) return;
   uint64_t p = gid - ch.plane_base;
   uint64_t tb = ch.task_base + p * ch.blocks_per_plane;
-  uint8_t* tag = ch.tag + gid * ch.tag_stride;
+  // the tag trees in LDS when every thread's share fits, else in the plane's global scratch
+  uint8_t* tag = 64ull * ch.tag_stride <= kWriteTagLds ? ltag + threadIdx.x * ch.tag_stride
+                                                       : ch.tag + gid * ch.tag_stride;
   uint64_t body = 0;
   for (uint32_t r = 0; r <= ch.L; ++r) {
     const GeoPacket& pk = gp[ch.geo_packet + r];
@@ -1175,41 +1181,84 @@ __global__ void write_planes_kernel(const DevChunk* __restrict__ chunks, uint32_
 
 // Container header and coefficient-plane index (container.rs:249-280,
 // PlaneEntry::from_codestream container.rs:80-100).
-__global__ void write_chunks_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks, DevPlane* planes,
-                                    uint64_t* chunk_len) {
-  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+// Container header and coefficient-plane index of each chunk
+// (container.rs:80-100, 249-280), one 256-thread workgroup per chunk: the
+// planes' codestream offsets are a workgroup prefix sum of their lengths,
+// and each thread writes the index entries of its planes.
+__global__ __launch_bounds__(256) void write_chunks_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks,
+                                                           DevPlane* planes, uint64_t* chunk_len) {
+  __shared__ uint64_t part[256];
+  __shared__ uint64_t carry;
+  const uint32_t c = blockIdx.x, tid = threadIdx.x;
   if (c >= nchunks) return;
   const DevChunk& ch = chunks[c];
   uint8_t* o = ch.chunk_head;
-  uint64_t n = 0;
-  o[n++] = 'n';
-  o[n++] = 'd';
-  o[n++] = 'h';
-  o[n++] = 't';
-  o[n++] = 1;
-  o[n++] = ch.index ? 1 : 0;
-  o[n++] = (uint8_t)ch.L;
-  o[n++] = (uint8_t)ch.ndim;
-  for (uint32_t i = 0; i < ch.ndim; ++i)
-    for (int k = 0; k < 4; ++k) o[n++] = uint8_t(ch.dims[i] >> (8 * k));
-  uint64_t entry = 12 + 4ull * (ch.L + 1);
-  uint64_t off = n + (ch.index ? (uint64_t)ch.P * entry : 0);
-  for (uint64_t p = 0; p < ch.P; ++p) {
-    DevPlane& pl = planes[ch.plane_base + p];
-    pl.cs_off = off;
-    if (ch.index) {
-      for (int k = 0; k < 8; ++k) o[n++] = uint8_t(off >> (8 * k));
-      uint32_t len = (uint32_t)pl.total_len;
-      for (int k = 0; k < 4; ++k) o[n++] = uint8_t(len >> (8 * k));
-      uint32_t pre = pl.head_len;
-      for (uint32_t r = 0; r <= ch.L; ++r) {
-        pre += pl.packet_len[r];
-        for (int k = 0; k < 4; ++k) o[n++] = uint8_t(pre >> (8 * k));
+  const uint64_t fixed = 8 + 4ull * ch.ndim;
+  if (tid == 0) {
+    o[0] = 'n';
+    o[1] = 'd';
+    o[2] = 'h';
+    o[3] = 't';
+    o[4] = 1;
+    o[5] = ch.index ? 1 : 0;
+    o[6] = (uint8_t)ch.L;
+    o[7] = (uint8_t)ch.ndim;
+    carry = 0;
+  }
+  for (uint32_t i = tid; i < 4 * ch.ndim; i += 256) o[8 + i] = uint8_t(ch.dims[i / 4] >> (8 * (i % 4)));
+  const uint64_t entry = 12 + 4ull * (ch.L + 1);
+  const uint64_t first = fixed + (ch.index ? (uint64_t)ch.P * entry : 0);
+  for (uint64_t p0 = 0; p0 < ch.P; p0 += 256) {
+    const uint64_t p = p0 + tid;
+    const uint64_t len = p < ch.P ? planes[ch.plane_base + p].total_len : 0;
+    // inclusive scan of the lengths over the workgroup
+    part[tid] = len;
+    __syncthreads();
+    for (uint32_t d = 1; d < 256; d <<= 1) {
+      const uint64_t v = tid >= d ? part[tid - d] : 0;
+      __syncthreads();
+      part[tid] += v;
+      __syncthreads();
+    }
+    const uint64_t off = first + carry + part[tid] - len;
+    if (p < ch.P) {
+      DevPlane& pl = planes[ch.plane_base + p];
+      pl.cs_off = off;
+      if (ch.index) {
+        uint8_t* e = o + fixed + p * entry;
+        for (int k = 0; k < 8; ++k) e[k] = uint8_t(off >> (8 * k));
+        const uint32_t l32 = (uint32_t)len;
+        for (int k = 0; k < 4; ++k) e[8 + k] = uint8_t(l32 >> (8 * k));
+        uint32_t pre = pl.head_len;
+        for (uint32_t r = 0; r <= ch.L; ++r) {
+          pre += pl.packet_len[r];
+          for (int k = 0; k < 4; ++k) e[12 + 4 * r + k] = uint8_t(pre >> (8 * k));
+        }
       }
     }
-    off += pl.total_len;
+    __syncthreads();
+    if (tid == 255) carry += part[255];
+    __syncthreads();
   }
-  chunk_len[c] = off;
+  if (tid == 0) chunk_len[c] = first + carry;
+}
+
+// The workgroup copies n bytes (src read backward when rev), four loads in
+// flight per thread.
+__device__ __forceinline__ void wg_copy(uint8_t* dst, const uint8_t* src, uint32_t n, bool rev) {
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += 1024) {
+    uint8_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t i = i0 + 256 * k;
+      v[k] = i < n ? (rev ? src[n - 1 - i] : src[i]) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t i = i0 + 256 * k;
+      if (i < n) dst[i] = v[k];
+    }
+  }
 }
 
 // One workgroup per plane copies [chunk header (plane 0)] plane header,
@@ -1224,17 +1273,16 @@ __global__ __launch_bounds__(256) void assemble_kernel(const DevChunk* __restric
   const DevPlane& pl = planes[gid];
   uint64_t p = gid - ch.plane_base;
   uint8_t* dst = out + ch.out_off;
-  if (p == 0)
-    for (uint64_t i = threadIdx.x; i < ch.chunk_head_len; i += 256) dst[i] = ch.chunk_head[i];
+  if (p == 0) wg_copy(dst, ch.chunk_head, (uint32_t)ch.chunk_head_len, false);
   dst += pl.cs_off;
   const uint8_t* head = ch.head + gid * (uint64_t)ch.head_stride;
-  for (uint32_t i = threadIdx.x; i < pl.head_len; i += 256) dst[i] = head[i];
+  wg_copy(dst, head, pl.head_len, false);
   uint64_t pos = pl.head_len;
   uint64_t tb = ch.task_base + p * ch.blocks_per_plane;
   for (uint32_t r = 0; r <= ch.L; ++r) {
     const uint8_t* hdr = ch.pkt_hdr + (gid * (uint64_t)ch.pkt_slots + r) * (uint64_t)ch.pkt_stride;
     uint32_t hl = pl.packet_hdr[r];
-    for (uint32_t i = threadIdx.x; i < hl; i += 256) dst[pos + i] = hdr[i];
+    wg_copy(dst + pos, hdr, hl, false);
     pos += hl;
     const GeoPacket& pk = gp[ch.geo_packet + r];
     if (hl == 1 && pl.packet_len[r] == 1) continue;  // empty packet
@@ -1242,10 +1290,12 @@ __global__ __launch_bounds__(256) void assemble_kernel(const DevChunk* __restric
       uint64_t ti = tb + pk.block_first + b;
       const uint32_t* l = lens + 4 * ti;
       if (!l[3]) continue;
-      uint32_t sl = l[0] + l[1] + l[2];
+      // [MagSgn][MEL][VLC reversed] (the layout ht_seg_byte reads)
       const HtEncTask& t = tasks[ti];
-      for (uint32_t i = threadIdx.x; i < sl; i += 256) dst[pos + i] = ht_seg_byte(t.slot, t.ms_cap, l, i);
-      pos += sl;
+      wg_copy(dst + pos, t.slot, l[0], false);
+      wg_copy(dst + pos + l[0], t.slot + t.ms_cap, l[1], false);
+      wg_copy(dst + pos + l[0] + l[1], t.slot + t.ms_cap + kMelCap, l[2], true);
+      pos += (uint64_t)l[0] + l[1] + l[2];
     }
   }
   if (threadIdx.x == 0) {
@@ -1337,8 +1387,7 @@ void launch_write_planes(const DevChunk* d_chunks, uint32_t nchunks, uint64_t np
 void launch_write_chunks(const DevChunk* d_chunks, uint32_t nchunks, DevPlane* planes, uint64_t* chunk_len,
                          hipStream_t stream) {
   if (!nchunks) return;
-  hipLaunchKernelGGL(write_chunks_kernel, dim3((uint32_t)ceil_div(nchunks, 64)), dim3(64), 0, stream, d_chunks,
-                     nchunks, planes, chunk_len);
+  hipLaunchKernelGGL(write_chunks_kernel, dim3(nchunks), dim3(256), 0, stream, d_chunks, nchunks, planes, chunk_len);
   NDIC_HIP(hipGetLastError());
 }
 
