@@ -2,10 +2,11 @@
 
 Requests arrive over time (thumbnail, full-chunk decode, ZFP brick, encode)
 and go to one of three lanes by kind: small reads (thumbnails and bricks),
-full-chunk decodes, encodes.  Each lane is a host thread, which the library
-gives its own HIP stream, so the lanes' work overlaps on the device.  Each
-pass of a lane admits every request of its kind that has arrived, as far as
-the lane's output memory allows, and runs them as device batches: thumbnails
+full-chunk decodes, encodes.  Each lane has one or more worker threads, each
+of which the library gives its own HIP stream, so their work overlaps on the
+device.  Each pass of a worker admits every request of its lane that has
+arrived, as far as its output memory allows, and runs them as device
+batches: thumbnails
 through ``ndic_htj2k_decode_plane_batch`` (plane 0 cut at the ndht index's
 ``prefix[r]``, resolutions 0..r), bricks through ``ndic_zfp_decode_bricks``,
 full decodes through ``ndic_lift_ht_decode_batch``, encodes through
@@ -103,6 +104,7 @@ class Request:
     out_bytes: int = 0
     done: float = -1.0
     out_offset: int = -1
+    out_buf: object = None
 
 
 def parse_plane0(head: bytes):
@@ -187,24 +189,30 @@ class Stats:
 class Server:
     """Continuous-batching request loop on one GPU.
 
-    Requests are routed by kind to three lanes, each a host thread with its
-    own library context and so its own HIP stream: small reads (thumbnails
-    and ZFP bricks), full-chunk decodes, and encodes.  A lane runs passes:
-    each admits every request of its kind that has arrived, as far as the
-    lane's output memory allows, and runs them as one device batch per kind.
-    The lanes' work overlaps on the device, so a thumbnail never waits behind
-    a large chunk decode or an encode.
+    Requests are routed by kind to three lanes: small reads (thumbnails and
+    ZFP bricks), full-chunk decodes, and encodes.  A lane has `workers[lane]`
+    host threads, each with its own library context (and so its own HIP
+    stream) and output memory, taking requests from the lane's queue.  A
+    worker runs passes: each admits every request of its lane that has
+    arrived, as far as its output memory allows, and runs them as one device
+    batch per kind.  All workers' work overlaps on the device, so a thumbnail
+    never waits behind a large chunk decode or an encode, and an encode that
+    arrives during another starts at once.
     """
 
-    def __init__(self, store: Store, out_budget: int = 4 << 30, encode_sources=None, small_budget: int = 64 << 20):
+    def __init__(self, store: Store, out_budget: int = 4 << 30, encode_sources=None, small_budget: int = 64 << 20,
+                 workers: Optional[Dict[str, int]] = None):
         self.store = store
-        self.out_budget = int(out_budget)      # full-chunk decodes
-        self.small_budget = int(small_budget)  # thumbnails and bricks
-        self.out = dev.DeviceBuffer(self.out_budget)
-        self.small_out = dev.DeviceBuffer(self.small_budget)
+        self.out_budget = int(out_budget)      # full-chunk decodes, per worker
+        self.small_budget = int(small_budget)  # thumbnails and bricks, per worker
+        # worker threads per lane: each has its own library context (stream) and output memory
+        self.workers = dict(workers or {"small": 1, "bulk": 2, "encode": 2})
+        self.outs = {"bulk": [dev.DeviceBuffer(self.out_budget) for _ in range(self.workers["bulk"])],
+                     "small": [dev.DeviceBuffer(self.small_budget) for _ in range(self.workers["small"])]}
+        self.out = self.outs["bulk"][0]
         # encode requests take samples from these (device buffer, shape, dtype, steps, levels) sources
         self.encode_sources = encode_sources or []
-        self.enc_out: Optional[dev.DeviceBuffer] = None
+        self.enc_out: Dict[int, dev.DeviceBuffer] = {}
 
     # -- request sizing
     def out_bytes(self, r: Request) -> int:
@@ -226,7 +234,7 @@ class Server:
         return 0
 
     def buffer_of(self, r: Request) -> "dev.DeviceBuffer":
-        return self.out if r.kind == DECODE else self.small_out
+        return r.out_buf
 
     def result_ptr(self, r: Request) -> int:
         """Device address of a served request's output (valid until its lane's next pass)."""
@@ -235,7 +243,7 @@ class Server:
     def result(self, r: Request) -> bytes:
         return self.buffer_of(r).download(r.out_offset, self.out_bytes(r))
 
-    def _thumbs(self, thumbs: List[Request], stats: Optional[Stats]):
+    def _thumbs(self, thumbs: List[Request], out: "dev.DeviceBuffer", stats: Optional[Stats]):
         s, lib = self.store, _lib()
         t0 = time.perf_counter()
         arr = (PlaneDesc * len(thumbs))()
@@ -250,12 +258,12 @@ class Server:
             d.width, d.height, d.levels = c.shape[-1], c.shape[-2], c.levels
             d.max_res, d.prefix = res, 1
         err = _pkg.Error()
-        _pkg._check(lib.ndic_htj2k_decode_plane_batch(C.c_void_p(s.ht_arena.ptr), C.c_void_p(self.small_out.ptr), arr,
+        _pkg._check(lib.ndic_htj2k_decode_plane_batch(C.c_void_p(s.ht_arena.ptr), C.c_void_p(out.ptr), arr,
                                                       len(thumbs), None, C.byref(err)), err)
         if stats:
             stats.record(THUMB, (time.perf_counter() - t0) * 1e3, len(thumbs))
 
-    def _bricks(self, bricks: List[Request], stats: Optional[Stats]):
+    def _bricks(self, bricks: List[Request], out: "dev.DeviceBuffer", stats: Optional[Stats]):
         s, lib = self.store, _lib()
         t0 = time.perf_counter()
         arr = (BrickDesc * len(bricks))()
@@ -270,17 +278,17 @@ class Server:
             d.dtype = _pkg._zfp_dtype(f.dtype)
             d.rate = f.rate
         err = _pkg.Error()
-        _pkg._check(lib.ndic_zfp_decode_bricks(C.c_void_p(s.zfp_arena.ptr), C.c_void_p(self.small_out.ptr), arr,
+        _pkg._check(lib.ndic_zfp_decode_bricks(C.c_void_p(s.zfp_arena.ptr), C.c_void_p(out.ptr), arr,
                                                len(bricks), None, C.byref(err)), err)
         if stats:
             stats.record(BRICK, (time.perf_counter() - t0) * 1e3, len(bricks))
 
-    def _decodes(self, decs: List[Request], stats: Optional[Stats]):
+    def _decodes(self, decs: List[Request], out: "dev.DeviceBuffer", stats: Optional[Stats]):
         s = self.store
         t0 = time.perf_counter()
         descs = [_pkg.make_desc(s.ht[r.key].shape, s.ht[r.key].dtype, s.ht[r.key].steps, s.ht[r.key].levels,
                                 s.ht[r.key].offset, s.ht[r.key].length, r.out_offset) for r in decs]
-        _pkg.decode_batch(s.ht_arena.ptr, self.out.ptr, descs)
+        _pkg.decode_batch(s.ht_arena.ptr, out.ptr, descs)
         if stats:
             stats.record(DECODE, (time.perf_counter() - t0) * 1e3, len(decs))
 
@@ -296,29 +304,32 @@ class Server:
             by_src.setdefault(buf.ptr, []).append(_pkg.make_desc(shape, dt, steps, lv, 0, n, 0))
             total += n
         cap = 2 * total + (64 << 20)
-        if self.enc_out is None or self.enc_out.nbytes < cap:
-            self.enc_out = dev.DeviceBuffer(cap)
+        me = threading.get_ident()
+        buf = self.enc_out.get(me)
+        if buf is None or buf.nbytes < cap:
+            buf = self.enc_out[me] = dev.DeviceBuffer(cap)
         for ptr, ds in by_src.items():
-            _pkg.encode_batch(ptr, self.enc_out.ptr, self.enc_out.nbytes, ds)
+            _pkg.encode_batch(ptr, buf.ptr, buf.nbytes, ds)
         if stats:
             stats.record(ENCODE, (time.perf_counter() - t0) * 1e3, len(encs))
 
-    def _run_pass(self, lane: str, batch: List[Request], stats: Optional[Stats]):
-        """One pass of a lane; every call returns with its work done on the
-        calling thread's stream."""
+    def _run_pass(self, lane: str, out: Optional["dev.DeviceBuffer"], batch: List[Request], stats: Optional[Stats]):
+        """One pass of a lane worker into its output memory; every call
+        returns with its work done on the calling thread's stream."""
         cursor = 0
         for r in batch:
             r.out_offset = cursor
+            r.out_buf = out
             cursor += (self.out_bytes(r) + 255) // 256 * 256
         if lane == "small":
             thumbs = [r for r in batch if r.kind == THUMB]
             bricks = [r for r in batch if r.kind == BRICK]
             if thumbs:
-                self._thumbs(thumbs, stats)
+                self._thumbs(thumbs, out, stats)
             if bricks:
-                self._bricks(bricks, stats)
+                self._bricks(bricks, out, stats)
         elif lane == "bulk":
-            self._decodes(batch, stats)
+            self._decodes(batch, out, stats)
         else:
             self._run_encodes(batch, stats)
 
@@ -343,37 +354,54 @@ class Server:
     def _lanes(self):
         if getattr(self, "_lane_q", None) is None:
             self._lane_q = {k: queue.Queue() for k in ("small", "bulk", "encode")}
-            self._lane_th = [threading.Thread(target=self._lane_loop, args=(k, q), daemon=True)
-                             for k, q in self._lane_q.items()]
-            for th in self._lane_th:
-                th.start()
+            self._lane_th = []
+            for k, q in self._lane_q.items():
+                for w in range(self.workers[k]):
+                    out = self.outs[k][w] if k in self.outs else None
+                    th = threading.Thread(target=self._lane_loop, args=(k, q, out), daemon=True)
+                    th.start()
+                    self._lane_th.append(th)
         return self._lane_q
 
     def close(self):
         """Stops the lane threads (their streams drain first)."""
         if getattr(self, "_lane_q", None) is not None:
-            for q in self._lane_q.values():
-                q.put(("stop", None))
+            for k, q in self._lane_q.items():
+                for _ in range(self.workers[k]):
+                    q.put(("stop", None))
             for th in self._lane_th:
                 th.join()
             self._lane_q = None
 
-    def _lane_loop(self, name: str, q: "queue.Queue"):
+    def _lane_loop(self, name: str, q: "queue.Queue", out: Optional["dev.DeviceBuffer"]):
+        """One worker of a lane.  Workers of a lane share its queue; a worker
+        stops taking messages after the first end-of-run token it receives
+        (every request of the run was queued before the tokens), finishes its
+        pending requests, drains its stream and signals the token."""
         pending: List[Request] = []
-        ends: List[threading.Event] = []
+        end_ev: Optional[threading.Event] = None
+
+        def take(msg) -> bool:
+            nonlocal end_ev
+            kind, val = msg
+            if kind == "req":
+                pending.append(val)
+            elif kind == "end":
+                end_ev = val
+            else:
+                return False
+            return True
+
         while True:
-            block = not pending
-            while True:
+            if not pending and end_ev is None:
+                if not take(q.get()):
+                    return
+            while end_ev is None:
                 try:
-                    kind, val = q.get(block=block)
+                    msg = q.get_nowait()
                 except queue.Empty:
                     break
-                block = False
-                if kind == "req":
-                    pending.append(val)
-                elif kind == "end":
-                    ends.append(val)
-                else:
+                if not take(msg):
                     return
             if pending:
                 run = self._run_ctx
@@ -388,7 +416,7 @@ class Server:
                     pending = pending[len(batch):]
                     try:
                         tp = time.perf_counter()
-                        self._run_pass(name, batch, stats)
+                        self._run_pass(name, out, batch, stats)
                         t = time.perf_counter() - run["t0"]
                         with stats.lock:
                             stats.pass_ms.append((time.perf_counter() - tp) * 1e3)
@@ -402,14 +430,13 @@ class Server:
                             run["verify"](self, batch)
                     except BaseException as e:
                         run["errors"].append(e)
-            if not pending and ends:
+            if not pending and end_ev is not None:
                 try:
-                    dev.synchronize()  # work the verify hook queued on this lane's stream
+                    dev.synchronize()  # work the verify hook queued on this worker's stream
                 except BaseException as e:
                     self._run_ctx["errors"].append(e)
-                for ev in ends:
-                    ev.set()
-                ends = []
+                end_ev.set()
+                end_ev = None
 
     def run(self, requests: List[Request], verify=None) -> Stats:
         """Serves requests (sorted by arrival, seconds from start) in real time
@@ -420,7 +447,7 @@ class Server:
         stats = Stats()
         self._run_ctx = {"stats": stats, "verify": verify, "t0": time.perf_counter(), "errors": []}
         t0 = self._run_ctx["t0"]
-        done = {k: threading.Event() for k in lanes}
+        done = []
         try:
             for r in requests:
                 wait = r.arrival - (time.perf_counter() - t0)
@@ -429,8 +456,11 @@ class Server:
                 lanes[self._ROUTE[r.kind]].put(("req", r))
         finally:
             for k, q in lanes.items():
-                q.put(("end", done[k]))
-            for ev in done.values():
+                for _ in range(self.workers[k]):
+                    ev = threading.Event()
+                    done.append(ev)
+                    q.put(("end", ev))
+            for ev in done:
                 ev.wait()
         stats.wall_s = time.perf_counter() - t0
         if self._run_ctx["errors"]:
