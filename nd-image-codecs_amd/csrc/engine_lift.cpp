@@ -256,6 +256,7 @@ void run_lift(Context& ctx, std::vector<LiftChunk>& chunks, bool fwd, hipStream_
     uint64_t items;
     bool wide;
     uint32_t max_n;
+    bool col;
   };
   std::vector<LiftJob> all;
   std::vector<Launch> launches;
@@ -319,23 +320,30 @@ void run_lift(Context& ctx, std::vector<LiftChunk>& chunks, bool fwd, hipStream_
       // LDS jobs go out in segment-length classes so that short segments
       // are not launched with the LDS (and occupancy) of the longest one
       (void)lds_max;
+      // (inverse int32 segments of at most two synthesis levels go to the
+      // register-resident kernel, the rest to the LDS tile kernel)
       const uint32_t classes[] = {4, 8, 16, 32, 64, 128};
       uint32_t lo = 0;
       for (uint32_t cap : classes) {
-        std::vector<LiftJob> jobs;
+        std::vector<LiftJob> jobs[2];
         for (auto& j : lds_jobs) {
           uint32_t n = (uint32_t)std::min<uint64_t>(j.group, 0xFFFFFFFFu);
-          if (n > lo && n <= cap) jobs.push_back(j);
+          if (n <= lo || n > cap) continue;
+          bool col = !fwd && !wide && cap <= lift_col_limit() &&
+                     (j.kind == 0 || j.kind == 3 || j.levels <= 2 || n <= 4);
+          jobs[col].push_back(j);
         }
         lo = cap;
-        if (jobs.empty()) continue;
-        uint64_t items = 0;
-        for (auto& j : jobs) {
-          j.item_begin = items;
-          items += j.items;
+        for (int col = 0; col < 2; ++col) {
+          if (jobs[col].empty()) continue;
+          uint64_t items = 0;
+          for (auto& j : jobs[col]) {
+            j.item_begin = items;
+            items += j.items;
+          }
+          launches.push_back({all.size(), jobs[col].size(), items, wide == 1, cap, col == 1});
+          all.insert(all.end(), jobs[col].begin(), jobs[col].end());
         }
-        launches.push_back({all.size(), jobs.size(), items, wide == 1, cap});
-        all.insert(all.end(), jobs.begin(), jobs.end());
       }
       if (!glob_jobs.empty()) {
         uint64_t items = 0;
@@ -343,7 +351,7 @@ void run_lift(Context& ctx, std::vector<LiftChunk>& chunks, bool fwd, hipStream_
           j.item_begin = items;
           items += j.items;
         }
-        launches.push_back({all.size(), glob_jobs.size(), items, wide == 1, glob_max});
+        launches.push_back({all.size(), glob_jobs.size(), items, wide == 1, glob_max, false});
         all.insert(all.end(), glob_jobs.begin(), glob_jobs.end());
       }
     }
@@ -353,7 +361,7 @@ void run_lift(Context& ctx, std::vector<LiftChunk>& chunks, bool fwd, hipStream_
   // kept list; callers synchronise before the job buffer is reused
   LiftJob* d_jobs = (LiftJob*)ctx.jobs.ensure(all.size() * sizeof(LiftJob));
   ctx.up(d_jobs, all.data(), all.size() * sizeof(LiftJob), stream);
-  for (auto& l : launches) launch_lift_step(d_jobs + l.first, (uint32_t)l.count, l.items, fwd, l.wide, l.max_n, stream);
+  for (auto& l : launches) launch_lift_step(d_jobs + l.first, (uint32_t)l.count, l.items, fwd, l.wide, l.max_n, l.col, stream);
 }
 
 namespace {

@@ -224,8 +224,8 @@ __global__ __launch_bounds__(256) void lift_tile_kernel(const LiftJob* __restric
   const uint64_t elem0 = o * job.len * job.inner + g0 * job.inner + j;
   T* plane = (T*)job.plane;
   const uint64_t inner = job.inner;
-  // load (unrolled: four rows' loads in flight per thread)
-#pragma unroll 4
+  // load (unrolled: eight rows' loads in flight per thread)
+#pragma unroll 8
   for (uint32_t k = grp; k < n; k += 4) {
     T v = 0;
     if (valid) {
@@ -299,11 +299,133 @@ __global__ __launch_bounds__(256) void lift_tile_kernel(const LiftJob* __restric
   }
   // store
   bool ok = true;
+#pragma unroll 8
   for (uint32_t k = grp; k < n; k += 4) {
     if (!valid) break;
     T v = X[k * 64 + col];
     if (!fwd && job.dst) ok &= store_dtype<T>(job.dst + (elem0 + k * inner) * job.dsize, job.dtype, v);
     else plane[elem0 + k * inner] = v;
+  }
+  if (!ok) atomicOr(job.flag, 1);
+}
+
+// One inverse level on registers: s[0..ns) low band, d[0..nd) high band,
+// x[0..len) the synthesised samples (same arithmetic as level_inv).  Every
+// index is a compile-time constant; entries past ns / nd / len hold values
+// that are never stored.
+template <uint32_t H>
+__device__ __forceinline__ void synth_regs(uint32_t kind, const int32_t (&s)[H], const int32_t (&d)[H], uint32_t nd,
+                                           uint32_t len, int32_t (&x)[2 * H]) {
+  if (kind == 1) {  // haar: x0 = s - (d >> 1), x1 = x0 + d; odd length keeps the last low sample
+#pragma unroll
+    for (uint32_t i = 0; i < H; ++i) {
+      int32_t x0 = wsub(s[i], (int32_t)(d[i] >> 1));
+      x[2 * i] = i < nd ? x0 : s[i];
+      x[2 * i + 1] = wadd(x0, d[i]);
+    }
+  } else {  // 5/3: even samples from the two neighbouring details (mirrored), then odd ones
+#pragma unroll
+    for (uint32_t i = 0; i < H; ++i) {
+      int32_t dl = d[i == 0 ? 0 : i - 1];
+      int32_t dr = i < nd ? d[i] : dl;
+      x[2 * i] = wsub(s[i], (int32_t)(wadd(wadd(dl, dr), 2) >> 2));
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < H; ++i) {
+      int32_t r = 2 * i + 2 < len ? x[2 * i + 2 < 2 * H ? 2 * i + 2 : 2 * i] : x[2 * i];
+      x[2 * i + 1] = wadd(d[i], (int32_t)(wadd(wadd(x[2 * i], r), 1) >> 1));
+    }
+  }
+}
+
+template <class D> __device__ __forceinline__ bool fits(int32_t v);
+template <> __device__ __forceinline__ bool fits<uint8_t>(int32_t v) { return v >= 0 && v <= 255; }
+template <> __device__ __forceinline__ bool fits<int8_t>(int32_t v) { return v >= -128 && v <= 127; }
+template <> __device__ __forceinline__ bool fits<uint16_t>(int32_t v) { return v >= 0 && v <= 65535; }
+template <> __device__ __forceinline__ bool fits<int16_t>(int32_t v) { return v >= -32768 && v <= 32767; }
+template <> __device__ __forceinline__ bool fits<uint32_t>(int32_t v) { return v >= 0; }
+template <> __device__ __forceinline__ bool fits<int32_t>(int32_t) { return true; }
+
+template <class D, uint32_t N>
+__device__ __forceinline__ bool store_rows(D* p, uint64_t inner, const int32_t (&v)[N], uint32_t n) {
+  bool ok = true;
+#pragma unroll
+  for (uint32_t k = 0; k < N; ++k) {
+    if (k < n) {
+      p[k * inner] = (D)v[k];
+      ok &= fits<D>(v[k]);
+    }
+  }
+  return ok;
+}
+
+// Register-resident inverse step of int32 planes: one lane per column of one
+// segment of at most N rows and at most two synthesis levels, four
+// independent waves per workgroup (one work item each).  Every row of the
+// segment is loaded at once (the bands from their run-time row offsets into
+// compile-time register slots), both levels run on registers, and each row is
+// stored once, narrowed to the dtype on the last step.  No LDS, no barriers:
+// occupancy is set by registers alone and each lane has up to N loads in
+// flight.
+template <uint32_t N>
+__global__ __launch_bounds__(256) void lift_col_kernel(const LiftJob* __restrict__ jobs, uint32_t njobs) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t item = (blockIdx.x + (uint64_t)blockIdx.y * gridDim.x) * 4 + wave;
+  const LiftJob& job = jobs[find_job(jobs, njobs, item)];
+  const uint64_t local = item - job.item_begin;
+  if (local >= job.items) return;
+  const uint64_t t = local % job.tiles, rest = local / job.tiles;
+  const uint64_t sg = rest % job.segs, o = rest / job.segs;
+  const uint64_t inner = job.inner;
+  if (t * 64 + lane >= inner) return;
+  const uint64_t g0 = sg * job.group;
+  const uint32_t n = (uint32_t)(job.len - g0 < job.group ? job.len - g0 : job.group);
+  const uint64_t row0 = (o * job.len + g0) * inner + t * 64 + lane;
+  const int32_t* in = (const int32_t*)job.plane + row0;
+  int32_t out[N];
+  const uint32_t bc = job.kind == 1 || job.kind == 2 ? band_count(n, job.levels) : 0;
+  if (bc == 0) {
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) out[k] = k < n ? in[k * inner] : 0;
+    if (job.kind == 0) {  // delta: wrapping prefix sum down the column
+#pragma unroll
+      for (uint32_t k = 1; k < N; ++k) out[k] = wadd(out[k], out[k - 1]);
+    }
+  } else {
+    // level 0 bands: low rows [0, len1), high rows [len1, n)
+    const uint32_t len1 = (n + 1) / 2, nd1 = n / 2;
+    int32_t x1[N / 2], d1[N / 2];
+#pragma unroll
+    for (uint32_t i = 0; i < N / 2; ++i) d1[i] = i < nd1 ? in[(len1 + i) * inner] : 0;
+    if (bc == 2) {  // level 1 bands inside the low rows: [0, ns2), [ns2, len1)
+      const uint32_t ns2 = (len1 + 1) / 2, nd2 = len1 / 2;
+      int32_t s2[N / 4], d2[N / 4];
+#pragma unroll
+      for (uint32_t i = 0; i < N / 4; ++i) {
+        s2[i] = i < ns2 ? in[i * inner] : 0;
+        d2[i] = i < nd2 ? in[(ns2 + i) * inner] : 0;
+      }
+      synth_regs<N / 4>(job.kind, s2, d2, nd2, len1, x1);
+    } else {
+#pragma unroll
+      for (uint32_t i = 0; i < N / 2; ++i) x1[i] = i < len1 ? in[i * inner] : 0;
+    }
+    synth_regs<N / 2>(job.kind, x1, d1, nd1, n, out);
+  }
+  bool ok = true;
+  if (job.dst) {
+    uint8_t* p = job.dst + row0 * job.dsize;
+    switch (job.dtype) {
+      case NDIC_U8: ok = store_rows<uint8_t, N>(p, inner, out, n); break;
+      case NDIC_I8: ok = store_rows<int8_t, N>((int8_t*)p, inner, out, n); break;
+      case NDIC_U16: ok = store_rows<uint16_t, N>((uint16_t*)p, inner, out, n); break;
+      case NDIC_I16: ok = store_rows<int16_t, N>((int16_t*)p, inner, out, n); break;
+      case NDIC_U32: ok = store_rows<uint32_t, N>((uint32_t*)p, inner, out, n); break;
+      default: ok = store_rows<int32_t, N>((int32_t*)p, inner, out, n); break;
+    }
+  } else {
+    store_rows<int32_t, N>((int32_t*)job.plane + row0, inner, out, n);
   }
   if (!ok) atomicOr(job.flag, 1);
 }
@@ -366,9 +488,22 @@ __global__ void convert_kernel(const uint8_t* __restrict__ src, uint32_t src_dty
 
 size_t lift_lds_limit(bool wide) { return wide ? 64 : 128; }
 
+uint32_t lift_col_limit() { return 64; }
+
 void launch_lift_step(const LiftJob* d_jobs, uint32_t njobs, uint64_t items, bool fwd, bool wide, uint32_t max_n,
-                      hipStream_t stream) {
+                      bool col, hipStream_t stream) {
   if (items == 0) return;
+  if (col) {
+    if (fwd || wide || max_n > lift_col_limit()) fail(NDIC_IO, "internal: register lift launch out of its range");
+    uint64_t blocks = ceil_div(items, 4);
+    uint32_t bx = (uint32_t)(blocks < 65535 ? blocks : 65535);
+    uint32_t by = (uint32_t)ceil_div(blocks, bx);
+    auto k = max_n <= 4 ? lift_col_kernel<4> : max_n <= 8 ? lift_col_kernel<8> : max_n <= 16 ? lift_col_kernel<16>
+           : max_n <= 32 ? lift_col_kernel<32> : lift_col_kernel<64>;
+    hipLaunchKernelGGL(k, dim3(bx, by), dim3(256), 0, stream, d_jobs, njobs);
+    NDIC_HIP(hipGetLastError());
+    return;
+  }
   uint32_t gx = (uint32_t)(items < 65535 ? items : 65535);
   uint32_t gy = (uint32_t)ceil_div(items, gx);
   bool lds = max_n <= lift_lds_limit(wide);

@@ -23,8 +23,14 @@ struct LiftJob {
 
 // Largest segment length staged in LDS (longer ones run from global memory).
 size_t lift_lds_limit(bool wide);
+// Longest segment the register-resident inverse kernel takes (int32 planes,
+// at most two synthesis levels per segment).
+uint32_t lift_col_limit();
+// col: the register-resident inverse kernel (every job of the launch must be
+// an inverse int32 job whose segments have at most two bands levels and at
+// most lift_col_limit() rows).
 void launch_lift_step(const LiftJob* d_jobs, uint32_t njobs, uint64_t items, bool fwd, bool wide, uint32_t max_n,
-                      hipStream_t stream);
+                      bool col, hipStream_t stream);
 // Elementwise widen/narrow with optional per-call min/max (2 x int64) and
 // range flag.
 void launch_convert(const uint8_t* src, uint32_t src_dtype, void* plane, bool wide, uint8_t* dst, uint32_t dst_dtype,
