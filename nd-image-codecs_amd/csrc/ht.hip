@@ -116,6 +116,10 @@ __device__ __forceinline__ uint32_t ld32(const uint8_t* p) { return *(gwords)(ui
 __device__ __forceinline__ uint32_t ld8(const uint8_t* p) {
   return *(const __attribute__((address_space(1))) uint8_t*)(uintptr_t)p;
 }
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 ld128(const uint8_t* p) {
+  return *(const __attribute__((address_space(1))) u32x4*)(uintptr_t)p;
+}
 __device__ __forceinline__ void st8(uint8_t* p, uint8_t v) { *(__attribute__((address_space(1))) uint8_t*)(uintptr_t)p = v; }
 __device__ __forceinline__ void st32(void* p, uint32_t v) { *(__attribute__((address_space(1))) uint32_t*)(uintptr_t)p = v; }
 __device__ __forceinline__ int32_t ldi32(const int32_t* p) { return *(const __attribute__((address_space(1))) int32_t*)(uintptr_t)p; }
@@ -207,6 +211,11 @@ struct MelDec {
   }
 };
 
+// The MagSgn and VLC readers take their bytes 16 at a time: a queue of four
+// 32-bit words (q0 first) is refilled with one 16-byte load as soon as it
+// empties, so the load is in flight while the words before it are decoded,
+// and a lane touches each cache line of its stream a quarter as often.  The
+// last < 16 bytes of a stream go through the 4-byte / byte path as before.
 struct RevDec {
   const uint8_t* data;
   long pos;
@@ -214,6 +223,19 @@ struct RevDec {
   uint64_t tmp;
   uint32_t bits;
   bool unstuff;
+  uint32_t q0, q1, q2, q3, nq;  // bytes pos+1.. backward: q0 = the 4 highest
+  __device__ void refill() {
+    if (size >= 16 && pos >= 15) {
+      u32x4 v = ld128(data + pos - 15);
+      q0 = v.w;
+      q1 = v.z;
+      q2 = v.y;
+      q3 = v.x;
+      nq = 4;
+      pos -= 16;
+      size -= 16;
+    }
+  }
   __device__ void init_vlc(const uint8_t* coded, uint32_t lcup, uint32_t scup) {
     data = coded;
     uint8_t d = (uint8_t)ld8(coded + lcup - 2);
@@ -222,6 +244,8 @@ struct RevDec {
     size = (int)scup - 2;
     bits = 4 - uint32_t((tmp & 7) == 7);
     unstuff = (d | 0xF) > 0x8F;
+    nq = 0;
+    refill();
     read();
   }
   __device__ void init_mrp(const uint8_t* coded, uint32_t lcup, uint32_t len2) {
@@ -231,12 +255,20 @@ struct RevDec {
     tmp = 0;
     bits = 0;
     unstuff = true;
+    nq = 0;
+    refill();
     read();
   }
   __device__ void read() {
     if (bits > 32) return;
     uint32_t val = 0;
-    if (size > 3 && pos >= 3) {
+    if (nq) {
+      val = q0;
+      q0 = q1;
+      q1 = q2;
+      q2 = q3;
+      if (--nq == 0) refill();
+    } else if (size > 3 && pos >= 3) {
       val = ld32(data + pos - 3);
       pos -= 4;
       size -= 4;
@@ -286,18 +318,39 @@ template <uint32_t FILL> struct FwdDec {
   uint64_t tmp;
   uint32_t bits;
   bool unstuff;
+  uint32_t q0, q1, q2, q3, nq;  // the 16 bytes before data, q0 first
+  __device__ void refill() {
+    if (size >= 16) {
+      u32x4 v = ld128(data);
+      q0 = v.x;
+      q1 = v.y;
+      q2 = v.z;
+      q3 = v.w;
+      nq = 4;
+      data += 16;
+      size -= 16;
+    }
+  }
   __device__ void init(const uint8_t* d, uint32_t len) {
     data = d;
     size = (int)len;
     tmp = 0;
     bits = 0;
     unstuff = false;
+    nq = 0;
+    refill();
     read();
   }
   __device__ void read() {
     if (bits > 32) return;
     uint32_t val;
-    if (size > 3) {
+    if (nq) {
+      val = q0;
+      q0 = q1;
+      q1 = q2;
+      q2 = q3;
+      if (--nq == 0) refill();
+    } else if (size > 3) {
       val = ld32(data);
       data += 4;
       size -= 4;
