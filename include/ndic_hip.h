@@ -245,7 +245,10 @@ ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp
  * "parse_wave_max_planes" -- windows of at most this many planes parse one
  * plane per wavefront (bytes and tag trees staged in LDS), larger ones one
  * plane per thread; "ht_wave_grain" -- code-blocks a wavefront of the
- * persistent wave-per-block coders takes from its work queue at a time.
+ * persistent wave-per-block coders takes from its work queue at a time;
+ * "fuse_dwt_lift" -- 1 (default): a lift-ht decode whose single lift step
+ * runs along a plane axis synthesises the finest DWT level, lifts and
+ * narrows in one pass; 0: three passes (same output).
  * ndic_tune_set returns 0, -1 for an unknown key, -2 for a value out of
  * range; ndic_tune_get returns the value in force (-1: unknown key);
  * ndic_tune_reset restores the built-in default; ndic_tune_name lists the

@@ -807,6 +807,7 @@ __global__ __launch_bounds__(256) void unshift_kernel(const DevChunk* __restrict
   const DevPlane& pl = planes[gid];
   if (pl.is_signed || !pl.ok) return;
   uint32_t c = find_chunk(chunks, nchunks, gid);
+  if (!data[c]) return;  // un-shifted by a later fused pass
   const DevChunk& ch = chunks[c];
   uint64_t ow = dims[2 * c], oh = dims[2 * c + 1];
   int32_t* d = data[c] + (gid - ch.plane_base) * (uint64_t)ch.W * ch.H;

@@ -19,12 +19,14 @@
 // level order of dwt/mod.rs:133-198 (vertical first on analysis, horizontal
 // first on synthesis).
 #include "dwt.hpp"
+#include "codestream.hpp"
 
 namespace ndic {
 
 namespace {
 
-__device__ __forceinline__ uint32_t find_job(const DwtJob* jobs, uint32_t njobs, uint64_t item) {
+template <class Job>
+__device__ __forceinline__ uint32_t find_job(const Job* jobs, uint32_t njobs, uint64_t item) {
   uint32_t lo = 0, hi = njobs;
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) / 2;
@@ -204,43 +206,64 @@ __global__ __launch_bounds__(256) void dwt_inv_level(const DwtJob* __restrict__ 
 
   // ---- horizontal synthesis of T rows into LDS
   // rows 0..32: low rows i0 + m (m < 33, i0 + m < hl); rows 33..66: high rows
-  // clamp(i0 - 1 + m, 0, hh - 1), m < 34
-  for (uint32_t it = tid; it < (kTile + 3) * (kTile / 2); it += 256) {
-    uint32_t r = it / (kTile / 2), jl = it % (kTile / 2);
-    uint32_t y;       // T row (level-region row)
-    int32_t* dstrow;
-    if (r < kTile / 2 + 1) {
-      if (i0 + r >= hl) continue;
-      y = i0 + r;
-      dstrow = TLo[r];
-    } else {
-      if (hh == 0) continue;
-      int32_t k = (int32_t)i0 - 1 + (int32_t)(r - (kTile / 2 + 1));
-      k = k < 0 ? 0 : (k > (int32_t)hh - 1 ? (int32_t)hh - 1 : k);
-      y = hl + (uint32_t)k;
-      dstrow = THi[r - (kTile / 2 + 1)];
-    }
-    uint32_t jj = jb + jl;
-    if (jj >= nl) continue;
-    const int32_t* low = (y < hl ? j.src_ll : j.src) + base + y * S;
-    const int32_t* high = j.src + base + y * S + nl;
-    uint32_t c = 2 * jl;  // tile column of the pair
-    if (w == 1) {
-      dstrow[0] = low[0];
-      continue;
-    }
-    int32_t hm = high[umin(jj == 0 ? 0 : jj - 1, nh - 1)], h0 = high[umin(jj, nh - 1)];
-    int32_t x0 = low[jj] - ((hm + h0 + 2) >> 2);
-    dstrow[c] = x0;
-    if (2 * jj + 1 < w) {
-      int32_t xr;
-      if (2 * jj + 2 < w) {
-        int32_t h1 = high[umin(jj + 1, nh - 1)];
-        xr = low[jj + 1] - ((h0 + h1 + 2) >> 2);
+  // clamp(i0 - 1 + m, 0, hh - 1), m < 34.  Thread tid owns column pair
+  // jl = tid % 32 of T rows tid / 32 + 8 k (k < 9); it takes them in two
+  // batches and issues the loads of a whole batch before it uses any.
+  constexpr uint32_t kTRows = kTile + 3, kIter = (kTRows * (kTile / 2) + 255) / 256, kBatch = 5;
+  const uint32_t jl = tid & 31, rb = tid >> 5;
+  const uint32_t jj = jb + jl;
+  const uint32_t jm = umin(jj == 0 ? 0 : jj - 1, nh - 1), j0 = umin(jj, nh - 1), j1 = umin(jj + 1, nh - 1);
+  const bool has_r = 2 * jj + 1 < w, has_r2 = 2 * jj + 2 < w;
+#pragma unroll 1
+  for (uint32_t k0 = 0; k0 < kIter; k0 += kBatch) {
+    int32_t vl0[kBatch], vl1[kBatch], vhm[kBatch], vh0[kBatch], vh1[kBatch];
+    bool ok[kBatch];
+#pragma unroll
+    for (uint32_t b = 0; b < kBatch; ++b) {
+      const uint32_t r = rb + 8 * (k0 + b);
+      uint32_t y = 0;
+      ok[b] = r < kTRows && jj < nl;
+      if (r < kTile / 2 + 1) {
+        ok[b] = ok[b] && i0 + r < hl;
+        y = i0 + r;
       } else {
-        xr = x0;
+        ok[b] = ok[b] && hh > 0;
+        int32_t m = (int32_t)i0 - 1 + (int32_t)(r - (kTile / 2 + 1));
+        m = m < 0 ? 0 : (m > (int32_t)hh - 1 ? (int32_t)hh - 1 : m);
+        y = hl + (uint32_t)m;
       }
-      dstrow[c + 1] = high[jj] + ((x0 + xr) >> 1);
+      vl0[b] = vl1[b] = vhm[b] = vh0[b] = vh1[b] = 0;
+      if (ok[b]) {
+        const int32_t* low = (y < hl ? j.src_ll : j.src) + base + y * S;
+        const int32_t* high = j.src + base + y * S + nl;
+        if (w == 1) {
+          vl0[b] = low[0];
+        } else {
+          vl0[b] = low[jj];
+          vhm[b] = high[jm];
+          vh0[b] = high[j0];
+          if (has_r2) {
+            vl1[b] = low[jj + 1];
+            vh1[b] = high[j1];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t b = 0; b < kBatch; ++b) {
+      if (!ok[b]) continue;
+      const uint32_t r = rb + 8 * (k0 + b);
+      int32_t* dstrow = r < kTile / 2 + 1 ? TLo[r] : THi[r - (kTile / 2 + 1)];
+      if (w == 1) {
+        dstrow[0] = vl0[b];
+        continue;
+      }
+      const int32_t x0 = vl0[b] - ((vhm[b] + vh0[b] + 2) >> 2);
+      dstrow[2 * jl] = x0;
+      if (has_r) {
+        const int32_t xr = has_r2 ? vl1[b] - ((vh0[b] + vh1[b] + 2) >> 2) : x0;
+        dstrow[2 * jl + 1] = vh0[b] + ((x0 + xr) >> 1);
+      }
     }
   }
   __syncthreads();
@@ -267,7 +290,352 @@ __global__ __launch_bounds__(256) void dwt_inv_level(const DwtJob* __restrict__ 
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Finest synthesis level fused with the inverse nd_lift step along a plane
+// axis and the narrowing store.
+//
+// A workgroup owns one 64 x 32 tile of the plane and one segment of the lift
+// axis (z).  It synthesises the tile of one z-row's plane at a time -- the
+// horizontal pass of the T rows it needs into LDS, a barrier, the vertical
+// pass into registers: 8 samples per thread, column tid % 64, row pairs
+// tid / 64 + 4 k -- and runs the lift's inverse levels along z as a stream
+// over those register tiles: each level keeps its last even sample and detail
+// (5/3) or nothing (haar, delta) and asks the level below for the next low
+// sample when it needs one, so every plane is synthesised exactly once and
+// each output plane is stored (narrowed to the dtype) as soon as it is known.
+// Two LDS buffers alternate between planes, so one barrier per plane
+// suffices.  Arithmetic: dwt_inv_level above (T.800 F.4.8.2), the DC
+// un-shift of unsigned planes (reader.rs:626-638), kernel.rs:186-200 and
+// 243-272 (inverse haar / 5-3 levels, wrapping) and the inverse delta.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kFx = 64, kFy = 32;                       // output tile
+constexpr uint32_t kFLo = kFy / 2 + 1, kFHi = kFy / 2 + 2;   // T rows: low, high
+constexpr uint32_t kFTRows = kFLo + kFHi, kFIter = (kFTRows * (kFx / 2) + 255) / 256;
+
+struct V8 {
+  int32_t v[8];
+};
+__device__ __forceinline__ int32_t wadd32(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+__device__ __forceinline__ int32_t wsub32(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+
+struct FusedTile {
+  const DwtLiftJob& j;
+  int32_t (*TLo)[kFLo][kFx];
+  int32_t (*THi)[kFHi][kFx];
+  uint64_t zplane0;  // plane index of z-row 0 of the segment, z-rows R planes apart
+  uint32_t X0, Y0, i0, jb, nl, nh, hl, hh;
+  uint32_t buf;
+  bool ok;
+  int32_t lo, hi;  // range mode
+
+  __device__ __forceinline__ uint64_t plane_of(uint32_t k) const { return zplane0 + (uint64_t)k * j.R; }
+
+  // Synthesised tile of z-row k: sample e of this thread is column
+  // X0 + tid % 64, row Y0 + 2 (tid / 64 + 4 (e / 2)) + e % 2.
+  __device__ __forceinline__ V8 fetch(uint32_t k) {
+    const uint32_t tid = threadIdx.x, w = j.w, h = j.h;
+    const uint64_t p = plane_of(k);
+    const int32_t* a = j.A + p * j.plane_elems;
+    const int32_t* ll = j.LL + p * j.plane_elems;
+    int32_t(*lo)[kFx] = TLo[buf];
+    int32_t(*hi)[kFx] = THi[buf];
+    buf ^= 1;
+    // horizontal synthesis of the T rows (loads of all rows first)
+    {
+      const uint32_t jl = tid & 31, rb = tid >> 5, jj = jb + jl;
+      const uint32_t jm = umin(jj == 0 ? 0 : jj - 1, nh - 1), j0 = umin(jj, nh - 1), j1 = umin(jj + 1, nh - 1);
+      const bool has_r = 2 * jj + 1 < w, has_r2 = 2 * jj + 2 < w;
+      int32_t vl0[kFIter], vl1[kFIter], vhm[kFIter], vh0[kFIter], vh1[kFIter];
+      bool okr[kFIter];
+#pragma unroll
+      for (uint32_t b = 0; b < kFIter; ++b) {
+        const uint32_t r = rb + 8 * b;
+        uint32_t y = 0;
+        okr[b] = r < kFTRows && jj < nl;
+        if (r < kFLo) {
+          okr[b] = okr[b] && i0 + r < hl;
+          y = i0 + r;
+        } else {
+          okr[b] = okr[b] && hh > 0;
+          int32_t m = (int32_t)i0 - 1 + (int32_t)(r - kFLo);
+          m = m < 0 ? 0 : (m > (int32_t)hh - 1 ? (int32_t)hh - 1 : m);
+          y = hl + (uint32_t)m;
+        }
+        vl0[b] = vl1[b] = vhm[b] = vh0[b] = vh1[b] = 0;
+        if (okr[b]) {
+          const int32_t* low = (y < hl ? ll : a) + (uint64_t)y * w;
+          const int32_t* high = a + (uint64_t)y * w + nl;
+          if (w == 1) {
+            vl0[b] = low[0];
+          } else {
+            vl0[b] = low[jj];
+            vhm[b] = high[jm];
+            vh0[b] = high[j0];
+            if (has_r2) {
+              vl1[b] = low[jj + 1];
+              vh1[b] = high[j1];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (uint32_t b = 0; b < kFIter; ++b) {
+        if (!okr[b]) continue;
+        const uint32_t r = rb + 8 * b;
+        int32_t* dstrow = r < kFLo ? lo[r] : hi[r - kFLo];
+        if (w == 1) {
+          dstrow[0] = vl0[b];
+          continue;
+        }
+        const int32_t x0 = vl0[b] - ((vhm[b] + vh0[b] + 2) >> 2);
+        dstrow[2 * jl] = x0;
+        if (has_r) {
+          const int32_t xr = has_r2 ? vl1[b] - ((vh0[b] + vh1[b] + 2) >> 2) : x0;
+          dstrow[2 * jl + 1] = vh0[b] + ((x0 + xr) >> 1);
+        }
+      }
+    }
+    __syncthreads();
+    // vertical synthesis into registers
+    V8 out;
+    const uint32_t c = tid & 63, ivb = tid >> 6;
+#pragma unroll
+    for (uint32_t kk = 0; kk < 4; ++kk) {
+      const uint32_t iv = ivb + 4 * kk, ii = i0 + iv;
+      int32_t x0 = 0, x1 = 0;
+      if (h == 1) {
+        x0 = lo[0][c];
+      } else if (ii < hl) {
+        const int32_t hm = hi[iv][c], h0 = hi[iv + 1][c];
+        x0 = lo[iv][c] - ((hm + h0 + 2) >> 2);
+        const int32_t xr = 2 * ii + 2 < h ? lo[iv + 1][c] - ((h0 + hi[iv + 2][c] + 2) >> 2) : x0;
+        x1 = h0 + ((x0 + xr) >> 1);
+      }
+      out.v[2 * kk] = x0;
+      out.v[2 * kk + 1] = x1;
+    }
+    // DC un-shift with clamping of unsigned planes
+    const DevPlane& pl = j.planes[p];
+    if (!pl.is_signed && pl.ok) {
+      const int32_t half = 1 << (pl.depth - 1);
+      const int64_t mx = ((int64_t)1 << pl.depth) - 1;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int64_t v = (int64_t)out.v[e] + half;
+        out.v[e] = (int32_t)(v < 0 ? 0 : v > mx ? mx : v);
+      }
+    }
+    return out;
+  }
+
+  template <class D>
+  __device__ __forceinline__ void put_as(uint8_t* base, const V8& x, uint32_t c, uint32_t ivb) {
+    D* d = (D*)base;
+#pragma unroll
+    for (uint32_t e = 0; e < 8; ++e) {
+      const uint32_t y = Y0 + 2 * (ivb + 4 * (e / 2)) + (e & 1), xx = X0 + c;
+      if (xx < j.w && y < j.h) {
+        d[(uint64_t)y * j.w + xx] = (D)x.v[e];
+        const int32_t v = x.v[e];
+        bool f = true;
+        if (sizeof(D) == 1) f = (D)-1 > 0 ? (v >= 0 && v <= 255) : (v >= -128 && v <= 127);
+        else if (sizeof(D) == 2) f = (D)-1 > 0 ? (v >= 0 && v <= 65535) : (v >= -32768 && v <= 32767);
+        else f = (D)-1 > 0 ? v >= 0 : true;
+        ok &= f;
+      }
+    }
+  }
+  // Stores z-row k of the output, narrowed to the dtype (range mode: folds
+  // it into lo / hi).
+  __device__ __forceinline__ void put(uint32_t k, const V8& x) {
+    const uint32_t c = threadIdx.x & 63, ivb = threadIdx.x >> 6;
+    if (!j.dst) {
+#pragma unroll
+      for (uint32_t e = 0; e < 8; ++e) {
+        const uint32_t y = Y0 + 2 * (ivb + 4 * (e / 2)) + (e & 1), xx = X0 + c;
+        if (xx < j.w && y < j.h) {
+          lo = x.v[e] < lo ? x.v[e] : lo;
+          hi = x.v[e] > hi ? x.v[e] : hi;
+        }
+      }
+      return;
+    }
+    uint8_t* base = j.dst + plane_of(k) * j.plane_elems * j.dsize;
+    switch (j.dtype) {
+      case NDIC_U8: put_as<uint8_t>(base, x, c, ivb); break;
+      case NDIC_I8: put_as<int8_t>(base, x, c, ivb); break;
+      case NDIC_U16: put_as<uint16_t>(base, x, c, ivb); break;
+      case NDIC_I16: put_as<int16_t>(base, x, c, ivb); break;
+      case NDIC_U32: put_as<uint32_t>(base, x, c, ivb); break;
+      default: put_as<int32_t>(base, x, c, ivb); break;
+    }
+  }
+};
+
+// Low samples read straight from z-rows base, base + 1, ...
+struct RowSrc {
+  FusedTile* t;
+  uint32_t next_row;
+  __device__ __forceinline__ V8 next() { return t->fetch(next_row++); }
+};
+
+// One inverse level along z: yields X[0], X[1], ... X[len - 1] from the low
+// samples of Src (in order) and the detail z-rows dbase + m.
+template <class Src>
+struct ZLevel {
+  FusedTile* t;
+  Src* src;
+  uint32_t kind, len, nd, dbase, m;
+  bool has_odd;
+  V8 xe, xo, dc;
+  __device__ __forceinline__ void init(FusedTile* tile, Src* s, uint32_t k, uint32_t n, uint32_t db) {
+    t = tile;
+    src = s;
+    kind = k;
+    len = n;
+    nd = n / 2;
+    dbase = db;
+    m = 0;
+    has_odd = false;
+    if (kind == 2) {  // X[0] needs S[0] and D[0] (len >= 2, so nd >= 1)
+      dc = t->fetch(dbase);
+      const V8 s0 = src->next();
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xe.v[e] = wsub32(s0.v[e], wadd32(wadd32(dc.v[e], dc.v[e]), 2) >> 2);
+    }
+  }
+  __device__ __forceinline__ V8 next() {
+    if (has_odd) {
+      has_odd = false;
+      return xo;
+    }
+    if (kind == 1) {  // haar: X[2m] = S[m] - (D[m] >> 1), X[2m+1] = X[2m] + D[m]
+      const V8 s = src->next();
+      if (m >= nd) {  // odd length: the last low sample passes through
+        ++m;
+        return s;
+      }
+      const V8 d = t->fetch(dbase + m);
+      V8 x0;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        x0.v[e] = wsub32(s.v[e], d.v[e] >> 1);
+        xo.v[e] = wadd32(x0.v[e], d.v[e]);
+      }
+      ++m;
+      has_odd = true;
+      return x0;
+    }
+    // 5/3: xe = X[2m], dc = D[m]
+    const V8 out = xe;
+    if (m < nd) {
+      V8 xn = xe, dn = dc;
+      if (2 * m + 2 < len) {
+        if (m + 1 < nd) dn = t->fetch(dbase + m + 1);
+        const V8 s = src->next();
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xn.v[e] = wsub32(s.v[e], wadd32(wadd32(dc.v[e], dn.v[e]), 2) >> 2);
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xo.v[e] = wadd32(dc.v[e], wadd32(wadd32(xe.v[e], xn.v[e]), 1) >> 1);
+      has_odd = true;
+      xe = xn;
+      dc = dn;
+    }
+    ++m;
+    return out;
+  }
+};
+
+__device__ __forceinline__ uint32_t zbands(uint64_t n, uint32_t levels) {
+  uint32_t c = 0;
+  for (uint64_t m = n; c < levels && m >= 2; m = (m + 1) / 2) ++c;
+  return c;
+}
+
+__global__ __launch_bounds__(256) void dwt_lift_kernel(const DwtLiftJob* __restrict__ jobs, uint32_t njobs) {
+  __shared__ int32_t TLo[2][kFLo][kFx];
+  __shared__ int32_t THi[2][kFHi][kFx];
+  const uint64_t item = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  const DwtLiftJob& j = jobs[find_job(jobs, njobs, item)];
+  const uint64_t local = item - j.item_begin;
+  if (local >= j.items) return;
+  uint64_t rest = local / j.tiles;
+  const uint32_t t = (uint32_t)(local % j.tiles);
+  const uint64_t r = rest % j.R;
+  rest /= j.R;
+  const uint64_t sg = rest % j.segs, o = rest / j.segs;
+  const uint64_t g0 = sg * j.group;
+  const uint32_t n = (uint32_t)(j.len - g0 < j.group ? j.len - g0 : j.group);
+  FusedTile ft{j, TLo, THi};
+  ft.zplane0 = (o * j.len + g0) * j.R + r;
+  ft.X0 = (t % j.tiles_x) * kFx;
+  ft.Y0 = (t / j.tiles_x) * kFy;
+  ft.i0 = ft.Y0 / 2;
+  ft.jb = ft.X0 / 2;
+  ft.nl = (j.w + 1) / 2;
+  ft.nh = j.w / 2;
+  ft.hl = (j.h + 1) / 2;
+  ft.hh = j.h / 2;
+  ft.buf = 0;
+  ft.ok = true;
+  ft.lo = INT32_MAX;
+  ft.hi = INT32_MIN;
+  const uint32_t bc = j.kind == 1 || j.kind == 2 ? zbands(n, j.levels) : 0;
+  if (bc == 0) {
+    V8 acc = ft.fetch(0);
+    ft.put(0, acc);
+    for (uint32_t k = 1; k < n; ++k) {
+      V8 v = ft.fetch(k);
+      if (j.kind == 0) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v.v[e] = wadd32(v.v[e], acc.v[e]);
+      }
+      ft.put(k, v);
+      acc = v;
+    }
+  } else {
+    const uint32_t len1 = (n + 1) / 2;
+    if (bc == 1) {
+      RowSrc s0{&ft, 0};
+      ZLevel<RowSrc> l1;
+      l1.init(&ft, &s0, j.kind, n, len1);
+      for (uint32_t k = 0; k < n; ++k) ft.put(k, l1.next());
+    } else {
+      const uint32_t ns2 = (len1 + 1) / 2;
+      RowSrc s0{&ft, 0};
+      ZLevel<RowSrc> l2;
+      l2.init(&ft, &s0, j.kind, len1, ns2);
+      ZLevel<ZLevel<RowSrc>> l1;
+      l1.init(&ft, &l2, j.kind, n, len1);
+      for (uint32_t k = 0; k < n; ++k) ft.put(k, l1.next());
+    }
+  }
+  if (!ft.ok) atomicOr(j.flag, 1);
+  if (!j.dst) {  // range mode: one atomic pair per wave
+    int32_t lo = ft.lo, hi = ft.hi;
+    for (int off = 32; off > 0; off >>= 1) {
+      const int32_t a = __shfl_xor(lo, off), b = __shfl_xor(hi, off);
+      lo = a < lo ? a : lo;
+      hi = b > hi ? b : hi;
+    }
+    if ((threadIdx.x & 63) == 0 && lo <= hi) {
+      atomicMin((long long*)&j.minmax[0], (long long)lo);
+      atomicMax((long long*)&j.minmax[1], (long long)hi);
+    }
+  }
+}
+
 }  // namespace
+
+void launch_dwt_lift(const DwtLiftJob* d_jobs, uint32_t njobs, uint64_t items, hipStream_t stream) {
+  if (items == 0) return;
+  uint32_t gx = (uint32_t)(items < 65535 ? items : 65535);
+  uint32_t gy = (uint32_t)ceil_div(items, gx);
+  hipLaunchKernelGGL(dwt_lift_kernel, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs);
+  NDIC_HIP(hipGetLastError());
+}
 
 void launch_dwt_pass(int pass, const DwtJob* d_jobs, uint32_t njobs, uint64_t items, hipStream_t stream) {
   if (items == 0) return;

@@ -24,4 +24,28 @@ struct DwtJob {
 
 void launch_dwt_pass(int pass, const DwtJob* d_jobs, uint32_t njobs, uint64_t items, hipStream_t stream);
 
+struct DevPlane;
+
+// The finest synthesis level of one chunk's planes fused with its inverse
+// nd_lift step along a plane axis and the narrowing store (dwt_lift_kernel).
+// Plane p of the chunk is ((o * len + z) * R + r) for outer slab o, lift-axis
+// position z and remaining plane index r.  Work items: outer x segs x R x
+// tiles (64 x 32 output tiles of a plane).
+struct DwtLiftJob {
+  const int32_t* A;         // Mallat coefficients (planes w * h apart, rows w apart)
+  const int32_t* LL;        // reconstruction of the next coarser level (same layout)
+  const DevPlane* planes;   // the chunk's planes: DC un-shift of unsigned ones
+  uint8_t* dst;             // dtype samples in shape order (null: range only)
+  int32_t* flag;            // set when a sample does not fit the dtype
+  int64_t* minmax;          // range mode (dst null): lifted sample min / max
+  uint64_t plane_elems;     // w * h
+  uint32_t w, h;
+  uint32_t tiles_x, tiles;  // tiles per tile row / per plane
+  uint64_t outer, len, R, group, segs;
+  uint32_t kind, levels, dtype, dsize;
+  uint64_t items, item_begin;
+};
+constexpr uint32_t kDwtLiftTileW = 64, kDwtLiftTileH = 32;
+void launch_dwt_lift(const DwtLiftJob* d_jobs, uint32_t njobs, uint64_t items, hipStream_t stream);
+
 }  // namespace ndic

@@ -40,7 +40,7 @@ struct Context {
   std::recursive_mutex mu;
   hipStream_t stream = nullptr;
   int device = 0;
-  DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, small;
+  DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, fusedjobs, small;
   uint64_t* queues = nullptr;  // work-queue counter pairs of the persistent kernels (zeroed)
   // Counter pair k (0: HT decode, 1: HT encode) of this context's queues.
   uint64_t* queue(int k);
@@ -86,6 +86,7 @@ struct DwtChunk {
   uint64_t stride = 0;                 // plane row pitch (0: W)
   uint64_t plane_elems = 0;            // plane spacing (0: W * H)
   uint32_t levels = 0;
+  bool skip_finest = false;  // inverse: stop before the finest level (result = its LL source)
 };
 void level_dims(uint64_t w, uint64_t h, uint32_t level, uint64_t* ow, uint64_t* oh);
 void run_dwt_forward(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t stream);
@@ -116,5 +117,9 @@ void lift_check_budget(int64_t lo, int64_t hi, const std::vector<size_t>& shape,
 // Runs every step of every chunk, forward (widening src into plane first)
 // or inverse (narrowing plane into dst last).
 void run_lift(Context& ctx, std::vector<LiftChunk>& chunks, bool fwd, hipStream_t stream);
+// After an inverse run_lift whose last step narrowed straight into dst: runs
+// that step again in place (no dst), so the plane holds the lifted samples the
+// narrowing saw (the range an out-of-range error reports).
+void lift_complete_plane(Context& ctx, const LiftChunk& c, hipStream_t stream);
 
 }  // namespace ndic

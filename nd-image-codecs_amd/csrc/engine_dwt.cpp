@@ -62,7 +62,7 @@ void run_dwt_inverse(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t st
   for (uint32_t l = max_l; l-- > 0;) {
     std::vector<DwtJob> jobs;
     for (auto& c : chunks) {
-      if (c.levels <= l || c.nplanes == 0) continue;
+      if (c.levels <= l || c.nplanes == 0 || (l == 0 && c.skip_finest)) continue;
       uint64_t w, h;
       level_dims(c.W, c.H, l, &w, &h);
       if (w == 1 && h == 1) continue;

@@ -16,6 +16,7 @@
 #include <tuple>
 
 #include "codestream.hpp"
+#include "dwt.hpp"
 #include "engine.hpp"
 #include "ht.hpp"
 #include "items.hpp"
@@ -166,6 +167,22 @@ uint64_t planes_of(const std::vector<size_t>& shape) {
   return p;
 }
 
+// Whether a chunk's finest synthesis level, its inverse nd_lift step and the
+// narrowing store can run as one pass (dwt_lift_kernel): a lift-ht chunk
+// decoded at full resolution with at least one DWT level, one lift step along
+// a plane axis (not y or x) with at most two synthesis levels, and a dtype of
+// at most 32 bits (int32 planes).
+bool fusable(const DecodeItem& it, const DevChunk& c) {
+  if (it.bare || !it.lift || !it.d_out || it.steps.size() != 1 || c.P == 0) return false;
+  if (c.L == 0 || c.max_res != (int32_t)c.L || (c.W == 1 && c.H == 1)) return false;
+  if (lift_wide(it.out_dtype)) return false;
+  const ndic_lift_step& st = it.steps[0];
+  if ((size_t)st.dimension + 2 >= it.shape.size() || st.kind > 2) return false;
+  const uint64_t len = it.shape[st.dimension];
+  const uint64_t group = st.group == 0 ? len : std::min<uint64_t>(st.group, len);
+  return st.kind == 0 || st.levels <= 2 || group <= 4;
+}
+
 // Device memory for one window of a decode batch.  Up to
 // ht_decode_wave_max_blocks code-blocks a window decodes one block per
 // wavefront (latency); above, one block per lane (throughput).
@@ -247,9 +264,13 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   else launch_ht_decode_wave(d_tasks, ntasks, ctx.queue(0), s);
   launch_ht_refine(d_tasks, ntasks, s);
   timer.mark();
+  std::vector<char> fuse(n, 0);
+  if (tune(kTuneFuseDwtLift))
+    for (size_t i = 0; i < n; ++i) fuse[i] = fusable(*items[i], dc[i]);
   std::vector<DwtChunk> dw(n);
   for (size_t i = 0; i < n; ++i) {
     DwtChunk& w = dw[i];
+    w.skip_finest = fuse[i] != 0;
     w.A = dc[i].coef;
     uint64_t off = (uint64_t)(dc[i].coef - A);
     w.T = T + off;
@@ -268,7 +289,7 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   std::vector<int32_t*> res(n);
   std::vector<uint64_t> dims(2 * n);
   for (size_t i = 0; i < n; ++i) {
-    res[i] = dw[i].result;
+    res[i] = fuse[i] ? nullptr : dw[i].result;  // fused chunks un-shift in dwt_lift_kernel
     items[i]->d_result = dw[i].result;
     dims[2 * i] = dw[i].W;
     dims[2 * i + 1] = dw[i].H;
@@ -279,9 +300,45 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   timer.mark();
   // outputs
   std::vector<LiftChunk> lifts;
+  std::vector<DwtLiftJob> fused;
+  std::vector<size_t> fused_of(n, SIZE_MAX);
+  uint64_t fused_items = 0;
   for (size_t i = 0; i < n; ++i) {
     DecodeItem& it = *items[i];
     uint64_t total = (uint64_t)dc[i].P * dc[i].W * dc[i].H;
+    if (fuse[i]) {
+      const ndic_lift_step& st = it.steps[0];
+      DwtLiftJob f{};
+      f.A = dc[i].coef;
+      f.LL = dw[i].result;  // the reconstruction of level 1 (A when there is one level)
+      f.planes = d_planes + dc[i].plane_base;
+      f.dst = it.d_out;
+      f.flag = d_flag + i;
+      f.w = dc[i].W;
+      f.h = dc[i].H;
+      f.plane_elems = (uint64_t)f.w * f.h;
+      f.tiles_x = (uint32_t)ceil_div(f.w, kDwtLiftTileW);
+      f.tiles = f.tiles_x * (uint32_t)ceil_div(f.h, kDwtLiftTileH);
+      f.len = it.shape[st.dimension];
+      f.outer = 1;
+      for (size_t d = 0; d < st.dimension; ++d) f.outer *= it.shape[d];
+      f.R = 1;
+      for (size_t d = st.dimension + 1; d + 2 < it.shape.size(); ++d) f.R *= it.shape[d];
+      f.group = st.group == 0 ? f.len : std::min<uint64_t>(st.group, f.len);
+      f.segs = ceil_div(f.len, f.group);
+      f.kind = st.kind;
+      f.levels = st.levels;
+      f.dtype = (uint32_t)it.out_dtype;
+      f.dsize = (uint32_t)dtype_size(it.out_dtype);
+      f.items = f.outer * f.segs * f.R * f.tiles;
+      f.item_begin = fused_items;
+      fused_items += f.items;
+      if (f.items) {
+        fused_of[i] = fused.size();
+        fused.push_back(f);
+      }
+      continue;
+    }
     if (it.bare) {
       // plane decodes hand back the int32 region: into the caller's buffer
       // when one is given (the window's planes are reused afterwards)
@@ -304,6 +361,12 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
       launch_convert(nullptr, 0, res[i], false, it.d_out, it.ht_dtype, total, nullptr, d_flag + i, s);
     }
   }
+  if (!fused.empty()) {
+    // (the host table stays alive until the window's synchronisation below)
+    DwtLiftJob* d_fused = (DwtLiftJob*)ctx.fusedjobs.ensure(fused.size() * sizeof(DwtLiftJob));
+    ctx.up(d_fused, fused.data(), fused.size() * sizeof(DwtLiftJob), s);
+    launch_dwt_lift(d_fused, (uint32_t)fused.size(), fused_items, s);
+  }
   if (!lifts.empty()) run_lift(ctx, lifts, false, s);
   timer.mark();
   const int32_t* p_err = (const int32_t*)ctx.down(d_err, n * 16, s);
@@ -314,14 +377,20 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
     // algorithmic bytes per stage: parse reads the streams; HT decode reads
     // the segments and writes 4 B per coefficient; the DWT reads and writes
     // 4 B per sample; the lift reads 4 B and writes the dtype per sample
-    double in = 0, samples = 0, lift = 0;
+    // (a fused chunk's finest level counts in the last stage, which then reads
+    // 4 B per sample once and writes the dtype)
+    double in = 0, dwt = 0, lift = 0;
     for (size_t i = 0; i < n; ++i) {
       in += (double)items[i]->len;
       double sm = (double)dc[i].P * dw[i].W * dw[i].H;
-      samples += sm;
+      for (uint32_t l = fuse[i] ? 1 : 0; l < dw[i].levels; ++l) {
+        uint64_t lw, lh;
+        level_dims(dw[i].W, dw[i].H, l, &lw, &lh);
+        if (lw * lh > 1) dwt += 8.0 * (double)dc[i].P * (double)(lw * lh);
+      }
       if (!items[i]->bare) lift += sm * (4 + dtype_size(items[i]->lift ? items[i]->out_dtype : items[i]->ht_dtype));
     }
-    double bytes[5] = {in, in + 4 * (double)coef, 8 * samples, 0, lift};
+    double bytes[5] = {in, in + 4 * (double)coef, dwt, 0, lift};
     timer.collect(bytes);
   }
   for (size_t i = 0; i < n; ++i) {
@@ -339,15 +408,27 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
       int64_t init[2] = {INT64_MAX, INT64_MIN}, h[2];
       uint64_t total = (uint64_t)dc[i].P * dc[i].W * dc[i].H;
       int32_t* src = it.lift ? nullptr : res[i];
-      if (it.lift) {
+      if (it.lift && fused_of[i] == SIZE_MAX) {
         // the narrowing happened inside the lift: recompute on the lifted plane
         LiftChunk lc = lifts[0];
         for (auto& l : lifts)
           if (l.dst == it.d_out) lc = l;
+        lift_complete_plane(ctx, lc, s);
         src = (int32_t*)lc.plane;
       }
       NDIC_HIP(hipMemcpyAsync(mm, init, 16, hipMemcpyHostToDevice, s));
-      launch_convert(nullptr, 0, src, false, nullptr, 0, total, mm, nullptr, s);
+      if (fused_of[i] != SIZE_MAX) {
+        // no lifted plane was written: the fused pass again, in range mode
+        DwtLiftJob f = fused[fused_of[i]];
+        f.dst = nullptr;
+        f.minmax = mm;
+        f.item_begin = 0;
+        DwtLiftJob* d_f = (DwtLiftJob*)ctx.fusedjobs.ensure(sizeof(DwtLiftJob));
+        NDIC_HIP(hipMemcpyAsync(d_f, &f, sizeof(f), hipMemcpyHostToDevice, s));
+        launch_dwt_lift(d_f, 1, f.items, s);
+      } else {
+        launch_convert(nullptr, 0, src, false, nullptr, 0, total, mm, nullptr, s);
+      }
       NDIC_HIP(hipMemcpyAsync(h, mm, 16, hipMemcpyDeviceToHost, s));
       NDIC_HIP(hipStreamSynchronize(s));
       std::ostringstream msg;

@@ -364,6 +364,15 @@ void run_lift(Context& ctx, std::vector<LiftChunk>& chunks, bool fwd, hipStream_
   for (auto& l : launches) launch_lift_step(d_jobs + l.first, (uint32_t)l.count, l.items, fwd, l.wide, l.max_n, l.col, stream);
 }
 
+void lift_complete_plane(Context& ctx, const LiftChunk& c, hipStream_t stream) {
+  if (c.steps.empty()) return;  // the narrowing read the plane as it is
+  LiftChunk last = c;
+  last.dst = nullptr;
+  last.steps.assign(1, c.steps[0]);
+  std::vector<LiftChunk> cs{last};
+  run_lift(ctx, cs, false, stream);
+}
+
 namespace {
 
 void set_error(ndic_error* err, const Failure& f) {
@@ -502,6 +511,7 @@ ndic_status lift_chunk_call(bool fwd, const uint8_t* bytes, size_t len, const si
         // error path: the range of the reconstructed plane, for the message
         int64_t* mm = (int64_t*)(d_small + 4);
         int64_t init[2] = {INT64_MAX, INT64_MIN}, hmm[2];
+        lift_complete_plane(ctx, c, s);
         NDIC_HIP(hipMemcpyAsync(mm, init, 16, hipMemcpyHostToDevice, s));
         launch_convert(nullptr, 0, d_plane, wide, nullptr, 0, n, mm, nullptr, s);
         NDIC_HIP(hipMemcpyAsync(hmm, mm, 16, hipMemcpyDeviceToHost, s));

@@ -20,6 +20,7 @@ const Knob kKnobs[kTuneCount] = {
     {"ht_encode_wave_max_blocks", INT64_MAX, 0, INT64_MAX},
     {"parse_wave_max_planes", 4096, 0, INT64_MAX},
     {"ht_wave_grain", 1, 1, 64},
+    {"fuse_dwt_lift", 1, 0, 1},
 };
 
 std::atomic<int64_t> g_vals[kTuneCount] = {};

@@ -18,6 +18,10 @@ enum TuneKey : int {
   // code-blocks a wave of the persistent wave-per-block coders takes from
   // the work queue at a time
   kTuneHtWaveGrain,
+  // 1: the finest inverse DWT level, the inverse nd_lift step along a plane
+  // axis and the narrowing store run as one pass (dwt_lift_kernel) wherever
+  // a chunk allows it; 0: as separate passes
+  kTuneFuseDwtLift,
   kTuneCount
 };
 

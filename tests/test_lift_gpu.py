@@ -79,6 +79,20 @@ def test_config1_shape(ndic, oracle):
     _roundtrip(ndic, oracle, arr, "u16", steps)
 
 
+def test_narrow_error_reports_lifted_range(ndic, oracle):
+    """The out-of-range message gives the range of the lifted plane that was
+    narrowed (crates/ndic-zarr/src/lift.rs:95-101), also when the last step
+    narrows straight into the output."""
+    rng = np.random.default_rng(4)
+    vol = rng.integers(-3000, 3000, size=(6, 9, 70)).astype(np.int16)
+    for steps in ([{"dimension": 0, "kind": "lift53", "levels": 2}],
+                  [{"dimension": 0, "kind": "delta"}, {"dimension": 2, "kind": "haar", "levels": 2}]):
+        coeffs = oracle.lift_chunk(True, vol.tobytes(), vol.shape, "i16", steps)
+        lo, hi = int(vol.min()), int(vol.max())
+        with pytest.raises(ndic.NdicError, match=rf"coefficient range \[{lo}, {hi}\] does not narrow back to u8"):
+            ndic.lift_inverse_bytes(coeffs, vol.shape, "uint8", steps)
+
+
 def test_refusals_match(ndic, oracle):
     with pytest.raises(ndic.NdicError, match="leave the 32-bit plane") as e:
         ndic.lift_forward_bytes(np.array([-2**31, 2**31 - 1], np.int32).tobytes(), [2], "int32",
