@@ -33,7 +33,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBPS = 8000.0
 # the kernel that dominates each decode stage (stages are timed per batch
 # window with HIP events on the library's stream)
-KERNELS = ["parse_planes_kernel", "ht_decode_kernel", "dwt_inv_level", "unshift_kernel", "lift_col_kernel"]
+KERNELS = ["parse_planes_kernel", "ht_decode_kernel", "dwt_inv_level", "unshift_kernel", "dwt_lift_kernel"]
 COUNTERS = ROOT / "profiles" / "c2_decode_counters.json"
 
 

@@ -248,7 +248,10 @@ ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp
  * persistent wave-per-block coders takes from its work queue at a time;
  * "fuse_dwt_lift" -- 1 (default): a lift-ht decode whose single lift step
  * runs along a plane axis synthesises the finest DWT level, lifts and
- * narrows in one pass; 0: three passes (same output).
+ * narrows in one pass; 0: three passes (same output); "pipeline_windows" --
+ * 1 (default): the windows of a large decode batch alternate between two
+ * streams so that their passes overlap; 0: one after the other;
+ * "window_elems" -- samples per batch window (0: NDIC_WINDOW_ELEMS, else 2^31).
  * ndic_tune_set returns 0, -1 for an unknown key, -2 for a value out of
  * range; ndic_tune_get returns the value in force (-1: unknown key);
  * ndic_tune_reset restores the built-in default; ndic_tune_name lists the

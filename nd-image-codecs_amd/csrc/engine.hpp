@@ -59,6 +59,11 @@ struct Context {
   void* down(const void* d, size_t n, hipStream_t s);
   // Reclaims the staging; only between windows (no copy of it in flight).
   void release_kept() { pin.reset(); }
+  // A second context of this one (same device, own buffers, staging and
+  // stream) for a window queued while one of this context is in flight;
+  // created on first use, used under this context's lock.
+  Context* twin_ctx = nullptr;
+  Context& twin();
   static Context& get();
 };
 

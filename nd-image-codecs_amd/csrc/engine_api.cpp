@@ -19,6 +19,8 @@ namespace ndic {
 namespace {
 
 uint64_t window_elems() {
+  const int64_t k = tune(kTuneWindowElems);
+  if (k > 0) return (uint64_t)k;
   static uint64_t w = [] {
     const char* e = std::getenv("NDIC_WINDOW_ELEMS");
     uint64_t v = e ? std::strtoull(e, nullptr, 10) : 0;

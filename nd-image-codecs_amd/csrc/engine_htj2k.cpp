@@ -10,6 +10,8 @@
 // every codestream and container.  Large batches are cut into windows that
 // bound the device memory of the intermediate planes.
 #include <algorithm>
+#include <functional>
+#include <memory>
 #include <cstring>
 #include <map>
 #include <sstream>
@@ -187,7 +189,9 @@ bool fusable(const DecodeItem& it, const DevChunk& c) {
 // ht_decode_wave_max_blocks code-blocks a window decodes one block per
 // wavefront (latency); above, one block per lane (throughput).
 
-void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s) {
+// Queues every pass of one window on s and returns the step that waits for
+// it and fills in the items' results (status, message, decoded region).
+std::function<void()> decode_window_begin(Context& ctx, const std::vector<DecodeItem*>& items, hipStream_t s) {
   ctx.release_kept();  // every earlier window of this context ended synchronised
   GeoTables geo;
   std::vector<DevChunk> dc(items.size());
@@ -254,16 +258,16 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   ctx.up(d_chunks, dc.data(), n * sizeof(DevChunk), s);
   ctx.up(d_gp, geo.packets.data(), geo.packets.size() * sizeof(GeoPacket), s);
   ctx.up(d_gb, geo.bands.data(), geo.bands.size() * sizeof(GeoBand), s);
-  StageTimer timer(s);
-  timer.mark();
+  auto timer = std::make_shared<StageTimer>(s);
+  timer->mark();
   launch_walk_chunks(d_chunks, (uint32_t)n, d_planes, s);
   launch_init_tasks(d_chunks, (uint32_t)n, ntasks, d_tasks, d_gp, d_gb, s);
   launch_parse_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_gp, d_gb, s);
-  timer.mark();
+  timer->mark();
   if ((int64_t)ntasks > tune(kTuneHtDecodeWaveMaxBlocks)) launch_ht_decode(d_tasks, nullptr, ntasks, s);
   else launch_ht_decode_wave(d_tasks, ntasks, ctx.queue(0), s);
   launch_ht_refine(d_tasks, ntasks, s);
-  timer.mark();
+  timer->mark();
   std::vector<char> fuse(n, 0);
   if (tune(kTuneFuseDwtLift))
     for (size_t i = 0; i < n; ++i) fuse[i] = fusable(*items[i], dc[i]);
@@ -285,7 +289,7 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
     items[i]->out_h = w.H;
   }
   run_dwt_inverse(ctx, dw, s);
-  timer.mark();
+  timer->mark();
   std::vector<int32_t*> res(n);
   std::vector<uint64_t> dims(2 * n);
   for (size_t i = 0; i < n; ++i) {
@@ -297,7 +301,7 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
   ctx.up(d_ptr, res.data(), n * sizeof(void*), s);
   ctx.up(d_dims, dims.data(), n * 16, s);
   launch_unshift(d_chunks, (uint32_t)n, nplanes, d_planes, d_ptr, d_dims, s);
-  timer.mark();
+  timer->mark();
   // outputs
   std::vector<LiftChunk> lifts;
   std::vector<DwtLiftJob> fused;
@@ -368,9 +372,12 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
     launch_dwt_lift(d_fused, (uint32_t)fused.size(), fused_items, s);
   }
   if (!lifts.empty()) run_lift(ctx, lifts, false, s);
-  timer.mark();
+  timer->mark();
   const int32_t* p_err = (const int32_t*)ctx.down(d_err, n * 16, s);
   const int32_t* p_flag = (const int32_t*)ctx.down(d_flag, n * 4, s);
+  return [&ctx, s, items, n, p_err, p_flag, coef, timer, dc = std::move(dc), dw = std::move(dw),
+          fuse = std::move(fuse), fused = std::move(fused), fused_of = std::move(fused_of),
+          lifts = std::move(lifts), res = std::move(res)]() {
   NDIC_HIP(hipStreamSynchronize(s));
   std::vector<int32_t> herr(p_err, p_err + 4 * n), hflag(p_flag, p_flag + n);
   if (g_prof.on) {
@@ -391,7 +398,7 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
       if (!items[i]->bare) lift += sm * (4 + dtype_size(items[i]->lift ? items[i]->out_dtype : items[i]->ht_dtype));
     }
     double bytes[5] = {in, in + 4 * (double)coef, dwt, 0, lift};
-    timer.collect(bytes);
+    timer->collect(bytes);
   }
   for (size_t i = 0; i < n; ++i) {
     DecodeItem& it = *items[i];
@@ -444,26 +451,66 @@ void decode_window(Context& ctx, std::vector<DecodeItem*>& items, hipStream_t s)
       it.message = msg.str();
     }
   }
+  };
 }
 
 }  // namespace
 
+// Windows alternate between ctx (on s) and its twin context (own buffers and
+// stream), so that one window's parse and entropy decode overlap the other's
+// synthesis and lift; a window's results are collected once the next one is
+// queued.  Under the stage profiler, or with pipeline_windows = 0, windows run
+// one after the other.
 void decode_items(Context& ctx, std::vector<DecodeItem>& items, hipStream_t s, uint64_t window_elems) {
   ht_init_tables();
-  std::vector<DecodeItem*> win;
+  std::vector<std::vector<DecodeItem*>> wins(1);
   uint64_t acc = 0;
   for (auto& it : items) {
     uint64_t e = 1;
     for (size_t d : it.shape) e *= d;
-    if (!win.empty() && acc + e > window_elems) {
-      decode_window(ctx, win, s);
-      win.clear();
+    if (!wins.back().empty() && acc + e > window_elems) {
+      wins.emplace_back();
       acc = 0;
     }
-    win.push_back(&it);
+    wins.back().push_back(&it);
     acc += e;
   }
-  if (!win.empty()) decode_window(ctx, win, s);
+  if (wins.back().empty()) return;
+  if (wins.size() == 1 || g_prof.on || !tune(kTunePipelineWindows)) {
+    for (auto& w : wins) decode_window_begin(ctx, w, s)();
+    return;
+  }
+  Context& twin = ctx.twin();
+  hipEvent_t ready;
+  NDIC_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+  std::function<void()> pending[2];
+  try {
+    NDIC_HIP(hipEventRecord(ready, s));  // the twin starts after what s holds
+    NDIC_HIP(hipStreamWaitEvent(twin.stream, ready, 0));
+    for (size_t k = 0; k < wins.size(); ++k) {
+      auto& slot = pending[k & 1];
+      if (slot) {
+        auto f = std::move(slot);
+        slot = nullptr;
+        f();
+      }
+      slot = (k & 1) ? decode_window_begin(twin, wins[k], twin.stream) : decode_window_begin(ctx, wins[k], s);
+    }
+    for (size_t k = wins.size(); k < wins.size() + 2; ++k) {
+      auto& slot = pending[k & 1];
+      if (slot) {
+        auto f = std::move(slot);
+        slot = nullptr;
+        f();
+      }
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamSynchronize(twin.stream);
+    (void)hipEventDestroy(ready);
+    throw;
+  }
+  NDIC_HIP(hipEventDestroy(ready));
 }
 
 namespace {

@@ -75,28 +75,48 @@ void* Context::down(const void* d, size_t n, hipStream_t s) {
 namespace {
 const std::thread::id g_loader_thread = std::this_thread::get_id();
 
+void destroy_context(Context* c) {
+  if (c->twin_ctx) destroy_context(c->twin_ctx);
+  (void)hipStreamSynchronize(c->stream);
+  for (DevBuf* b : {&c->in, &c->out, &c->plane, &c->tmp, &c->coef, &c->work, &c->jobs, &c->dwtjobs, &c->fusedjobs,
+                    &c->small})
+    b->release();
+  if (c->queues) (void)hipFree(c->queues);
+  c->pin.reset();
+  if (c->pin.base) (void)hipHostFree(c->pin.base);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+Context* new_context() {
+  auto* c = new Context();
+  NDIC_HIP(hipGetDevice(&c->device));
+  NDIC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  return c;
+}
+
 struct ContextOwner {
   Context* c = nullptr;
   ~ContextOwner() {
     if (!c || std::this_thread::get_id() == g_loader_thread) return;
-    (void)hipStreamSynchronize(c->stream);
-    for (DevBuf* b : {&c->in, &c->out, &c->plane, &c->tmp, &c->coef, &c->work, &c->jobs, &c->dwtjobs, &c->small})
-      b->release();
-    if (c->queues) (void)hipFree(c->queues);
-    c->pin.reset();
-    if (c->pin.base) (void)hipHostFree(c->pin.base);
-    (void)hipStreamDestroy(c->stream);
-    delete c;
+    destroy_context(c);
   }
 };
 }  // namespace
 
 uint64_t* Context::queue(int k) {
   if (!queues) {
+    // zeroed before any stream (the caller's may not be this one) uses them
     NDIC_HIP(hipMalloc((void**)&queues, 256));
     NDIC_HIP(hipMemsetAsync(queues, 0, 256, stream));
+    NDIC_HIP(hipStreamSynchronize(stream));
   }
   return queues + 2 * k;
+}
+
+Context& Context::twin() {
+  if (!twin_ctx) twin_ctx = new_context();
+  return *twin_ctx;
 }
 
 Context& Context::get() {
@@ -105,10 +125,7 @@ Context& Context::get() {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
       fail(NDIC_IO, "no HIP device is visible: the nd-image-codecs MI355X path needs a GPU");
-    auto* c = new Context();
-    NDIC_HIP(hipGetDevice(&c->device));
-    NDIC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    owner.c = c;
+    owner.c = new_context();
   }
   return *owner.c;
 }

@@ -21,6 +21,8 @@ const Knob kKnobs[kTuneCount] = {
     {"parse_wave_max_planes", 4096, 0, INT64_MAX},
     {"ht_wave_grain", 1, 1, 64},
     {"fuse_dwt_lift", 1, 0, 1},
+    {"pipeline_windows", 1, 0, 1},
+    {"window_elems", 0, 0, (int64_t)1 << 40},
 };
 
 std::atomic<int64_t> g_vals[kTuneCount] = {};

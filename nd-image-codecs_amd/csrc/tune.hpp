@@ -22,6 +22,13 @@ enum TuneKey : int {
   // axis and the narrowing store run as one pass (dwt_lift_kernel) wherever
   // a chunk allows it; 0: as separate passes
   kTuneFuseDwtLift,
+  // 1: the windows of a large decode batch alternate between two streams
+  // (one window's parse / entropy decode overlaps the other's synthesis);
+  // 0: one window after the other
+  kTunePipelineWindows,
+  // samples per batch window (bounds the intermediate planes' memory);
+  // 0: NDIC_WINDOW_ELEMS from the environment, else 2^31
+  kTuneWindowElems,
   kTuneCount
 };
 

@@ -129,3 +129,37 @@ def test_fused_range_error_matches_separate_passes(ndic, oracle):
         assert f"coefficient range [{lo}, {hi}] does not narrow back to u8" in err, (fuse, err)
         assert out[:good[0].nbytes] == good[0].tobytes()
     assert res[1][1] == res[0][1] and res[1][2] == res[0][2]
+
+
+@pytest.mark.parametrize("window", [60_000, 250_000])
+def test_pipelined_windows_match(ndic, oracle, window):
+    """Small windows so that the batch spans many: alternating streams
+    (pipeline_windows = 1) and one window after the other give the same
+    bit-exact samples and the same per-chunk statuses, a failing chunk in the
+    middle included."""
+    rng = np.random.default_rng(9)
+    specs = _cases()
+    vol = rng.integers(-3000, 3000, size=(6, 33, 47)).astype(np.int16)
+    bad_at = len(specs) // 2
+    specs.insert(bad_at, (vol, "int16", [{"dimension": 0, "kind": "haar", "levels": 2, "group": 0}], 2))
+    out_dt = [dt for _, dt, _, _ in specs]
+    out_dt[bad_at] = "uint8"
+    dev, d_in, d_out, descs, al = _batch(ndic, oracle, specs, out_dtypes=out_dt)
+    ndic.tune_set("window_elems", window)
+    try:
+        res = {}
+        for pipe in (1, 0):
+            ndic.tune_set("pipeline_windows", pipe)
+            res[pipe] = _decode(ndic, dev, d_in, d_out, descs, 1)
+    finally:
+        ndic.tune_reset("pipeline_windows")
+        ndic.tune_reset("window_elems")
+    for pipe, (out, status, err) in res.items():
+        assert [i for i, st in enumerate(status) if st] == [bad_at], (pipe, status)
+        assert "does not narrow back to u8" in err, (pipe, err)
+        o = 0
+        for i, (v, dt, steps, lv) in enumerate(specs):
+            if i != bad_at:
+                assert out[o:o + v.nbytes] == v.tobytes(), (pipe, i, v.shape, dt, steps, lv)
+            o += al(v.nbytes)
+    assert res[1] == res[0]
