@@ -409,10 +409,29 @@ constexpr int kRowCap = 34;  // quads per row (64 columns) + terminators
 constexpr uint32_t kStageCols = NDIC_HT_STAGE_COLS;
 constexpr int kStagePitch = 65;  // LDS pitch of a staged sample row (bank-conflict free)
 
+// Where a decoder looks up the CxtVLC and UVLC prefix tables: the constant
+// arrays, or a workgroup's copy in LDS (a few hundred cycles closer than a
+// vector-L1 line the lanes' byte streams keep evicting).
+struct ConstTables {
+  __device__ __forceinline__ uint32_t dec(bool initial, uint32_t i) const { return initial ? c_dec0[i] : c_dec1[i]; }
+  __device__ __forceinline__ uint32_t uvlc(bool initial, uint32_t i) const { return initial ? c_uvlc0[i] : c_uvlc1[i]; }
+};
+typedef const __attribute__((address_space(3))) uint16_t* lds_u16p;
+constexpr uint32_t kTabDec1 = 1024, kTabUv0 = 2048, kTabUv1 = 2048 + 320, kTabAll = 2048 + 320 + 256;
+struct LdsTables {
+  lds_u16p t;  // dec0 | dec1 | uvlc0 | uvlc1
+  __device__ __forceinline__ uint32_t dec(bool initial, uint32_t i) const { return t[(initial ? 0 : kTabDec1) + i]; }
+  __device__ __forceinline__ uint32_t uvlc(bool initial, uint32_t i) const {
+    return t[(initial ? kTabUv0 : kTabUv1) + i];
+  }
+};
+
 // Per-lane decoder of one code-block's cleanup pass (decode.rs:54-396).  The
 // caller drives it through uniform row-pair / column-pair loops so that the
 // whole wave can flush decoded rows together.
+template <class Tab>
 struct BlockDec {
+  Tab tab;
   HtDecTask t;
   MelDec mel;
   RevDec vlc;
@@ -489,7 +508,7 @@ struct BlockDec {
       c_q |= (a1 & 0x20) << 4;
     }
     uint32_t vlc_val = vlc.fetch();
-    uint32_t t0 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
+    uint32_t t0 = tab.dec(initial, c_q | (vlc_val & 0x7F));
     if (c_q == 0) {
       run -= 2;
       t0 = run == -1 ? t0 : 0;
@@ -504,7 +523,7 @@ struct BlockDec {
       c_q |= (a2 & 0x20) << 4;
     }
     vlc_val = vlc.advance(t0 & 0x7);
-    uint32_t t1 = initial ? c_dec0[c_q | (vlc_val & 0x7F)] : c_dec1[c_q | (vlc_val & 0x7F)];
+    uint32_t t1 = tab.dec(initial, c_q | (vlc_val & 0x7F));
     if (c_q == 0 && x + 2 < width) {
       run -= 2;
       t1 = run == -1 ? t1 : 0;
@@ -524,7 +543,7 @@ struct BlockDec {
       uvlc_mode += run == -1 ? 0x40 : 0;
       if (run < 0) run = mel.get_run();
     }
-    uint32_t e = initial ? c_uvlc0[uvlc_mode + (vlc_val & 0x3F)] : c_uvlc1[uvlc_mode + (vlc_val & 0x3F)];
+    uint32_t e = tab.uvlc(initial, uvlc_mode + (vlc_val & 0x3F));
     vlc_val = vlc.advance(e & 0x7);
     e >>= 3;
     uint32_t len = e & 0xF;
@@ -612,14 +631,46 @@ __device__ void fill_zero(const HtDecTask& t) {
     for (uint32_t x = 0; x < t.w; ++x) st32((uint32_t*)t.out + (uint64_t)y * t.stride + x, 0);
 }
 
-// Lane l of wave w decodes block 64 w + l (in `order`).
-__global__ __launch_bounds__(64) void ht_decode_kernel(const HtDecTask* __restrict__ tasks,
-                                                      const uint32_t* __restrict__ order, uint64_t n) {
-  __shared__ uint8_t lrho[kRowCap * 64];
-  __shared__ uint8_t lbl[kRowCap * 64];
-  __shared__ uint32_t stage[2 * kStageCols * kStagePitch];
-  const uint32_t lane = threadIdx.x;
-  uint64_t i = blockIdx.x * (uint64_t)64 + lane;
+// A wave-local ordering point: the lanes' LDS writes before it are seen by
+// reads after it (a wave's LDS instructions execute in order; this keeps the
+// compiler from moving memory operations across).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Four independent waves per workgroup; lane l of wave w decodes block
+// 256 b + 64 w + l (in `order`).  The workgroup shares one LDS copy of the
+// decode tables; each wave has its own context rows and stage.
+constexpr uint32_t kLaneWaves = 4;
+__global__ __launch_bounds__(64 * kLaneWaves) void ht_decode_kernel(const HtDecTask* __restrict__ tasks,
+                                                                   const uint32_t* __restrict__ order, uint64_t n) {
+  __shared__ __align__(16) uint16_t tabs[kTabAll];
+  __shared__ uint8_t lrho_all[kLaneWaves][kRowCap * 64];
+  __shared__ uint8_t lbl_all[kLaneWaves][kRowCap * 64];
+  __shared__ uint32_t stage_all[kLaneWaves][2 * kStageCols * kStagePitch];
+  {
+    uint32_t* tw = (uint32_t*)tabs;
+    const uint32_t* s0 = (const uint32_t*)c_dec0;
+    const uint32_t* s1 = (const uint32_t*)c_dec1;
+    const uint32_t* s2 = (const uint32_t*)c_uvlc0;
+    const uint32_t* s3 = (const uint32_t*)c_uvlc1;
+    for (uint32_t k = threadIdx.x; k < kTabAll / 2; k += blockDim.x) {
+      uint32_t v;
+      if (k < kTabDec1 / 2) v = s0[k];
+      else if (k < kTabUv0 / 2) v = s1[k - kTabDec1 / 2];
+      else if (k < kTabUv1 / 2) v = s2[k - kTabUv0 / 2];
+      else v = s3[k - kTabUv1 / 2];
+      tw[k] = v;
+    }
+  }
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint8_t* lrho = lrho_all[wv];
+  uint8_t* lbl = lbl_all[wv];
+  uint32_t* stage = stage_all[wv];
+  uint64_t i = blockIdx.x * (uint64_t)(64 * kLaneWaves) + threadIdx.x;
   HtDecTask task{};
   if (i < n) {
     if (order) i = order[i];
@@ -629,7 +680,8 @@ __global__ __launch_bounds__(64) void ht_decode_kernel(const HtDecTask* __restri
     lrho[k * 64 + lane] = 0;
     lbl[k * 64 + lane] = 0;
   }
-  BlockDec d;
+  BlockDec<LdsTables> d;
+  d.tab.t = (lds_u16p)tabs;
   d.start(task);
   const uint32_t w = task.w, h = task.h;
   const uint64_t out = i < n ? (uint64_t)task.out : 0;
@@ -651,9 +703,9 @@ __global__ __launch_bounds__(64) void ht_decode_kernel(const HtDecTask* __restri
           }
         }
       }
-      __syncthreads();  // the staged window is complete (one wave per workgroup)
+      wave_sync();  // the staged window is complete
       flush_stage(stage, out, task.stride, w, h, y, xb);
-      __syncthreads();  // reads done before the next window overwrites the stage
+      wave_sync();  // reads done before the next window overwrites the stage
     }
     if (d.active && y < h) d.row_end(lrho + lane, lbl + lane);
   }
@@ -818,7 +870,7 @@ __device__ void decode_block_serial(const HtDecTask& task, uint8_t* lds, HtDecTa
   uint8_t* lbl = lds + kRowCap * 64;
   uint32_t* stage = (uint32_t*)(lds + 2 * kRowCap * 64);
   for (int k = 0; k < kRowCap; ++k) lrho[k * 64] = lbl[k * 64] = 0;
-  BlockDec d;
+  BlockDec<ConstTables> d;
   d.start(task);
   const uint32_t w = task.w, h = task.h;
   uint32_t* out = (uint32_t*)task.out;
@@ -2110,7 +2162,8 @@ void ht_init_tables() {
 
 void launch_ht_decode(const HtDecTask* d_tasks, const uint32_t* d_order, uint64_t n, hipStream_t stream) {
   if (n == 0) return;
-  hipLaunchKernelGGL(ht_decode_kernel, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, d_order, n);
+  hipLaunchKernelGGL(ht_decode_kernel, dim3((uint32_t)ceil_div(n, 64 * kLaneWaves)), dim3(64 * kLaneWaves), 0, stream,
+                     d_tasks, d_order, n);
   NDIC_HIP(hipGetLastError());
 }
 
