@@ -44,8 +44,9 @@ def crossover(sizes: Sequence[int], fast_small: Sequence[float], fast_big: Seque
     """Largest size up to which the small-batch form is never slower than the
     large-batch form (0 when it loses at the smallest size).  Between the last
     winning size and the first losing one the costs are interpolated
-    linearly."""
-    last = 0
+    linearly.  A form that wins at every measured size is chosen up to the
+    largest one only: beyond it nothing was measured, and the large-batch form
+    (the one built for throughput) stays in force."""
     for k, (n, a, b) in enumerate(zip(sizes, fast_small, fast_big)):
         if a > b:
             if k == 0:
@@ -53,8 +54,7 @@ def crossover(sizes: Sequence[int], fast_small: Sequence[float], fast_big: Seque
             n0, a0, b0 = sizes[k - 1], fast_small[k - 1], fast_big[k - 1]
             d0, d1 = b0 - a0, b - a  # >= 0 then < 0
             return int(n0 + (n - n0) * d0 / (d0 - d1))
-        last = n
-    return last if last < sizes[-1] else 1 << 62
+    return int(sizes[-1])
 
 
 class _Window:

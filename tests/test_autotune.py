@@ -21,8 +21,8 @@ def test_crossover_rule(autotune):
     assert autotune.crossover(sizes, [1, 2, 5, 9], [4, 4, 4, 4]) == 200 + int(200 * 2 / (2 + 1))
     # loses at once
     assert autotune.crossover(sizes, [5, 6, 7, 8], [1, 1, 1, 1]) == 0
-    # wins everywhere: no limit
-    assert autotune.crossover(sizes, [1, 1, 1, 1], [2, 2, 2, 2]) >= 1 << 60
+    # wins everywhere: up to the largest size measured, not beyond
+    assert autotune.crossover(sizes, [1, 1, 1, 1], [2, 2, 2, 2]) == 800
     # a tie counts as a win; with the tie just before the loss the crossing is the tie
     assert autotune.crossover(sizes, [1, 2, 3, 5], [1, 2, 3, 4]) == 400
 
