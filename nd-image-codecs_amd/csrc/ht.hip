@@ -1150,13 +1150,7 @@ __global__ __launch_bounds__(64) void ht_decode_wave_kernel(const HtDecTask* __r
 // mask per block row in LDS.
 __constant__ uint32_t c_spp_prop[4] = {0x33, 0x76, 0xEC, 0xC8};
 
-__global__ __launch_bounds__(64) void ht_refine_kernel(const HtDecTask* __restrict__ tasks, uint64_t n) {
-  __shared__ uint64_t sigrow[64 * 64];
-  __shared__ uint16_t prevrow[18 * 64];
-  const uint32_t lane = threadIdx.x;
-  uint64_t i = blockIdx.x * (uint64_t)64 + lane;
-  if (i >= n) return;
-  const HtDecTask t = tasks[i];
+__device__ __forceinline__ void refine_block(const HtDecTask& t, uint64_t* sigrow, uint16_t* prevrow, uint32_t lane) {
   if (!(t.flags & kTaskIncluded) || t.passes < 2 || t.len2 == 0 || t.mmsb >= 29) return;
   const uint32_t num_passes = t.passes;
   const uint32_t width = t.w, height = t.h;
@@ -1310,6 +1304,19 @@ __global__ __launch_bounds__(64) void ht_refine_kernel(const HtDecTask* __restri
         int32_t mag = (int32_t)((sm & 0x7FFFFFFFu) >> shift);
         out[y * stride + x] = (uint32_t)((sm & 0x80000000u) ? -mag : mag);
       }
+  }
+}
+
+// Grid-stride over the blocks: the launch holds at most what the GPU keeps
+// resident at once, so that it is cheap to place next to another stream's
+// kernels when (as in cleanup-only streams) no block has the passes.
+__global__ __launch_bounds__(64) void ht_refine_kernel(const HtDecTask* __restrict__ tasks, uint64_t n) {
+  __shared__ uint64_t sigrow[64 * 64];
+  __shared__ uint16_t prevrow[18 * 64];
+  const uint32_t lane = threadIdx.x;
+  for (uint64_t i = blockIdx.x * (uint64_t)64 + lane; i < n; i += (uint64_t)gridDim.x * 64) {
+    const HtDecTask t = tasks[i];
+    refine_block(t, sigrow, prevrow, lane);
   }
 }
 
@@ -2168,18 +2175,26 @@ void launch_ht_decode(const HtDecTask* d_tasks, const uint32_t* d_order, uint64_
 }
 
 namespace {
-// Workgroups of a persistent launch: enough to fill every CU at the
-// kernel's occupancy, fewer when the queue holds fewer grains.
+// Workgroups of a persistent (or grid-stride) launch of a 64-thread kernel:
+// enough to fill every CU at the kernel's occupancy, fewer when there are
+// fewer grains.  The resident count is measured once per kernel.
 template <class K> uint32_t persistent_grid(K kernel, uint64_t grains) {
-  static int cap = 0;
-  static std::once_flag once;
-  std::call_once(once, [&] {
-    int dev = 0, cus = 0, per = 0;
-    NDIC_HIP(hipGetDevice(&dev));
-    NDIC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    NDIC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64, 0));
-    cap = std::max(1, cus * std::max(1, per));
-  });
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, int>> caps;
+  int cap = 0;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& c : caps)
+      if (c.first == (const void*)kernel) cap = c.second;
+    if (!cap) {
+      int dev = 0, cus = 0, per = 0;
+      NDIC_HIP(hipGetDevice(&dev));
+      NDIC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      NDIC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64, 0));
+      cap = std::max(1, cus * std::max(1, per));
+      caps.emplace_back((const void*)kernel, cap);
+    }
+  }
   return (uint32_t)std::min<uint64_t>(grains, (uint64_t)cap);
 }
 }  // namespace
@@ -2194,7 +2209,8 @@ void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, uint64_t* queue
 
 void launch_ht_refine(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream) {
   if (n == 0) return;
-  hipLaunchKernelGGL(ht_refine_kernel, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, n);
+  hipLaunchKernelGGL(ht_refine_kernel, dim3(persistent_grid(ht_refine_kernel, ceil_div(n, 64))), dim3(64), 0, stream,
+                     d_tasks, n);
   NDIC_HIP(hipGetLastError());
 }
 
