@@ -309,9 +309,14 @@ __global__ __launch_bounds__(256) void dwt_inv_level(const DwtJob* __restrict__ 
 // un-shift of unsigned planes (reader.rs:626-638), kernel.rs:186-200 and
 // 243-272 (inverse haar / 5-3 levels, wrapping) and the inverse delta.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kFx = 64, kFy = 32;                       // output tile
-constexpr uint32_t kFLo = kFy / 2 + 1, kFHi = kFy / 2 + 2;   // T rows: low, high
-constexpr uint32_t kFTRows = kFLo + kFHi, kFIter = (kFTRows * (kFx / 2) + 255) / 256;
+constexpr uint32_t kFx = kDwtLiftTileW, kFy = kDwtLiftTileH;   // output tile
+constexpr uint32_t kFLo = kFy / 2 + 1, kFHi = kFy / 2 + 2;     // T rows: low, high
+constexpr uint32_t kFTRows = kFLo + kFHi;
+// horizontal pass: kFx / 2 column pairs per T row, kFHRows T rows at a time
+constexpr uint32_t kFPairs = kFx / 2, kFHRows = 256 / kFPairs, kFIter = (kFTRows + kFHRows - 1) / kFHRows;
+// vertical pass: kFx columns, kFVGroups row-pair groups, 4 row pairs per thread
+constexpr uint32_t kFVGroups = 256 / kFx;
+static_assert(kFVGroups * 4 * 2 == kFy, "a thread holds 4 row pairs of one column");
 
 struct V8 {
   int32_t v[8];
@@ -332,7 +337,7 @@ struct FusedTile {
   __device__ __forceinline__ uint64_t plane_of(uint32_t k) const { return zplane0 + (uint64_t)k * j.R; }
 
   // Synthesised tile of z-row k: sample e of this thread is column
-  // X0 + tid % 64, row Y0 + 2 (tid / 64 + 4 (e / 2)) + e % 2.
+  // X0 + tid % kFx, row Y0 + 2 (tid / kFx + kFVGroups (e / 2)) + e % 2.
   __device__ __forceinline__ V8 fetch(uint32_t k) {
     const uint32_t tid = threadIdx.x, w = j.w, h = j.h;
     const uint64_t p = plane_of(k);
@@ -343,14 +348,14 @@ struct FusedTile {
     buf ^= 1;
     // horizontal synthesis of the T rows (loads of all rows first)
     {
-      const uint32_t jl = tid & 31, rb = tid >> 5, jj = jb + jl;
+      const uint32_t jl = tid % kFPairs, rb = tid / kFPairs, jj = jb + jl;
       const uint32_t jm = umin(jj == 0 ? 0 : jj - 1, nh - 1), j0 = umin(jj, nh - 1), j1 = umin(jj + 1, nh - 1);
       const bool has_r = 2 * jj + 1 < w, has_r2 = 2 * jj + 2 < w;
       int32_t vl0[kFIter], vl1[kFIter], vhm[kFIter], vh0[kFIter], vh1[kFIter];
       bool okr[kFIter];
 #pragma unroll
       for (uint32_t b = 0; b < kFIter; ++b) {
-        const uint32_t r = rb + 8 * b;
+        const uint32_t r = rb + kFHRows * b;
         uint32_t y = 0;
         okr[b] = r < kFTRows && jj < nl;
         if (r < kFLo) {
@@ -382,7 +387,7 @@ struct FusedTile {
 #pragma unroll
       for (uint32_t b = 0; b < kFIter; ++b) {
         if (!okr[b]) continue;
-        const uint32_t r = rb + 8 * b;
+        const uint32_t r = rb + kFHRows * b;
         int32_t* dstrow = r < kFLo ? lo[r] : hi[r - kFLo];
         if (w == 1) {
           dstrow[0] = vl0[b];
@@ -399,10 +404,10 @@ struct FusedTile {
     __syncthreads();
     // vertical synthesis into registers
     V8 out;
-    const uint32_t c = tid & 63, ivb = tid >> 6;
+    const uint32_t c = tid % kFx, ivb = tid / kFx;
 #pragma unroll
     for (uint32_t kk = 0; kk < 4; ++kk) {
-      const uint32_t iv = ivb + 4 * kk, ii = i0 + iv;
+      const uint32_t iv = ivb + kFVGroups * kk, ii = i0 + iv;
       int32_t x0 = 0, x1 = 0;
       if (h == 1) {
         x0 = lo[0][c];
@@ -434,7 +439,7 @@ struct FusedTile {
     D* d = (D*)base;
 #pragma unroll
     for (uint32_t e = 0; e < 8; ++e) {
-      const uint32_t y = Y0 + 2 * (ivb + 4 * (e / 2)) + (e & 1), xx = X0 + c;
+      const uint32_t y = Y0 + 2 * (ivb + kFVGroups * (e / 2)) + (e & 1), xx = X0 + c;
       if (xx < j.w && y < j.h) {
         d[(uint64_t)y * j.w + xx] = (D)x.v[e];
         const int32_t v = x.v[e];
@@ -449,11 +454,11 @@ struct FusedTile {
   // Stores z-row k of the output, narrowed to the dtype (range mode: folds
   // it into lo / hi).
   __device__ __forceinline__ void put(uint32_t k, const V8& x) {
-    const uint32_t c = threadIdx.x & 63, ivb = threadIdx.x >> 6;
+    const uint32_t c = threadIdx.x % kFx, ivb = threadIdx.x / kFx;
     if (!j.dst) {
 #pragma unroll
       for (uint32_t e = 0; e < 8; ++e) {
-        const uint32_t y = Y0 + 2 * (ivb + 4 * (e / 2)) + (e & 1), xx = X0 + c;
+        const uint32_t y = Y0 + 2 * (ivb + kFVGroups * (e / 2)) + (e & 1), xx = X0 + c;
         if (xx < j.w && y < j.h) {
           lo = x.v[e] < lo ? x.v[e] : lo;
           hi = x.v[e] > hi ? x.v[e] : hi;

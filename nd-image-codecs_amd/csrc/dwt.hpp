@@ -45,7 +45,10 @@ struct DwtLiftJob {
   uint32_t kind, levels, dtype, dsize;
   uint64_t items, item_begin;
 };
-constexpr uint32_t kDwtLiftTileW = 64, kDwtLiftTileH = 32;
+#ifndef NDIC_DWT_LIFT_TILE_W
+#define NDIC_DWT_LIFT_TILE_W 64
+#endif
+constexpr uint32_t kDwtLiftTileW = NDIC_DWT_LIFT_TILE_W, kDwtLiftTileH = 2048 / kDwtLiftTileW;
 void launch_dwt_lift(const DwtLiftJob* d_jobs, uint32_t njobs, uint64_t items, hipStream_t stream);
 
 }  // namespace ndic
