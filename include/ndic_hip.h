@@ -246,12 +246,17 @@ ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp
  * plane per wavefront (bytes and tag trees staged in LDS), larger ones one
  * plane per thread; "ht_wave_grain" -- code-blocks a wavefront of the
  * persistent wave-per-block coders takes from its work queue at a time;
- * "fuse_dwt_lift" -- 1 (default): a lift-ht decode whose single lift step
- * runs along a plane axis synthesises the finest DWT level, lifts and
- * narrows in one pass; 0: three passes (same output); "pipeline_windows" --
+ * "fuse_dwt_lift" -- a lift-ht decode window whose chunks have a single lift
+ * step along a plane axis synthesises the finest DWT level, lifts and
+ * narrows in one pass when it holds at least this many fused work items
+ * (64 x 32 tiles x lift segments; default 4096); 0: three passes (same
+ * output); "pipeline_windows" --
  * 1 (default): the windows of a large decode batch alternate between two
  * streams so that their passes overlap; 0: one after the other;
- * "window_elems" -- samples per batch window (0: NDIC_WINDOW_ELEMS, else 2^31).
+ * "window_elems" -- samples per batch window (0: NDIC_WINDOW_ELEMS, else 2^31);
+ * "ht_lane_waves" -- waves per workgroup of the lane-per-block HT decoder
+ * (0: four on windows with at least four waves per CU, else one; 1-3: one;
+ * 4: four).
  * ndic_tune_set returns 0, -1 for an unknown key, -2 for a value out of
  * range; ndic_tune_get returns the value in force (-1: unknown key);
  * ndic_tune_reset restores the built-in default; ndic_tune_name lists the

@@ -59,11 +59,16 @@ struct Context {
   void* down(const void* d, size_t n, hipStream_t s);
   // Reclaims the staging; only between windows (no copy of it in flight).
   void release_kept() { pin.reset(); }
-  // A second context of this one (same device, own buffers, staging and
-  // stream) for a window queued while one of this context is in flight;
-  // created on first use, used under this context's lock.
+  // A second context of this one (same device, own buffers and staging) for
+  // a window queued while one of this context is in flight; created on first
+  // use, used under this context's lock.  Its stream exists only while a
+  // batch uses it (open_stream / close_stream): HIP maps a process's streams
+  // onto a few hardware queues, and an idle extra stream there changes which
+  // of the other threads' streams share one.
   Context* twin_ctx = nullptr;
   Context& twin();
+  void open_stream();
+  void close_stream();
   static Context& get();
 };
 

@@ -268,9 +268,22 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   else launch_ht_decode_wave(d_tasks, ntasks, ctx.queue(0), s);
   launch_ht_refine(d_tasks, ntasks, s);
   timer->mark();
+  // fused only when the window has the work items to fill the GPU many
+  // times over: each fused workgroup walks its tile's whole lift axis, so a
+  // small window (a serving read of one chunk) runs faster as separate passes
   std::vector<char> fuse(n, 0);
-  if (tune(kTuneFuseDwtLift))
-    for (size_t i = 0; i < n; ++i) fuse[i] = fusable(*items[i], dc[i]);
+  if (const int64_t min_items = tune(kTuneFuseDwtLift)) {
+    uint64_t total = 0;
+    for (size_t i = 0; i < n; ++i) {
+      if (!(fuse[i] = fusable(*items[i], dc[i]))) continue;
+      const ndic_lift_step& st = items[i]->steps[0];
+      const uint64_t len = items[i]->shape[st.dimension];
+      const uint64_t group = st.group == 0 ? len : std::min<uint64_t>(st.group, len);
+      total += (uint64_t)dc[i].P / len * ceil_div(len, group) * ceil_div(dc[i].W, kDwtLiftTileW) *
+               ceil_div(dc[i].H, kDwtLiftTileH);
+    }
+    if (total < (uint64_t)min_items) std::fill(fuse.begin(), fuse.end(), 0);
+  }
   std::vector<DwtChunk> dw(n);
   for (size_t i = 0; i < n; ++i) {
     DwtChunk& w = dw[i];
@@ -481,6 +494,7 @@ void decode_items(Context& ctx, std::vector<DecodeItem>& items, hipStream_t s, u
     return;
   }
   Context& twin = ctx.twin();
+  twin.open_stream();
   hipEvent_t ready;
   NDIC_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
   std::function<void()> pending[2];
@@ -506,10 +520,11 @@ void decode_items(Context& ctx, std::vector<DecodeItem>& items, hipStream_t s, u
     }
   } catch (...) {
     (void)hipStreamSynchronize(s);
-    (void)hipStreamSynchronize(twin.stream);
+    twin.close_stream();
     (void)hipEventDestroy(ready);
     throw;
   }
+  twin.close_stream();
   NDIC_HIP(hipEventDestroy(ready));
 }
 

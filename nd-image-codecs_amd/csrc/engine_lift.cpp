@@ -77,14 +77,14 @@ const std::thread::id g_loader_thread = std::this_thread::get_id();
 
 void destroy_context(Context* c) {
   if (c->twin_ctx) destroy_context(c->twin_ctx);
-  (void)hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->in, &c->out, &c->plane, &c->tmp, &c->coef, &c->work, &c->jobs, &c->dwtjobs, &c->fusedjobs,
                     &c->small})
     b->release();
   if (c->queues) (void)hipFree(c->queues);
   c->pin.reset();
   if (c->pin.base) (void)hipHostFree(c->pin.base);
-  (void)hipStreamDestroy(c->stream);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -115,8 +115,22 @@ uint64_t* Context::queue(int k) {
 }
 
 Context& Context::twin() {
-  if (!twin_ctx) twin_ctx = new_context();
+  if (!twin_ctx) {
+    twin_ctx = new Context();
+    twin_ctx->device = device;
+  }
   return *twin_ctx;
+}
+
+void Context::open_stream() {
+  if (!stream) NDIC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+}
+
+void Context::close_stream() {
+  if (!stream) return;
+  (void)hipStreamSynchronize(stream);
+  (void)hipStreamDestroy(stream);
+  stream = nullptr;
 }
 
 Context& Context::get() {

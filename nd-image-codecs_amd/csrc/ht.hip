@@ -640,10 +640,12 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Four independent waves per workgroup; lane l of wave w decodes block
-// 256 b + 64 w + l (in `order`).  The workgroup shares one LDS copy of the
-// decode tables; each wave has its own context rows and stage.
-constexpr uint32_t kLaneWaves = 4;
+// kW independent waves per workgroup; lane l of wave w decodes block
+// 64 (kW b + w) + l (in `order`).  The workgroup shares one LDS copy of the
+// decode tables; each wave has its own context rows and stage.  Four waves
+// share a copy on large windows; a window with fewer waves than four per CU
+// runs one wave per workgroup, spread over every CU.
+template <uint32_t kLaneWaves>
 __global__ __launch_bounds__(64 * kLaneWaves) void ht_decode_kernel(const HtDecTask* __restrict__ tasks,
                                                                    const uint32_t* __restrict__ order, uint64_t n) {
   __shared__ __align__(16) uint16_t tabs[kTabAll];
@@ -2169,8 +2171,18 @@ void ht_init_tables() {
 
 void launch_ht_decode(const HtDecTask* d_tasks, const uint32_t* d_order, uint64_t n, hipStream_t stream) {
   if (n == 0) return;
-  hipLaunchKernelGGL(ht_decode_kernel, dim3((uint32_t)ceil_div(n, 64 * kLaneWaves)), dim3(64 * kLaneWaves), 0, stream,
-                     d_tasks, d_order, n);
+  static int cus = [] {
+    int dev = 0, c = 0;
+    NDIC_HIP(hipGetDevice(&dev));
+    NDIC_HIP(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
+    return std::max(1, c);
+  }();
+  int64_t waves = tune(kTuneHtLaneWaves);
+  if (waves == 0) waves = ceil_div(n, 64) >= 4 * (uint64_t)cus ? 4 : 1;
+  if (waves == 4)
+    hipLaunchKernelGGL(ht_decode_kernel<4>, dim3((uint32_t)ceil_div(n, 256)), dim3(256), 0, stream, d_tasks, d_order, n);
+  else
+    hipLaunchKernelGGL(ht_decode_kernel<1>, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, d_order, n);
   NDIC_HIP(hipGetLastError());
 }
 

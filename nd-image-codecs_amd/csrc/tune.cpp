@@ -20,9 +20,10 @@ const Knob kKnobs[kTuneCount] = {
     {"ht_encode_wave_max_blocks", INT64_MAX, 0, INT64_MAX},
     {"parse_wave_max_planes", 4096, 0, INT64_MAX},
     {"ht_wave_grain", 1, 1, 64},
-    {"fuse_dwt_lift", 1, 0, 1},
+    {"fuse_dwt_lift", 4096, 0, INT64_MAX},
     {"pipeline_windows", 1, 0, 1},
     {"window_elems", 0, 0, (int64_t)1 << 40},
+    {"ht_lane_waves", 0, 0, 4},
 };
 
 std::atomic<int64_t> g_vals[kTuneCount] = {};

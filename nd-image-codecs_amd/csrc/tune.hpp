@@ -18,9 +18,10 @@ enum TuneKey : int {
   // code-blocks a wave of the persistent wave-per-block coders takes from
   // the work queue at a time
   kTuneHtWaveGrain,
-  // 1: the finest inverse DWT level, the inverse nd_lift step along a plane
-  // axis and the narrowing store run as one pass (dwt_lift_kernel) wherever
-  // a chunk allows it; 0: as separate passes
+  // the finest inverse DWT level, the inverse nd_lift step along a plane
+  // axis and the narrowing store run as one pass (dwt_lift_kernel) in
+  // windows with at least this many fused work items (64 x 32 tiles x lift
+  // segments); 0: always as separate passes
   kTuneFuseDwtLift,
   // 1: the windows of a large decode batch alternate between two streams
   // (one window's parse / entropy decode overlaps the other's synthesis);
@@ -29,6 +30,10 @@ enum TuneKey : int {
   // samples per batch window (bounds the intermediate planes' memory);
   // 0: NDIC_WINDOW_ELEMS from the environment, else 2^31
   kTuneWindowElems,
+  // waves per workgroup of the lane-per-block HT decoder (they share one LDS
+  // copy of the decode tables): 0 = 4 when the window has at least four
+  // waves per CU, else 1; 1-3 = one wave, 4 = four
+  kTuneHtLaneWaves,
   kTuneCount
 };
 
