@@ -158,6 +158,60 @@ typedef struct {
 ndic_status ndic_htj2k_decode_plane_batch(const uint8_t* d_in, uint8_t* d_out, ndic_plane_desc* descs, size_t n,
                                           void* stream, ndic_error* err);
 
+/* ---- byte-range plans and plane-selective decodes ---------------------- */
+
+/* Replaces RangeIndex::from_chunk_prefix + thumbnail / plane / thumbnail_3d
+ * (crates/ndic-codestream/src/range.rs:141-369): what to fetch from an ndht
+ * chunk, planned from its first `len` bytes (at least header + index).
+ * target: 0 thumbnail (arg = max_px, plane 0), 1 plane (arg = z), 2
+ * thumbnail_3d (arg = max_px; steps = the chunk's nd_lift steps: the
+ * low-pass planes of every lifted leading axis).  `ranges` holds (start, end)
+ * uint64 pairs, end inclusive, coalesced and ascending; `planes` the chunk
+ * plane indices in fetch order (uint64); decoded_size the result's shape,
+ * outermost first.  Both buffers are released with ndic_buffer_free. */
+typedef struct {
+  int32_t target;
+  uint32_t max_res;
+  uint32_t ndims;
+  uint64_t decoded_size[34];
+  uint64_t total_bytes;
+  uint64_t header_len;   /* bytes of [header | index] (the first range) */
+  ndic_buffer planes;
+  ndic_buffer ranges;
+} ndic_range_plan;
+ndic_status ndic_range_plan_chunk(const uint8_t* prefix, size_t len, int32_t target, uint64_t arg,
+                                  const ndic_lift_step* steps, size_t nsteps, ndic_range_plan* out, ndic_error* err);
+
+/* Plane-selective decodes of device-resident ndht chunks, driven from each
+ * chunk's own index in device memory (only the selected planes' codestreams
+ * are read and decoded).  target 0 replaces the zarrs partial decoder
+ * (crates/ndic-zarr/src/htj2k_codec.rs:319-411): the planes an axis-aligned
+ * subset (start/count per axis) touches, decoded in full, their (y, x)
+ * window written in C order as `dtype` samples (a sample that does not fit
+ * is an error, as in narrow_samples).  target 1 decodes what thumbnail_3d
+ * plans: the low-pass planes under `steps` at the largest resolution whose
+ * long side fits max_px, as int32 samples.  The library fills max_res,
+ * out_ndim / out_shape and out_bytes; with d_out NULL it only fills them. */
+typedef struct {
+  uint64_t in_offset, in_len, out_offset;
+  uint32_t ndim;
+  uint32_t shape[NDIC_MAX_NDIM];
+  int32_t dtype;         /* target 0: the htj2k codec's dtype */
+  uint32_t xy_levels;
+  int32_t target;        /* 0: subset, 1: thumbnail_3d */
+  uint32_t start[NDIC_MAX_NDIM], count[NDIC_MAX_NDIM];
+  uint32_t max_px;
+  uint32_t nsteps;
+  ndic_lift_step steps[NDIC_MAX_STEPS];
+  uint32_t max_res;      /* filled by the library */
+  uint32_t out_ndim;
+  uint64_t out_shape[NDIC_MAX_NDIM];
+  uint64_t out_bytes;
+  int32_t status;
+} ndic_select_desc;
+ndic_status ndic_htj2k_decode_select_batch(const uint8_t* d_in, uint8_t* d_out, ndic_select_desc* descs, size_t n,
+                                           void* stream, ndic_error* err);
+
 /* ---- nd_zfp / zfp (LLNL ZFP streams, codec version 5) ------------------ */
 
 /* ZfpMode (crates/ndic-zfp/src/lib.rs:77-86) as carried by NdZfpConfig

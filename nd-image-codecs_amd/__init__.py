@@ -86,6 +86,21 @@ class ZfpDesc(C.Structure):
                 ("dtype", C.c_int32), ("config", ZfpConfig), ("status", C.c_int32)]
 
 
+class RangePlanC(C.Structure):
+    _fields_ = [("target", C.c_int32), ("max_res", C.c_uint32), ("ndims", C.c_uint32),
+                ("decoded_size", C.c_uint64 * 34), ("total_bytes", C.c_uint64), ("header_len", C.c_uint64),
+                ("planes", Buffer), ("ranges", Buffer)]
+
+
+class SelectDesc(C.Structure):
+    _fields_ = [("in_offset", C.c_uint64), ("in_len", C.c_uint64), ("out_offset", C.c_uint64),
+                ("ndim", C.c_uint32), ("shape", C.c_uint32 * MAX_NDIM), ("dtype", C.c_int32),
+                ("xy_levels", C.c_uint32), ("target", C.c_int32), ("start", C.c_uint32 * MAX_NDIM),
+                ("count", C.c_uint32 * MAX_NDIM), ("max_px", C.c_uint32), ("nsteps", C.c_uint32),
+                ("steps", LiftStep * MAX_STEPS), ("max_res", C.c_uint32), ("out_ndim", C.c_uint32),
+                ("out_shape", C.c_uint64 * MAX_NDIM), ("out_bytes", C.c_uint64), ("status", C.c_int32)]
+
+
 ZFP_MODES = {"fixed_rate": 0, "fixed_accuracy": 1, "fixed_precision": 2, "reversible": 3}
 
 
@@ -111,6 +126,8 @@ def _load() -> C.CDLL:
                                             P, E]),
         "ndic_zfp_encode_batch": (C.c_int, [P, P, S, P, S, P, E]),
         "ndic_zfp_decode_batch": (C.c_int, [P, P, P, S, P, E]),
+        "ndic_range_plan_chunk": (C.c_int, [C.c_char_p, S, C.c_int32, C.c_uint64, P, S, C.POINTER(RangePlanC), E]),
+        "ndic_htj2k_decode_select_batch": (C.c_int, [P, P, P, S, P, E]),
         "ndic_tune_set": (C.c_int32, [C.c_char_p, C.c_int64]),
         "ndic_tune_get": (C.c_int64, [C.c_char_p]),
         "ndic_tune_reset": (C.c_int32, [C.c_char_p]),
@@ -290,6 +307,73 @@ def encode_batch(d_in_ptr: int, d_out_ptr: int, capacity: int, descs, plain: boo
     rc = _lib.ndic_lift_ht_encode_batch(C.c_void_p(d_in_ptr), C.c_void_p(d_out_ptr), int(capacity), arr, len(arr),
                                         1 if plain else 0, C.c_void_p(stream), C.byref(err))
     _check(rc, err)
+
+
+# ---------------------------------------------------------------------------
+# byte-range plans and plane-selective decodes (crates/ndic-codestream/src/
+# range.rs:141-369, crates/ndic-zarr/src/htj2k_codec.rs:319-411)
+# ---------------------------------------------------------------------------
+PLAN_TARGETS = {"thumbnail": 0, "plane": 1, "thumbnail_3d": 2}
+
+
+def range_plan(prefix: bytes, target: str, arg: int, transforms: Sequence[dict] = ()) -> dict:
+    """RangeIndex::from_chunk_prefix(prefix) then thumbnail(arg = max_px),
+    plane(arg = z) or thumbnail_3d(arg = max_px, transforms): the Plan as a
+    dict (target, decoded_size, max_res, planes, ranges as end-inclusive
+    (start, end) pairs, total_bytes)."""
+    data = _as_bytes(prefix)
+    st, ns = _steps(transforms)
+    out = RangePlanC()
+    err = Error()
+    rc = _lib.ndic_range_plan_chunk(data, len(data), PLAN_TARGETS[target], int(arg), st, ns, C.byref(out),
+                                    C.byref(err))
+    _check(rc, err)
+    planes = np.frombuffer(_take(out.planes), np.uint64).tolist()
+    flat = np.frombuffer(_take(out.ranges), np.uint64).tolist()
+    return {"target": target, "decoded_size": [int(out.decoded_size[i]) for i in range(out.ndims)],
+            "max_res": int(out.max_res), "planes": planes,
+            "ranges": [(flat[2 * i], flat[2 * i + 1]) for i in range(len(flat) // 2)],
+            "total_bytes": int(out.total_bytes)}
+
+
+def make_select_desc(shape: Sequence[int], dtype: Any, xy_levels: int, in_offset: int, in_len: int,
+                     out_offset: int = 0, *, subset: Sequence[tuple] = None, max_px: int = None,
+                     transforms: Sequence[dict] = ()) -> SelectDesc:
+    """A plane-selective decode request: subset = [(start, count)] per axis
+    (the partial decoder), or max_px (+ the chunk's nd_lift transforms) for
+    the thumbnail_3d planes."""
+    if len(shape) > MAX_NDIM or len(transforms) > MAX_STEPS:
+        raise NdicError(1, 0, "select descriptors hold at most 8 dimensions and 4 lift steps")
+    d = SelectDesc()
+    d.in_offset, d.in_len, d.out_offset = int(in_offset), int(in_len), int(out_offset)
+    d.ndim = len(shape)
+    for i, x in enumerate(shape):
+        d.shape[i] = int(x)
+    d.dtype = _dtype_code(dtype)
+    d.xy_levels = int(xy_levels)
+    if subset is not None:
+        d.target = 0
+        for i, (a, n) in enumerate(subset):
+            d.start[i], d.count[i] = int(a), int(n)
+    else:
+        d.target = 1
+        d.max_px = int(max_px)
+        st, _ = _steps(transforms)
+        d.nsteps = len(transforms)
+        for i in range(len(transforms)):
+            d.steps[i] = st[i]
+    return d
+
+
+def decode_select_batch(d_in_ptr: int, d_out_ptr: int, descs, stream: int = 0):
+    """Plane-selective decodes of device-resident chunks; with d_out_ptr 0 the
+    descriptors only receive their output shapes and sizes."""
+    arr = descs if isinstance(descs, C.Array) else (SelectDesc * len(descs))(*descs)
+    err = Error()
+    rc = _lib.ndic_htj2k_decode_select_batch(C.c_void_p(d_in_ptr), C.c_void_p(d_out_ptr), arr, len(arr),
+                                             C.c_void_p(stream), C.byref(err))
+    _check(rc, err)
+    return arr
 
 
 # ---------------------------------------------------------------------------

@@ -253,21 +253,24 @@ __device__ bool locate(const DevChunk& ch, uint64_t p, uint64_t gid, DevPlane& p
     if (pl.cs_len == 0) return raise(ch, kErrChunkTruncated, n, gid), false;
     return true;
   }
+  // the container plane this decoded plane is (a plane-selective decode
+  // reads only the index entries of the planes it decodes)
+  const uint64_t cp = ch.plane_sel ? ch.plane_sel[p] : p;
   uint64_t entry = 12 + 4ull * (lv + 1);
-  uint64_t need = fixed + (uint64_t)ch.P * entry;
+  uint64_t need = fixed + (uint64_t)ch.P_all * entry;
   if (n < need) return raise(ch, kErrChunkTruncated, n, gid), false;
-  const uint8_t* e = b + fixed + p * entry;
+  const uint8_t* e = b + fixed + cp * entry;
   uint64_t off = le64(e);
   uint32_t len = le32(e + 8);
   uint64_t min_off = need;
-  if (p > 0) {
+  if (cp > 0) {
     const uint8_t* pe = e - entry;
     min_off = le64(pe) + le32(pe + 8);
   }
-  if (off < min_off) return raise(ch, kErrChunkIndex, fixed + p * entry, gid), false;
+  if (off < min_off) return raise(ch, kErrChunkIndex, fixed + cp * entry, gid), false;
   for (uint32_t r = 0; r + 1 <= lv; ++r)
-    if (le32(e + 12 + 4 * r) > le32(e + 16 + 4 * r)) return raise(ch, kErrChunkIndex, fixed + p * entry, gid), false;
-  if (le32(e + 12 + 4 * lv) > len) return raise(ch, kErrChunkIndex, fixed + p * entry, gid), false;
+    if (le32(e + 12 + 4 * r) > le32(e + 16 + 4 * r)) return raise(ch, kErrChunkIndex, fixed + cp * entry, gid), false;
+  if (le32(e + 12 + 4 * lv) > len) return raise(ch, kErrChunkIndex, fixed + cp * entry, gid), false;
   if (off + len > n) return raise(ch, kErrPlaneRange, off, gid), false;
   pl.cs_off = off;
   pl.cs_len = len;
@@ -763,10 +766,17 @@ __global__ void walk_chunks_kernel(const DevChunk* __restrict__ chunks, uint32_t
   if (ch.no_container || ch.in_len < 8 || (ch.in[5] & 1)) return;
   uint32_t ndim = ch.in[7];
   uint64_t off = 8 + 4ull * ndim;
-  for (uint64_t p = 0; p < ch.P; ++p) {
-    DevPlane& pl = planes[ch.plane_base + p];
-    pl.cs_off = off;
-    pl.cs_len = 0;
+  // every container plane is walked (lengths are only known in order); the
+  // decoded planes (all, or the selected ones in order) take their offsets
+  uint64_t j = 0;
+  for (uint64_t cp = 0; cp < ch.P_all && j < ch.P; ++cp) {
+    DevPlane* pl = nullptr;
+    if (!ch.plane_sel || ch.plane_sel[j] == cp) {
+      pl = &planes[ch.plane_base + j];
+      ++j;
+      pl->cs_off = off;
+      pl->cs_len = 0;
+    }
     const uint8_t* cs = ch.in + off;
     if (off > ch.in_len) return;
     uint64_t len = ch.in_len - off, pos = 2;
@@ -794,7 +804,7 @@ __global__ void walk_chunks_kernel(const DevChunk* __restrict__ chunks, uint32_t
       }
       pos += psot;
     }
-    pl.cs_len = pos;
+    if (pl) pl->cs_len = pos;
     off += pos;
   }
 }
@@ -817,6 +827,60 @@ __global__ __launch_bounds__(256) void unshift_kernel(const DevChunk* __restrict
     uint64_t y = i / ow, x = i % ow;
     int64_t v = (int64_t)d[y * ch.W + x] + half;
     d[y * ch.W + x] = (int32_t)(v < 0 ? 0 : v > mx ? mx : v);
+  }
+}
+
+// Window copy of decoded planes with narrowing (htj2k_codec.rs:393-407: the
+// (y, x) window of each selected plane, narrowed like htj2k.rs
+// narrow_samples); one workgroup per output row.
+__device__ __forceinline__ uint32_t find_window_job(const WindowJob* jobs, uint32_t njobs, uint64_t item) {
+  uint32_t lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) / 2;
+    if (jobs[mid].item_begin <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void window_kernel(const WindowJob* __restrict__ jobs, uint32_t njobs) {
+  const uint64_t item = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  const WindowJob& j = jobs[find_window_job(jobs, njobs, item)];
+  const uint64_t local = item - j.item_begin;
+  if (local >= j.items) return;
+  const uint64_t plane = local / j.h, row = local % j.h;
+  const int32_t* src = j.src + plane * j.plane_elems + (uint64_t)(j.y0 + row) * j.W + j.x0;
+  const uint64_t o = (plane * j.h + row) * j.w;
+  bool ok = true;
+  int32_t lo = INT32_MAX, hi = INT32_MIN;
+  for (uint32_t x = threadIdx.x; x < j.w; x += blockDim.x) {
+    const int32_t v = src[x];
+    if (!j.dst) {
+      lo = v < lo ? v : lo;
+      hi = v > hi ? v : hi;
+      continue;
+    }
+    uint8_t* d = j.dst + (o + x) * j.dsize;
+    switch (j.dtype) {
+      case NDIC_U8: *d = (uint8_t)v; ok &= v >= 0 && v <= 255; break;
+      case NDIC_I8: *d = (uint8_t)v; ok &= v >= -128 && v <= 127; break;
+      case NDIC_U16: *(uint16_t*)d = (uint16_t)v; ok &= v >= 0 && v <= 65535; break;
+      case NDIC_I16: *(int16_t*)d = (int16_t)v; ok &= v >= -32768 && v <= 32767; break;
+      case NDIC_U32: *(uint32_t*)d = (uint32_t)v; ok &= v >= 0; break;
+      default: *(int32_t*)d = v; break;
+    }
+  }
+  if (!ok) atomicOr(j.flag, 1);
+  if (!j.dst) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const int32_t a = __shfl_xor(lo, off), b = __shfl_xor(hi, off);
+      lo = a < lo ? a : lo;
+      hi = b > hi ? b : hi;
+    }
+    if ((threadIdx.x & 63) == 0 && lo <= hi) {
+      atomicMin((long long*)&j.minmax[0], (long long)lo);
+      atomicMax((long long*)&j.minmax[1], (long long)hi);
+    }
   }
 }
 
@@ -1333,6 +1397,14 @@ void launch_walk_chunks(const DevChunk* d_chunks, uint32_t nchunks, DevPlane* pl
   if (!nchunks) return;
   hipLaunchKernelGGL(walk_chunks_kernel, dim3((uint32_t)ceil_div(nchunks, 64)), dim3(64), 0, stream, d_chunks,
                      nchunks, planes);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_window(const WindowJob* d_jobs, uint32_t njobs, uint64_t rows, hipStream_t stream) {
+  if (!rows) return;
+  uint32_t gx = (uint32_t)(rows < 65535 ? rows : 65535);
+  uint32_t gy = (uint32_t)ceil_div(rows, gx);
+  hipLaunchKernelGGL(window_kernel, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs);
   NDIC_HIP(hipGetLastError());
 }
 

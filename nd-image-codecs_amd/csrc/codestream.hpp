@@ -54,6 +54,8 @@ struct DevChunk {
   uint8_t* tag;           // tag-tree scratch, tag_stride bytes per plane
   uint32_t tag_stride;
   uint32_t no_container;  // decode: bit 0 `in` is one bare codestream, bit 1 it may be a prefix
+  const uint32_t* plane_sel;  // decode: container plane of each decoded plane (ascending), or null: plane p
+  uint32_t P_all;             // decode: planes in the container (P when plane_sel is null)
   // encode
   int32_t* planes_in;     // the int32 spatial planes (P x W x H)
   int32_t sample_signed;  // the sample dtype is signed (SIZ signedness)
@@ -70,6 +72,22 @@ struct DevChunk {
   uint64_t out_off;       // encode: byte offset of the chunk in the output
   uint64_t out_len;
 };
+
+// One chunk of a window copy: the top-left-anchored window (x0, y0, w, h)
+// of each of `planes` int32 planes (W x H, row pitch W, plane_elems apart)
+// written packed (plane, row, column) as `dtype` samples, with a range flag;
+// range mode (dst null) folds the window's min / max into minmax[0..1].
+struct WindowJob {
+  const int32_t* src;
+  uint8_t* dst;
+  int32_t* flag;
+  int64_t* minmax;
+  uint64_t plane_elems;
+  uint32_t W, x0, y0, w, h, planes;
+  uint32_t dtype, dsize;
+  uint64_t items, item_begin;  // rows: planes x h
+};
+void launch_window(const WindowJob* d_jobs, uint32_t njobs, uint64_t rows, hipStream_t stream);
 
 // Tag-tree scratch bytes a plane needs for a band grid of nx x ny blocks.
 uint32_t tag_scratch_bytes(uint32_t nx, uint32_t ny);

@@ -40,7 +40,7 @@ struct Context {
   std::recursive_mutex mu;
   hipStream_t stream = nullptr;
   int device = 0;
-  DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, fusedjobs, small;
+  DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, fusedjobs, winjobs, small;
   uint64_t* queues = nullptr;  // work-queue counter pairs of the persistent kernels (zeroed)
   // Counter pair k (0: HT decode, 1: HT encode) of this context's queues.
   uint64_t* queue(int k);
@@ -113,6 +113,9 @@ struct EncodeItem;
 // window_elems bounds the samples whose intermediate planes are resident at
 // once.
 void decode_items(Context& ctx, std::vector<DecodeItem>& items, hipStream_t s, uint64_t window_elems);
+// Samples per batch window: the window_elems knob, else NDIC_WINDOW_ELEMS,
+// else 2^31.
+uint64_t window_elems_knob();
 void encode_items(Context& ctx, std::vector<EncodeItem>& items, uint8_t* d_out, uint64_t capacity, hipStream_t s,
                   uint64_t window_elems);
 

@@ -16,9 +16,7 @@
 
 namespace ndic {
 
-namespace {
-
-uint64_t window_elems() {
+uint64_t window_elems_knob() {
   const int64_t k = tune(kTuneWindowElems);
   if (k > 0) return (uint64_t)k;
   static uint64_t w = [] {
@@ -28,6 +26,10 @@ uint64_t window_elems() {
   }();
   return w;
 }
+
+namespace {
+
+uint64_t window_elems() { return window_elems_knob(); }
 
 bool ht_dtype_ok(int d) {
   return d == NDIC_U8 || d == NDIC_I8 || d == NDIC_U16 || d == NDIC_I16 || d == NDIC_U32 || d == NDIC_I32;

@@ -209,7 +209,8 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
     for (size_t d = 0; d < it.shape.size() && d < 32; ++d) c.dims[d] = (uint32_t)it.shape[d];
     c.W = W;
     c.H = H;
-    c.P = (uint32_t)planes_of(it.shape);
+    c.P_all = (uint32_t)planes_of(it.shape);
+    c.P = it.plane_sel.empty() ? c.P_all : (uint32_t)it.plane_sel.size();
     c.L = it.L;
     c.max_res = it.max_res < 0 ? (int32_t)it.L : std::min<int32_t>(it.max_res, it.L);
     c.geo_packet = g.packet_first;
@@ -230,7 +231,10 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
            sz_gb = al(geo.bands.size() * sizeof(GeoBand)), sz_planes = al(nplanes * sizeof(DevPlane)),
            sz_tasks = al(ntasks * sizeof(HtDecTask)), sz_tag = al(nplanes * tag_stride), sz_err = al(n * 16),
            sz_ptr = al(n * sizeof(void*)), sz_dims = al(n * 16), sz_flag = al(n * 4);
-  uint64_t meta = sz_chunks + sz_gp + sz_gb + sz_planes + sz_tasks + sz_tag + sz_err + sz_ptr + sz_dims + sz_flag;
+  uint64_t nsel = 0;
+  for (auto* it : items) nsel += it->plane_sel.size();
+  const uint64_t sz_sel = al(nsel * 4);
+  uint64_t meta = sz_chunks + sz_gp + sz_gb + sz_planes + sz_tasks + sz_tag + sz_err + sz_ptr + sz_dims + sz_flag + sz_sel;
   uint8_t* m = (uint8_t*)ctx.work.ensure(meta);
   DevChunk* d_chunks = (DevChunk*)m;
   GeoPacket* d_gp = (GeoPacket*)(m + sz_chunks);
@@ -242,6 +246,16 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   int32_t** d_ptr = (int32_t**)((uint8_t*)d_err + sz_err);
   uint64_t* d_dims = (uint64_t*)((uint8_t*)d_ptr + sz_ptr);
   int32_t* d_flag = (int32_t*)((uint8_t*)d_dims + sz_dims);
+  uint32_t* d_sel = (uint32_t*)((uint8_t*)d_flag + sz_flag);
+  if (nsel) {
+    std::vector<uint32_t> sel;
+    sel.reserve(nsel);
+    for (size_t i = 0; i < n; ++i) {
+      dc[i].plane_sel = items[i]->plane_sel.empty() ? nullptr : d_sel + sel.size();
+      sel.insert(sel.end(), items[i]->plane_sel.begin(), items[i]->plane_sel.end());
+    }
+    ctx.up(d_sel, sel.data(), nsel * 4, s);
+  }
   int32_t* A = (int32_t*)ctx.coef.ensure(al(3 * coef * 4 + 64));
   int32_t* T = A + coef;
   int32_t* O = A + 2 * coef;
@@ -320,6 +334,9 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   std::vector<DwtLiftJob> fused;
   std::vector<size_t> fused_of(n, SIZE_MAX);
   uint64_t fused_items = 0;
+  std::vector<WindowJob> windows;
+  std::vector<size_t> window_of(n, SIZE_MAX);
+  uint64_t window_rows = 0;
   for (size_t i = 0; i < n; ++i) {
     DecodeItem& it = *items[i];
     uint64_t total = (uint64_t)dc[i].P * dc[i].W * dc[i].H;
@@ -356,6 +373,27 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
       }
       continue;
     }
+    if (it.window) {
+      WindowJob wj{};
+      wj.src = res[i];
+      wj.dst = it.d_out;
+      wj.flag = d_flag + i;
+      wj.plane_elems = (uint64_t)dc[i].W * dc[i].H;
+      wj.W = dc[i].W;
+      wj.x0 = it.win[0];
+      wj.y0 = it.win[1];
+      wj.w = it.win[2];
+      wj.h = it.win[3];
+      wj.planes = dc[i].P;
+      wj.dtype = (uint32_t)it.ht_dtype;
+      wj.dsize = (uint32_t)dtype_size(it.ht_dtype);
+      wj.items = (uint64_t)wj.planes * wj.h;
+      wj.item_begin = window_rows;
+      window_rows += wj.items;
+      window_of[i] = windows.size();
+      if (wj.items && wj.w) windows.push_back(wj);
+      continue;
+    }
     if (it.bare) {
       // plane decodes hand back the int32 region: into the caller's buffer
       // when one is given (the window's planes are reused afterwards)
@@ -385,11 +423,17 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
     launch_dwt_lift(d_fused, (uint32_t)fused.size(), fused_items, s);
   }
   if (!lifts.empty()) run_lift(ctx, lifts, false, s);
+  if (!windows.empty()) {
+    WindowJob* d_win = (WindowJob*)ctx.winjobs.ensure(windows.size() * sizeof(WindowJob));
+    ctx.up(d_win, windows.data(), windows.size() * sizeof(WindowJob), s);
+    launch_window(d_win, (uint32_t)windows.size(), window_rows, s);
+  }
   timer->mark();
   const int32_t* p_err = (const int32_t*)ctx.down(d_err, n * 16, s);
   const int32_t* p_flag = (const int32_t*)ctx.down(d_flag, n * 4, s);
   return [&ctx, s, items, n, p_err, p_flag, coef, timer, dc = std::move(dc), dw = std::move(dw),
           fuse = std::move(fuse), fused = std::move(fused), fused_of = std::move(fused_of),
+          windows = std::move(windows), window_of = std::move(window_of),
           lifts = std::move(lifts), res = std::move(res)]() {
   NDIC_HIP(hipStreamSynchronize(s));
   std::vector<int32_t> herr(p_err, p_err + 4 * n), hflag(p_flag, p_flag + n);
@@ -437,7 +481,16 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
         src = (int32_t*)lc.plane;
       }
       NDIC_HIP(hipMemcpyAsync(mm, init, 16, hipMemcpyHostToDevice, s));
-      if (fused_of[i] != SIZE_MAX) {
+      if (window_of[i] != SIZE_MAX) {
+        // the window that was narrowed, again in range mode
+        WindowJob wj = windows[window_of[i]];
+        wj.dst = nullptr;
+        wj.minmax = mm;
+        wj.item_begin = 0;
+        WindowJob* d_w = (WindowJob*)ctx.winjobs.ensure(sizeof(WindowJob));
+        NDIC_HIP(hipMemcpyAsync(d_w, &wj, sizeof(wj), hipMemcpyHostToDevice, s));
+        launch_window(d_w, 1, wj.items, s);
+      } else if (fused_of[i] != SIZE_MAX) {
         // no lifted plane was written: the fused pass again, in range mode
         DwtLiftJob f = fused[fused_of[i]];
         f.dst = nullptr;

@@ -79,7 +79,7 @@ void destroy_context(Context* c) {
   if (c->twin_ctx) destroy_context(c->twin_ctx);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->in, &c->out, &c->plane, &c->tmp, &c->coef, &c->work, &c->jobs, &c->dwtjobs, &c->fusedjobs,
-                    &c->small})
+                    &c->winjobs, &c->small})
     b->release();
   if (c->queues) (void)hipFree(c->queues);
   c->pin.reset();

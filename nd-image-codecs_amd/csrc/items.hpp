@@ -24,6 +24,13 @@ struct DecodeItem {
   bool prefix = false;      // ... possibly cut short (a byte-range prefix)
   uint64_t out_w = 0, out_h = 0;
   int32_t* d_result = nullptr;  // bare: the decoded region (row pitch = plane width)
+  // plane-selective decode: the container planes to decode (ascending;
+  // empty = every plane), and, with `window`, the (x0, y0, w, h) window of
+  // each decoded plane at the decoded resolution, written packed to d_out as
+  // ht_dtype samples
+  std::vector<uint32_t> plane_sel;
+  bool window = false;
+  uint32_t win[4] = {0, 0, 0, 0};
   // results
   int status = NDIC_OK;
   uint64_t offset = 0;
