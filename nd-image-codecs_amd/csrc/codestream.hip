@@ -15,6 +15,8 @@
 // (packet.rs:69-168), markers and PLT/TLM (writer.rs:198-322), the container
 // header and coefficient-plane index (container.rs:80-100, 249-280), then a
 // workgroup per plane copies every piece into the packed output.
+#include <algorithm>
+
 #include "codestream.hpp"
 #include "tune.hpp"
 
@@ -1397,6 +1399,19 @@ void launch_walk_chunks(const DevChunk* d_chunks, uint32_t nchunks, DevPlane* pl
   if (!nchunks) return;
   hipLaunchKernelGGL(walk_chunks_kernel, dim3((uint32_t)ceil_div(nchunks, 64)), dim3(64), 0, stream, d_chunks,
                      nchunks, planes);
+  NDIC_HIP(hipGetLastError());
+}
+
+__global__ void fill32_kernel(uint32_t* p, uint32_t value, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+void launch_fill32(void* p, uint32_t value, uint64_t bytes, hipStream_t stream) {
+  const uint64_t n = bytes / 4;
+  if (!n) return;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(ceil_div(n, 256), 1024);
+  hipLaunchKernelGGL(fill32_kernel, dim3(blocks), dim3(256), 0, stream, (uint32_t*)p, value, n);
   NDIC_HIP(hipGetLastError());
 }
 

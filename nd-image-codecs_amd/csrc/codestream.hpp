@@ -88,6 +88,9 @@ struct WindowJob {
   uint64_t items, item_begin;  // rows: planes x h
 };
 void launch_window(const WindowJob* d_jobs, uint32_t njobs, uint64_t rows, hipStream_t stream);
+// Sets `bytes` (a multiple of 4) at p to 32-bit `value`: a kernel rather
+// than hipMemsetAsync, so that it can stand in a captured graph.
+void launch_fill32(void* p, uint32_t value, uint64_t bytes, hipStream_t stream);
 
 // Tag-tree scratch bytes a plane needs for a band grid of nx x ny blocks.
 uint32_t tag_scratch_bytes(uint32_t nx, uint32_t ny);

@@ -21,6 +21,12 @@ struct DevBuf {
   void release();
 };
 
+// While a window's launch sequence is being captured into a HIP graph a
+// buffer that must grow cannot free its old allocation (a free synchronises
+// the device): DevBuf::ensure parks it here instead, and the context frees it
+// once the window has completed.
+extern thread_local std::vector<void*>* g_parked_frees;
+
 // Pinned host staging for the small tables a call uploads (descriptors, job
 // lists) and the statuses it reads back: copies from pinned memory are
 // queued without the host waiting, where a copy from pageable memory would
@@ -57,8 +63,20 @@ struct Context {
   // Queue a download into pinned staging; the bytes are there once s is
   // synchronised.
   void* down(const void* d, size_t n, hipStream_t s);
-  // Reclaims the staging; only between windows (no copy of it in flight).
-  void release_kept() { pin.reset(); }
+  // Reclaims the staging and the allocations parked during a capture; only
+  // between windows (nothing of them in flight).
+  std::vector<void*> parked;
+  void release_kept();
+  // Instantiated decode-window graphs, keyed by their node sequence (node
+  // types and kernel functions); a new capture with the same sequence
+  // updates one in place instead of instantiating again.
+  struct CachedGraph {
+    uint64_t sig;
+    hipGraphExec_t exec;
+  };
+  std::vector<CachedGraph> graphs;
+  // Launches graph g on s through the cache (g is consumed).
+  void launch_graph(hipGraph_t g, hipStream_t s);
   // A second context of this one (same device, own buffers and staging) for
   // a window queued while one of this context is in flight; created on first
   // use, used under this context's lock.  Its stream exists only while a

@@ -267,8 +267,27 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
     dc[i].tag_stride = tag_stride;
     co += (uint64_t)dc[i].P * dc[i].W * dc[i].H;
   }
-  NDIC_HIP(hipMemsetAsync(d_planes, 0, sz_planes, s));
-  NDIC_HIP(hipMemsetAsync(d_err, 0, sz_err + sz_ptr + sz_dims + sz_flag, s));
+  // With graph_windows the window's whole launch sequence, from here to the
+  // status downloads, is captured into a graph and launched as one.
+  const bool graph = tune(kTuneGraphWindows) != 0 && !g_prof.on;
+  struct Capture {
+    hipStream_t s = nullptr;
+    ~Capture() {  // an exception mid-capture: end it, drop the partial graph
+      if (!s) return;
+      g_parked_frees = nullptr;
+      hipGraph_t g = nullptr;
+      if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+    }
+  } capture;
+  if (graph) {
+    if ((int64_t)ntasks <= tune(kTuneHtDecodeWaveMaxBlocks)) (void)ctx.queue(0);  // allocated outside the capture
+    NDIC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    capture.s = s;
+    g_parked_frees = &ctx.parked;
+  }
+  launch_fill32(d_planes, 0, sz_planes, s);
+  launch_fill32(d_err, 0, sz_err + sz_ptr + sz_dims + sz_flag, s);
   ctx.up(d_chunks, dc.data(), n * sizeof(DevChunk), s);
   ctx.up(d_gp, geo.packets.data(), geo.packets.size() * sizeof(GeoPacket), s);
   ctx.up(d_gb, geo.bands.data(), geo.bands.size() * sizeof(GeoBand), s);
@@ -431,6 +450,13 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   timer->mark();
   const int32_t* p_err = (const int32_t*)ctx.down(d_err, n * 16, s);
   const int32_t* p_flag = (const int32_t*)ctx.down(d_flag, n * 4, s);
+  if (graph) {
+    g_parked_frees = nullptr;
+    capture.s = nullptr;
+    hipGraph_t g = nullptr;
+    NDIC_HIP(hipStreamEndCapture(s, &g));
+    ctx.launch_graph(g, s);
+  }
   return [&ctx, s, items, n, p_err, p_flag, coef, timer, dc = std::move(dc), dw = std::move(dw),
           fuse = std::move(fuse), fused = std::move(fused), fused_of = std::move(fused_of),
           windows = std::move(windows), window_of = std::move(window_of),

@@ -12,8 +12,15 @@
 
 namespace ndic {
 
+thread_local std::vector<void*>* g_parked_frees = nullptr;
+
 void* DevBuf::ensure(size_t bytes) {
   if (bytes <= cap && ptr) return ptr;
+  if (g_parked_frees && ptr) {
+    g_parked_frees->push_back(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
   release();
   // grow to the next size of the form m * 2^k (8 <= m < 16): at most 1/8
   // over, and a slowly growing sequence of requests reallocates (and
@@ -54,6 +61,54 @@ void PinnedArena::reset() {
   off = 0;
 }
 
+void Context::release_kept() {
+  pin.reset();
+  for (void* p : parked) (void)hipFree(p);
+  parked.clear();
+}
+
+void Context::launch_graph(hipGraph_t g, hipStream_t s) {
+  size_t nn = 0;
+  NDIC_HIP(hipGraphGetNodes(g, nullptr, &nn));
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (nn) NDIC_HIP(hipGraphGetNodes(g, nodes.data(), &nn));
+  uint64_t sig = 1469598103934665603ull ^ nn;
+  for (hipGraphNode_t nd : nodes) {
+    hipGraphNodeType t;
+    NDIC_HIP(hipGraphNodeGetType(nd, &t));
+    sig = (sig ^ (uint64_t)t) * 1099511628211ull;
+    if (t == hipGraphNodeTypeKernel) {
+      hipKernelNodeParams kp;
+      NDIC_HIP(hipGraphKernelNodeGetParams(nd, &kp));
+      sig = (sig ^ (uint64_t)(uintptr_t)kp.func) * 1099511628211ull;
+    }
+  }
+  hipGraphExec_t exec = nullptr;
+  for (size_t i = 0; i < graphs.size(); ++i) {
+    if (graphs[i].sig != sig) continue;
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(graphs[i].exec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+      exec = graphs[i].exec;
+    } else {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(graphs[i].exec);
+      graphs.erase(graphs.begin() + i);
+    }
+    break;
+  }
+  if (!exec) {
+    NDIC_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    if (graphs.size() >= 8) {
+      (void)hipGraphExecDestroy(graphs.front().exec);
+      graphs.erase(graphs.begin());
+    }
+    graphs.push_back({sig, exec});
+  }
+  (void)hipGraphDestroy(g);
+  NDIC_HIP(hipGraphLaunch(exec, s));
+}
+
 void Context::up(void* d, const void* h, size_t n, hipStream_t s) {
   if (!n) return;
   const void* p = stage_keep(h, n);
@@ -78,6 +133,8 @@ const std::thread::id g_loader_thread = std::this_thread::get_id();
 void destroy_context(Context* c) {
   if (c->twin_ctx) destroy_context(c->twin_ctx);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  for (void* p : c->parked) (void)hipFree(p);
   for (DevBuf* b : {&c->in, &c->out, &c->plane, &c->tmp, &c->coef, &c->work, &c->jobs, &c->dwtjobs, &c->fusedjobs,
                     &c->winjobs, &c->small})
     b->release();

@@ -24,6 +24,7 @@ const Knob kKnobs[kTuneCount] = {
     {"pipeline_windows", 1, 0, 1},
     {"window_elems", 0, 0, (int64_t)1 << 40},
     {"ht_lane_waves", 0, 0, 4},
+    {"graph_windows", 0, 0, 1},
 };
 
 std::atomic<int64_t> g_vals[kTuneCount] = {};

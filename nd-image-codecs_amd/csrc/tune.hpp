@@ -34,6 +34,10 @@ enum TuneKey : int {
   // copy of the decode tables): 0 = 4 when the window has at least four
   // waves per CU, else 1; 1-3 = one wave, 4 = four
   kTuneHtLaneWaves,
+  // 1: each decode window's launch sequence is captured into a HIP graph
+  // and launched as one (instantiated graphs are cached per context and
+  // updated in place); 0: launched one by one
+  kTuneGraphWindows,
   kTuneCount
 };
 
