@@ -310,7 +310,9 @@ ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp
  * "window_elems" -- samples per batch window (0: NDIC_WINDOW_ELEMS, else 2^31);
  * "ht_lane_waves" -- waves per workgroup of the lane-per-block HT decoder
  * (0: four on windows with at least four waves per CU, else one; 1-3: one;
- * 4: four).
+ * 4: four); "graph_windows" -- 1: each decode window's launch sequence is
+ * captured into a HIP graph (cached per thread context, updated in place)
+ * and launched as one; 0 (default): launched directly.
  * ndic_tune_set returns 0, -1 for an unknown key, -2 for a value out of
  * range; ndic_tune_get returns the value in force (-1: unknown key);
  * ndic_tune_reset restores the built-in default; ndic_tune_name lists the
