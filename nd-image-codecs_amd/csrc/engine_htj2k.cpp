@@ -234,7 +234,11 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   uint64_t nsel = 0;
   for (auto* it : items) nsel += it->plane_sel.size();
   const uint64_t sz_sel = al(nsel * 4);
-  uint64_t meta = sz_chunks + sz_gp + sz_gb + sz_planes + sz_tasks + sz_tag + sz_err + sz_ptr + sz_dims + sz_flag + sz_sel;
+  const bool lane_decode = (int64_t)ntasks > tune(kTuneHtDecodeWaveMaxBlocks);
+  const bool sort = lane_decode && tune(kTuneHtDecodeSort) != 0;
+  const uint64_t sz_order = sort ? al(ntasks * 4) + al(4 * (kHtClasses + 1)) : 0;
+  uint64_t meta = sz_chunks + sz_gp + sz_gb + sz_planes + sz_tasks + sz_tag + sz_err + sz_ptr + sz_dims + sz_flag + sz_sel +
+                  sz_order;
   uint8_t* m = (uint8_t*)ctx.work.ensure(meta);
   DevChunk* d_chunks = (DevChunk*)m;
   GeoPacket* d_gp = (GeoPacket*)(m + sz_chunks);
@@ -247,6 +251,8 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   uint64_t* d_dims = (uint64_t*)((uint8_t*)d_ptr + sz_ptr);
   int32_t* d_flag = (int32_t*)((uint8_t*)d_dims + sz_dims);
   uint32_t* d_sel = (uint32_t*)((uint8_t*)d_flag + sz_flag);
+  uint32_t* d_order = sort ? (uint32_t*)((uint8_t*)d_sel + sz_sel) : nullptr;
+  uint32_t* d_counts = sort ? d_order + al(ntasks * 4) / 4 : nullptr;
   if (nsel) {
     std::vector<uint32_t> sel;
     sel.reserve(nsel);
@@ -297,8 +303,12 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   launch_init_tasks(d_chunks, (uint32_t)n, ntasks, d_tasks, d_gp, d_gb, s);
   launch_parse_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_gp, d_gb, s);
   timer->mark();
-  if ((int64_t)ntasks > tune(kTuneHtDecodeWaveMaxBlocks)) launch_ht_decode(d_tasks, nullptr, ntasks, s);
-  else launch_ht_decode_wave(d_tasks, ntasks, ctx.queue(0), s);
+  if (lane_decode) {
+    if (sort) launch_ht_order(d_tasks, ntasks, d_counts, d_order, s);
+    launch_ht_decode(d_tasks, d_order, ntasks, s);
+  } else {
+    launch_ht_decode_wave(d_tasks, ntasks, ctx.queue(0), s);
+  }
   launch_ht_refine(d_tasks, ntasks, s);
   timer->mark();
   // fused only when the window has the work items to fill the GPU many

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "ht.hpp"
+#include "codestream.hpp"
 #include "tune.hpp"
 
 namespace ndic {
@@ -2167,6 +2168,69 @@ void ht_init_tables() {
   NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_uvlc0w), wu0.data(), 1280));
   NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_uvlc1w), wu1.data(), 1024));
   done = true;
+}
+
+namespace {
+// Work class of a task: 0 heaviest ... kHtClasses - 1 (left out).
+__device__ __forceinline__ uint32_t ht_class(const HtDecTask& t) {
+  if (!(t.flags & kTaskIncluded) || t.len1 == 0) return kHtClasses - 1;
+  const uint32_t bits = 32 - __clz(t.len1);                        // 1..16 (len1 < 65535)
+  const uint32_t half = (t.len1 >> (bits > 1 ? bits - 2 : 0)) & 1;  // the bit below the top one
+  return 33 - (2 * bits + half);                                     // 0 (longest) .. 31
+}
+
+__global__ __launch_bounds__(256) void ht_class_count(const HtDecTask* __restrict__ tasks, uint64_t n,
+                                                      uint32_t* counts) {
+  __shared__ uint32_t h[kHtClasses];
+  for (uint32_t k = threadIdx.x; k < kHtClasses; k += blockDim.x) h[k] = 0;
+  __syncthreads();
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&h[ht_class(tasks[i])], 1u);
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < kHtClasses; k += blockDim.x)
+    if (h[k]) atomicAdd(&counts[k], h[k]);
+}
+
+__global__ void ht_class_scan(uint32_t* counts) {
+  uint32_t acc = 0;
+  for (uint32_t k = 0; k < kHtClasses; ++k) {
+    const uint32_t c = counts[k];
+    counts[k] = acc;
+    acc += c;
+  }
+}
+
+// One wave per 64 consecutive tasks: each class's lanes take consecutive
+// slots (one atomic per class per wave), so neighbouring blocks of a class
+// stay neighbours in the order.
+__global__ __launch_bounds__(64) void ht_class_scatter(const HtDecTask* __restrict__ tasks, uint64_t n,
+                                                       uint32_t* offsets, uint32_t* order) {
+  const uint64_t i = blockIdx.x * (uint64_t)64 + threadIdx.x;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t cls = i < n ? ht_class(tasks[i]) : 0xFFFFFFFFu;
+  uint64_t pending = __ballot(i < n);
+  while (pending) {
+    const uint32_t leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t c = __shfl(cls, (int)leader);
+    const uint64_t same = __ballot(cls == c) & pending;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&offsets[c], (uint32_t)__popcll(same));
+    base = __shfl(base, (int)leader);
+    if (cls == c && i < n) order[base + __popcll(same & ((1ull << lane) - 1))] = (uint32_t)i;
+    pending &= ~same;
+  }
+}
+}  // namespace
+
+void launch_ht_order(const HtDecTask* d_tasks, uint64_t n, uint32_t* d_counts, uint32_t* d_order, hipStream_t stream) {
+  if (n == 0) return;
+  launch_fill32(d_counts, 0, 4 * (kHtClasses + 1), stream);
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(ceil_div(n, 256), 1024);
+  hipLaunchKernelGGL(ht_class_count, dim3(blocks), dim3(256), 0, stream, d_tasks, n, d_counts);
+  hipLaunchKernelGGL(ht_class_scan, dim3(1), dim3(1), 0, stream, d_counts);
+  hipLaunchKernelGGL(ht_class_scatter, dim3((uint32_t)ceil_div(n, 64)), dim3(64), 0, stream, d_tasks, n, d_counts,
+                     d_order);
+  NDIC_HIP(hipGetLastError());
 }
 
 void launch_ht_decode(const HtDecTask* d_tasks, const uint32_t* d_order, uint64_t n, hipStream_t stream) {

@@ -42,6 +42,11 @@ __host__ __device__ inline uint32_t ht_ms_cap(uint32_t w, uint32_t h, uint32_t k
 }
 
 void ht_init_tables();
+// Orders the tasks heaviest first by work class (the bit length of the
+// cleanup segment; blocks left out come last): a counting sort on the
+// device into d_order (n entries), using d_counts (kHtClasses + 1 words).
+constexpr uint32_t kHtClasses = 34;
+void launch_ht_order(const HtDecTask* d_tasks, uint64_t n, uint32_t* d_counts, uint32_t* d_order, hipStream_t stream);
 // d_order: the task indices to decode, in order (nullptr: 0..n-1)
 void launch_ht_decode(const HtDecTask* d_tasks, const uint32_t* d_order, uint64_t n, hipStream_t stream);
 // One code-block per wavefront with the lanes working on its columns

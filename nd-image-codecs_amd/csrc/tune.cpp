@@ -25,6 +25,7 @@ const Knob kKnobs[kTuneCount] = {
     {"window_elems", 0, 0, (int64_t)1 << 40},
     {"ht_lane_waves", 0, 0, 4},
     {"graph_windows", 0, 0, 1},
+    {"ht_decode_sort", 0, 0, 1},
 };
 
 std::atomic<int64_t> g_vals[kTuneCount] = {};

@@ -38,6 +38,10 @@ enum TuneKey : int {
   // and launched as one (instantiated graphs are cached per context and
   // updated in place); 0: launched one by one
   kTuneGraphWindows,
+  // 1: the lane-per-block HT decoder takes its code-blocks heaviest first
+  // (a device counting sort by cleanup-segment length) so that a wave's
+  // blocks carry similar work; 0: in window order
+  kTuneHtDecodeSort,
   kTuneCount
 };
 

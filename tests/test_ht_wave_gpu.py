@@ -33,14 +33,17 @@ ENC_KNOB = "ht_encode_wave_max_blocks"
 def force(ndic):
     """Selects the decoder: "wave" (wavefront per block), "lane" (lane per
     block, workgroup shape chosen by the library), "lane1" / "lane4" (lane
-    per block with one / four waves per workgroup sharing the tables)."""
+    per block with one / four waves per workgroup sharing the tables),
+    "lane_sorted" (lane per block, blocks taken heaviest first)."""
     def set_(mode: str) -> None:
         ndic.tune_set(KNOB, 1 << 40 if mode == "wave" else 0)
         ndic.tune_set("ht_lane_waves", {"lane1": 1, "lane4": 4}.get(mode, 0))
+        ndic.tune_set("ht_decode_sort", 1 if mode == "lane_sorted" else 0)
 
     yield set_
     ndic.tune_reset(KNOB)
     ndic.tune_reset("ht_lane_waves")
+    ndic.tune_reset("ht_decode_sort")
 
 
 @pytest.fixture
@@ -162,7 +165,7 @@ def test_decoders_agree_on_corrupt_segments(ndic, oracle, force):
     assert checked > 200 and accepted > 20, (checked, accepted)
 
 
-@pytest.mark.parametrize("mode", ["wave", "lane1", "lane4"])
+@pytest.mark.parametrize("mode", ["wave", "lane1", "lane4", "lane_sorted"])
 def test_chunks_decode_with_either_decoder(ndic, oracle, force, mode):
     force(mode)
     rng = np.random.default_rng(35)
@@ -180,7 +183,7 @@ def test_chunks_decode_with_either_decoder(ndic, oracle, force, mode):
         assert ndic.htj2k_decode(chunk, shape, name) == raw, (mode, shape, dt, lv)
 
 
-@pytest.mark.parametrize("mode", ["wave", "lane", "lane1", "lane4"])
+@pytest.mark.parametrize("mode", ["wave", "lane", "lane1", "lane4", "lane_sorted"])
 def test_window_of_9217_blocks_with_either_decoder(ndic, force, mode):
     """One window of 9,217 code-blocks (an odd count) of blob content,
     encoded on the device and decoded back bit-exactly by each decoder."""
