@@ -255,6 +255,83 @@ def zfp_bench(ndic, dev, args):
     return out
 
 
+# kernels of the decode path, as their names appear in rocprofv3's CSVs
+PROFILED = {"parse_planes_kernel": "::parse_planes_kernel(", "ht_decode_kernel": "::ht_decode_kernel<",
+            "ht_refine_kernel": "::ht_refine_kernel(", "dwt_inv_level": "::dwt_inv_level(",
+            "unshift_kernel": "::unshift_kernel(", "dwt_lift_kernel": "::dwt_lift_kernel(",
+            "lift_col_kernel": "::lift_col_kernel<", "ht_decode_wave_kernel": "::ht_decode_wave_kernel("}
+
+
+def write_counters(root: Path, cus: int = 256) -> dict:
+    """profiles/c2_decode_counters.json from one rocprofv3 kernel trace
+    (root/prof/trace: this script with --steps 2 --warmup 1 --no-autotune ...)
+    and three PMC passes (root/pmc/{fetch,write,sq}: --steps 1 --warmup 0
+    --no-autotune --copy-calibration 1073741824 ...).  HBM bytes per window
+    are FETCH_SIZE / WRITE_SIZE scaled by what they read for the 1 GiB copy
+    kernel; per-window times come from the stage-profiled pass (windows one
+    after the other); GRBM_GUI_ACTIVE is summed over the 8 XCDs in the CSV."""
+    import csv
+    import glob
+    from collections import defaultdict
+
+    def rows(pattern):
+        with open(glob.glob(str(root / pattern))[0]) as f:
+            return list(csv.DictReader(f))
+
+    def decode_phase(rs):
+        first = min(int(r["Dispatch_Id"]) for r in rs if "walk_chunks_kernel" in r["Kernel_Name"])
+        return [r for r in rs if int(r["Dispatch_Id"]) >= first]
+
+    def per_kernel(rs):
+        vals, disp = defaultdict(lambda: defaultdict(float)), defaultdict(set)
+        for r in rs:
+            for k, pat in PROFILED.items():
+                if pat in r["Kernel_Name"]:
+                    vals[k][r["Counter_Name"]] += float(r["Counter_Value"])
+                    disp[k].add(r["Dispatch_Id"])
+        return vals, {k: len(v) for k, v in disp.items()}
+
+    def calib(rs, name):
+        return next(float(r["Counter_Value"]) for r in rs
+                    if "copy_kernel" in r["Kernel_Name"] and r["Counter_Name"] == name)
+
+    fetch, write, sq = (rows(f"pmc/{p}/*counter_collection.csv") for p in ("fetch", "write", "sq"))
+    fscale = (1 << 30) / 1024 / calib(fetch, "FETCH_SIZE")
+    wscale = (1 << 30) / 1024 / calib(write, "WRITE_SIZE")
+    F, nF = per_kernel(decode_phase(fetch))
+    W, _ = per_kernel(decode_phase(write))
+    S, _ = per_kernel(decode_phase(sq))
+    windows = nF.get("parse_planes_kernel", 16) // 2  # a timed and a stage-profiled pass
+    dec = decode_phase(rows("prof/trace/*kernel_trace.csv"))
+    walks = [i for i, r in enumerate(dec) if "walk_chunks_kernel" in r["Kernel_Name"]]
+    dur = defaultdict(float)
+    for r in dec[walks[-windows]:]:
+        for k, pat in PROFILED.items():
+            if pat in r["Kernel_Name"]:
+                dur[k] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+    out = {"workload": "c2: 4096 uneven chunks, seed 2", "windows_per_pass": windows, "fetch_scale": fscale,
+           "write_scale": wscale, "cu": cus, "kernels": {}}
+    for k in PROFILED:
+        if k not in F:
+            continue
+        fb = F[k]["FETCH_SIZE"] * 1024 * fscale / (2 * windows)
+        wb = W[k]["WRITE_SIZE"] * 1024 * wscale / (2 * windows)
+        rec = {"dispatches_per_pass": nF[k] // 2, "fetch_bytes": round(fb), "write_bytes": round(wb),
+               "traffic_bytes": round(fb + wb), "avg_launch_ms": round(dur[k] / windows, 3)}
+        s = S.get(k)
+        if s and s.get("GRBM_GUI_ACTIVE"):
+            g = s["GRBM_GUI_ACTIVE"] / 8
+            rec["occupancy_waves_per_simd"] = round(s["SQ_WAVE_CYCLES"] / g / cus, 2)
+            rec["occupancy_pct"] = round(400 * s["SQ_WAVE_CYCLES"] / g / cus / 32, 1)
+            rec["valu_busy_pct"] = round(100 * s["SQ_ACTIVE_INST_VALU"] / cus / g, 1)
+            rec["valu_lane_util_pct"] = round(100 * s["SQ_THREAD_CYCLES_VALU"] / max(1.0, s["SQ_ACTIVE_INST_VALU"] * 64), 1)
+            rec["valu_insts_per_wave"] = round(s["SQ_INSTS_VALU"] / max(1.0, s["SQ_WAVES"]))
+            rec["salu_insts_per_wave"] = round(s["SQ_INSTS_SALU"] / max(1.0, s["SQ_WAVES"]))
+        out["kernels"][k] = rec
+    COUNTERS.write_text(json.dumps(out, indent=1) + "\n")
+    return out
+
+
 def counters_of(kernel: str, args, avg_ms: float):
     """The committed rocprofv3 record of a decode kernel on this same workload
     (HBM bytes per launch from FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy
@@ -339,7 +416,13 @@ def main():
                     help="keep the library's built-in launch parameters instead of tuning them first")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first run one device copy kernel of this many bytes (traffic-counter scale)")
+    ap.add_argument("--counters-from", default=None,
+                    help="only write profiles/c2_decode_counters.json from the rocprofv3 CSVs under this "
+                         "directory (prof/trace and pmc/{fetch,write,sq}) and exit")
     args = ap.parse_args()
+    if args.counters_from:
+        print(json.dumps(write_counters(Path(args.counters_from)), indent=1))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (got --gpus {args.gpus})",
               file=sys.stderr)
