@@ -306,7 +306,8 @@ ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp
  * (64 x 32 tiles x lift segments; default 4096); 0: three passes (same
  * output); "pipeline_windows" --
  * 1 (default): the windows of a large decode batch alternate between two
- * streams so that their passes overlap; 0: one after the other;
+ * streams so that their passes overlap; 2, 3: three or four windows in
+ * flight on as many streams; 0: one after the other;
  * "window_elems" -- samples per batch window (0: NDIC_WINDOW_ELEMS, else 2^31);
  * "ht_lane_waves" -- waves per workgroup of the lane-per-block HT decoder
  * (0: four on windows with at least four waves per CU, else one; 1-3: one;

@@ -558,10 +558,10 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
 
 }  // namespace
 
-// Windows alternate between ctx (on s) and its twin context (own buffers and
-// stream), so that one window's parse and entropy decode overlap the other's
-// synthesis and lift; a window's results are collected once the next one is
-// queued.  Under the stage profiler, or with pipeline_windows = 0, windows run
+// Windows take turns over ctx (on s) and its twin contexts (own buffers and
+// streams), so that one window's parse and entropy decode overlap another's
+// synthesis and lift; a window's results are collected once the windows after
+// it are queued.  Under the stage profiler, or with pipeline_windows = 0, windows run
 // one after the other.
 void decode_items(Context& ctx, std::vector<DecodeItem>& items, hipStream_t s, uint64_t window_elems) {
   ht_init_tables();
@@ -582,38 +582,42 @@ void decode_items(Context& ctx, std::vector<DecodeItem>& items, hipStream_t s, u
     for (auto& w : wins) decode_window_begin(ctx, w, s)();
     return;
   }
-  Context& twin = ctx.twin();
-  twin.open_stream();
+  // pipeline_windows = n keeps n + 1 windows in flight (at most four), each
+  // on its own context: ctx, its twin, the twin's twin ...
+  const size_t depth = 1 + std::min<size_t>(tune(kTunePipelineWindows), 3);
+  std::vector<Context*> cs{&ctx};
+  while (cs.size() < depth) cs.push_back(&cs.back()->twin());
   hipEvent_t ready;
   NDIC_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-  std::function<void()> pending[2];
+  std::vector<std::function<void()>> pending(depth);
+  auto close_all = [&] {
+    for (size_t j = 1; j < depth; ++j) cs[j]->close_stream();
+  };
   try {
-    NDIC_HIP(hipEventRecord(ready, s));  // the twin starts after what s holds
-    NDIC_HIP(hipStreamWaitEvent(twin.stream, ready, 0));
-    for (size_t k = 0; k < wins.size(); ++k) {
-      auto& slot = pending[k & 1];
+    NDIC_HIP(hipEventRecord(ready, s));  // the other contexts start after what s holds
+    for (size_t j = 1; j < depth; ++j) {
+      cs[j]->open_stream();
+      NDIC_HIP(hipStreamWaitEvent(cs[j]->stream, ready, 0));
+    }
+    for (size_t k = 0; k < wins.size() + depth; ++k) {
+      auto& slot = pending[k % depth];
       if (slot) {
         auto f = std::move(slot);
         slot = nullptr;
         f();
       }
-      slot = (k & 1) ? decode_window_begin(twin, wins[k], twin.stream) : decode_window_begin(ctx, wins[k], s);
-    }
-    for (size_t k = wins.size(); k < wins.size() + 2; ++k) {
-      auto& slot = pending[k & 1];
-      if (slot) {
-        auto f = std::move(slot);
-        slot = nullptr;
-        f();
+      if (k < wins.size()) {
+        Context& c = *cs[k % depth];
+        slot = decode_window_begin(c, wins[k], k % depth ? c.stream : s);
       }
     }
   } catch (...) {
     (void)hipStreamSynchronize(s);
-    twin.close_stream();
+    close_all();
     (void)hipEventDestroy(ready);
     throw;
   }
-  twin.close_stream();
+  close_all();
   NDIC_HIP(hipEventDestroy(ready));
 }
 

@@ -21,7 +21,7 @@ const Knob kKnobs[kTuneCount] = {
     {"parse_wave_max_planes", 4096, 0, INT64_MAX},
     {"ht_wave_grain", 1, 1, 64},
     {"fuse_dwt_lift", 4096, 0, INT64_MAX},
-    {"pipeline_windows", 1, 0, 1},
+    {"pipeline_windows", 1, 0, 3},
     {"window_elems", 0, 0, (int64_t)1 << 40},
     {"ht_lane_waves", 0, 0, 4},
     {"graph_windows", 0, 0, 1},

@@ -133,8 +133,8 @@ def test_fused_range_error_matches_separate_passes(ndic, oracle):
 
 @pytest.mark.parametrize("window", [60_000, 250_000])
 def test_pipelined_windows_match(ndic, oracle, window):
-    """Small windows so that the batch spans many: alternating streams
-    (pipeline_windows = 1) and one window after the other give the same
+    """Small windows so that the batch spans many: two, three or four windows
+    in flight (pipeline_windows = 1, 2, 3) and one window after the other give the same
     bit-exact samples and the same per-chunk statuses, a failing chunk in the
     middle included."""
     rng = np.random.default_rng(9)
@@ -148,7 +148,7 @@ def test_pipelined_windows_match(ndic, oracle, window):
     ndic.tune_set("window_elems", window)
     try:
         res = {}
-        for pipe in (1, 0):
+        for pipe in (1, 2, 3, 0):
             ndic.tune_set("pipeline_windows", pipe)
             res[pipe] = _decode(ndic, dev, d_in, d_out, descs, 1)
     finally:
@@ -162,4 +162,4 @@ def test_pipelined_windows_match(ndic, oracle, window):
             if i != bad_at:
                 assert out[o:o + v.nbytes] == v.tobytes(), (pipe, i, v.shape, dt, steps, lv)
             o += al(v.nbytes)
-    assert res[1] == res[0]
+    assert res[1] == res[0] and res[2] == res[0] and res[3] == res[0]
