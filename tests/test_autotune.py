@@ -27,6 +27,42 @@ def test_crossover_rule(autotune):
     assert autotune.crossover(sizes, [1, 2, 3, 5], [1, 2, 3, 4]) == 400
 
 
+
+def test_knob_registry(ndic):
+    """The knobs are host state: defaults, the accepted ranges, set / reset
+    round trips and the errors for unknown keys and values out of range."""
+    defaults = {"ht_decode_wave_max_blocks": 8192, "parse_wave_max_planes": 4096, "ht_wave_grain": 1,
+                "fuse_dwt_lift": 4096, "pipeline_windows": 1, "window_elems": 0, "ht_lane_waves": 0,
+                "graph_windows": 0, "ht_decode_sort": 0}
+    keys = ndic.tune_keys()
+    assert set(defaults) <= set(keys)
+    before = {k: ndic.tune_get(k) for k in keys}
+    try:
+        for k in keys:
+            ndic.tune_reset(k)
+        for k, v in defaults.items():
+            assert ndic.tune_get(k) == v, k
+        for k, lo, hi in [("pipeline_windows", 0, 3), ("ht_wave_grain", 1, 64), ("ht_lane_waves", 0, 4),
+                          ("graph_windows", 0, 1), ("ht_decode_sort", 0, 1)]:
+            for v in (lo, hi):
+                ndic.tune_set(k, v)
+                assert ndic.tune_get(k) == v
+            for v in (lo - 1, hi + 1):
+                with pytest.raises(ValueError):
+                    ndic.tune_set(k, v)
+            assert ndic.tune_get(k) == hi  # a refused value leaves the knob as it was
+            ndic.tune_reset(k)
+            assert ndic.tune_get(k) == defaults[k]
+        with pytest.raises(KeyError):
+            ndic.tune_set("no_such_knob", 1)
+        with pytest.raises(KeyError):
+            ndic.tune_get("no_such_knob")
+    finally:
+        for k, v in before.items():
+            ndic.tune_reset(k)
+            if ndic.tune_get(k) != v:
+                ndic.tune_set(k, v)
+
 @pytest.mark.gpu
 def test_short_tuning_run(autotune, ndic):
     keys = autotune.KNOBS
@@ -46,4 +82,6 @@ def test_short_tuning_run(autotune, ndic):
         assert np.frombuffer(w.out.download(0, 64), np.uint16).any() or True
     finally:
         for k, v in before.items():
-            ndic.tune_set(k, v)
+            ndic.tune_reset(k)
+            if ndic.tune_get(k) != v:
+                ndic.tune_set(k, v)
