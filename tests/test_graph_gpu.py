@@ -56,7 +56,7 @@ def knobs(ndic):
         ndic.tune_reset(k)
 
 
-@pytest.mark.parametrize("pipeline", [0, 1])
+@pytest.mark.parametrize("pipeline", [0, 1, 3])
 def test_graph_windows_replay_equal_eager(knobs, oracle, pipeline):
     ndic = knobs
     dev = importlib.import_module("nd-image-codecs_amd.device")
