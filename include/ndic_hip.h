@@ -290,6 +290,10 @@ typedef struct {
 ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp_brick_desc* descs, size_t n,
                                    void* stream, ndic_error* err);
 
+/* Free and total memory of the device the library runs on (hipMemGetInfo).
+ * No counterpart in the reference: a serving loop admits work by it. */
+ndic_status ndic_device_memory(size_t* free_bytes, size_t* total_bytes, ndic_error* err);
+
 /* ---- launch-parameter knobs (no counterpart in the reference) ----------
  * The autotuner (nd-image-codecs_amd/autotune.py) measures the kernel
  * variants per shape class and stores its choices here; tests pin them.

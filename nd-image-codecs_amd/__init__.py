@@ -128,6 +128,7 @@ def _load() -> C.CDLL:
         "ndic_zfp_decode_batch": (C.c_int, [P, P, P, S, P, E]),
         "ndic_range_plan_chunk": (C.c_int, [C.c_char_p, S, C.c_int32, C.c_uint64, P, S, C.POINTER(RangePlanC), E]),
         "ndic_htj2k_decode_select_batch": (C.c_int, [P, P, P, S, P, E]),
+        "ndic_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), E]),
         "ndic_tune_set": (C.c_int32, [C.c_char_p, C.c_int64]),
         "ndic_tune_get": (C.c_int64, [C.c_char_p]),
         "ndic_tune_reset": (C.c_int32, [C.c_char_p]),
@@ -508,6 +509,13 @@ def zfp_decode_batch(d_in_ptr: int, d_out_ptr: int, descs, stream: int = 0):
                                     C.c_void_p(stream), C.byref(err))
     _check(rc, err)
     return arr
+
+
+def device_memory() -> tuple:
+    """(free, total) bytes of device memory, measured now."""
+    f, t, err = C.c_size_t(), C.c_size_t(), Error()
+    _check(_lib.ndic_device_memory(C.byref(f), C.byref(t), C.byref(err)), err)
+    return int(f.value), int(t.value)
 
 
 def version() -> str:

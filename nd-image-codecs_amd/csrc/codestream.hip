@@ -595,7 +595,7 @@ __device__ void parse_plane(const DevChunk& ch, uint64_t p, uint64_t gid, DevPla
           t.mmsb = (uint8_t)min(mmsbs, 255u);
           t.passes = (uint8_t)min(np, 255u);  // more than 3 is refused by the block decoder
           t.kmax = (uint8_t)kmax;
-          t.flags = kTaskIncluded | ((h.cb_style & 0x08) ? kTaskCausal : 0);
+          t.flags = kTaskHeader | ((h.cb_style & 0x08) ? kTaskCausal : 0);
           nincl += 1;
         }
     }
@@ -613,11 +613,12 @@ __device__ void parse_plane(const DevChunk& ch, uint64_t p, uint64_t gid, DevPla
       const GeoBand& bd = gb[ch.geo_band + pk.band_first + b];
       for (uint32_t i = 0; i < bd.nx * bd.ny; ++i) {
         HtDecTask& t = pt[bd.block_first + i];
-        if (!(t.flags & kTaskIncluded)) continue;
+        if (!(t.flags & kTaskHeader)) continue;
         uint64_t sl = (uint64_t)t.len1 + t.len2;
         if (q + sl > blen) return raise(ch, kErrPacket, pl.cs_off + bs + q, gid);
         t.seg = cs.g + bs + q;
         t.err_offset = (uint32_t)(pl.cs_off + bs + q);
+        t.flags = (uint8_t)((t.flags & ~kTaskHeader) | kTaskIncluded);
         q += sl;
       }
     }

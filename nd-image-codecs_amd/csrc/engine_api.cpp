@@ -418,6 +418,17 @@ ndic_status ndic_dev_ht_decode_block(const uint8_t* coded, size_t len, uint32_t*
 
 }  // extern "C"
 
+// Free and total device memory of the GPU the calling thread's context uses
+// (hipMemGetInfo): what a serving loop admits work against.
+extern "C" ndic_status ndic_device_memory(size_t* free_bytes, size_t* total_bytes, ndic_error* err) {
+  return ndic::guard_call(err, [&] {
+    using namespace ndic;
+    if (!free_bytes || !total_bytes) fail(NDIC_INVALID_ARGUMENT, "null output pointer");
+    Context::get();
+    NDIC_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  });
+}
+
 extern "C" ndic_status ndic_htj2k_decode_plane_batch(const uint8_t* d_in, uint8_t* d_out, ndic_plane_desc* descs,
                                                      size_t n, void* stream, ndic_error* err) {
   return ndic::guard_call(err, [&] {

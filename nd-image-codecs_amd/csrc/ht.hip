@@ -451,7 +451,7 @@ struct BlockDec {
     active = false;
     bad = false;
     refine = false;
-    if (!(t.flags & kTaskIncluded) || t.w == 0 || t.h == 0) return;
+    if (!(t.flags & kTaskIncluded) || !t.seg || t.w == 0 || t.h == 0) return;
     uint32_t num_passes = t.passes, missing_msbs = t.mmsb;
     if (num_passes > 1 && t.len2 == 0) num_passes = 1;
     if (num_passes > 3 || missing_msbs >= 30 || t.len1 < 2 || t.w > 64 || t.h > 64) {
@@ -914,7 +914,7 @@ __device__ void ht_decode_wave_block(const HtDecTask* __restrict__ tasks, uint64
   // decode.rs:54-110, as BlockDec::start
   bool active = false, refine = false;
   uint32_t p = 0, shift = 0, mmsbp2 = 0, lcup = 0, scup = 0;
-  if ((t.flags & kTaskIncluded) && width && height) {
+  if ((t.flags & kTaskIncluded) && t.seg && width && height) {
     uint32_t num_passes = t.passes;
     const uint32_t missing_msbs = t.mmsb;
     if (num_passes > 1 && t.len2 == 0) num_passes = 1;
@@ -1154,7 +1154,7 @@ __global__ __launch_bounds__(64) void ht_decode_wave_kernel(const HtDecTask* __r
 __constant__ uint32_t c_spp_prop[4] = {0x33, 0x76, 0xEC, 0xC8};
 
 __device__ __forceinline__ void refine_block(const HtDecTask& t, uint64_t* sigrow, uint16_t* prevrow, uint32_t lane) {
-  if (!(t.flags & kTaskIncluded) || t.passes < 2 || t.len2 == 0 || t.mmsb >= 29) return;
+  if (!(t.flags & kTaskIncluded) || !t.seg || t.passes < 2 || t.len2 == 0 || t.mmsb >= 29) return;
   const uint32_t num_passes = t.passes;
   const uint32_t width = t.w, height = t.h;
   if (width == 0 || height == 0 || width > 64 || height > 64) return;
@@ -2173,7 +2173,7 @@ void ht_init_tables() {
 namespace {
 // Work class of a task: 0 heaviest ... kHtClasses - 1 (left out).
 __device__ __forceinline__ uint32_t ht_class(const HtDecTask& t) {
-  if (!(t.flags & kTaskIncluded) || t.len1 == 0) return kHtClasses - 1;
+  if (!(t.flags & kTaskIncluded) || !t.seg || t.len1 == 0) return kHtClasses - 1;
   const uint32_t bits = 32 - __clz(t.len1);                        // 1..16 (len1 < 65535)
   const uint32_t half = (t.len1 >> (bits > 1 ? bits - 2 : 0)) & 1;  // the bit below the top one
   return 33 - (2 * bits + half);                                     // 0 (longest) .. 31

@@ -14,7 +14,7 @@ struct HtDecTask {
   uint32_t len1, len2;  // cleanup / refinement lengths
   uint16_t w, h;
   uint8_t mmsb, passes, kmax;
-  uint8_t flags;        // kTaskIncluded | kTaskSignMag
+  uint8_t flags;        // kTaskIncluded (seg is set) | kTaskSignMag | kTaskCausal | kTaskHeader
   uint32_t err_offset;  // byte offset reported with an error
 };
 
@@ -32,8 +32,11 @@ struct HtEncTask {
 };
 
 // kTaskCausal: the COD style's vertically causal bit (SigProp ignores the
-// stripe below)
-enum : uint8_t { kTaskIncluded = 1, kTaskSignMag = 2, kTaskCausal = 4 };
+// stripe below).  kTaskHeader: the packet header included the block; the
+// parser turns it into kTaskIncluded only when it also sets the segment
+// pointer from the packet body, so a packet that fails part-way leaves no
+// included task without a segment.
+enum : uint8_t { kTaskIncluded = 1, kTaskSignMag = 2, kTaskCausal = 4, kTaskHeader = 8 };
 constexpr uint32_t kMelCap = 4096, kVlcCap = 4096;
 
 // Upper bound of the MagSgn bytes of a w x h block at K_max.

@@ -1,5 +1,7 @@
 """The serving loop: every response equals the same request served alone and
-the oracle's answer; Poisson arrivals are all served with latency reported."""
+the oracle's answer (encodes: the oracle's chunk bytes); Poisson arrivals are
+all served with latency reported; a corrupt chunk and an oversize request
+fail alone while the rest of their passes are served."""
 
 from __future__ import annotations
 
@@ -44,8 +46,19 @@ def world(ndic, oracle):
     arena = dev.DeviceBuffer(cap, zero=True)
     arr = (ndic.ChunkDesc * len(descs))(*descs)
     ndic.encode_batch(d_raw.ptr, arena.ptr, cap, arr)
+    # a corrupt copy of chunk 0 after the others: plane 0's SOC marker broken
+    end = max(d.out_offset + d.out_len for d in arr)
+    bad_off = (end + 255) // 256 * 256
+    good0 = arena.download(arr[0].out_offset, arr[0].out_len)
+    p0 = serve.parse_plane0(good0)[0]
+    bad = bytearray(good0)
+    bad[p0:p0 + 2] = b"\x00\x00"
+    arena.upload(bytes(bad), bad_off)
+    bad_desc = ndic.make_desc(specs[0][0], specs[0][1], specs[0][2], specs[0][3], 0, 0, 0)
+    bad_desc.out_offset, bad_desc.out_len = bad_off, len(bad)
     store = serve.Store()
-    store.add_htj2k_arena(arena, arr, specs, d_raw, raw_offs)
+    store.add_htj2k_arena(arena, list(arr) + [bad_desc], specs + [specs[0]], d_raw, raw_offs + [raw_offs[0]])
+    store.bad_key = len(arr)
     # ZFP fields at fixed rate, streams at 8-byte aligned offsets
     fields, blob = [], b""
     for k, (shape, dt, rate) in enumerate([((9, 10, 11), "float32", 8.0), ((3, 5, 6, 7), "float64", 16.0),
@@ -57,18 +70,34 @@ def world(ndic, oracle):
         blob += s
     store.add_zfp_arena(dev.DeviceBuffer.from_bytes(blob), fields)
     zfp_blob = blob
+    # encode sources: small chunks of the 5D series recipe (t, c, z, y, x after
+    # the transpose to c, t, z, y, x), lift53 along t and haar along z
+    store.enc_vols = []
+    enc_src = []
+    for k, shape in enumerate([(1, 4, 6, 40, 56), (1, 2, 3, 64, 64)]):
+        v = datagen.blobs_volume((shape[1] * shape[2], shape[3], shape[4]), n_blobs=4, base=3000.0,
+                                 amp=(2000.0, 30000.0), sigma=(3.0, 12.0), lo=0, hi=65535, seed=70 + k)
+        v = v.reshape(shape)
+        steps = [{"dimension": 1, "kind": "lift53", "levels": 2, "group": 0},
+                 {"dimension": 2, "kind": "haar", "levels": 2, "group": 0}]
+        enc_src.append((dev.DeviceBuffer.from_bytes(v.tobytes()), shape, "uint16", steps, 4))
+        store.enc_vols.append(v)
+    store.enc_src = enc_src
     return serve, dev, store, vols, zfp_blob
 
 
-def make_requests(serve, store, n, seed):
+def make_requests(serve, store, n, seed, encodes=False):
     rng = np.random.default_rng(seed)
     reqs = []
+    good = len(store.ht) - 1  # the last chunk is the corrupt copy
     for i in range(n):
-        k = i % 3
-        if k == 0:
-            reqs.append(serve.Request(serve.THUMB, 0.0, key=int(rng.integers(0, len(store.ht))), res=3))
+        k = i % (4 if encodes else 3)
+        if k == 3:
+            reqs.append(serve.Request(serve.ENCODE, 0.0, key=int(rng.integers(0, len(store.enc_src)))))
+        elif k == 0:
+            reqs.append(serve.Request(serve.THUMB, 0.0, key=int(rng.integers(0, good)), res=3))
         elif k == 1:
-            reqs.append(serve.Request(serve.DECODE, 0.0, key=int(rng.integers(0, len(store.ht)))))
+            reqs.append(serve.Request(serve.DECODE, 0.0, key=int(rng.integers(0, good))))
         else:
             key = int(rng.integers(0, len(store.zfp)))
             f = store.zfp[key]
@@ -77,10 +106,35 @@ def make_requests(serve, store, n, seed):
     return reqs
 
 
+def check_response(serve, ndic, oracle, store, vols, zfp_blob, r, got):
+    if r.kind == serve.ENCODE:
+        v = store.enc_vols[r.key % len(store.enc_src)]
+        _, shape, dt, steps, lv = store.enc_src[r.key % len(store.enc_src)]
+        assert got == oracle.lift_ht_encode(v, "u16", steps, xy_levels=lv), r.key
+        return
+    c = store.ht[r.key] if r.kind != serve.BRICK else None
+    if r.kind == serve.DECODE:
+        assert got == vols[r.key].tobytes()
+    elif r.kind == serve.THUMB:
+        chunk = store.ht_arena.download(c.offset, c.length)
+        off, ln, pre = c.plane0
+        res = min(r.res, c.levels)
+        exp = oracle.decode_codestream(chunk[off:off + pre[res]], res, prefix=True)
+        assert got == exp.tobytes(), (r.key, res)
+    else:
+        f = store.zfp[r.key]
+        s = zfp_blob[f.offset:f.offset + f.length]
+        cfg = {"mode": "fixed_rate", "rate": f.rate}
+        odt = {"float32": "f32", "float64": "f64", "int32": "i32"}[f.dtype]
+        full = np.frombuffer(oracle.zfp_decode_chunk(s, f.shape, odt, cfg), f.dtype).reshape(f.shape)
+        sl = tuple(slice(4 * b, 4 * b + 4) for b in r.brick)
+        assert got == np.ascontiguousarray(full[sl]).tobytes()
+
+
 def test_mix_equals_alone_and_oracle(world, ndic, oracle):
     serve, dev, store, vols, zfp_blob = world
-    server = serve.Server(store, out_budget=64 << 20)
-    reqs = make_requests(serve, store, 30, 1)
+    server = serve.Server(store, out_budget=64 << 20, encode_sources=store.enc_src)
+    reqs = make_requests(serve, store, 40, 1, encodes=True)
     results = {}
 
     def keep(srv, batch):
@@ -88,45 +142,63 @@ def test_mix_equals_alone_and_oracle(world, ndic, oracle):
             results[id(r)] = srv.result(r)
 
     stats = server.run(reqs, verify=keep)
+    assert not stats.failed
     assert stats.passes >= 1 and sum(len(v) for v in stats.latencies_ms.values()) == len(reqs)
+    assert any(r.kind == serve.ENCODE for r in reqs)
     mixed = [results[id(r)] for r in reqs]
     for r, got in zip(reqs, mixed):
         alone = serve.Request(r.kind, 0.0, key=r.key, brick=r.brick, res=r.res)
-        server.run([alone])
-        assert server.result(alone) == got, (r.kind, r.key)
-        c = store.ht[r.key] if r.kind != serve.BRICK else None
-        if r.kind == serve.DECODE:
-            assert got == vols[r.key].tobytes()
-        elif r.kind == serve.THUMB:
-            chunk = store.ht_arena.download(c.offset, c.length)
-            off, ln, pre = c.plane0
-            res = min(r.res, c.levels)
-            exp = oracle.decode_codestream(chunk[off:off + pre[res]], res, prefix=True)
-            assert got == exp.tobytes(), (r.key, res)
-        else:
-            f = store.zfp[r.key]
-            s = zfp_blob[f.offset:f.offset + f.length]
-            full = np.frombuffer(ndic.nd_zfp_decode(s, f.shape, f.dtype, mode="fixed_rate", rate=f.rate),
-                                 f.dtype).reshape(f.shape)
-            sl = tuple(slice(4 * b, 4 * b + 4) for b in r.brick)
-            assert got == np.ascontiguousarray(full[sl]).tobytes()
+        server.run([alone], verify=keep)
+        assert results[id(alone)] == got, (r.kind, r.key)
+        check_response(serve, ndic, oracle, store, vols, zfp_blob, r, got)
+
+
+def test_failures_stay_with_their_request(world, ndic, oracle):
+    """One corrupt chunk (decoded in full and as a thumbnail) and one request
+    whose output exceeds its lane's memory fail alone; every other request
+    of the mix, in the same passes, is served and correct."""
+    serve, dev, store, vols, zfp_blob = world
+    sizes = [int(np.prod(c.shape)) * np.dtype(c.dtype).itemsize for c in store.ht[:store.bad_key]]
+    big = int(np.argmax(sizes))
+    assert sorted(sizes)[-2] + 256 < sizes[big]
+    # the decode lane's memory holds every chunk but the largest
+    server = serve.Server(store, out_budget=sizes[big] - 256, encode_sources=store.enc_src)
+    reqs = make_requests(serve, store, 24, 3, encodes=True)
+    reqs = [r for r in reqs if not (r.kind == serve.DECODE and r.key == big)]
+    bad = [serve.Request(serve.DECODE, 0.0, key=store.bad_key), serve.Request(serve.THUMB, 0.0, key=store.bad_key),
+           serve.Request(serve.DECODE, 0.0, key=big)]
+    mix = reqs[:5] + bad[:1] + reqs[5:12] + bad[1:] + reqs[12:]
+    results = {}
+    stats = server.run(mix, verify=lambda srv, batch: results.update((id(r), srv.result(r)) for r in batch))
+    assert {id(r) for r in stats.failed} == {id(r) for r in bad}
+    assert bad[0].status == 2 and bad[1].status == 2 and bad[2].status == 1
+    for r in bad:
+        assert r.error and id(r) not in results
+        with pytest.raises(ndic.NdicError):
+            server.result(r)
+    assert sum(len(v) for v in stats.latencies_ms.values()) == len(reqs)
+    assert stats.summary()["failed"] == {"decode": 2, "thumbnail": 1}
+    for r in reqs:
+        assert r.status == 0, (r.kind, r.error)
+        check_response(serve, ndic, oracle, store, vols, zfp_blob, r, results[id(r)])
 
 
 def test_poisson_arrivals_all_served(world):
     serve, dev, store, vols, _ = world
     rng = np.random.default_rng(2)
-    reqs = serve.poisson_requests(rng, rate=400.0, duration=0.5, store=store, n_encode_sources=0)
-    server = serve.Server(store, out_budget=32 << 20)
+    reqs = serve.poisson_requests(rng, rate=400.0, duration=0.5, store=store, n_encode_sources=2)
+    reqs = [r for r in reqs if r.key != store.bad_key or r.kind == serve.BRICK]
+    server = serve.Server(store, out_budget=32 << 20, encode_sources=store.enc_src)
     stats = server.run(reqs)
     summ = stats.summary()
-    assert summ["requests"] == len(reqs)
+    assert summ["requests"] == len(reqs) and summ["failed"] == {}
     assert summ["p50_ms"] is not None and summ["p99_ms"] >= summ["p50_ms"]
     assert all(r.done >= r.arrival for r in reqs)
 
 
 def test_admission_splits_passes_by_memory(world):
     serve, dev, store, vols, _ = world
-    big = max(range(len(store.ht)), key=lambda k: int(np.prod(store.ht[k].shape)) * 2)
+    big = max(range(store.bad_key), key=lambda k: int(np.prod(store.ht[k].shape)) * 2)
     server = serve.Server(store, out_budget=int(np.prod(store.ht[big].shape)) * 2 + 4096)
     reqs = [serve.Request(serve.DECODE, 0.0, key=big) for _ in range(3)]
     outs = []
