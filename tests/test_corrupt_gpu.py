@@ -150,8 +150,14 @@ def test_chunk_mutations_match_oracle(ndic, oracle, form):
     chunk = oracle.htj2k_encode_chunk(vals.tobytes(), vals.shape, "u16", xy_levels=2)
     ndim, lv = 3, 2
     head = 8 + 4 * ndim + 4 * (12 + 4 * (lv + 1))
-    cands = [m for _, _, m in mutations(chunk, list(range(head)) + list(range(head, len(chunk), 7)))]
-    cands += [chunk[:c] for c in range(len(chunk))]
+    # codestream header bytes: from each plane's offset to its first packet
+    planes = [struct.unpack_from("<QI", chunk, 8 + 4 * ndim + (12 + 4 * (lv + 1)) * p) for p in range(4)]
+    in_header = set()
+    for off, ln in planes:
+        in_header.update(range(off, off + cs_tools.layout(chunk[off:off + ln]).body_start))
+    muts = list(mutations(chunk, list(range(head)) + list(range(head, len(chunk), 7))))
+    cands = [m for _, _, m in muts] + [chunk[:c] for c in range(len(chunk))]
+    where = [pos for pos, _, _ in muts] + [-1] * len(chunk)
     got = batch_tools.chunk_batch(ndic, cands, vals.shape, "uint16", [], lv)
     bad = []
     for i, ((st, out), data) in enumerate(zip(got, cands)):
@@ -159,6 +165,8 @@ def test_chunk_mutations_match_oracle(ndic, oracle, form):
             exp = oracle.htj2k_decode_chunk(data, vals.shape, "u16")
         except oracle.OracleError:
             exp = None
+        if exp is not None and st == batch_tools.NDIC_UNSUPPORTED and where[i] in in_header:
+            continue  # outside the device path's codestream scope (e.g. other code-block sizes)
         if (exp is None) != (st != 0) or (exp is not None and out != exp):
             bad.append((i, st, exp is None))
     assert not bad, f"{len(bad)} of {len(cands)} chunk outcomes differ from the oracle: {bad[:12]}"
