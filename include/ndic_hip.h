@@ -290,6 +290,19 @@ typedef struct {
 ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp_brick_desc* descs, size_t n,
                                    void* stream, ndic_error* err);
 
+/* The allocation-free form of ndic_zfp_decode_bricks, with the same
+ * arguments, results and errors.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc, hipFree or other allocating routine: its device tables live in
+ * a fixed library-owned array on each GPU (under 1 MiB, held until the
+ * process ends), so a call takes at most NDIC_ZFP_NOALLOC_MAX_BRICKS bricks
+ * (more is an InvalidArgument error).  Calls on one GPU share that array and
+ * the library serialises them; each returns after its kernels complete. */
+#define NDIC_ZFP_NOALLOC_MAX_BRICKS 4096
+ndic_status ndic_zfp_decode_bricks_noalloc(const uint8_t* d_in, uint8_t* d_out, ndic_zfp_brick_desc* descs,
+                                           size_t n, void* stream, ndic_error* err);
+
 /* Free and total memory of the device the library runs on (hipMemGetInfo).
  * No counterpart in the reference: a serving loop admits work by it. */
 ndic_status ndic_device_memory(size_t* free_bytes, size_t* total_bytes, ndic_error* err);

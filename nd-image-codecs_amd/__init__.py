@@ -129,6 +129,8 @@ def _load() -> C.CDLL:
         "ndic_range_plan_chunk": (C.c_int, [C.c_char_p, S, C.c_int32, C.c_uint64, P, S, C.POINTER(RangePlanC), E]),
         "ndic_htj2k_decode_select_batch": (C.c_int, [P, P, P, S, P, E]),
         "ndic_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), E]),
+        "ndic_zfp_decode_bricks": (C.c_int, [P, P, P, S, P, E]),
+        "ndic_zfp_decode_bricks_noalloc": (C.c_int, [P, P, P, S, P, E]),
         "ndic_tune_set": (C.c_int32, [C.c_char_p, C.c_int64]),
         "ndic_tune_get": (C.c_int64, [C.c_char_p]),
         "ndic_tune_reset": (C.c_int32, [C.c_char_p]),

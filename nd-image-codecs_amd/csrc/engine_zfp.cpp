@@ -809,10 +809,84 @@ extern "C" ndic_status ndic_zfp_decode_batch(const uint8_t* d_in, uint8_t* d_out
   return batch_status(descs, items, err);
 }
 
+namespace ndic {
+
+void plan_bricks(const uint8_t* d_in, uint8_t* d_out, const ndic_zfp_brick_desc* descs, size_t n,
+                 const uint8_t* heads, BrickPlan& bp) {
+  bp.status.assign(n, NDIC_OK);
+  bp.msg.assign(n, std::string());
+  bp.classes.clear();
+  std::vector<Plan> plans(n);
+  std::vector<uint64_t> block(n, 0);
+  for (size_t i = 0; i < n; ++i) {
+    const ndic_zfp_brick_desc& d = descs[i];
+    try {
+      if (d.ndim < 1 || d.ndim > 4) fail(NDIC_INVALID_ARGUMENT, "nd_zfp: brick descriptor ndim must be 1..=4");
+      if (d.in_offset % 8) fail(NDIC_INVALID_ARGUMENT, "zfp bricks: stream offset is not 8-byte aligned");
+      ndic_zfp_config cfg{};
+      cfg.mode = NDIC_ZFP_FIXED_RATE;
+      cfg.rate = d.rate;
+      std::vector<size_t> shape(d.shape, d.shape + d.ndim);
+      Plan pl = make_plan(shape, d.dtype, cfg);
+      if (d.in_len != pl.fixed_bytes)
+        fail(NDIC_CODESTREAM, "nd_zfp: fixed-rate stream is " + std::to_string(d.in_len) +
+                                  " bytes, computed size is " + std::to_string(pl.fixed_bytes));
+      check_stream(pl, &heads[i * kZfpHeadBytes], kZfpHeadBytes, d.in_len);
+      uint64_t k = 0;
+      for (uint32_t a = 0; a < pl.dims; ++a) {
+        uint32_t g = pl.nb[pl.dims - 1 - a];
+        if (d.brick[a] >= g) fail(NDIC_INVALID_ARGUMENT, "nd_zfp: brick coordinates exceed the grid");
+        k = k * g + d.brick[a];
+      }
+      if (d.out_offset % dtype_size(d.dtype)) fail(NDIC_INVALID_ARGUMENT, "zfp bricks: output offset is not aligned");
+      plans[i] = pl;
+      block[i] = k;
+    } catch (const Failure& f) {
+      bp.status[i] = f.status;
+      bp.msg[i] = f.message;
+    }
+  }
+  for (uint32_t cls = 8; cls < 40; ++cls) {
+    BrickClass bc;
+    bc.dims = cls / 8;
+    bc.scalar = cls % 8;
+    for (size_t i = 0; i < n; ++i) {
+      if (bp.status[i] != NDIC_OK || class_of(plans[i]) != cls) continue;
+      ZfpChunk c = chunk_of(plans[i]);
+      c.src = d_in + descs[i].in_offset;
+      c.src_len = descs[i].in_len;
+      ZfpBrick b{};
+      b.chunk = (uint32_t)bc.chunks.size();
+      b.block = block[i];
+      b.out = d_out + descs[i].out_offset;
+      bc.chunks.push_back(c);
+      bc.bricks.push_back(b);
+    }
+    if (!bc.chunks.empty()) bp.classes.push_back(std::move(bc));
+  }
+}
+
+ndic_status report_bricks(ndic_zfp_brick_desc* descs, size_t n, const BrickPlan& bp, ndic_error* err) {
+  ndic_status first = NDIC_OK;
+  for (size_t i = 0; i < n; ++i) {
+    descs[i].status = bp.status[i];
+    if (bp.status[i] != NDIC_OK && first == NDIC_OK) {
+      first = (ndic_status)bp.status[i];
+      if (err) {
+        err->status = bp.status[i];
+        err->offset = 0;
+        std::snprintf(err->message, sizeof(err->message), "brick %zu: %s", i, bp.msg[i].c_str());
+      }
+    }
+  }
+  return first;
+}
+
+}  // namespace ndic
+
 extern "C" ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_out, ndic_zfp_brick_desc* descs,
                                               size_t n, void* stream, ndic_error* err) {
-  std::vector<int> status(n, NDIC_OK);
-  std::vector<std::string> msg(n);
+  BrickPlan bp;
   ndic_status st = guard_call(err, [&] {
     if (n && !descs) fail(NDIC_INVALID_ARGUMENT, "null descriptors");
     Context& ctx = Context::get();
@@ -820,8 +894,6 @@ extern "C" ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_ou
     zfp_init_tables();
     ctx.release_kept();
     hipStream_t s = stream ? (hipStream_t)stream : ctx.stream;
-    std::vector<Plan> plans(n);
-    std::vector<uint64_t> block(n, 0);
     // header bytes of every stream for the checks of BrickReader::fixed_rate
     std::vector<ZfpHead> hv(n);
     for (size_t i = 0; i < n; ++i) hv[i] = {d_in + descs[i].in_offset, descs[i].in_len};
@@ -835,70 +907,17 @@ extern "C" ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_ou
       NDIC_HIP(hipStreamSynchronize(s));
       std::memcpy(heads.data(), got, heads.size());
     }
-    for (size_t i = 0; i < n; ++i) {
-      const ndic_zfp_brick_desc& d = descs[i];
-      try {
-        if (d.ndim < 1 || d.ndim > 4) fail(NDIC_INVALID_ARGUMENT, "nd_zfp: brick descriptor ndim must be 1..=4");
-        if (d.in_offset % 8) fail(NDIC_INVALID_ARGUMENT, "zfp bricks: stream offset is not 8-byte aligned");
-        ndic_zfp_config cfg{};
-        cfg.mode = NDIC_ZFP_FIXED_RATE;
-        cfg.rate = d.rate;
-        std::vector<size_t> shape(d.shape, d.shape + d.ndim);
-        Plan pl = make_plan(shape, d.dtype, cfg);
-        if (d.in_len != pl.fixed_bytes)
-          fail(NDIC_CODESTREAM, "nd_zfp: fixed-rate stream is " + std::to_string(d.in_len) +
-                                    " bytes, computed size is " + std::to_string(pl.fixed_bytes));
-        check_stream(pl, &heads[i * kZfpHeadBytes], kZfpHeadBytes, d.in_len);
-        uint64_t k = 0;
-        for (uint32_t a = 0; a < pl.dims; ++a) {
-          uint32_t g = pl.nb[pl.dims - 1 - a];
-          if (d.brick[a] >= g) fail(NDIC_INVALID_ARGUMENT, "nd_zfp: brick coordinates exceed the grid");
-          k = k * g + d.brick[a];
-        }
-        if (d.out_offset % dtype_size(d.dtype)) fail(NDIC_INVALID_ARGUMENT, "zfp bricks: output offset is not aligned");
-        plans[i] = pl;
-        block[i] = k;
-      } catch (const Failure& f) {
-        status[i] = f.status;
-        msg[i] = f.message;
-      }
-    }
-    for (uint32_t cls = 8; cls < 40; ++cls) {
-      std::vector<ZfpChunk> ch;
-      std::vector<ZfpBrick> br;
-      for (size_t i = 0; i < n; ++i) {
-        if (status[i] != NDIC_OK || class_of(plans[i]) != cls) continue;
-        ZfpChunk c = chunk_of(plans[i]);
-        c.src = d_in + descs[i].in_offset;
-        c.src_len = descs[i].in_len;
-        ZfpBrick b{};
-        b.chunk = (uint32_t)ch.size();
-        b.block = block[i];
-        b.out = d_out + descs[i].out_offset;
-        ch.push_back(c);
-        br.push_back(b);
-      }
-      if (ch.empty()) continue;
-      uint64_t sz_ch = al256(ch.size() * sizeof(ZfpChunk));
-      uint8_t* w = (uint8_t*)ctx.work.ensure(sz_ch + br.size() * sizeof(ZfpBrick));
-      ctx.up(w, ch.data(), ch.size() * sizeof(ZfpChunk), s);
-      ctx.up(w + sz_ch, br.data(), br.size() * sizeof(ZfpBrick), s);
-      launch_zfp_decode((ZfpChunk*)w, (uint32_t)ch.size(), br.size(), cls / 8, cls % 8, (ZfpBrick*)(w + sz_ch), s);
+    plan_bricks(d_in, d_out, descs, n, heads.data(), bp);
+    for (const BrickClass& bc : bp.classes) {
+      uint64_t sz_ch = al256(bc.chunks.size() * sizeof(ZfpChunk));
+      uint8_t* w = (uint8_t*)ctx.work.ensure(sz_ch + bc.bricks.size() * sizeof(ZfpBrick));
+      ctx.up(w, bc.chunks.data(), bc.chunks.size() * sizeof(ZfpChunk), s);
+      ctx.up(w + sz_ch, bc.bricks.data(), bc.bricks.size() * sizeof(ZfpBrick), s);
+      launch_zfp_decode((ZfpChunk*)w, (uint32_t)bc.chunks.size(), bc.bricks.size(), bc.dims, bc.scalar,
+                        (ZfpBrick*)(w + sz_ch), s);
       NDIC_HIP(hipStreamSynchronize(s));
     }
   });
   if (st != NDIC_OK) return st;
-  ndic_status first = NDIC_OK;
-  for (size_t i = 0; i < n; ++i) {
-    descs[i].status = status[i];
-    if (status[i] != NDIC_OK && first == NDIC_OK) {
-      first = (ndic_status)status[i];
-      if (err) {
-        err->status = status[i];
-        err->offset = 0;
-        std::snprintf(err->message, sizeof(err->message), "brick %zu: %s", i, msg[i].c_str());
-      }
-    }
-  }
-  return first;
+  return report_bricks(descs, n, bp, err);
 }

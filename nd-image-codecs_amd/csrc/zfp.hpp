@@ -80,4 +80,23 @@ void launch_zfp_index(const ZfpChunk* d_chunks, uint32_t nchunks, uint32_t dims,
 void launch_zfp_decode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, uint32_t dims, uint32_t scalar,
                        const ZfpBrick* d_bricks, hipStream_t stream);
 
+// Host plan of a brick batch (NdZfpBrickDecoder::decode_brick, chunk.rs:484,
+// over BrickReader::fixed_rate's checks, lib.rs:509-599): per descriptor a
+// status and message, and the launch entries of the good ones grouped by
+// (dims, scalar) class.  heads holds kZfpHeadBytes stream bytes per brick.
+struct BrickClass {
+  uint32_t dims = 0, scalar = 0;
+  std::vector<ZfpChunk> chunks;
+  std::vector<ZfpBrick> bricks;
+};
+struct BrickPlan {
+  std::vector<int> status;
+  std::vector<std::string> msg;
+  std::vector<BrickClass> classes;
+};
+void plan_bricks(const uint8_t* d_in, uint8_t* d_out, const ndic_zfp_brick_desc* descs, size_t n,
+                 const uint8_t* heads, BrickPlan& bp);
+// Writes each descriptor's status; returns the first failure (its message in err).
+ndic_status report_bricks(ndic_zfp_brick_desc* descs, size_t n, const BrickPlan& bp, ndic_error* err);
+
 }  // namespace ndic
