@@ -193,6 +193,32 @@ int orc_zfp_decode(int chunk, const uint8_t* in, size_t len, const size_t* shape
   });
 }
 
+// BrickIndex (lib.rs:398-500): header bits, bits per brick, stream length
+// and the grid; byte_range of brick k = [bit / 8, ceil((bit + len) / 8)).
+int orc_zfp_brick_index(const size_t* shape, size_t ndim, int type, double rate, uint64_t* header_bits,
+                        uint64_t* bits_per_brick, uint64_t* stream_len, uint64_t* grid, orc_err* err) {
+  return guard(err, [&] {
+    orc::zfp::BrickIndex bi = orc::zfp::brick_index(std::vector<size_t>(shape, shape + ndim), type, rate);
+    *header_bits = bi.header_bits;
+    *bits_per_brick = bi.bits_per_brick;
+    *stream_len = bi.stream_len;
+    for (size_t a = 0; a < bi.grid.size(); ++a) grid[a] = bi.grid[a];
+  });
+}
+
+// chunk = 1: decode_chunk_brick (type is a dtype code); 0: decompress_brick (type is a zfp type)
+int orc_zfp_decode_brick(int chunk, const uint8_t* in, size_t len, const size_t* shape, size_t ndim, int type,
+                         double rate, const size_t* brick, uint8_t** out, size_t* out_len, size_t* brick_shape,
+                         orc_err* err) {
+  return guard(err, [&] {
+    std::vector<size_t> sh(shape, shape + ndim), br(brick, brick + ndim), bs;
+    std::vector<uint8_t> v = chunk ? orc::zfp::decode_chunk_brick(in, len, sh, type, rate, br, &bs)
+                                   : orc::zfp::decompress_brick(in, len, type, sh, rate, br, &bs);
+    for (size_t a = 0; a < bs.size(); ++a) brick_shape[a] = bs[a];
+    emit(v, out, out_len);
+  });
+}
+
 void orc_zfp_perm(int dims, uint8_t* out) {
   const uint8_t* p = orc::zfp::perm_of(dims);
   std::memcpy(out, p, (size_t)1 << (2 * dims));

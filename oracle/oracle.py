@@ -70,6 +70,8 @@ def _declare(L) -> None:
         "orc_zfp_encode": [I, C.c_char_p, S, P, S, I, I, C.c_double, P, P, P, E],
         "orc_zfp_decode": [I, C.c_char_p, S, P, S, I, I, C.c_double, P, P, E],
         "orc_zfp_perm": [I, P],
+        "orc_zfp_brick_index": [P, S, I, C.c_double, P, P, P, P, E],
+        "orc_zfp_decode_brick": [I, C.c_char_p, S, P, S, I, C.c_double, P, P, P, P, E],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -307,3 +309,36 @@ def zfp_perm(dims: int) -> list:
     buf = (C.c_uint8 * (1 << (2 * dims)))()
     lib().orc_zfp_perm(dims, buf)
     return list(buf)
+
+
+def zfp_brick_index(shape, np_dtype, rate: float) -> dict:
+    """BrickIndex::fixed_rate (lib.rs:398-500) over a row-major shape."""
+    t = {np.dtype(np.int32): 1, np.dtype(np.int64): 2, np.dtype(np.float32): 3, np.dtype(np.float64): 4}[
+        np.dtype(np_dtype)]
+    sh, nd = _shape(shape)
+    hb, bpb, sl, err = C.c_uint64(), C.c_uint64(), C.c_uint64(), _Err()
+    grid = (C.c_uint64 * 4)()
+    _check(lib().orc_zfp_brick_index(sh, nd, t, float(rate), C.byref(hb), C.byref(bpb), C.byref(sl), grid,
+                                     C.byref(err)), err)
+    return {"header_bits": hb.value, "bits_per_brick": bpb.value, "stream_len": sl.value,
+            "grid": [int(grid[a]) for a in range(len(shape))]}
+
+
+def zfp_byte_range(index: dict, k: int):
+    """BrickIndex::byte_range (lib.rs:486-500): (offset, length) in bytes."""
+    bit = index["header_bits"] + k * index["bits_per_brick"]
+    start = bit // 8
+    end = -(-(bit + index["bits_per_brick"]) // 8)
+    return start, end - start
+
+
+def zfp_decode_brick(stream: bytes, shape, dtype: str, rate: float, brick):
+    """decode_chunk_brick (chunk.rs:532): (brick bytes, clipped brick shape);
+    shape is the effective shape, dtype an oracle dtype name (u8 ... f64)."""
+    sh, nd = _shape(shape)
+    br = (C.c_size_t * len(shape))(*brick)
+    bs = (C.c_size_t * len(shape))()
+    out, n, err = C.c_void_p(), C.c_size_t(), _Err()
+    _check(lib().orc_zfp_decode_brick(1, stream, len(stream), sh, nd, DTYPES[dtype], float(rate), br, C.byref(out),
+                                      C.byref(n), bs, C.byref(err)), err)
+    return _take(out, n.value), [int(bs[a]) for a in range(len(shape))]

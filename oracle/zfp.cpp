@@ -739,6 +739,139 @@ void decompress(const uint8_t* bytes, size_t len, int type, const std::vector<si
   }
 }
 
+// lib.rs:386-500 BrickIndex::fixed_rate: brick k spans bits
+// [header_bits + k * bits_per_brick, + bits_per_brick); the stream is padded
+// to a 64-bit word.
+BrickIndex brick_index(const std::vector<size_t>& shape, int type, double rate) {
+  if (shape.empty() || shape.size() > 4) fail(kInvalidArgument, "nd_zfp: shape must have 1..=4 dimensions");
+  for (size_t d : shape)
+    if (d == 0) fail(kInvalidArgument, "nd_zfp: shape has a zero extent");
+  Mode m;
+  m.kind = Mode::kRate;
+  m.rate = rate;
+  int dims = (int)shape.size();
+  Cfg c = cfg_of(m, type, dims);
+  uint32_t mb;
+  (void)mode_word(m, c, &mb);
+  BrickIndex bi;
+  bi.header_bits = 32 + 52 + mb;
+  bi.bits_per_brick = c.maxbits;
+  uint64_t nbricks = 1;
+  for (size_t d : shape) {
+    bi.grid.push_back((d + 3) / 4);
+    nbricks *= (d + 3) / 4;
+  }
+  bi.stream_len = (size_t)(((bi.header_bits + nbricks * bi.bits_per_brick + 63) / 64) * 8);
+  return bi;
+}
+
+// BrickIndex::linear (lib.rs:453-474): row-major over the grid, last axis fastest
+uint64_t brick_linear(const BrickIndex& bi, const std::vector<size_t>& coords) {
+  if (coords.size() != bi.grid.size()) fail(kInvalidArgument, "nd_zfp: brick coordinates do not match the grid");
+  uint64_t k = 0;
+  for (size_t a = 0; a < coords.size(); ++a) {
+    if (coords[a] >= bi.grid[a]) fail(kInvalidArgument, "nd_zfp: brick coordinates exceed the grid");
+    k = k * bi.grid[a] + coords[a];
+  }
+  return k;
+}
+
+// BrickReader::fixed_rate + brick (lib.rs:509-578): the stream length and
+// full header are checked as decompress checks them, then a seek to brick k
+// and one block decode, stored clipped to the array bounds in C order.
+std::vector<uint8_t> decompress_brick(const uint8_t* bytes, size_t len, int type, const std::vector<size_t>& shape,
+                                      double rate, const std::vector<size_t>& brick, std::vector<size_t>* brick_shape) {
+  BrickIndex bi = brick_index(shape, type, rate);
+  if (len != bi.stream_len) fail(kCodestream, "nd_zfp: fixed-rate stream length differs from the computed size");
+  int dims = (int)shape.size();
+  uint32_t nd[4];
+  rev_dims(shape, nd);
+  Mode m;
+  m.kind = Mode::kRate;
+  m.rate = rate;
+  Cfg c = cfg_of(m, type, dims);
+  BitReader r(bytes, len);
+  if (r.bits(8) != 'z' || r.bits(8) != 'f' || r.bits(8) != 'p' || r.bits(8) != 5)
+    fail(kCodestream, "nd_zfp: bad ZFP magic");
+  if (r.bits(52) != field_meta(type, dims, nd)) fail(kCodestream, "nd_zfp: stream header does not declare the expected field");
+  uint32_t mb;
+  uint64_t mw = mode_word(m, c, &mb);
+  uint64_t got = r.bits(12);
+  if (got == 0xfff) got |= r.bits(52) << 12;
+  if (got != mw) fail(kCodestream, "nd_zfp: stream compression parameters do not match the configuration");
+  if (r.pos != bi.header_bits) fail(kCodestream, "nd_zfp: stream header has an unexpected length");
+  uint64_t k = brick_linear(bi, brick);
+  r.pos = bi.header_bits + k * bi.bits_per_brick;
+  brick_shape->clear();
+  size_t n = 1;
+  for (size_t a = 0; a < shape.size(); ++a) {
+    brick_shape->push_back(std::min<size_t>(4, shape[a] - 4 * brick[a]));
+    n *= brick_shape->back();
+  }
+  size_t es = (type == kInt32 || type == kFloat) ? 4 : 8;
+  std::vector<uint8_t> out(n * es);
+  auto store = [&](auto* q) {
+    using S = std::remove_pointer_t<decltype(q)>;
+    decompress_block_t<S>(r, c, q, dims);
+    // block value i sits at (x, y, z, w) = (i & 3, i >> 2 & 3, ...), x = the last axis
+    size_t o = 0;
+    std::vector<size_t> idx(shape.size(), 0);
+    for (size_t j = 0; j < n; ++j) {
+      uint32_t i = 0;
+      for (size_t a = 0; a < shape.size(); ++a) i |= (uint32_t)idx[a] << (2 * (shape.size() - 1 - a));
+      std::memcpy(&out[o], &q[i], es);
+      o += es;
+      for (size_t a = shape.size(); a-- > 0;) {
+        if (++idx[a] < (*brick_shape)[a]) break;
+        idx[a] = 0;
+      }
+    }
+  };
+  switch (type) {
+    case kInt32: { int32_t q[256]; store(q); break; }
+    case kInt64: { int64_t q[256]; store(q); break; }
+    case kFloat: { float q[256]; store(q); break; }
+    default: { double q[256]; store(q); break; }
+  }
+  return out;
+}
+
+// chunk.rs:446-540 NdZfpBrickDecoder / decode_chunk_brick: the brick over the
+// effective shape, narrow integers demoted from the int32 reader.
+std::vector<uint8_t> decode_chunk_brick(const uint8_t* bytes, size_t len, const std::vector<size_t>& shape, int dtype,
+                                        double rate, const std::vector<size_t>& brick,
+                                        std::vector<size_t>* brick_shape) {
+  if (dtype == kF32) return decompress_brick(bytes, len, kFloat, shape, rate, brick, brick_shape);
+  if (dtype == kF64) return decompress_brick(bytes, len, kDouble, shape, rate, brick, brick_shape);
+  if (dtype == kI32) return decompress_brick(bytes, len, kInt32, shape, rate, brick, brick_shape);
+  if (dtype == kI64) return decompress_brick(bytes, len, kInt64, shape, rate, brick, brick_shape);
+  std::vector<uint8_t> v = decompress_brick(bytes, len, kInt32, shape, rate, brick, brick_shape);
+  size_t n = v.size() / 4, es = dtype_size(dtype);
+  std::vector<uint8_t> out(n * es);
+  for (size_t i = 0; i < n; ++i) {
+    int32_t x;
+    std::memcpy(&x, &v[4 * i], 4);
+    switch (dtype) {
+      case kU8: out[i] = (uint8_t)std::min(std::max((x >> 23) + 0x80, 0), 0xff); break;
+      case kI8: out[i] = (uint8_t)(int8_t)std::min(std::max(x >> 23, -0x80), 0x7f); break;
+      case kU16: {
+        uint16_t u = (uint16_t)std::min(std::max((x >> 15) + 0x8000, 0), 0xffff);
+        out[2 * i] = u & 0xff;
+        out[2 * i + 1] = u >> 8;
+        break;
+      }
+      case kI16: {
+        uint16_t u = (uint16_t)(int16_t)std::min(std::max(x >> 15, -0x8000), 0x7fff);
+        out[2 * i] = u & 0xff;
+        out[2 * i + 1] = u >> 8;
+        break;
+      }
+      default: fail(kUnsupported, "zfp has no path for this dtype");
+    }
+  }
+  return out;
+}
+
 // chunk.rs:305-345: narrow integers promoted to int32 (and back)
 std::vector<uint8_t> encode_chunk(const uint8_t* bytes, size_t len, const std::vector<size_t>& shape, int dtype,
                                   const Mode& mode) {

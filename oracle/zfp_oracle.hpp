@@ -32,6 +32,20 @@ std::vector<uint8_t> encode_chunk(const uint8_t* bytes, size_t len, const std::v
                                   const Mode& mode);
 std::vector<uint8_t> decode_chunk(const uint8_t* bytes, size_t len, const std::vector<size_t>& shape, int dtype,
                                   const Mode& mode);
+// Fixed-rate brick addressing and single-brick decodes (lib.rs:386-599,
+// chunk.rs:446-540).  Coordinates and brick shapes are row-major.
+struct BrickIndex {
+  uint64_t header_bits = 0, bits_per_brick = 0;
+  std::vector<size_t> grid;
+  size_t stream_len = 0;
+};
+BrickIndex brick_index(const std::vector<size_t>& shape, int type, double rate);
+uint64_t brick_linear(const BrickIndex& bi, const std::vector<size_t>& coords);
+std::vector<uint8_t> decompress_brick(const uint8_t* bytes, size_t len, int type, const std::vector<size_t>& shape,
+                                      double rate, const std::vector<size_t>& brick, std::vector<size_t>* brick_shape);
+std::vector<uint8_t> decode_chunk_brick(const uint8_t* bytes, size_t len, const std::vector<size_t>& shape, int dtype,
+                                        double rate, const std::vector<size_t>& brick,
+                                        std::vector<size_t>* brick_shape);
 // Order of the 4^d block coefficients (d = 1..4).
 const uint8_t* perm_of(int dims);
 
