@@ -41,12 +41,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_workload(dev, ndic, args):
-    rng = np.random.default_rng(args.seed)
+def draw_specs(dev, rng, workload: str, n: int):
+    """n chunk specs (shape, dtype, steps, levels, content, blobs, seed) of a
+    workload: "c2" the uneven batch, "c0" the 32x256x256 u16 chunk."""
     kinds = ["delta", "haar", "lift53"]
     specs = []
-    for i in range(args.chunks):
-        if args.workload == "c2":
+    for i in range(n):
+        if workload == "c2":
             y = int(round(math.exp(rng.uniform(math.log(64), math.log(1024)))))
             x = int(round(math.exp(rng.uniform(math.log(64), math.log(1024)))))
             z = int(rng.integers(1, 65))
@@ -66,6 +67,12 @@ def make_workload(dev, ndic, args):
                                           rng.uniform(4, 20))) for _ in range(8)]
         seed = int(rng.integers(0, 2**63))
         specs.append((shape, dt, steps, lv, content, blobs, seed))
+    return specs
+
+
+def make_workload(dev, ndic, args):
+    rng = np.random.default_rng(args.seed)
+    specs = draw_specs(dev, rng, args.workload, args.chunks)
     sizes = [int(np.prod(s[0])) * np.dtype(s[1]).itemsize for s in specs]
     raw_total = sum(sizes)
     padded = sum((n + 15) // 16 * 16 for n in sizes)

@@ -49,3 +49,44 @@ def sm_block(rng, w, h, kmax, dens):
     sm = (sign | (mag.astype(np.uint32) << (31 - kmax))).astype(np.uint32)
     sm[mag == 0] = 0
     return sm
+
+
+def series_5d(shape=(8, 1, 32, 512, 512), n_blobs=24, sigma=(3.0, 30.0), amp=(2000.0, 40000.0), base=1000.0,
+              fade=0.02, saturated=0.01, seed=1):
+    """The 5D series recipe of the single-chunk 8x1x32x512x512 configuration:
+    uint16 t,c,z,y,x over the full 16-bit range -- n_blobs 3D Gaussian blobs
+    (sigma and amplitude uniform in the given ranges) that move 1 px per t step
+    along a random direction in the (y, x) plane and fade by `fade` per t step,
+    Poisson noise around the blob intensity, and a `saturated` fraction of
+    pixels at 65535."""
+    rng = np.random.default_rng(seed)
+    T, C, Z, Y, X = shape
+    lam = np.full(shape, base, np.float32)
+    cz = rng.uniform(0, Z, n_blobs)
+    cy = rng.uniform(0, Y, n_blobs)
+    cx = rng.uniform(0, X, n_blobs)
+    s = rng.uniform(*sigma, n_blobs)
+    a = rng.uniform(*amp, n_blobs)
+    ang = rng.uniform(0, 2 * np.pi, n_blobs)
+    zz = np.arange(Z, dtype=np.float32)
+    for t in range(T):
+        for b in range(n_blobs):
+            y0c, x0c = cy[b] + t * np.sin(ang[b]), cx[b] + t * np.cos(ang[b])
+            r = 3.0 * s[b]
+            ya, yb = max(0, int(y0c - r)), min(Y, int(y0c + r) + 1)
+            xa, xb = max(0, int(x0c - r)), min(X, int(x0c + r) + 1)
+            if ya >= yb or xa >= xb:
+                continue
+            k = -1.0 / (2 * s[b] * s[b])
+            gz = np.exp(k * (zz - cz[b]) ** 2)
+            gy = np.exp(k * (np.arange(ya, yb, dtype=np.float32) - y0c) ** 2)
+            gx = np.exp(k * (np.arange(xa, xb, dtype=np.float32) - x0c) ** 2)
+            w = np.float32(a[b] * (1.0 - fade) ** t)
+            for c in range(C):
+                lam[t, c, :, ya:yb, xa:xb] += w * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+    vol = rng.poisson(lam).astype(np.float64)
+    np.clip(vol, 0, 65535, out=vol)
+    out = vol.astype(np.uint16)
+    flat = out.reshape(-1)
+    flat[rng.choice(flat.size, size=int(flat.size * saturated), replace=False)] = 65535
+    return out

@@ -101,8 +101,12 @@ def test_special_float_values_match_oracle(ndic, oracle):
     assert rev == a.tobytes()
 
 
-def test_bricks_equal_full_decode(ndic):
+def test_bricks_match_oracle_brick_reader(ndic, oracle):
+    """ndic_zfp_decode_brick against the oracle's brick reader
+    (decode_chunk_brick, crates/ndic-zfp/src/chunk.rs:532), every brick of
+    each field, and the device's full decode sliced to the same brick."""
     rng = np.random.default_rng(3)
+    short = {"float32": "f32", "float64": "f64", "int32": "i32", "uint16": "u16", "int8": "i8"}
     for shape, dtype in [((9, 10, 11), "float32"), ((5, 6, 7, 9), "float64"), ((13, 6), "int32"), ((19,), "uint16"),
                          ((6, 7, 5), "int8")]:
         npt = np.dtype(dtype)
@@ -116,6 +120,8 @@ def test_bricks_equal_full_decode(ndic):
         grid = [(n + 3) // 4 for n in shape]
         for coords in itertools.product(*[range(g) for g in grid]):
             data, bshape = ndic.nd_zfp_decode_brick(s, shape, dtype, coords, rate=12.0)
+            exp, eshape = oracle.zfp_decode_brick(s, shape, short[dtype], 12.0, coords)
+            assert data == exp and list(bshape) == eshape, (shape, coords)
             sl = tuple(slice(4 * c, 4 * c + b) for c, b in zip(coords, bshape))
             assert bshape == [min(4, n - 4 * c) for n, c in zip(shape, coords)]
             assert np.array_equal(np.frombuffer(data, npt).reshape(bshape), full[sl]), (shape, coords)
