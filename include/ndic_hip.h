@@ -263,6 +263,8 @@ typedef struct {
   int32_t dtype;
   ndic_zfp_config config;
   int32_t status;      /* filled by the library */
+  uint64_t index_offset; /* block index of a variable-size stream in the index arena (the *_indexed calls) */
+  uint64_t index_len;    /* its bytes: 4 * (blocks + 1); 0: none */
 } ndic_zfp_desc;
 
 /* Batch forms of ndic_zfp_encode_chunk / ndic_zfp_decode_chunk over device
@@ -272,6 +274,28 @@ ndic_status ndic_zfp_encode_batch(const uint8_t* d_in, uint8_t* d_out, size_t ca
                                   size_t n, void* stream, ndic_error* err);
 ndic_status ndic_zfp_decode_batch(const uint8_t* d_in, uint8_t* d_out, ndic_zfp_desc* descs, size_t n,
                                   void* stream, ndic_error* err);
+
+/* Block index of a variable-size stream (reversible, fixed-accuracy,
+ * fixed-precision).  No counterpart in the reference, whose docs list an
+ * explicit table for these modes as future work (docs/architecture/zfp.md,
+ * "Brick index"): the stream bytes stay exactly the ZFP stream, and beside
+ * it the index holds blocks + 1 little-endian uint32 bit offsets from the
+ * stream's first block bit (the bit after the header), the last one the end
+ * of the last block; block k occupies stream bits [header_bits + idx[k],
+ * header_bits + idx[k + 1]).  Streams of 2^32 block bits or more get none.
+ * ndic_zfp_encode_batch_indexed writes each variable-size stream's index
+ * packed into d_index (8-byte aligned entries; index_capacity bytes) and
+ * reports it in descs[i].index_offset / index_len (0 for fixed-rate
+ * streams, which need none).  ndic_zfp_decode_batch_indexed decodes every
+ * block of an indexed stream in parallel instead of walking the stream to
+ * find them; a block whose bits disagree with the index fails its chunk with
+ * a Codestream error.  Descriptors with index_len 0 decode as in
+ * ndic_zfp_decode_batch. */
+ndic_status ndic_zfp_encode_batch_indexed(const uint8_t* d_in, uint8_t* d_out, size_t capacity, uint8_t* d_index,
+                                          size_t index_capacity, ndic_zfp_desc* descs, size_t n, void* stream,
+                                          ndic_error* err);
+ndic_status ndic_zfp_decode_batch_indexed(const uint8_t* d_in, const uint8_t* d_index, uint8_t* d_out,
+                                          ndic_zfp_desc* descs, size_t n, void* stream, ndic_error* err);
 
 /* One 4^d brick of a device-resident fixed-rate stream (the serving form of
  * NdZfpBrickDecoder::decode_brick, chunk.rs:484): shape is the effective

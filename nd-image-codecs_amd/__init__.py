@@ -83,7 +83,8 @@ class ChunkDesc(C.Structure):
 class ZfpDesc(C.Structure):
     _fields_ = [("in_offset", C.c_uint64), ("in_len", C.c_uint64), ("out_offset", C.c_uint64),
                 ("out_len", C.c_uint64), ("ndim", C.c_uint32), ("shape", C.c_uint32 * MAX_NDIM),
-                ("dtype", C.c_int32), ("config", ZfpConfig), ("status", C.c_int32)]
+                ("dtype", C.c_int32), ("config", ZfpConfig), ("status", C.c_int32),
+                ("index_offset", C.c_uint64), ("index_len", C.c_uint64)]
 
 
 class RangePlanC(C.Structure):
@@ -126,6 +127,8 @@ def _load() -> C.CDLL:
                                             P, E]),
         "ndic_zfp_encode_batch": (C.c_int, [P, P, S, P, S, P, E]),
         "ndic_zfp_decode_batch": (C.c_int, [P, P, P, S, P, E]),
+        "ndic_zfp_encode_batch_indexed": (C.c_int, [P, P, S, P, S, P, S, P, E]),
+        "ndic_zfp_decode_batch_indexed": (C.c_int, [P, P, P, P, S, P, E]),
         "ndic_range_plan_chunk": (C.c_int, [C.c_char_p, S, C.c_int32, C.c_uint64, P, S, C.POINTER(RangePlanC), E]),
         "ndic_htj2k_decode_select_batch": (C.c_int, [P, P, P, S, P, E]),
         "ndic_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), E]),
@@ -481,7 +484,7 @@ def nd_zfp_decode_brick(chunk: Any, shape: Sequence[int], dtype: Any, brick: Seq
 
 
 def make_zfp_desc(shape: Sequence[int], dtype: Any, config: ZfpConfig, in_offset: int = 0, in_len: int = 0,
-                  out_offset: int = 0) -> ZfpDesc:
+                  out_offset: int = 0, index_offset: int = 0, index_len: int = 0) -> ZfpDesc:
     if len(shape) > MAX_NDIM:
         raise NdicError(1, 0, "batch descriptors hold at most 8 dimensions")
     d = ZfpDesc()
@@ -491,24 +494,39 @@ def make_zfp_desc(shape: Sequence[int], dtype: Any, config: ZfpConfig, in_offset
         d.shape[i] = int(s)
     d.dtype = _zfp_dtype(dtype)
     d.config = config
+    d.index_offset, d.index_len = int(index_offset), int(index_len)
     return d
 
 
-def zfp_encode_batch(d_in_ptr: int, d_out_ptr: int, capacity: int, descs, stream: int = 0):
-    """Encode device-resident chunks; descs[i].out_offset/out_len report the streams."""
+def zfp_encode_batch(d_in_ptr: int, d_out_ptr: int, capacity: int, descs, stream: int = 0, d_index_ptr: int = 0,
+                     index_capacity: int = 0):
+    """Encode device-resident chunks; descs[i].out_offset/out_len report the
+    streams.  With an index arena (d_index_ptr) each variable-size stream's
+    block index is written there too (descs[i].index_offset/index_len)."""
     arr = descs if isinstance(descs, C.Array) else (ZfpDesc * len(descs))(*descs)
     err = Error()
-    rc = _lib.ndic_zfp_encode_batch(C.c_void_p(d_in_ptr), C.c_void_p(d_out_ptr), int(capacity), arr, len(arr),
-                                    C.c_void_p(stream), C.byref(err))
+    if d_index_ptr:
+        rc = _lib.ndic_zfp_encode_batch_indexed(C.c_void_p(d_in_ptr), C.c_void_p(d_out_ptr), int(capacity),
+                                                C.c_void_p(d_index_ptr), int(index_capacity), arr, len(arr),
+                                                C.c_void_p(stream), C.byref(err))
+    else:
+        rc = _lib.ndic_zfp_encode_batch(C.c_void_p(d_in_ptr), C.c_void_p(d_out_ptr), int(capacity), arr, len(arr),
+                                        C.c_void_p(stream), C.byref(err))
     _check(rc, err)
     return arr
 
 
-def zfp_decode_batch(d_in_ptr: int, d_out_ptr: int, descs, stream: int = 0):
+def zfp_decode_batch(d_in_ptr: int, d_out_ptr: int, descs, stream: int = 0, d_index_ptr: int = 0):
+    """Decode device-resident streams; with an index arena, streams whose
+    descriptor carries a block index decode without walking the stream."""
     arr = descs if isinstance(descs, C.Array) else (ZfpDesc * len(descs))(*descs)
     err = Error()
-    rc = _lib.ndic_zfp_decode_batch(C.c_void_p(d_in_ptr), C.c_void_p(d_out_ptr), arr, len(arr),
-                                    C.c_void_p(stream), C.byref(err))
+    if d_index_ptr:
+        rc = _lib.ndic_zfp_decode_batch_indexed(C.c_void_p(d_in_ptr), C.c_void_p(d_index_ptr), C.c_void_p(d_out_ptr),
+                                                arr, len(arr), C.c_void_p(stream), C.byref(err))
+    else:
+        rc = _lib.ndic_zfp_decode_batch(C.c_void_p(d_in_ptr), C.c_void_p(d_out_ptr), arr, len(arr),
+                                        C.c_void_p(stream), C.byref(err))
     _check(rc, err)
     return arr
 

@@ -355,6 +355,11 @@ struct ZfpItem {
   uint64_t out_off = 0, out_len = 0;
   Plan plan;
   const uint8_t* host_head = nullptr;  // decode: first bytes on the host, if there
+  // Block index of a variable-size stream (nblocks + 1 uint32 bit offsets):
+  // encode writes it at index_off in the index arena when one is given;
+  // decode seeks with it when index_len > 0.
+  uint8_t* d_index = nullptr;
+  uint64_t index_off = 0, index_len = 0;
 };
 
 namespace {
@@ -369,7 +374,8 @@ uint32_t class_of(const Plan& p) { return p.dims * 8 + p.scalar; }
 
 }  // namespace
 
-void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out, uint64_t capacity, hipStream_t s) {
+void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out, uint64_t capacity, hipStream_t s,
+                      uint8_t* d_index, uint64_t index_capacity) {
   ctx.release_kept();  // earlier calls on this context ended synchronised
   zfp_init_tables();
   for (auto& it : items) {
@@ -385,7 +391,7 @@ void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out,
       set_fail(it, f);
     }
   }
-  uint64_t cursor = 0;
+  uint64_t cursor = 0, icursor = 0;
   for (uint32_t cls = 8; cls < 40; ++cls) {
     std::vector<ZfpItem*> group;
     for (auto& it : items)
@@ -393,6 +399,7 @@ void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out,
     if (group.empty()) continue;
     std::vector<ZfpChunk> ch;
     uint64_t total = 0, vblocks = 0;
+    bool any_index = false;
     uint32_t slot_words = 1;
     for (ZfpItem* it : group) {
       ZfpChunk c = chunk_of(it->plan);
@@ -468,9 +475,14 @@ void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out,
       for (size_t i = 0; i < group.size(); ++i) {
         if (ch[i].fixed || group[i]->status != NDIC_OK) continue;
         uint64_t len = ((ch[i].header_bits + tot[i] + 63) / 64) * 8;
-        if (cursor + len > capacity) {
+        // its block index: 32-bit offsets, so only streams of fewer than
+        // 2^32 block bits have one
+        const bool with_index = d_index && tot[i] < (1ull << 32);
+        const uint64_t ilen = with_index ? 4 * (ch[i].nblocks + 1) : 0;
+        if (cursor + len > capacity || icursor + ilen > index_capacity) {
           group[i]->status = NDIC_INVALID_ARGUMENT;
-          group[i]->message = "zfp encode output capacity exceeded";
+          group[i]->message = cursor + len > capacity ? "zfp encode output capacity exceeded"
+                                                      : "zfp encode block-index capacity exceeded";
           ch[i].offs = nullptr;
           continue;
         }
@@ -483,6 +495,13 @@ void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out,
         header_words(group[i]->plan, h.w, &h.nwords);
         hdr.push_back(h);
         NDIC_HIP(hipMemsetAsync(ch[i].dst, 0, len, s));
+        if (with_index) {
+          group[i]->index_off = icursor;
+          group[i]->index_len = ilen;
+          ch[i].index = (uint32_t*)(d_index + icursor);
+          icursor += (ilen + 7) & ~uint64_t(7);
+          any_index = true;
+        }
       }
       // fixed-rate chunks of the class take no part in placement
       for (size_t i = 0; i < ch.size(); ++i)
@@ -493,6 +512,7 @@ void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out,
         launch_zfp_headers(d_hdr, (uint32_t)hdr.size(), s);
       }
       launch_zfp_place(d_ch, (uint32_t)ch.size(), total, scan, s);
+      if (any_index) launch_zfp_index_write(d_ch, (uint32_t)ch.size(), total, scan, s);
     }
     NDIC_HIP(hipStreamSynchronize(s));
   }
@@ -532,6 +552,10 @@ void zfp_decode_items(Context& ctx, std::vector<ZfpItem>& items, hipStream_t s) 
     try {
       it.plan = make_plan(it.shape, it.dtype, it.cfg);
       check_stream(it.plan, &heads[i * kZfpHeadBytes], kZfpHeadBytes, it.in_len);
+      if (it.index_len && !it.plan.fixed && it.index_len != 4 * (it.plan.nblocks + 1))
+        fail(NDIC_INVALID_ARGUMENT, "nd_zfp: the block index has " + std::to_string(it.index_len) +
+                                        " bytes, the stream's " + std::to_string(it.plan.nblocks) +
+                                        " blocks need " + std::to_string(4 * (it.plan.nblocks + 1)));
     } catch (const Failure& f) {
       set_fail(it, f);
     }
@@ -543,7 +567,7 @@ void zfp_decode_items(Context& ctx, std::vector<ZfpItem>& items, hipStream_t s) 
     if (group.empty()) continue;
     std::vector<ZfpChunk> ch;
     uint64_t total = 0;
-    bool variable = false;
+    bool walk = false, indexed = false;
     for (ZfpItem* it : group) {
       ZfpChunk c = chunk_of(it->plan);
       c.block_begin = total;
@@ -551,18 +575,34 @@ void zfp_decode_items(Context& ctx, std::vector<ZfpItem>& items, hipStream_t s) 
       c.src_len = it->in_len;
       c.dst = it->d_out;
       total += c.nblocks;
-      variable |= !c.fixed;
+      if (!c.fixed && it->index_len) {
+        c.index = (uint32_t*)it->d_index;
+        indexed = true;
+      } else {
+        walk |= !c.fixed;
+      }
       ch.push_back(c);
     }
-    uint64_t sz_ch = al256(ch.size() * sizeof(ZfpChunk));
-    uint8_t* meta = (uint8_t*)ctx.work.ensure(sz_ch + (variable ? al256(total * 8) : 0));
-    uint64_t* offs = variable ? (uint64_t*)(meta + sz_ch) : nullptr;
-    for (auto& c : ch) c.offs = offs;
+    const uint64_t sz_ch = al256(ch.size() * sizeof(ZfpChunk)), sz_offs = walk ? al256(total * 8) : 0;
+    uint8_t* meta = (uint8_t*)ctx.work.ensure(sz_ch + sz_offs + (indexed ? al256(ch.size() * 4) : 0));
+    uint64_t* offs = walk ? (uint64_t*)(meta + sz_ch) : nullptr;
+    uint32_t* bad = indexed ? (uint32_t*)(meta + sz_ch + sz_offs) : nullptr;
+    for (size_t j = 0; j < ch.size(); ++j) {
+      ch[j].offs = offs;
+      if (ch[j].index) ch[j].bad = bad + j;
+    }
     ZfpChunk* d_ch = (ZfpChunk*)meta;
     ctx.up(d_ch, ch.data(), ch.size() * sizeof(ZfpChunk), s);
-    if (variable) launch_zfp_index(d_ch, (uint32_t)ch.size(), cls / 8, cls % 8, s);
+    if (indexed) NDIC_HIP(hipMemsetAsync(bad, 0, ch.size() * 4, s));
+    if (walk) launch_zfp_index(d_ch, (uint32_t)ch.size(), cls / 8, cls % 8, s);
     launch_zfp_decode(d_ch, (uint32_t)ch.size(), total, cls / 8, cls % 8, nullptr, s);
+    const uint32_t* flags = indexed ? (const uint32_t*)ctx.down(bad, ch.size() * 4, s) : nullptr;
     NDIC_HIP(hipStreamSynchronize(s));
+    for (size_t j = 0; flags && j < ch.size(); ++j)
+      if (flags[j]) {
+        group[j]->status = NDIC_CODESTREAM;
+        group[j]->message = "nd_zfp: the block index disagrees with the stream";
+      }
   }
 }
 
@@ -616,7 +656,7 @@ extern "C" ndic_status ndic_zfp_encode_chunk(const uint8_t* bytes, size_t len, c
     uint8_t* d_out = (uint8_t*)ctx.out.ensure(cap);
     it.d_in = d_in;
     std::vector<ZfpItem> items{std::move(it)};
-    zfp_encode_items(ctx, items, d_out, cap, s);
+    zfp_encode_items(ctx, items, d_out, cap, s, nullptr, 0);
     raise_item(items[0]);
     emit_host(d_out + items[0].out_off, items[0].out_len, out, s);
   });
@@ -761,8 +801,10 @@ ndic_status batch_status(ndic_zfp_desc* descs, std::vector<ZfpItem>& items, ndic
 
 }  // namespace
 
-extern "C" ndic_status ndic_zfp_encode_batch(const uint8_t* d_in, uint8_t* d_out, size_t capacity,
-                                             ndic_zfp_desc* descs, size_t n, void* stream, ndic_error* err) {
+namespace {
+
+ndic_status encode_batch_call(const uint8_t* d_in, uint8_t* d_out, size_t capacity, uint8_t* d_index,
+                              size_t index_capacity, ndic_zfp_desc* descs, size_t n, void* stream, ndic_error* err) {
   std::vector<ZfpItem> items(n);
   ndic_status st = guard_call(err, [&] {
     if (n && !descs) fail(NDIC_INVALID_ARGUMENT, "null descriptors");
@@ -777,24 +819,32 @@ extern "C" ndic_status ndic_zfp_encode_batch(const uint8_t* d_in, uint8_t* d_out
     Context& ctx = Context::get();
     std::lock_guard<std::recursive_mutex> lock(ctx.mu);
     hipStream_t s = stream ? (hipStream_t)stream : ctx.stream;
-    zfp_encode_items(ctx, items, d_out, capacity, s);
+    zfp_encode_items(ctx, items, d_out, capacity, s, d_index, index_capacity);
     for (size_t i = 0; i < n; ++i) {
+      const bool ok = items[i].status == NDIC_OK;
       descs[i].out_offset = items[i].out_off;
-      descs[i].out_len = items[i].status == NDIC_OK ? items[i].out_len : 0;
+      descs[i].out_len = ok ? items[i].out_len : 0;
+      descs[i].index_offset = ok ? items[i].index_off : 0;
+      descs[i].index_len = ok ? items[i].index_len : 0;
     }
   });
   if (st != NDIC_OK) return st;
   return batch_status(descs, items, err);
 }
 
-extern "C" ndic_status ndic_zfp_decode_batch(const uint8_t* d_in, uint8_t* d_out, ndic_zfp_desc* descs, size_t n,
-                                             void* stream, ndic_error* err) {
+ndic_status decode_batch_call(const uint8_t* d_in, const uint8_t* d_index, uint8_t* d_out, ndic_zfp_desc* descs,
+                              size_t n, void* stream, ndic_error* err) {
   std::vector<ZfpItem> items(n);
   ndic_status st = guard_call(err, [&] {
     if (n && !descs) fail(NDIC_INVALID_ARGUMENT, "null descriptors");
     for (size_t i = 0; i < n; ++i) {
       try {
         items[i] = item_of(descs[i], d_in, d_out, false);
+        if (d_index && descs[i].index_len) {
+          if (descs[i].index_offset % 4) fail(NDIC_INVALID_ARGUMENT, "zfp batch: block index offset is not 4-byte aligned");
+          items[i].d_index = const_cast<uint8_t*>(d_index) + descs[i].index_offset;
+          items[i].index_len = descs[i].index_len;
+        }
       } catch (const Failure& f) {
         items[i].status = f.status;
         items[i].message = f.message;
@@ -807,6 +857,29 @@ extern "C" ndic_status ndic_zfp_decode_batch(const uint8_t* d_in, uint8_t* d_out
   });
   if (st != NDIC_OK) return st;
   return batch_status(descs, items, err);
+}
+
+}  // namespace
+
+extern "C" ndic_status ndic_zfp_encode_batch(const uint8_t* d_in, uint8_t* d_out, size_t capacity,
+                                             ndic_zfp_desc* descs, size_t n, void* stream, ndic_error* err) {
+  return encode_batch_call(d_in, d_out, capacity, nullptr, 0, descs, n, stream, err);
+}
+
+extern "C" ndic_status ndic_zfp_decode_batch(const uint8_t* d_in, uint8_t* d_out, ndic_zfp_desc* descs, size_t n,
+                                             void* stream, ndic_error* err) {
+  return decode_batch_call(d_in, nullptr, d_out, descs, n, stream, err);
+}
+
+extern "C" ndic_status ndic_zfp_encode_batch_indexed(const uint8_t* d_in, uint8_t* d_out, size_t capacity,
+                                                     uint8_t* d_index, size_t index_capacity, ndic_zfp_desc* descs,
+                                                     size_t n, void* stream, ndic_error* err) {
+  return encode_batch_call(d_in, d_out, capacity, d_index, index_capacity, descs, n, stream, err);
+}
+
+extern "C" ndic_status ndic_zfp_decode_batch_indexed(const uint8_t* d_in, const uint8_t* d_index, uint8_t* d_out,
+                                                     ndic_zfp_desc* descs, size_t n, void* stream, ndic_error* err) {
+  return decode_batch_call(d_in, d_index, d_out, descs, n, stream, err);
 }
 
 namespace ndic {

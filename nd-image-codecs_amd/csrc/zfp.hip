@@ -932,7 +932,7 @@ __global__ __launch_bounds__(64) void zfp_index_kernel(const ZfpChunk* __restric
   uint32_t ci = blockIdx.x;
   if (ci >= nchunks) return;
   const ZfpChunk& c = chunks[ci];
-  if (c.fixed) return;
+  if (c.fixed || c.index) return;  // computed addressing, or a block index
   BitReaderT<WindowSrc> r;
   r.src.init(c.src, c.src_len, c.header_bits >> 6);
   r.seek(c.header_bits);
@@ -964,12 +964,35 @@ __global__ __launch_bounds__(256) void zfp_decode_kernel(const ZfpChunk* __restr
   BitReader r;
   r.src.base = c.src;
   r.src.nbytes = c.src_len;
-  r.seek(c.fixed ? c.header_bits + k * c.maxbits : c.offs[c.block_begin + k]);
+  const bool indexed = !c.fixed && c.index;
+  r.seek(c.fixed ? c.header_bits + k * c.maxbits : indexed ? c.header_bits + c.index[k] : c.offs[c.block_begin + k]);
   S vals[R];
-  decode_block<D, S, false>(r, c, vals);
+  const uint32_t used = decode_block<D, S, false>(r, c, vals);
+  if (indexed && lane() == 0) {
+    // the block must take exactly the bits the index gives it, the index
+    // must start at the first block and end inside the stream
+    const uint32_t lo = c.index[k], hi = c.index[k + 1];
+    if (hi < lo || hi - lo != used || (k == 0 && lo != 0) ||
+        (k + 1 == c.nblocks && c.header_bits + (uint64_t)hi > 8 * c.src_len))
+      atomicOr(c.bad, 1u);
+  }
   uint32_t b[4];
   block_coords(c, k, b);
   scatter<D, S>(c, b, vals, brick);
+}
+
+// Block index of each variable-size chunk that asks for one: entry k is
+// block k's bit offset from the first block, entry nblocks the end.
+__global__ __launch_bounds__(256) void zfp_index_write_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks,
+                                                              uint64_t total, const uint64_t* __restrict__ scan) {
+  const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const ZfpChunk& c = chunks[find_chunk(chunks, nchunks, g)];
+  if (c.fixed || !c.index || !c.dst || !c.offs) return;
+  const uint64_t k = g - c.block_begin;
+  const uint64_t at = scan[g] - scan[c.block_begin];
+  c.index[k] = (uint32_t)at;
+  if (k + 1 == c.nblocks) c.index[k + 1] = (uint32_t)(at + c.offs[g]);
 }
 
 template <int D, class S> void enc(const ZfpChunk* d, uint32_t n, uint64_t total, hipStream_t s) {
@@ -1132,6 +1155,14 @@ void launch_zfp_index(const ZfpChunk* d_chunks, uint32_t nchunks, uint32_t dims,
                       hipStream_t stream) {
   if (nchunks == 0) return;
   NDIC_ZFP_DISPATCH(idx, d_chunks, nchunks, stream);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_zfp_index_write(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, const uint64_t* d_scan,
+                            hipStream_t stream) {
+  if (total == 0) return;
+  hipLaunchKernelGGL(zfp_index_write_kernel, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, stream, d_chunks,
+                     nchunks, total, d_scan);
   NDIC_HIP(hipGetLastError());
 }
 

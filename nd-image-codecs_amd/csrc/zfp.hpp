@@ -37,6 +37,11 @@ struct ZfpChunk {
   uint8_t* dst;           // encode: stream words (zeroed); decode: samples
   uint64_t* offs;         // variable modes: per-block bit lengths (encode) / bit offsets (decode)
   uint64_t* slots;        // encode, variable modes: per-block scratch
+  // Block index of a variable-size stream (nblocks + 1 bit offsets from the
+  // first block bit; the last entry is the end of the last block): encode
+  // writes it, decode seeks with it instead of walking the stream.
+  uint32_t* index;
+  uint32_t* bad;          // decode with an index: set when a block's bits disagree with it
 };
 
 // One brick (block) decoded into its own clipped, C-ordered output.
@@ -79,6 +84,9 @@ void launch_zfp_index(const ZfpChunk* d_chunks, uint32_t nchunks, uint32_t dims,
                       hipStream_t stream);
 void launch_zfp_decode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, uint32_t dims, uint32_t scalar,
                        const ZfpBrick* d_bricks, hipStream_t stream);
+// Variable-size modes, after the scan: writes each indexed chunk's block index.
+void launch_zfp_index_write(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, const uint64_t* d_scan,
+                            hipStream_t stream);
 
 // Host plan of a brick batch (NdZfpBrickDecoder::decode_brick, chunk.rs:484,
 // over BrickReader::fixed_rate's checks, lib.rs:509-599): per descriptor a

@@ -219,6 +219,17 @@ int orc_zfp_decode_brick(int chunk, const uint8_t* in, size_t len, const size_t*
   });
 }
 
+int orc_zfp_block_starts(const uint8_t* in, size_t len, const size_t* shape, size_t ndim, int type, int kind,
+                         double value, uint8_t** out, size_t* out_len, orc_err* err) {
+  return guard(err, [&] {
+    std::vector<uint64_t> v =
+        orc::zfp::block_starts(in, len, type, std::vector<size_t>(shape, shape + ndim), zfp_mode_of(kind, value));
+    std::vector<uint8_t> b(v.size() * 8);
+    std::memcpy(b.data(), v.data(), b.size());
+    emit(b, out, out_len);
+  });
+}
+
 void orc_zfp_perm(int dims, uint8_t* out) {
   const uint8_t* p = orc::zfp::perm_of(dims);
   std::memcpy(out, p, (size_t)1 << (2 * dims));

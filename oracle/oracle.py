@@ -71,6 +71,7 @@ def _declare(L) -> None:
         "orc_zfp_decode": [I, C.c_char_p, S, P, S, I, I, C.c_double, P, P, E],
         "orc_zfp_perm": [I, P],
         "orc_zfp_brick_index": [P, S, I, C.c_double, P, P, P, P, E],
+        "orc_zfp_block_starts": [C.c_char_p, S, P, S, I, I, C.c_double, P, P, E],
         "orc_zfp_decode_brick": [I, C.c_char_p, S, P, S, I, C.c_double, P, P, P, P, E],
     }
     for name, args in sig.items():
@@ -342,3 +343,16 @@ def zfp_decode_brick(stream: bytes, shape, dtype: str, rate: float, brick):
     _check(lib().orc_zfp_decode_brick(1, stream, len(stream), sh, nd, DTYPES[dtype], float(rate), br, C.byref(out),
                                       C.byref(n), bs, C.byref(err)), err)
     return _take(out, n.value), [int(bs[a]) for a in range(len(shape))]
+
+
+def zfp_block_starts(stream: bytes, shape, np_dtype, config: dict) -> np.ndarray:
+    """Absolute bit position of each block of a raw ZFP stream, then the end
+    of the last block (uint64)."""
+    t = {np.dtype(np.int32): 1, np.dtype(np.int64): 2, np.dtype(np.float32): 3, np.dtype(np.float64): 4}[
+        np.dtype(np_dtype)]
+    sh, nd = _shape(shape)
+    kind, value = zfp_mode(config)
+    out, n, err = C.c_void_p(), C.c_size_t(), _Err()
+    _check(lib().orc_zfp_block_starts(stream, len(stream), sh, nd, t, kind, value, C.byref(out), C.byref(n),
+                                      C.byref(err)), err)
+    return np.frombuffer(_take(out, n.value), np.uint64).copy()

@@ -739,6 +739,42 @@ void decompress(const uint8_t* bytes, size_t len, int type, const std::vector<si
   }
 }
 
+// Where each block of a stream begins (absolute bit positions, found by
+// walking the blocks the way decompress does), with the end of the last
+// block as a final entry: what a block index of the stream records.
+std::vector<uint64_t> block_starts(const uint8_t* bytes, size_t len, int type, const std::vector<size_t>& shape,
+                                   const Mode& mode) {
+  if (shape.empty() || shape.size() > 4) fail(kInvalidArgument, "nd_zfp: shape must have 1..=4 dimensions");
+  int dims = (int)shape.size();
+  uint32_t nd[4];
+  rev_dims(shape, nd);
+  Cfg c = cfg_of(mode, type, dims);
+  BitReader r(bytes, len);
+  if (r.bits(8) != 'z' || r.bits(8) != 'f' || r.bits(8) != 'p' || r.bits(8) != 5)
+    fail(kCodestream, "nd_zfp: bad ZFP magic");
+  if (r.bits(52) != field_meta(type, dims, nd)) fail(kCodestream, "nd_zfp: stream header does not declare the expected field");
+  uint64_t got = r.bits(12);
+  if (got == 0xfff) got |= r.bits(52) << 12;
+  uint64_t blocks = 1;
+  for (int a = 0; a < dims; ++a) blocks *= (nd[a] + 3) / 4;
+  std::vector<uint64_t> out;
+  auto walk = [&](auto* q) {
+    using S = std::remove_pointer_t<decltype(q)>;
+    for (uint64_t k = 0; k < blocks; ++k) {
+      out.push_back(r.pos);
+      decompress_block_t<S>(r, c, q, dims);
+    }
+    out.push_back(r.pos);
+  };
+  switch (type) {
+    case kInt32: { int32_t q[256]; walk(q); break; }
+    case kInt64: { int64_t q[256]; walk(q); break; }
+    case kFloat: { float q[256]; walk(q); break; }
+    default: { double q[256]; walk(q); break; }
+  }
+  return out;
+}
+
 // lib.rs:386-500 BrickIndex::fixed_rate: brick k spans bits
 // [header_bits + k * bits_per_brick, + bits_per_brick); the stream is padded
 // to a 64-bit word.

@@ -46,6 +46,9 @@ std::vector<uint8_t> decompress_brick(const uint8_t* bytes, size_t len, int type
 std::vector<uint8_t> decode_chunk_brick(const uint8_t* bytes, size_t len, const std::vector<size_t>& shape, int dtype,
                                         double rate, const std::vector<size_t>& brick,
                                         std::vector<size_t>* brick_shape);
+// Absolute bit position of every block of a stream, then the end of the last.
+std::vector<uint64_t> block_starts(const uint8_t* bytes, size_t len, int type, const std::vector<size_t>& shape,
+                                   const Mode& mode);
 // Order of the 4^d block coefficients (d = 1..4).
 const uint8_t* perm_of(int dims);
 

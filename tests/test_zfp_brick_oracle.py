@@ -129,3 +129,23 @@ def test_brick_reader_refusals(oracle):
     bad[0] ^= 0xFF
     with pytest.raises(oracle.OracleError):  # magic
         oracle.zfp_decode_brick(bytes(bad), shape, "f32", 8.0, [0, 0])
+
+
+@pytest.mark.parametrize("shape,dtype,rate", CASES[:4])
+def test_block_starts_walk(oracle, shape, dtype, rate):
+    """The oracle's block walk (what a block index records): at fixed rate it
+    reproduces the computed addressing header_bits + k * bits_per_brick; in
+    every mode it starts at the header's end, increases, and ends where the
+    stream's 64-bit padding begins."""
+    data = smooth(shape, dtype, seed=5)
+    idx = oracle.zfp_brick_index(shape, dtype, rate)
+    stream, hb = oracle.zfp_compress(data, {"mode": "fixed_rate", "rate": rate})
+    starts = oracle.zfp_block_starts(stream, shape, dtype, {"mode": "fixed_rate", "rate": rate})
+    n = int(np.prod(idx["grid"]))
+    assert list(starts) == [hb + k * idx["bits_per_brick"] for k in range(n + 1)]
+    for cfg in ({"mode": "reversible"}, {"mode": "fixed_precision", "precision": 17}):
+        stream, hb = oracle.zfp_compress(data, cfg)
+        starts = oracle.zfp_block_starts(stream, shape, dtype, cfg)
+        assert starts[0] == hb and len(starts) == n + 1
+        assert np.all(np.diff(starts.astype(np.int64)) > 0)
+        assert len(stream) == (int(starts[-1]) + 63) // 64 * 8
