@@ -70,8 +70,10 @@ def draw_specs(dev, rng, workload: str, n: int):
     return specs
 
 
-def make_workload(dev, ndic, args):
-    rng = np.random.default_rng(args.seed)
+def make_workload(dev, ndic, args, seed=None, compact=False):
+    """Generates a batch on the GPU and encodes it there.  compact: move the
+    encoded chunks into an arena of exactly their size (and drop the samples)."""
+    rng = np.random.default_rng(args.seed if seed is None else seed)
     specs = draw_specs(dev, rng, args.workload, args.chunks)
     sizes = [int(np.prod(s[0])) * np.dtype(s[1]).itemsize for s in specs]
     raw_total = sum(sizes)
@@ -88,8 +90,9 @@ def make_workload(dev, ndic, args):
         off += (nb + 15) // 16 * 16
     dev.synchronize()
     log(f"generated in {time.time() - t0:.1f}s")
-    # encode on the GPU
-    cap = int(raw_total * 2.5) + (256 << 20)
+    # encode on the GPU (blob chunks compress to ~0.7, uniform-random ones
+    # grow by a few per cent)
+    cap = int(raw_total * 1.3) + (256 << 20)
     d_enc = dev.DeviceBuffer(cap)
     edescs = (ndic.ChunkDesc * len(specs))(*[
         ndic.make_desc(s[0], s[1], s[2], s[3], offsets[i], 0, 0) for i, s in enumerate(specs)])
@@ -98,72 +101,163 @@ def make_workload(dev, ndic, args):
     dev.synchronize()
     enc_s = time.time() - t0
     comp_total = sum(d.out_len for d in edescs)
+    if any(d.status for d in edescs):
+        raise RuntimeError("workload encode failed for some chunks")
     log(f"encoded in {enc_s:.1f}s: {comp_total / 1e9:.3f} GB compressed ({comp_total / raw_total:.3f} of raw)")
+    if compact:
+        # content hashes of the samples, then the chunks packed tightly and the samples dropped
+        hashes = dev.DeviceBuffer(8 * len(specs), zero=True)
+        for i, (shape, dt) in enumerate((s_[0], s_[1]) for s_ in specs):
+            dev.hash_async(d_raw.ptr + offsets[i], int(np.prod(shape)) * np.dtype(dt).itemsize, hashes.ptr + 8 * i)
+        dev.synchronize()
+        raw_hash = [int(v) for v in np.frombuffer(hashes.download(), np.uint64)]
+        packed = dev.DeviceBuffer(comp_total + 256 * len(specs))
+        o, new_off = 0, []
+        for d in edescs:
+            dev.memcpy_d2d(packed.ptr + o, d_enc.ptr + d.out_offset, d.out_len)
+            new_off.append(o)
+            o += (d.out_len + 255) // 256 * 256
+        dev.synchronize()
+        del d_enc, d_raw
+        ddescs = (ndic.ChunkDesc * len(specs))(*[
+            ndic.make_desc(s[0], s[1], s[2], s[3], new_off[i], edescs[i].out_len, 0) for i, s in enumerate(specs)])
+        return specs, raw_hash, packed, ddescs, raw_total, comp_total
     ddescs = (ndic.ChunkDesc * len(specs))(*[
         ndic.make_desc(s[0], s[1], s[2], s[3], edescs[i].out_offset, edescs[i].out_len, offsets[i])
         for i, s in enumerate(specs)])
     return specs, d_raw, d_enc, ddescs, raw_total, comp_total, enc_s, padded
 
 
-def serve_bench(ndic, dev, d_raw, d_enc, ddescs, specs, args):
-    """The serving loop over the encoded batch kept in HBM: Poisson arrivals
-    at each rate with the 60/25/10/5 thumbnail/decode/brick/encode mix.  Full
-    decodes are checked against the original samples after each pass."""
+def serve_store(ndic, dev, d_enc, ddescs, specs, raw_hashes, args):
+    """The serving store kept in HBM: the benchmark batch plus two more
+    batches of the same generator (other seeds), and fixed-rate ZFP fields of
+    the C3 generators; returns (store, per-chunk sample hashes or device raw
+    offsets, ZFP stream bytes by field, store bytes)."""
     serve = importlib.import_module("nd-image-codecs_amd.serve")
     store = serve.Store()
-    offs = [d.out_offset for d in ddescs]  # decode descriptors: out_offset = sample offset
-    store.add_htj2k_arena(d_enc, [type("D", (), {"out_offset": d.in_offset, "out_len": d.in_len})() for d in ddescs],
-                          [s_[:4] for s_ in specs], d_raw, offs)
-    # ZFP: the f32 64x256x256 sine field at rates 8 and 16 (fixed rate: brick index)
+    arenas, meta = [d_enc], []
+    for i, d in enumerate(ddescs):
+        meta.append((0, d.in_offset, d.in_len, specs[i][:4], raw_hashes[i]))
+    for extra in range(args.serve_batches - 1):
+        sp, hashes, packed, dd, _, comp = make_workload(dev, ndic, args, seed=args.seed + 1000 + extra, compact=True)
+        arenas.append(packed)
+        for i, d in enumerate(dd):
+            meta.append((len(arenas) - 1, d.in_offset, d.in_len, sp[i][:4], hashes[i]))
+        log(f"serving store: batch {extra + 2}: {comp / 1e9:.2f} GB compressed")
+    # one arena for the lanes: the batches side by side
+    total = sum(a.nbytes for a in arenas)
+    base = [0]
+    for a in arenas[:-1]:
+        base.append(base[-1] + a.nbytes)
+    ht = dev.DeviceBuffer(total)
+    for a, b0 in zip(arenas, base):
+        dev.memcpy_d2d(ht.ptr + b0, a.ptr, a.nbytes)
+    del arenas[1:]
+    comp_bytes = sum(m[2] for m in meta)
+    store.add_htj2k_arena(ht, [type("D", (), {"out_offset": base[m[0]] + m[1], "out_len": m[2]})() for m in meta],
+                          [m[3] for m in meta])
+    checks = [m[4] for m in meta]
+    # ZFP: C3 fields (f32 64x256x256, f64 4x64x128x128, several seeds) at rates 8 and 16
     rng = np.random.default_rng(3)
-    z, y, x = np.meshgrid(np.arange(64), np.arange(256), np.arange(256), indexing="ij")
-    fld = np.zeros((64, 256, 256), np.float64)
-    for _ in range(6):
-        f = rng.uniform(-0.1, 0.1, 3)
-        fld += np.sin(2 * np.pi * (f[0] * z + f[1] * y + f[2] * x) + rng.uniform(0, 6.28))
-    fld = (fld + rng.normal(0, 1e-3, fld.shape)).astype(np.float32)
-    d_fld = dev.DeviceBuffer.from_bytes(fld.tobytes())
-    zdescs = [ndic.make_zfp_desc(fld.shape, "float32", ndic.zfp_config("fixed_rate", rate=r), 0, fld.nbytes)
-              for r in (8.0, 16.0)]
-    zcap = 3 * fld.nbytes
-    d_z = dev.DeviceBuffer(zcap, zero=True)
-    zarr = ndic.zfp_encode_batch(d_fld.ptr, d_z.ptr, zcap, zdescs)
-    store.add_zfp_arena(d_z, [serve.ZfpField(d.out_offset, d.out_len, fld.shape, "float32", r)
-                              for d, r in zip(zarr, (8.0, 16.0))])
-    # encode requests: 8x1x32x256x256 u16 chunks, lift53 along t and haar along z, 5 DWT levels
-    src = dev.DeviceBuffer(8 * 32 * 256 * 256 * 2)
-    dev.synth_chunk(src.ptr, (256, 256, 256), "uint16", dev.SYNTH_BLOBS,
-                    [(float(a), float(b), float(c), 30000.0, 12.0) for a, b, c in rng.uniform(0, 256, (8, 3))], 7)
-    enc_src = [(src, (8, 1, 32, 256, 256), "uint16",
-                [{"dimension": 0, "kind": "lift53", "levels": 2, "group": 0},
-                 {"dimension": 2, "kind": "haar", "levels": 2, "group": 0}], 5)]
-    server = serve.Server(store, out_budget=8 << 30, encode_sources=enc_src)
-    d_bad = dev.DeviceBuffer(256, zero=True)
+    fields = [sine_field(rng, (64, 256, 256)).astype(np.float32) for _ in range(4)] + \
+             [sine_field(rng, (4, 64, 128, 128), drift_axis=0) for _ in range(2)]
+    zstreams, zfields, zoff = [], [], 0
+    for f in fields:
+        d_f = dev.DeviceBuffer.from_bytes(f.tobytes())
+        for r in (8.0, 16.0):
+            cap = f.nbytes + (1 << 20)
+            d_z = dev.DeviceBuffer(cap, zero=True)
+            arr = ndic.zfp_encode_batch(d_f.ptr, d_z.ptr, cap, [
+                ndic.make_zfp_desc(f.shape, f.dtype.name, ndic.zfp_config("fixed_rate", rate=r), 0, f.nbytes)])
+            zstreams.append(d_z.download(arr[0].out_offset, arr[0].out_len))
+            zfields.append(serve.ZfpField(zoff, arr[0].out_len, f.shape, f.dtype.name, r))
+            zoff += (arr[0].out_len + 255) // 256 * 256
+    zblob = b"".join(z + bytes((-len(z)) % 256) for z in zstreams)
+    store.add_zfp_arena(dev.DeviceBuffer.from_bytes(zblob), zfields)
+    return store, checks, zstreams, comp_bytes + sum(len(z) for z in zstreams)
 
-    def verify(srv, batch):
-        # queued behind the pass on the serving thread's stream; read at the end
-        for r in batch:
-            if r.kind == serve.DECODE:
-                c = store.ht[r.key]
-                n = int(np.prod(c.shape)) * np.dtype(c.dtype).itemsize
-                dev.count_mismatch_async(srv.result_ptr(r), d_raw.ptr + c.raw_offset, n, d_bad.ptr)
 
-    out = {"mix": "60% thumbnail (plane 0, resolution 3, index prefix) / 25% full decode / 10% ZFP 4^3 brick "
-                  "(f32 64x256x256, rate 8|16) / 5% encode (u16 8x1x32x256x256)",
-           "store_bytes": d_enc.nbytes + d_z.nbytes, "rates": {}}
-    # warm-up: every lane's library context and device buffers exist before timing
-    server.run(serve.poisson_requests(np.random.default_rng(9), 400.0, 0.25, store, n_encode_sources=1),
-               verify=verify)
-    for rate in [float(v) for v in args.serve_rates.split(",") if v]:
-        reqs = serve.poisson_requests(np.random.default_rng(int(rate)), rate, args.serve_seconds, store,
-                                      n_encode_sources=1)
+def serve_bench(ndic, dev, d_raw, store, checks, zstreams, store_bytes, expected, args):
+    """The serving loop over the HBM-resident store: Poisson arrivals at each
+    rate with the 60/25/10/5 thumbnail/decode/brick/encode mix.  Every
+    response is hashed on the device after its pass and compared at the end
+    with the expected output: full decodes with the original samples
+    (generated on the device), thumbnails, bricks and encodes with the
+    oracle's outputs (computed in the CPU leg)."""
+    serve = importlib.import_module("nd-image-codecs_amd.serve")
+    enc_src = expected["encode_source"]
+    server = serve.Server(store, out_budget=8 << 30, encode_sources=[enc_src])
+    out = {"mix": "60% thumbnail (plane 0, resolution 3, index prefix) / 25% full decode / 10% ZFP 4^d brick "
+                  "(f32 64x256x256 and f64 4x64x128x128, rate 8|16) / 5% encode (u16 8x1x32x256x256 series)",
+           "store_bytes": store_bytes, "store_chunks": len(store.ht), "store_zfp_fields": len(store.zfp),
+           "rates": {}}
+    bad = {"decode": 0, "thumbnail": 0, "brick": 0, "encode": 0}
+    for rate in [None] + [float(v) for v in args.serve_rates.split(",") if v]:
+        reqs = expected["requests"][rate]
+        slots = dev.DeviceBuffer(8 * max(1, len(reqs)), zero=True)
+        slot_of = {id(r): k for k, r in enumerate(reqs)}
+
+        def verify(srv, batch):
+            # queued behind the pass on the serving thread's stream; read at the end
+            for r in batch:
+                dev.hash_async(srv.result_ptr(r), srv.out_bytes(r), slots.ptr + 8 * slot_of[id(r)])
+
         st = server.run(reqs, verify=verify)
+        dev.synchronize()
+        got = np.frombuffer(slots.download(), np.uint64)
+        for k, r in enumerate(reqs):
+            want = expected["hash"](r)
+            if r.status != 0 or int(got[k]) != want or (r.kind == serve.ENCODE and r.out_len != expected["encode_len"]):
+                bad[r.kind] += 1
+        if rate is None:
+            continue  # warm-up: every lane's library context and device buffers exist before timing
         summ = st.summary()
         out["rates"][str(int(rate))] = summ
         log(f"serving {rate:.0f} req/s: {json.dumps(summ)}")
-    dev.synchronize()
-    out["decode_responses_bit_exact"] = int(np.frombuffer(d_bad.download(0, 8), np.uint64)[0]) == 0
+    out["responses_checked"] = {k: v == 0 for k, v in bad.items()}
+    out["mismatches"] = bad
+    out["all_responses_bit_exact"] = not any(bad.values())
     return out
+
+
+def single_chunk_lines(ndic, dev, args):
+    """One chunk per call, device-resident, at the two single-chunk shapes:
+    the u16 32x256x256 chunk (nd_lift 5/3 along z) and the u16
+    8x1x32x512x512 series chunk (transposed to c,t,z,y,x; 5/3 along t, haar
+    along z); both with 5 DWT levels.  Median of `reps` encodes and decodes,
+    each checked bit-exact against the samples."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import datagen
+
+    out = {}
+    cases = [("u16 32x256x256", datagen.c0_volume((32, 256, 256), seed=0),
+              [{"dimension": 0, "kind": "lift53", "levels": 2, "group": 0}]),
+             ("u16 8x1x32x512x512 series", np.ascontiguousarray(datagen.series_5d(seed=1).transpose(1, 0, 2, 3, 4)),
+              [{"dimension": 1, "kind": "lift53", "levels": 2, "group": 0},
+               {"dimension": 2, "kind": "haar", "levels": 2, "group": 0}])]
+    for name, vol, steps in cases:
+        d_raw = dev.DeviceBuffer.from_bytes(vol.tobytes())
+        cap = 2 * vol.nbytes + (16 << 20)
+        d_enc = dev.DeviceBuffer(cap)
+        d_out = dev.DeviceBuffer(vol.nbytes)
+        te, td = [], []
+        for _ in range(args.single_reps):
+            e = (ndic.ChunkDesc * 1)(ndic.make_desc(vol.shape, "uint16", steps, 5, 0, vol.nbytes, 0))
+            dev.synchronize()
+            t0 = time.perf_counter()
+            ndic.encode_batch(d_raw.ptr, d_enc.ptr, cap, e)
+            te.append(time.perf_counter() - t0)
+            dd = [ndic.make_desc(vol.shape, "uint16", steps, 5, e[0].out_offset, e[0].out_len, 0)]
+            t0 = time.perf_counter()
+            ndic.decode_batch(d_enc.ptr, d_out.ptr, dd)
+            td.append(time.perf_counter() - t0)
+        exact = dev.count_mismatch(d_out.ptr, d_raw.ptr, vol.nbytes) == 0
+        me, md = sorted(te)[len(te) // 2], sorted(td)[len(td) // 2]
+        out[name] = {"encode_ms": round(me * 1e3, 3), "decode_ms": round(md * 1e3, 3),
+                     "encode_GBps": round(vol.nbytes / me / 1e9, 2), "decode_GBps": round(vol.nbytes / md / 1e9, 2),
+                     "compressed_bytes": int(e[0].out_len), "bit_exact": exact, "reps": args.single_reps}
+        log(f"single chunk {name}: {out[name]}")
+    return out, cases
 
 
 def sine_field(rng, shape, drift_axis=None):
@@ -181,46 +275,64 @@ def sine_field(rng, shape, drift_axis=None):
 
 def zfp_bench(ndic, dev, args):
     """ZFP at the two field shapes (f32 64x256x256, f64 4x64x128x128), fixed
-    rate 8 / 16 and reversible: device-resident encode and decode rates, the
-    oracle on one core beside them, and a batch of 2,048 independent bricks
-    of 16^3..64^3 samples."""
+    rate 8 / 16, reversible, fixed accuracy and fixed precision:
+    device-resident encode and decode rates (median of 5), the variable-size
+    modes decoded through their block index and, once, by walking the
+    stream; the oracle on one core beside them, and a batch of 2,048
+    independent fields of 16^3..64^3 samples."""
     sys.path.insert(0, str(ROOT / "oracle"))
     rng = np.random.default_rng(4)
     fields = [("f32 64x256x256", sine_field(rng, (64, 256, 256)).astype(np.float32)),
               ("f64 4x64x128x128", sine_field(rng, (4, 64, 128, 128), drift_axis=0))]
     modes = [("rate8", ndic.zfp_config("fixed_rate", rate=8.0)), ("rate16", ndic.zfp_config("fixed_rate", rate=16.0)),
-             ("reversible", ndic.zfp_config("reversible"))]
+             ("reversible", ndic.zfp_config("reversible")),
+             ("fixed_accuracy 1e-3", ndic.zfp_config("fixed_accuracy", tolerance=1e-3)),
+             ("fixed_precision 20", ndic.zfp_config("fixed_precision", precision=20))]
     out = {}
+
+    def timed(fn, reps):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2]
+
     for name, fld in fields:
         d_in = dev.DeviceBuffer.from_bytes(fld.tobytes())
         cap = 3 * fld.nbytes + (1 << 20)
         d_enc = dev.DeviceBuffer(cap, zero=True)
+        icap = 8 << 20
+        d_idx = dev.DeviceBuffer(icap, zero=True)
         d_out = dev.DeviceBuffer(fld.nbytes)
         for mname, cfg in modes:
             desc = [ndic.make_zfp_desc(fld.shape, fld.dtype.name, cfg, 0, fld.nbytes)]
-            arr = ndic.zfp_encode_batch(d_in.ptr, d_enc.ptr, cap, desc)  # warm-up
-            reps = 5
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                arr = ndic.zfp_encode_batch(d_in.ptr, d_enc.ptr, cap, desc)
-            enc_s = (time.perf_counter() - t0) / reps
+            arr = ndic.zfp_encode_batch(d_in.ptr, d_enc.ptr, cap, desc, d_index_ptr=d_idx.ptr, index_capacity=icap)
+            enc_s = timed(lambda: ndic.zfp_encode_batch(d_in.ptr, d_enc.ptr, cap, desc, d_index_ptr=d_idx.ptr,
+                                                        index_capacity=icap), 5)
             clen = arr[0].out_len
-            ddesc = [ndic.make_zfp_desc(fld.shape, fld.dtype.name, cfg, arr[0].out_offset, clen, 0)]
-            ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, ddesc)
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, ddesc)
-            dec_s = (time.perf_counter() - t0) / reps
-            exact = None
+            rec = {"encode_GBps": round(fld.nbytes / enc_s / 1e9, 2), "compressed_bytes": clen}
+            variable = arr[0].index_len > 0
+            walk = [ndic.make_zfp_desc(fld.shape, fld.dtype.name, cfg, arr[0].out_offset, clen, 0)]
+            if variable:
+                idx = [ndic.make_zfp_desc(fld.shape, fld.dtype.name, cfg, arr[0].out_offset, clen, 0,
+                                          arr[0].index_offset, arr[0].index_len)]
+                dec_s = timed(lambda: ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, idx, d_index_ptr=d_idx.ptr), 5)
+                walk_s = timed(lambda: ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, walk), 1)
+                rec.update({"decode_GBps": round(fld.nbytes / dec_s / 1e9, 2), "decode_ms": round(dec_s * 1e3, 3),
+                            "block_index_bytes": int(arr[0].index_len),
+                            "decode_without_index_GBps": round(fld.nbytes / walk_s / 1e9, 3),
+                            "decode_without_index_ms": round(walk_s * 1e3, 1)})
+            else:
+                dec_s = timed(lambda: ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, walk), 5)
+                rec.update({"decode_GBps": round(fld.nbytes / dec_s / 1e9, 2), "decode_ms": round(dec_s * 1e3, 3)})
+            rec["decode_algorithmic_GBps"] = round((fld.nbytes + clen) / dec_s / 1e9, 2)
             if mname == "reversible":
-                exact = dev.count_mismatch(d_out.ptr, d_in.ptr, fld.nbytes) == 0
-            out[f"{name} {mname}"] = {
-                "encode_GBps": round(fld.nbytes / enc_s / 1e9, 2), "decode_GBps": round(fld.nbytes / dec_s / 1e9, 2),
-                "decode_ms": round(dec_s * 1e3, 3), "compressed_bytes": clen,
-                "decode_algorithmic_GBps": round((fld.nbytes + clen) / dec_s / 1e9, 2),
-                **({"lossless_exact": exact} if exact is not None else {})}
-            log(f"zfp {name} {mname}: {out[f'{name} {mname}']}")
-        del d_in, d_enc, d_out
+                rec["lossless_exact"] = dev.count_mismatch(d_out.ptr, d_in.ptr, fld.nbytes) == 0
+            out[f"{name} {mname}"] = rec
+            log(f"zfp {name} {mname}: {rec}")
+        del d_in, d_enc, d_out, d_idx
     # 2,048 independent bricks (chunks) of 16^3..64^3 f32 at rate 8
     sides = rng.integers(16, 65, size=(2048, 3))
     sizes = [int(np.prod(sd)) * 4 for sd in sides]
@@ -353,22 +465,49 @@ def counters_of(kernel: str, args, avg_ms: float):
     return rec
 
 
-def cpu_baseline(d_enc, ddescs, specs, budget_s):
-    """The oracle (single-threaded C++ restatement of the reference path)
-    decoding chunks of the same workload, chunk-parallel over the host cores
-    (a thread per core, as zarrs decodes chunks concurrently), until the time
-    budget is spent; one thread alone for a third of the budget first."""
-    from concurrent.futures import ThreadPoolExecutor
+def host_cpu() -> dict:
+    model = "?"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "affinity": aff}
 
+
+# ---------------------------------------------------------------------------
+# The CPU leg: the oracle (the C++ restatement of the reference path, test
+# infrastructure) timed as the CPU baseline, and, as the checker, the
+# cross-checks of the device results and the expected serving outputs.
+# ---------------------------------------------------------------------------
+CODES = {"uint8": "u8", "uint16": "u16", "int16": "i16", "float32": "f32", "float64": "f64"}
+
+
+def _oracle():
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
+    return oracle
 
-    codes = {"uint8": "u8", "uint16": "u16", "int16": "i16"}
+
+def cpu_baseline(d_raw, d_enc, ddescs, specs, budget_s, single_cases):
+    """The oracle decoding chunks of the same workload, chunk-parallel over
+    the host cores the GPU box gives one GPU (16 threads: the box's CPU share,
+    whatever nproc reports), as zarrs decodes chunks concurrently, until the
+    time budget is spent; one thread alone for a third of the budget first.
+    Outside the timing, the decoded chunks of the sample are compared with the
+    original samples and the oracle's encode of some of them with the device's
+    bytes; the single-chunk u16 32x256x256 case is timed on one core."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    oracle = _oracle()
     order = [int(i) for i in np.random.default_rng(1).permutation(len(specs))]
 
     def one(i):
         shape, dt, steps = specs[i][:3]
-        return oracle.lift_ht_decode(chunks[i], shape, codes[dt], steps).nbytes
+        return oracle.lift_ht_decode(chunks[i], shape, CODES[dt], steps).nbytes
 
     # inputs come off the GPU before any timing
     chunks = {}
@@ -378,7 +517,6 @@ def cpu_baseline(d_enc, ddescs, specs, budget_s):
             if i not in chunks:
                 chunks[i] = d_enc.download(ddescs[i].in_offset, ddescs[i].in_len)
 
-    # one thread
     single_bytes, single_t, k = 0, 0.0, 0
     while single_t < budget_s / 3 and k < len(order):
         fetch([order[k]])
@@ -387,9 +525,8 @@ def cpu_baseline(d_enc, ddescs, specs, budget_s):
         single_t += time.perf_counter() - t0
         k += 1
     single = single_bytes / single_t / 1e9
-    # all cores: enough chunks for the rest of the budget at the expected rate
-    # (the sample wraps around the batch if the batch is smaller)
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cpu = host_cpu()
+    cores = max(1, min(16, cpu["affinity"] or 1))
     want = int((budget_s * 2 / 3) * single * 1e9 * cores / max(1, single_bytes / k)) + cores
     rest = [order[(k + j) % len(order)] for j in range(want)]
     fetch(rest)
@@ -397,11 +534,92 @@ def cpu_baseline(d_enc, ddescs, specs, budget_s):
     with ThreadPoolExecutor(max_workers=cores) as pool:
         par_bytes = sum(pool.map(one, rest))
     par_t = time.perf_counter() - t0
+
+    # cross-checks, untimed: oracle decode == original samples; oracle encode == device bytes
+    def check(i):
+        shape, dt, steps, lv = specs[i][:4]
+        raw = d_raw.download(ddescs[i].out_offset, int(np.prod(shape)) * np.dtype(dt).itemsize)
+        dec = oracle.lift_ht_decode(chunks[i], shape, CODES[dt], steps).tobytes() == raw
+        enc = oracle.lift_ht_encode(np.frombuffer(raw, dt).reshape(shape), CODES[dt], steps, xy_levels=lv) == chunks[i]
+        return dec, enc
+
+    sample = sorted(set(rest))[:48]
+    with ThreadPoolExecutor(max_workers=cores) as pool:
+        res = list(pool.map(check, sample))
+    # the u16 32x256x256 single chunk on one core (the shape the reference's CPU path is quoted at)
+    name, vol, steps = single_cases[0]
+    t0 = time.perf_counter()
+    c0 = oracle.lift_ht_encode(vol, "u16", steps, xy_levels=5)
+    te = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.lift_ht_decode(c0, vol.shape, "u16", steps)
+    td = time.perf_counter() - t0
     return {"value": round(par_bytes / par_t / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
-            "single_thread_GBps": round(single, 4),
+            "host_cpu": cpu, "single_thread_GBps": round(single, 4),
             "sample": f"{len(rest)} chunks of the same workload ({par_bytes / 1e6:.1f} MB decoded) by the C++ oracle, "
-                      f"chunk-parallel on {cores} threads, {par_t:.1f} s; one thread: {k} chunks, "
-                      f"{single_bytes / 1e6:.1f} MB, {single_t:.1f} s"}
+                      f"chunk-parallel on {cores} threads (the box's CPU share for one GPU), {par_t:.1f} s; one "
+                      f"thread: {k} chunks, {single_bytes / 1e6:.1f} MB, {single_t:.1f} s",
+            "cross_checks": {"chunks": len(sample), "oracle_decode_equals_samples": sum(r[0] for r in res),
+                             "oracle_encode_equals_device_bytes": sum(r[1] for r in res)},
+            "single_chunk_1_core": {name: {"encode_GBps": round(vol.nbytes / te / 1e9, 4),
+                                           "decode_GBps": round(vol.nbytes / td / 1e9, 4)}}}
+
+
+def serving_expectations(ndic, dev, store, checks, zstreams, args):
+    """The serving runs' requests (seeded Poisson arrivals per rate, a short
+    warm-up first) and each response's expected content hash: full decodes
+    hash like the original samples, thumbnails like the oracle's decode of
+    the plane-0 prefix at the resolution, bricks like the oracle's brick
+    reader, encodes like the oracle's chunk bytes."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import datagen
+
+    serve = importlib.import_module("nd-image-codecs_amd.serve")
+    oracle = _oracle()
+    series = datagen.series_5d((8, 1, 32, 256, 256), seed=7)
+    enc_vol = np.ascontiguousarray(series.transpose(1, 0, 2, 3, 4))
+    steps = [{"dimension": 1, "kind": "lift53", "levels": 2, "group": 0},
+             {"dimension": 2, "kind": "haar", "levels": 2, "group": 0}]
+    enc_bytes = oracle.lift_ht_encode(enc_vol, "u16", steps, xy_levels=5)
+    enc_src = (dev.DeviceBuffer.from_bytes(enc_vol.tobytes()), enc_vol.shape, "uint16", steps, 5)
+    reqs = {None: serve.poisson_requests(np.random.default_rng(9), 400.0, 0.25, store, n_encode_sources=1)}
+    for rate in [float(v) for v in args.serve_rates.split(",") if v]:
+        reqs[rate] = serve.poisson_requests(np.random.default_rng(int(rate)), rate, args.serve_seconds, store,
+                                            n_encode_sources=1)
+    thumbs = {(r.key, r.res) for rs in reqs.values() for r in rs if r.kind == serve.THUMB}
+    bricks = {(r.key, r.brick) for rs in reqs.values() for r in rs if r.kind == serve.BRICK}
+
+    def thumb(kr):
+        key, res = kr
+        c = store.ht[key]
+        off, ln, pre = c.plane0
+        rr = min(res, c.levels)
+        prefix = store.ht_arena.download(c.offset + off, pre[rr])
+        return kr, dev.host_hash(oracle.decode_codestream(prefix, rr, prefix=True).tobytes())
+
+    def brick(kb):
+        key, b = kb
+        f = store.zfp[key]
+        got, _ = oracle.zfp_decode_brick(zstreams[key], f.shape, CODES[f.dtype], f.rate, b)
+        return kb, dev.host_hash(got)
+
+    with ThreadPoolExecutor(max_workers=16) as pool:
+        th = dict(pool.map(thumb, sorted(thumbs)))
+        bh = dict(pool.map(brick, sorted(bricks)))
+    enc_hash = dev.host_hash(enc_bytes)
+
+    def expected_hash(r):
+        if r.kind == serve.THUMB:
+            return th[(r.key, r.res)]
+        if r.kind == serve.BRICK:
+            return bh[(r.key, r.brick)]
+        if r.kind == serve.ENCODE:
+            return enc_hash
+        return checks[r.key]
+
+    return {"requests": reqs, "hash": expected_hash, "encode_source": enc_src, "encode_len": len(enc_bytes)}
 
 
 def main():
@@ -418,6 +636,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="seconds of Poisson arrivals per serving rate (0: no serving run)")
     ap.add_argument("--serve-rates", default="500,2000,5000")
+    ap.add_argument("--serve-batches", type=int, default=3,
+                    help="batches of the uneven-chunk generator in the serving store (~21.7 GB each)")
+    ap.add_argument("--single-reps", type=int, default=5, help="repetitions of each single-chunk line")
     ap.add_argument("--no-zfp", dest="zfp", action="store_false", help="skip the ZFP section")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the library's built-in launch parameters instead of tuning them first")
@@ -471,11 +692,15 @@ def main():
         exact = exact_now()
         log(f"bit-exact after warmup: {exact}")
     t0 = time.perf_counter()
+    step_s = []
     for _ in range(args.steps):
-        run()
+        ts = time.perf_counter()
+        run()  # returns with the batch decoded
+        step_s.append(time.perf_counter() - ts)
     dev.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     gbps = raw_total / dt / 1e9
+    median_gbps = raw_total / sorted(step_s)[len(step_s) // 2] / 1e9
     exact &= exact_now()
     log(f"batched decode: {dt * 1e3:.1f} ms/step, {gbps:.2f} GB/s, bit-exact {exact}")
     # plain path
@@ -515,13 +740,25 @@ def main():
     entropy = {k: ht_rec.get(k) for k in ("occupancy_waves_per_simd", "occupancy_pct", "valu_busy_pct",
                                           "valu_lane_util_pct", "valu_insts_per_wave")} if ht_rec else None
     zfp = zfp_bench(ndic, dev, args) if args.zfp else None
-    serving = None
+    single, single_cases = single_chunk_lines(ndic, dev, args)
+    del d_out
+    serving = store = None
     if args.serve_seconds > 0 and args.workload == "c2":
-        serving = serve_bench(ndic, dev, d_raw, d_enc, ddescs, specs, args)
+        # the original samples' hashes, for checking served full decodes
+        hs = dev.DeviceBuffer(8 * len(specs), zero=True)
+        for i, d in enumerate(ddescs):
+            dev.hash_async(d_raw.ptr + d.out_offset, int(np.prod(specs[i][0])) * np.dtype(specs[i][1]).itemsize,
+                           hs.ptr + 8 * i)
+        dev.synchronize()
+        raw_hashes = [int(v) for v in np.frombuffer(hs.download(), np.uint64)]
+        store, checks, zstreams, store_bytes = serve_store(ndic, dev, d_enc, ddescs, specs, raw_hashes, args)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(d_enc, ddescs, specs, args.cpu_seconds)
+        cpu = cpu_baseline(d_raw, d_enc, ddescs, specs, args.cpu_seconds, single_cases)
         log("cpu baseline:", cpu)
+    if store is not None and not args.no_cpu:
+        expected = serving_expectations(ndic, dev, store, checks, zstreams, args)
+        serving = serve_bench(ndic, dev, d_raw, store, checks, zstreams, store_bytes, expected, args)
     line = {
         "metric": "decoded GB/s lossless uint16 nd-lift-ht (bit-exact) vs plain GPU; p50/p99 latency",
         "value": round(gbps, 3),
@@ -534,7 +771,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
-        "data": "synthetic (seeded blobs + Poisson-like noise, generated on the GPU)",
+        "data": ("synthetic, generated on the GPU (seeded): 1000 + 8 Gaussian blobs with Poisson noise by its normal "
+                 "approximation (lambda >= 1000), clipped to 12 bits and scaled to the dtype (u8 >> 4, u16 << 4, "
+                 "i16 (<< 4) - 32768); 5% all-zero and 5% uniform-random chunks"),
         "bit_exact": exact,
         "config": {
             "workload": ("c2: 4096 uneven chunks, y/x log-uniform 64..1024, z 1..64, u8:u16:i16 2:5:3, "
@@ -543,6 +782,7 @@ def main():
             "chunks": len(specs),
             "raw_bytes": raw_total,
             "compressed_bytes": comp_total,
+            "median_step_gbps": round(median_gbps, 3),
             "plain_gbps": round(plain_gbps, 3) if plain_gbps > 0 else None,
             "batched_over_plain": round(gbps / plain_gbps, 2) if plain_gbps > 0 else None,
             "encode_seconds": round(enc_s, 2),
@@ -564,6 +804,7 @@ def main():
         "autotune": tuning,
         "entropy_coder": entropy,
         "cpu_baseline": cpu,
+        "single_chunk": single,
         "serving": serving,
         "zfp": zfp,
     }

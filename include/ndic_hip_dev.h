@@ -81,6 +81,14 @@ ndic_status ndic_dev_count_mismatch(const void* a, const void* b, size_t n, uint
 ndic_status ndic_dev_count_mismatch_async(const void* a, const void* b, size_t n, uint64_t* d_count,
                                           ndic_error* err);
 
+/* Content hash of n device bytes added to *d_slot (8-byte aligned) on the
+ * calling thread's stream, without waiting: the sum mod 2^64 over the 64-bit
+ * little-endian words w_i (the last zero-padded) of mix64(w_i ^ i*K) plus
+ * mix64(n ^ K), K = 0x9e3779b97f4a7c15, mix64 the splitmix64 finaliser.
+ * The serving benchmark checks responses against the hashes of expected
+ * outputs computed on the host. */
+ndic_status ndic_dev_hash_async(const void* p, size_t n, uint64_t* d_slot, ndic_error* err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,31 @@ __global__ __launch_bounds__(256) void mismatch_kernel(const uint8_t* __restrict
   if (local) atomicAdd(count, (unsigned long long)local);
 }
 
+// Content hash of n bytes: the sum (mod 2^64) over the 64-bit little-endian
+// words w_i (the last one zero-padded) of mix64(w_i ^ (i * K)), plus
+// mix64(n ^ K); order-dependent through the word index.  Added into *slot.
+constexpr uint64_t kHashK = 0x9e3779b97f4a7c15ull;
+__global__ __launch_bounds__(256) void hash_kernel(const uint8_t* __restrict__ p, uint64_t n,
+                                                   unsigned long long* slot) {
+  const uint64_t nw = (n + 7) / 8, full = n / 8;
+  uint64_t local = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nw; i += stride) {
+    uint64_t w = 0;
+    if (i < full && ((uintptr_t)p & 7) == 0) {
+      w = ((const uint64_t*)p)[i];
+    } else if (i < full) {
+      for (int b = 0; b < 8; ++b) w |= (uint64_t)p[8 * i + b] << (8 * b);
+    } else {
+      for (uint64_t b = 8 * i; b < n; ++b) w |= (uint64_t)p[b] << (8 * (b - 8 * i));
+    }
+    local += mix64(w ^ (i * kHashK));
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) local += mix64(n ^ kHashK);
+  for (int o = 32; o >= 1; o >>= 1) local += __shfl_xor(local, o);
+  if ((threadIdx.x & 63) == 0 && local) atomicAdd(slot, (unsigned long long)local);
+}
+
 __global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
@@ -179,6 +204,17 @@ extern "C" ndic_status ndic_dev_count_mismatch_async(const void* a, const void* 
     std::lock_guard<std::recursive_mutex> lock(ctx.mu);
     hipLaunchKernelGGL(mismatch_kernel, dim3(grid_for(n / 16 + 1)), dim3(256), 0, ctx.stream, (const uint8_t*)a,
                        (const uint8_t*)b, (uint64_t)n, (unsigned long long*)d_count);
+    NDIC_HIP(hipGetLastError());
+  });
+}
+
+extern "C" ndic_status ndic_dev_hash_async(const void* p, size_t n, uint64_t* d_slot, ndic_error* err) {
+  return guard_call(err, [&] {
+    if ((uintptr_t)d_slot % 8) fail(NDIC_INVALID_ARGUMENT, "ndic_dev_hash_async needs an 8-byte aligned slot");
+    Context& ctx = Context::get();
+    std::lock_guard<std::recursive_mutex> lock(ctx.mu);
+    hipLaunchKernelGGL(hash_kernel, dim3(grid_for((n + 7) / 8 + 1)), dim3(256), 0, ctx.stream, (const uint8_t*)p,
+                       (uint64_t)n, (unsigned long long*)d_slot);
     NDIC_HIP(hipGetLastError());
   });
 }

@@ -42,6 +42,7 @@ def _lib():
             ("ndic_dev_count_mismatch", C.c_int, [P, P, S, C.POINTER(C.c_uint64), E]),
             ("ndic_dev_count_mismatch_async", C.c_int, [P, P, S, P, E]),
             ("ndic_dev_copy", C.c_int, [P, P, S, E]),
+            ("ndic_dev_hash_async", C.c_int, [P, S, P, E]),
             ("ndic_dev_profile", None, [C.c_int32]),
             ("ndic_dev_stage_times", C.c_size_t, [C.POINTER(C.c_double), C.POINTER(C.c_double), S]),
         ]:
@@ -107,6 +108,12 @@ class DeviceBuffer:
             pass
 
 
+def memcpy_d2d(dst: int, src: int, n: int) -> None:
+    """Device-to-device copy of n bytes (the copy engine; returns when done)."""
+    if n:
+        _call(_lib().ndic_dev_memcpy, dst, src, int(n), 2)
+
+
 def synchronize() -> None:
     _call(_lib().ndic_dev_synchronize)
 
@@ -138,6 +145,35 @@ def count_mismatch_async(a: int, b: int, n: int, d_count: int) -> None:
     """Adds the mismatch count of a and b to the device counter at d_count
     on this thread's stream, without waiting."""
     _call(_lib().ndic_dev_count_mismatch_async, a, b, int(n), d_count)
+
+
+def hash_async(ptr: int, n: int, d_slot: int) -> None:
+    """Adds the content hash of n device bytes at ptr to the uint64 at d_slot
+    (on this thread's stream, without waiting); see host_hash."""
+    _call(_lib().ndic_dev_hash_async, ptr, int(n), d_slot)
+
+
+_K = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix64(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        x = x + _K
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def host_hash(data) -> int:
+    """The content hash ndic_dev_hash_async computes, on the host."""
+    b = bytes(data)
+    n = len(b)
+    w = np.frombuffer(b + bytes((-n) % 8), np.uint64)
+    with np.errstate(over="ignore"):
+        idx = np.arange(len(w), dtype=np.uint64) * _K
+        total = int(_mix64(w ^ idx).sum(dtype=np.uint64)) if len(w) else 0
+        total += int(_mix64(np.array([n], np.uint64) ^ _K)[0])
+    return total & 0xFFFFFFFFFFFFFFFF
 
 
 def profile(enable: bool) -> None:

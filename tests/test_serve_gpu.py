@@ -205,3 +205,22 @@ def test_admission_splits_passes_by_memory(world):
     stats = server.run(reqs, verify=lambda srv, batch: outs.extend(srv.result(r) for r in batch))
     assert stats.passes == 3
     assert outs == [vols[big].tobytes()] * 3
+
+
+def test_device_hash_matches_host(world):
+    """The content hash the serving benchmark checks responses with: the
+    device kernel and its host restatement agree on every length (tails of
+    1..7 bytes, unaligned starts, empty)."""
+    serve, dev, store, vols, _ = world
+    rng = np.random.default_rng(6)
+    data = rng.integers(0, 256, size=20000, dtype=np.uint8).tobytes()
+    buf = dev.DeviceBuffer.from_bytes(data)
+    cases = [(0, 0), (0, 1), (0, 7), (0, 8), (0, 9), (3, 5), (5, 4096), (8, 19991), (0, 20000)]
+    slots = dev.DeviceBuffer(8 * len(cases), zero=True)
+    for k, (off, n) in enumerate(cases):
+        dev.hash_async(buf.ptr + off, n, slots.ptr + 8 * k)
+    dev.synchronize()
+    got = np.frombuffer(slots.download(), np.uint64)
+    for k, (off, n) in enumerate(cases):
+        assert int(got[k]) == dev.host_hash(data[off:off + n]), (off, n)
+    assert dev.host_hash(b"a") != dev.host_hash(b"a\0") and dev.host_hash(data[:64]) != dev.host_hash(data[8:72])
