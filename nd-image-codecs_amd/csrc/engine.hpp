@@ -46,7 +46,7 @@ struct Context {
   std::recursive_mutex mu;
   hipStream_t stream = nullptr;
   int device = 0;
-  DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, fusedjobs, winjobs, small;
+  DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, fusedjobs, winjobs, small, htrec;
   uint64_t* queues = nullptr;  // work-queue counter pairs of the persistent kernels (zeroed)
   // Counter pair k (0: HT decode, 1: HT encode) of this context's queues.
   uint64_t* queue(int k);

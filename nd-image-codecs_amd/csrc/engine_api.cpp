@@ -406,6 +406,7 @@ ndic_status ndic_dev_ht_decode_block(const uint8_t* coded, size_t len, uint32_t*
     t.flags = kTaskIncluded | kTaskSignMag | (stripe_causal ? kTaskCausal : 0);
     NDIC_HIP(hipMemcpyAsync(d_task, &t, sizeof(t), hipMemcpyHostToDevice, s));
     if (tune(kTuneHtDecodeWaveMaxBlocks) >= 1) launch_ht_decode_wave(d_task, 1, ctx.queue(0), s);
+    else if (tune(kTuneHtDecodeSplit)) launch_ht_decode_split(d_task, 1, (uint32_t*)ctx.htrec.ensure(kHtRecBytes), s);
     else launch_ht_decode(d_task, nullptr, 1, s);
     launch_ht_refine(d_task, 1, s);
     int32_t e = 0;

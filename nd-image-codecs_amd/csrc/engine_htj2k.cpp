@@ -235,7 +235,10 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   for (auto* it : items) nsel += it->plane_sel.size();
   const uint64_t sz_sel = al(nsel * 4);
   const bool lane_decode = (int64_t)ntasks > tune(kTuneHtDecodeWaveMaxBlocks);
-  const bool sort = lane_decode && tune(kTuneHtDecodeSort) != 0;
+  const bool split = lane_decode && tune(kTuneHtDecodeSplit) != 0;
+  const bool sort = lane_decode && !split && tune(kTuneHtDecodeSort) != 0;
+  // per-quad records of the two-pass decoder (allocated before any capture)
+  uint32_t* d_rec = split ? (uint32_t*)ctx.htrec.ensure(ntasks * kHtRecBytes + 256) : nullptr;
   const uint64_t sz_order = sort ? al(ntasks * 4) + al(4 * (kHtClasses + 1)) : 0;
   uint64_t meta = sz_chunks + sz_gp + sz_gb + sz_planes + sz_tasks + sz_tag + sz_err + sz_ptr + sz_dims + sz_flag + sz_sel +
                   sz_order;
@@ -303,7 +306,9 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   launch_init_tasks(d_chunks, (uint32_t)n, ntasks, d_tasks, d_gp, d_gb, s);
   launch_parse_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_gp, d_gb, s);
   timer->mark();
-  if (lane_decode) {
+  if (split) {
+    launch_ht_decode_split(d_tasks, ntasks, d_rec, s);
+  } else if (lane_decode) {
     if (sort) launch_ht_order(d_tasks, ntasks, d_counts, d_order, s);
     launch_ht_decode(d_tasks, d_order, ntasks, s);
   } else {

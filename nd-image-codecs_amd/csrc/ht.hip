@@ -124,6 +124,7 @@ __device__ __forceinline__ u32x4 ld128(const uint8_t* p) {
 __device__ __forceinline__ void st8(uint8_t* p, uint8_t v) { *(__attribute__((address_space(1))) uint8_t*)(uintptr_t)p = v; }
 __device__ __forceinline__ void st32(void* p, uint32_t v) { *(__attribute__((address_space(1))) uint32_t*)(uintptr_t)p = v; }
 __device__ __forceinline__ int32_t ldi32(const int32_t* p) { return *(const __attribute__((address_space(1))) int32_t*)(uintptr_t)p; }
+__device__ __forceinline__ uint32_t ldi32u(const uint32_t* p) { return *(const __attribute__((address_space(1))) uint32_t*)(uintptr_t)p; }
 
 struct MelDec {
   const uint8_t* data;
@@ -1144,6 +1145,416 @@ __global__ __launch_bounds__(64) void ht_decode_wave_kernel(const HtDecTask* __r
     }
   }
   queue_done(queue);
+}
+
+// ---------------------------------------------------------------------------
+// Two-pass cleanup decode (large windows).
+//
+// Pass 1, ht_vlc_kernel: one lane per code-block runs only the VLC / MEL
+// decode (decode.rs:115-300), the serial chain whose context is the quad
+// significance of the row above, and writes per quad the CxtVLC entry and
+// the UVLC exponent offset before the row-above correction, u << 16 | entry
+// (what the wavefront decoder's step 2 leaves in its lanes).  A quad row of
+// the wave's 64 blocks is staged in LDS and stored 32 quads (128 bytes) per
+// block at a time.  The VLC and MEL streams are the short end of a segment.
+// Pass 2, ht_magsgn_kernel: one wavefront per code-block unstuffs the MagSgn
+// bytes into LDS with coalesced reads, loads the block's quad records, and
+// decodes each row pair lane-per-column (the wavefront decoder's step 3):
+// 64-column rows stored as full lines.  The result equals the other
+// decoders' sample for sample, errors included.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kRecWords = 32 * 32;  // quad records per block: 32 quad rows x 32 quads
+
+constexpr uint32_t kRecPitch = 33;       // LDS pitch of a lane's staged quad row
+
+template <class Tab>
+struct VlcMelDec {
+  Tab tab;
+  MelDec mel;
+  RevDec vlc;
+  int run;
+  bool active;
+  uint32_t c_q, q;
+
+  // decode.rs:54-110, the checks that decide whether the streams are read
+  __device__ void start(const HtDecTask& t) {
+    active = false;
+    if (!(t.flags & kTaskIncluded) || !t.seg || t.w == 0 || t.h == 0) return;
+    uint32_t num_passes = t.passes;
+    if (num_passes > 1 && t.len2 == 0) num_passes = 1;
+    if (num_passes > 3 || t.mmsb >= 30 || t.len1 < 2 || t.w > 64 || t.h > 64) return;
+    const uint32_t lcup = t.len1;
+    const uint32_t scup = (ld8(t.seg + lcup - 1) << 4) + (ld8(t.seg + lcup - 2) & 0xF);
+    if (scup < 2 || scup > lcup || scup > 4079) return;
+    mel.init(t.seg, lcup, scup);
+    vlc.init_vlc(t.seg, lcup, scup);
+    run = mel.get_run();
+    active = true;
+  }
+
+  __device__ void row_begin() {
+    c_q = 0;
+    q = 0;
+  }
+
+  // One pair of quads (columns x..x+3, rows y, y+1): their records.
+  __device__ void pair(uint32_t x, uint32_t y, uint32_t width, uint8_t* lrho, uint32_t* r0, uint32_t* r1) {
+    auto RHO = [&](int k) -> uint8_t& { return lrho[k * 64]; };
+    const bool initial = y == 0;
+    uint32_t a0 = 0, a1 = 0, a2 = 0;
+    if (!initial) {
+      a0 = uint32_t(RHO(q)) << 4;
+      a1 = uint32_t(RHO(q + 1)) << 4;
+      a2 = uint32_t(RHO(q + 2)) << 4;
+      c_q |= (a0 & 0xA0) << 2;
+      c_q |= (a1 & 0x20) << 4;
+    }
+    uint32_t vlc_val = vlc.fetch();
+    uint32_t t0 = tab.dec(initial, c_q | (vlc_val & 0x7F));
+    if (c_q == 0) {
+      run -= 2;
+      t0 = run == -1 ? t0 : 0;
+      if (run < 0) run = mel.get_run();
+    }
+    if (initial) {
+      c_q = ((t0 & 0x10) << 3) | ((t0 & 0xE0) << 2);
+    } else {
+      c_q = ((t0 & 0x40) << 2) | ((t0 & 0x80) << 1);
+      c_q |= a0 & 0x80;
+      c_q |= (a1 & 0xA0) << 2;
+      c_q |= (a2 & 0x20) << 4;
+    }
+    vlc_val = vlc.advance(t0 & 0x7);
+    uint32_t t1 = tab.dec(initial, c_q | (vlc_val & 0x7F));
+    if (c_q == 0 && x + 2 < width) {
+      run -= 2;
+      t1 = run == -1 ? t1 : 0;
+      if (run < 0) run = mel.get_run();
+    }
+    if (x + 2 >= width) t1 = 0;
+    if (initial) {
+      c_q = ((t1 & 0x10) << 3) | ((t1 & 0xE0) << 2);
+    } else {
+      c_q = ((t1 & 0x40) << 2) | ((t1 & 0x80) << 1);
+      c_q |= a1 & 0x80;
+    }
+    vlc_val = vlc.advance(t1 & 0x7);
+    uint32_t uvlc_mode = ((t0 & 0x8) << 3) | ((t1 & 0x8) << 4);
+    if (initial && uvlc_mode == 0xC0) {
+      run -= 2;
+      uvlc_mode += run == -1 ? 0x40 : 0;
+      if (run < 0) run = mel.get_run();
+    }
+    uint32_t e = tab.uvlc(initial, uvlc_mode + (vlc_val & 0x3F));
+    vlc_val = vlc.advance(e & 0x7);
+    e >>= 3;
+    const uint32_t len = e & 0xF;
+    const uint32_t tmp = vlc_val & ((1u << len) - 1);
+    vlc.advance(len);
+    e >>= 4;
+    const uint32_t len0 = e & 0x7;
+    e >>= 3;
+    const uint32_t base = initial ? 1 : 0;
+    *r0 = t0 | ((base + (e & 7) + (tmp & ~(0xFFu << len0))) << 16);
+    *r1 = t1 | ((base + (e >> 3) + (tmp >> len0)) << 16);
+    RHO(q) = uint8_t(t0 >> 4);
+    RHO(q + 1) = uint8_t(t1 >> 4);
+    q += 2;
+  }
+
+  // the quads after the row read as insignificant
+  __device__ void row_end(uint8_t* lrho) {
+    lrho[q * 64] = 0;
+    lrho[(q + 1) * 64] = 0;
+  }
+};
+
+// kW waves per workgroup share one LDS copy of the decode tables; lane l of
+// wave w decodes task 64 (kW b + w) + l; its records go to rec + task * kRecWords.
+template <uint32_t kLaneWaves>
+__global__ __launch_bounds__(64 * kLaneWaves) void ht_vlc_kernel(const HtDecTask* __restrict__ tasks, uint64_t n,
+                                                                uint32_t* __restrict__ rec) {
+  __shared__ __align__(16) uint16_t tabs[kTabAll];
+  __shared__ uint8_t lrho_all[kLaneWaves][kRowCap * 64];
+  __shared__ uint32_t recst_all[kLaneWaves][64 * kRecPitch];
+  {
+    uint32_t* tw = (uint32_t*)tabs;
+    const uint32_t* s0 = (const uint32_t*)c_dec0;
+    const uint32_t* s1 = (const uint32_t*)c_dec1;
+    const uint32_t* s2 = (const uint32_t*)c_uvlc0;
+    const uint32_t* s3 = (const uint32_t*)c_uvlc1;
+    for (uint32_t k = threadIdx.x; k < kTabAll / 2; k += blockDim.x) {
+      uint32_t v;
+      if (k < kTabDec1 / 2) v = s0[k];
+      else if (k < kTabUv0 / 2) v = s1[k - kTabDec1 / 2];
+      else if (k < kTabUv1 / 2) v = s2[k - kTabUv0 / 2];
+      else v = s3[k - kTabUv1 / 2];
+      tw[k] = v;
+    }
+  }
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint8_t* lrho = lrho_all[wv];
+  uint32_t* recst = recst_all[wv];
+  const uint64_t i = blockIdx.x * (uint64_t)(64 * kLaneWaves) + threadIdx.x;
+  HtDecTask task{};
+  if (i < n) task = tasks[i];
+  for (int k = 0; k < kRowCap; ++k) lrho[k * 64 + lane] = 0;
+  VlcMelDec<LdsTables> d;
+  d.tab.t = (lds_u16p)tabs;
+  d.start(task);
+  const uint32_t w = d.active ? task.w : 0, h = d.active ? task.h : 0;
+  const uint32_t nq = (w + 1) / 2;  // quads per row
+  for (uint32_t y = 0; y < 64; y += 2) {
+    if (__ballot(y < h) == 0) break;
+    if (y < h) {
+      d.row_begin();
+      for (uint32_t x = 0; x < w; x += 4) {
+        uint32_t r0, r1;
+        d.pair(x, y, w, lrho + lane, &r0, &r1);
+        recst[lane * kRecPitch + x / 2] = r0;
+        recst[lane * kRecPitch + x / 2 + 1] = r1;
+      }
+      d.row_end(lrho + lane);
+    }
+    wave_sync();  // the quad row of every lane's block is staged
+    // two blocks per store: lanes 0-31 the quads of block j, 32-63 of block j + 1
+    const uint32_t half = lane >> 5, qi = lane & 31;
+#pragma unroll 4
+    for (uint32_t j = 0; j < 64; j += 2) {
+      const uint32_t b = j + half;
+      const uint32_t bh = __shfl(h, (int)b), bq = __shfl(nq, (int)b);
+      const uint64_t bi = (uint64_t)__shfl((long long)i, (int)b);  // the task index of lane b
+      if (y < bh && qi < bq) rec[bi * kRecWords + (y / 2) * 32 + qi] = recst[b * kRecPitch + qi];
+    }
+    wave_sync();  // reads done before the next row overwrites the stage
+  }
+}
+
+// The MagSgn stream of one block unstuffed into an LDS ring of kMsRing words
+// as the row decode advances (bit b of the stream at word (b >> 5) mod
+// kMsRing), 1024 bytes per refill: lane l unstuffs bytes 16l..16l+15 of the
+// chunk, its bit offset a wave prefix sum of the widths the stuffing rule
+// gives (8 - [previous byte == 0xFF]).  The next chunk's bytes are loaded one
+// refill ahead.  Past the stream's end the bits read as ones.
+constexpr uint32_t kMsRing = 512;  // words: 16 Kbit, more than the 12 Kbit live window
+struct MsRing {
+  uint32_t* ring;
+  const uint8_t* seg;
+  uint32_t n;      // MagSgn bytes
+  uint32_t next;   // first byte of the next chunk
+  uint32_t end;    // bits unstuffed so far
+  uint32_t prev;   // last byte of the previous chunk (0 before the first)
+  uint32_t nvalid; // words that hold stream bits, once all are in (else ~0)
+  uint32_t pf[16]; // this lane's bytes of the next chunk
+
+  __device__ void load_next() {
+    const uint32_t k0 = next + 16 * (threadIdx.x & 63);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) pf[j] = k0 + j < n ? ld8(seg + k0 + j) : 0u;
+  }
+  __device__ void init(uint32_t* r, const uint8_t* s, uint32_t nbytes) {
+    ring = r;
+    seg = s;
+    n = nbytes;
+    next = 0;
+    end = 0;
+    prev = 0;
+    nvalid = ~0u;
+    load_next();
+  }
+  __device__ void refill() {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t k0 = next + 16 * lane;
+    uint32_t prev_l = __shfl(pf[15], (int)lane - 1);
+    if (lane == 0) prev_l = prev;
+    uint32_t w[16], sum = 0;
+    uint32_t pv = prev_l;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      w[j] = k0 + j < n ? 8 - uint32_t(pv == 0xFF) : 0u;
+      sum += w[j];
+      pv = pf[j];
+    }
+    // exclusive prefix of the lane sums (<= 128), bit-sliced over ballots
+    uint32_t excl = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint64_t bal = __ballot((sum >> k) & 1);
+      excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u)) << k;
+      total += (uint32_t)__popcll(bal) << k;
+    }
+    // clear the words this chunk reaches past the partly filled one (their
+    // previous contents are long consumed), then OR the bytes in
+    const uint32_t w0 = (end >> 5) + 1, w1 = ((end + total) >> 5) + 1;
+    for (uint32_t k = w0 + lane; k <= w1; k += 64) ring[k % kMsRing] = 0;
+    wave_sync();
+    uint32_t pos = end + excl;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      if (w[j]) {
+        const uint32_t sh = pos & 31, wi = pos >> 5;
+        atomicOr(&ring[wi % kMsRing], pf[j] << sh);
+        if (sh > 24) atomicOr(&ring[(wi + 1) % kMsRing], pf[j] >> (32 - sh));
+      }
+      pos += w[j];
+    }
+    prev = __shfl(pf[15], 63);
+    if (next + 1024 > n) prev = 0;  // unused: the stream ends in this chunk
+    end += total;
+    next += 1024;
+    if (next < n) {
+      load_next();
+    } else {
+      // the bits past the end read as ones
+      if (lane == 0) atomicOr(&ring[(end >> 5) % kMsRing], ~0u << (end & 31));
+      nvalid = (end >> 5) + 1;
+    }
+    wave_sync();
+  }
+  // the stream holds bits [from, from + need) (or has ended)
+  __device__ void ensure(uint32_t from, uint32_t need) {
+    while (end < from + need && nvalid == ~0u) refill();
+  }
+  __device__ uint32_t bits32(uint32_t pos) const {
+    const uint32_t wi = pos >> 5;
+    const uint32_t lo = wi < nvalid ? ring[wi % kMsRing] : ~0u;
+    const uint32_t hi = wi + 1 < nvalid ? ring[(wi + 1) % kMsRing] : ~0u;
+    return __builtin_amdgcn_alignbit(hi, lo, pos & 31);
+  }
+};
+
+// Task i with its quad records (the whole wave; returns together).
+__device__ void ht_magsgn_block(const HtDecTask* __restrict__ tasks, uint64_t i, const uint32_t* __restrict__ rec,
+                                uint32_t* ringbuf) {
+  const uint32_t lane = threadIdx.x;
+  const HtDecTask t = tasks[i];
+  HtDecTask* slot = (HtDecTask*)tasks + i;
+  const uint32_t width = t.w, height = t.h;
+  uint32_t* out = (uint32_t*)t.out;
+  const uint64_t stride = t.stride;
+  // decode.rs:54-110, as BlockDec::start
+  bool active = false, refine = false;
+  uint32_t p = 0, shift = 0, mmsbp2 = 0, lcup = 0, scup = 0;
+  if ((t.flags & kTaskIncluded) && t.seg && width && height) {
+    uint32_t num_passes = t.passes;
+    const uint32_t missing_msbs = t.mmsb;
+    if (num_passes > 1 && t.len2 == 0) num_passes = 1;
+    if (num_passes > 3 || missing_msbs >= 30 || t.len1 < 2 || width > 64 || height > 64) {
+      if (lane == 0) report(t, kErrBlock);
+    } else {
+      if (missing_msbs == 29) num_passes = 1;
+      refine = num_passes > 1;
+      lcup = t.len1;
+      scup = (ld8(t.seg + lcup - 1) << 4) + (ld8(t.seg + lcup - 2) & 0xF);
+      if (scup < 2 || scup > lcup || scup > 4079) {
+        if (lane == 0) report(t, kErrBlock);
+      } else {
+        active = true;
+        p = 30 - missing_msbs;
+        shift = 31 - t.kmax;
+        mmsbp2 = missing_msbs + 2;
+      }
+    }
+  }
+  const uint32_t wcap = width < 64 ? width : 64, hcap = height < 64 ? height : 64;
+  if (!active) {
+    if (out)
+      for (uint32_t y = 0; y < hcap; ++y)
+        if (lane < wcap) st32(out + (uint64_t)y * stride + lane, 0);
+    if (lane == 0 && t.passes > 1) slot->passes = 1;
+    return;
+  }
+  const bool raw = (t.flags & kTaskSignMag) || refine;
+  const bool col = lane < width;
+  MsRing ms;
+  ms.init(ringbuf, t.seg, lcup - scup);
+  if (lane == 0) ringbuf[0] = 0;  // the first refill clears from word 1 on
+  wave_sync();
+  uint32_t msbase = 0;
+  bool bad = false;
+  uint32_t blprev = 0;  // bit length of this column's sample in the row above
+  uint32_t qnext = col ? ldi32u(rec + lane / 2) : 0u;
+  for (uint32_t y = 0; y < height; y += 2) {
+    const bool initial = y == 0;
+    // ---- MagSgn of rows y, y+1, lane = column (decode.rs:302-396); the
+    // next row's records are in flight meanwhile
+    const uint32_t qinfo = qnext;
+    qnext = col && y + 2 < height ? ldi32u(rec + (y / 2 + 1) * 32 + lane / 2) : 0u;
+    const uint32_t tq = qinfo & 0xFFFF;
+    uint32_t U = qinfo >> 16;
+    if (!initial) {
+      uint32_t gamma = tq & 0xF0;
+      gamma &= gamma - 0x10;
+      // columns 2q-1 .. 2q+2 of the row above (q = lane / 2); past either edge 0
+      const int q2 = (int)(lane & ~1u);
+      const uint32_t b0 = __shfl(blprev, q2 - 1), b1 = __shfl(blprev, q2), b2 = __shfl(blprev, q2 + 1),
+                     b3 = __shfl(blprev, q2 + 2);
+      const uint32_t e4 = max(max(q2 >= 1 ? b0 : 0u, b1), max(b2, q2 + 2 < 64 ? b3 : 0u));
+      U += gamma != 0 ? max(e4, 2u) - 1 : 1;
+    }
+    if (col && !(lane & 1) && U > mmsbp2) bad = true;
+    const uint32_t nb0 = 2 * (lane & 1);
+    uint32_t m[2], sig[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const uint32_t nbit = nb0 + r;
+      sig[r] = col ? (tq >> (4 + nbit)) & 1 : 0u;
+      m[r] = (U - ((tq >> (12 + nbit)) & 1)) & 31;
+    }
+    const uint32_t c0 = sig[0] ? m[0] : 0u, c1 = sig[1] ? m[1] : 0u;
+    // exclusive prefix sum of the bit counts over the lanes, bit-sliced: for
+    // each bit of the count a ballot, and mbcnt counts the lanes below
+    const uint32_t cnt = c0 + c1;  // < 64
+    uint32_t excl = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const uint64_t bal = __ballot((cnt >> k) & 1);
+      excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u)) << k;
+      total += (uint32_t)__popcll(bal) << k;
+    }
+    ms.ensure(msbase, total + 64);
+    const uint32_t off = msbase + excl;
+    uint32_t blnew = 0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const uint32_t nbit = nb0 + r;
+      uint32_t val = 0, vv = 0;
+      if (sig[r]) {
+        const uint32_t ms_val = ms.bits32(off + (r ? c0 : 0u));
+        vv = ms_val & ((1u << m[r]) - 1);
+        vv |= ((tq >> (8 + nbit)) & 1) << m[r];
+        vv |= 1;
+        val = (ms_val << 31) | ((vv + 2) << ((p - 1) & 31));
+      }
+      if (r == 1) blnew = bitlen(vv);
+      if (col && y + r < height) {
+        uint32_t o = val;
+        if (!raw) {
+          const int32_t mag = (int32_t)((val & 0x7FFFFFFFu) >> shift);
+          o = (uint32_t)((val & 0x80000000u) ? -mag : mag);
+        }
+        st32(out + (uint64_t)(y + r) * stride + lane, o);
+      }
+    }
+    msbase += total;
+    blprev = blnew;
+  }
+  // a magnitude bound violated mid-block: the whole block reads as zero
+  if (__ballot(bad)) {
+    if (lane == 0) report(t, kErrBlock);
+    for (uint32_t y = 0; y < hcap; ++y)
+      if (lane < wcap) st32(out + (uint64_t)y * stride + lane, 0);
+    if (lane == 0 && t.passes > 1) slot->passes = 1;
+  }
+}
+
+__global__ __launch_bounds__(64) void ht_magsgn_kernel(const HtDecTask* __restrict__ tasks, uint64_t n,
+                                                       const uint32_t* __restrict__ rec) {
+  __shared__ uint32_t ring[kMsRing];
+  const uint64_t i = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  if (i >= n) return;
+  ht_magsgn_block(tasks, i, rec + i * kRecWords, ring);
 }
 
 // SigProp and MagRef passes (decode.rs:398-600) of the blocks that carry
@@ -2274,6 +2685,15 @@ template <class K> uint32_t persistent_grid(K kernel, uint64_t grains) {
   return (uint32_t)std::min<uint64_t>(grains, (uint64_t)cap);
 }
 }  // namespace
+
+void launch_ht_decode_split(const HtDecTask* d_tasks, uint64_t n, uint32_t* d_rec, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(ht_vlc_kernel<4>, dim3((uint32_t)ceil_div(n, 256)), dim3(256), 0, stream, d_tasks, n, d_rec);
+  NDIC_HIP(hipGetLastError());
+  const uint32_t gx = (uint32_t)std::min<uint64_t>(n, 1u << 20), gy = (uint32_t)ceil_div(n, gx);
+  hipLaunchKernelGGL(ht_magsgn_kernel, dim3(gx, gy), dim3(64), 0, stream, d_tasks, n, (const uint32_t*)d_rec);
+  NDIC_HIP(hipGetLastError());
+}
 
 void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, uint64_t* queue, hipStream_t stream) {
   if (n == 0) return;

@@ -57,6 +57,11 @@ void launch_ht_decode(const HtDecTask* d_tasks, const uint32_t* d_order, uint64_
 // latency per block.  A persistent launch fed by the work queue at `queue`
 // (two zeroed counters the calling context owns; left zeroed).
 void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, uint64_t* queue, hipStream_t stream);
+// The same decode in two passes: the VLC / MEL decode one lane per block,
+// writing per-quad records (kHtRecBytes per task) to d_rec, then the MagSgn
+// decode one wavefront per block from those records.
+constexpr uint64_t kHtRecBytes = 32 * 32 * 4;
+void launch_ht_decode_split(const HtDecTask* d_tasks, uint64_t n, uint32_t* d_rec, hipStream_t stream);
 // SigProp/MagRef of the tasks with more than one coding pass (after launch_ht_decode)
 void launch_ht_refine(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
 void launch_ht_encode(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream);

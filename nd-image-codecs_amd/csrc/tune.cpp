@@ -26,6 +26,7 @@ const Knob kKnobs[kTuneCount] = {
     {"ht_lane_waves", 0, 0, 4},
     {"graph_windows", 0, 0, 1},
     {"ht_decode_sort", 0, 0, 1},
+    {"ht_decode_split", 1, 0, 1},
 };
 
 std::atomic<int64_t> g_vals[kTuneCount] = {};

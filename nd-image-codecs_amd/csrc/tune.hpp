@@ -42,6 +42,10 @@ enum TuneKey : int {
   // (a device counting sort by cleanup-segment length) so that a wave's
   // blocks carry similar work; 0: in window order
   kTuneHtDecodeSort,
+  // 1: windows above the wavefront crossover decode their code-blocks in two
+  // passes (VLC / MEL one lane per block into per-quad records, then MagSgn
+  // one wavefront per block); 0: the one-pass lane-per-block decoder
+  kTuneHtDecodeSplit,
   kTuneCount
 };
 

@@ -1,7 +1,8 @@
-"""The two HT cleanup coders on the device against each other and the
-oracle: one code-block per wavefront (lanes over columns, wave-parallel
-stream unstuffing / bit packing and MagSgn offsets) and one code-block per
-lane.
+"""The HT cleanup coders on the device against each other and the oracle:
+one code-block per wavefront (lanes over columns, wave-parallel stream
+unstuffing / bit packing and MagSgn offsets), one code-block per lane, and
+the two-pass decoder (VLC / MEL one block per lane into per-quad records,
+then MagSgn one block per wavefront).
 
 Covers what the reference's block tests cover (crates/ndic-htj2k/tests/
 block_roundtrip.rs:110-140: every block size 1..64, K_max 2..28, densities
@@ -34,16 +35,17 @@ def force(ndic):
     """Selects the decoder: "wave" (wavefront per block), "lane" (lane per
     block, workgroup shape chosen by the library), "lane1" / "lane4" (lane
     per block with one / four waves per workgroup sharing the tables),
-    "lane_sorted" (lane per block, blocks taken heaviest first)."""
+    "lane_sorted" (lane per block, blocks taken heaviest first), "split"
+    (the two-pass decoder)."""
     def set_(mode: str) -> None:
         ndic.tune_set(KNOB, 1 << 40 if mode == "wave" else 0)
         ndic.tune_set("ht_lane_waves", {"lane1": 1, "lane4": 4}.get(mode, 0))
         ndic.tune_set("ht_decode_sort", 1 if mode == "lane_sorted" else 0)
+        ndic.tune_set("ht_decode_split", 1 if mode == "split" else 0)
 
     yield set_
-    ndic.tune_reset(KNOB)
-    ndic.tune_reset("ht_lane_waves")
-    ndic.tune_reset("ht_decode_sort")
+    for k in (KNOB, "ht_lane_waves", "ht_decode_sort", "ht_decode_split"):
+        ndic.tune_reset(k)
 
 
 @pytest.fixture
@@ -85,10 +87,12 @@ def test_wave_decoder_matches_oracle(ndic, oracle, force):
         assert np.array_equal(got, oracle.ht_decode_block(coded, w, h, kmax - 1)), (i, w, h, kmax)
 
 
-def test_wave_decoder_large_magsgn_segments(ndic, oracle, force):
-    """Dense 64x64 blocks at K_max 27-30 carry more than 12,280 MagSgn bytes:
-    lane 0 decodes them serially; smaller ones take the parallel path."""
-    force("wave")
+@pytest.mark.parametrize("mode", ["wave", "split"])
+def test_large_magsgn_segments(ndic, oracle, force, mode):
+    """Dense 64x64 blocks at K_max 20-30: the wavefront decoder hands MagSgn
+    parts of more than 12,280 bytes to lane 0's serial decode, the two-pass
+    decoder streams every size through its LDS ring."""
+    force(mode)
     rng = np.random.default_rng(32)
     sizes = []
     for kmax in (20, 24, 27, 28, 29, 30):
@@ -154,6 +158,9 @@ def test_decoders_agree_on_corrupt_segments(ndic, oracle, force):
         force("wave")
         rc_w, out_w = decode_rc(ndic, coded, w, h, mmsb)
         assert rc_l == rc_w and np.array_equal(out_l, out_w), (i, kind, w, h, kmax)
+        force("split")
+        rc_s, out_s = decode_rc(ndic, coded, w, h, mmsb)
+        assert rc_s == rc_w and np.array_equal(out_s, out_w), (i, kind, w, h, kmax)
         checked += 1
         try:
             exp = oracle.ht_decode_block(coded, w, h, mmsb)
@@ -165,7 +172,7 @@ def test_decoders_agree_on_corrupt_segments(ndic, oracle, force):
     assert checked > 200 and accepted > 20, (checked, accepted)
 
 
-@pytest.mark.parametrize("mode", ["wave", "lane1", "lane4", "lane_sorted"])
+@pytest.mark.parametrize("mode", ["wave", "lane1", "lane4", "lane_sorted", "split"])
 def test_chunks_decode_with_either_decoder(ndic, oracle, force, mode):
     force(mode)
     rng = np.random.default_rng(35)
@@ -183,7 +190,7 @@ def test_chunks_decode_with_either_decoder(ndic, oracle, force, mode):
         assert ndic.htj2k_decode(chunk, shape, name) == raw, (mode, shape, dt, lv)
 
 
-@pytest.mark.parametrize("mode", ["wave", "lane", "lane1", "lane4", "lane_sorted"])
+@pytest.mark.parametrize("mode", ["wave", "lane", "lane1", "lane4", "lane_sorted", "split"])
 def test_window_of_9217_blocks_with_either_decoder(ndic, force, mode):
     """One window of 9,217 code-blocks (an odd count) of blob content,
     encoded on the device and decoded back bit-exactly by each decoder."""
@@ -263,3 +270,26 @@ def test_chunks_encode_with_either_encoder(ndic, oracle, force_enc, mode):
     vol = datagen.c0_volume((4, 256, 256), seed=3)
     exp = oracle.htj2k_encode_chunk(vol.tobytes(), vol.shape, "u16", xy_levels=5)
     assert ndic.htj2k_encode(vol, vol.shape, "uint16", xy_levels=5) == exp
+
+
+@pytest.mark.parametrize("mode", ["split", "lane"])
+def test_decoders_match_oracle_per_block(ndic, oracle, force, mode):
+    """Every block size and density of the block round-trip test through the
+    lane-per-block and the two-pass decoders, and the refinement passes
+    after them."""
+    force(mode)
+    rng = np.random.default_rng(36)
+    for i in range(160):
+        w, h, kmax, sm = random_block(rng, i)
+        coded = oracle.ht_encode_block(sm, kmax - 1)
+        got = devhooks.ht_decode_block(ndic, coded, w, h, kmax - 1)
+        assert np.array_equal(got, oracle.ht_decode_block(coded, w, h, kmax - 1)), (mode, i, w, h, kmax)
+    for i in range(40):
+        w, h = int(rng.integers(1, 65)), int(rng.integers(1, 65))
+        kmax = int(rng.integers(3, 29))
+        sm = datagen.sm_block(rng, w, h, kmax, float(rng.uniform(0.02, 0.6)))
+        cleanup = oracle.ht_encode_block(sm, kmax - 1)
+        seg2 = rng.integers(0, 256, size=int(rng.integers(1, 300)), dtype=np.uint8).tobytes()
+        exp = oracle.ht_decode_block(cleanup + seg2, w, h, kmax - 1, 3, len(cleanup), len(seg2), bool(i & 1))
+        got = devhooks.ht_decode_block(ndic, cleanup + seg2, w, h, kmax - 1, 3, len(cleanup), len(seg2), bool(i & 1))
+        assert np.array_equal(got, exp), (mode, i)
