@@ -31,9 +31,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
-# the kernel that dominates each decode stage (stages are timed per batch
-# window with HIP events on the library's stream)
-KERNELS = ["parse_planes_kernel", "ht_decode_kernel", "dwt_inv_level", "unshift_kernel", "dwt_lift_kernel"]
+# the kernels of each decode stage (stages are timed per batch window with
+# HIP events on the library's stream); the HT stage is two passes (VLC/MEL
+# lane per code-block, MagSgn wavefront per code-block) and the refinement
+HT_PASSES = ["ht_vlc_kernel", "ht_magsgn_kernel", "ht_refine_kernel"]
+KERNELS = ["parse_planes_kernel", "+".join(HT_PASSES), "dwt_inv_level", "unshift_kernel", "dwt_lift_kernel"]
 COUNTERS = ROOT / "profiles" / "c2_decode_counters.json"
 
 
@@ -273,22 +275,54 @@ def sine_field(rng, shape, drift_axis=None):
     return a + rng.normal(0, 1e-3, shape)
 
 
+ZFP_FIELDS = [("f32 64x256x256", (64, 256, 256), np.float32, None),
+              ("f64 4x64x128x128", (4, 64, 128, 128), np.float64, 0)]
+ZFP_MODES = [("rate8", "fixed_rate", {"rate": 8.0}), ("rate16", "fixed_rate", {"rate": 16.0}),
+             ("reversible", "reversible", {}), ("fixed_accuracy 1e-3", "fixed_accuracy", {"tolerance": 1e-3}),
+             ("fixed_precision 20", "fixed_precision", {"precision": 20})]
+ZFP_REPS = 5
+ZFP_BRICKS = "2048 fields 16^3..64^3 f32 rate8"
+
+
+def zfp_schedule():
+    """The ZFP section's phases in launch order, each opened by one marker
+    copy kernel: (case, phase, calls).  An encode call launches
+    zfp_encode_kernel (+ zfp_place_kernel and zfp_index_write_kernel in the
+    variable-size modes), a decode call zfp_decode_kernel, a walk call
+    zfp_index_kernel then zfp_decode_kernel."""
+    out = []
+    for fname, *_ in ZFP_FIELDS:
+        for mname, mode, _ in ZFP_MODES:
+            case = f"{fname} {mname}"
+            out.append((case, "encode", 2 + 2 * ZFP_REPS))
+            out.append((case, "decode", 1 + 2 * ZFP_REPS))
+            if mode != "fixed_rate":
+                out.append((case, "walk", 3))
+    out.append((ZFP_BRICKS, "encode", 1))
+    out.append((ZFP_BRICKS, "decode", 1 + 2 * ZFP_REPS))
+    return out
+
+
 def zfp_bench(ndic, dev, args):
     """ZFP at the two field shapes (f32 64x256x256, f64 4x64x128x128), fixed
-    rate 8 / 16, reversible, fixed accuracy and fixed precision:
-    device-resident encode and decode rates (median of 5), the variable-size
-    modes decoded through their block index and, once, by walking the
-    stream; the oracle on one core beside them, and a batch of 2,048
-    independent fields of 16^3..64^3 samples."""
-    sys.path.insert(0, str(ROOT / "oracle"))
+    rate 8 / 16, reversible, fixed accuracy and fixed precision, and a batch
+    of 2,048 independent f32 fields of 16^3..64^3 samples at rate 8:
+    device-resident encode and decode rates of whole calls (host clock,
+    median of 5), the kernels alone (HIP events around their launches,
+    summed over 5 calls) against the HBM roofline, the variable-size modes
+    decoded through their block index and by walking the stream, and the
+    oracle on one core beside them.  The phases follow zfp_schedule()."""
     rng = np.random.default_rng(4)
-    fields = [("f32 64x256x256", sine_field(rng, (64, 256, 256)).astype(np.float32)),
-              ("f64 4x64x128x128", sine_field(rng, (4, 64, 128, 128), drift_axis=0))]
-    modes = [("rate8", ndic.zfp_config("fixed_rate", rate=8.0)), ("rate16", ndic.zfp_config("fixed_rate", rate=16.0)),
-             ("reversible", ndic.zfp_config("reversible")),
-             ("fixed_accuracy 1e-3", ndic.zfp_config("fixed_accuracy", tolerance=1e-3)),
-             ("fixed_precision 20", ndic.zfp_config("fixed_precision", precision=20))]
+    fields = [(name, sine_field(rng, shape, drift_axis=drift).astype(dt)) for name, shape, dt, drift in ZFP_FIELDS]
     out = {}
+    marker = dev.DeviceBuffer(8192, zero=True)
+    sched = iter(zfp_schedule())
+
+    def phase(case, name):
+        want = next(sched)
+        assert want[:2] == (case, name), (want, case, name)
+        dev.copy(marker.ptr + 4096, marker.ptr, 256)
+        dev.synchronize()
 
     def timed(fn, reps):
         fn()
@@ -299,6 +333,19 @@ def zfp_bench(ndic, dev, args):
             ts.append(time.perf_counter() - t0)
         return sorted(ts)[len(ts) // 2]
 
+    def kernel_ms(fn, groups):
+        """average device ms per call of the named kernel groups"""
+        dev.profile(True)
+        for _ in range(ZFP_REPS):
+            fn()
+        t = dev.zfp_times()
+        dev.profile(False)
+        return {g: t[g][0] / ZFP_REPS for g in groups}
+
+    def roof(nbytes, ms):
+        gbps = nbytes / ms / 1e6
+        return {"ms": round(ms, 4), "bytes": int(nbytes), "GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+
     for name, fld in fields:
         d_in = dev.DeviceBuffer.from_bytes(fld.tobytes())
         cap = 3 * fld.nbytes + (1 << 20)
@@ -306,34 +353,57 @@ def zfp_bench(ndic, dev, args):
         icap = 8 << 20
         d_idx = dev.DeviceBuffer(icap, zero=True)
         d_out = dev.DeviceBuffer(fld.nbytes)
-        for mname, cfg in modes:
+        for mname, mode, kw in ZFP_MODES:
+            case = f"{name} {mname}"
+            cfg = ndic.zfp_config(mode, **kw)
             desc = [ndic.make_zfp_desc(fld.shape, fld.dtype.name, cfg, 0, fld.nbytes)]
-            arr = ndic.zfp_encode_batch(d_in.ptr, d_enc.ptr, cap, desc, d_index_ptr=d_idx.ptr, index_capacity=icap)
-            enc_s = timed(lambda: ndic.zfp_encode_batch(d_in.ptr, d_enc.ptr, cap, desc, d_index_ptr=d_idx.ptr,
-                                                        index_capacity=icap), 5)
-            clen = arr[0].out_len
+
+            def enc():
+                return ndic.zfp_encode_batch(d_in.ptr, d_enc.ptr, cap, desc, d_index_ptr=d_idx.ptr,
+                                             index_capacity=icap)
+
+            phase(case, "encode")
+            arr = enc()
+            enc_s = timed(enc, ZFP_REPS)
+            kenc = kernel_ms(enc, ("encode", "place"))
+            clen, ilen = arr[0].out_len, arr[0].index_len
             rec = {"encode_GBps": round(fld.nbytes / enc_s / 1e9, 2), "compressed_bytes": clen}
-            variable = arr[0].index_len > 0
+            variable = ilen > 0
             walk = [ndic.make_zfp_desc(fld.shape, fld.dtype.name, cfg, arr[0].out_offset, clen, 0)]
+            idx = [ndic.make_zfp_desc(fld.shape, fld.dtype.name, cfg, arr[0].out_offset, clen, 0,
+                                      arr[0].index_offset, ilen)]
+
+            def dec():
+                ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, idx, d_index_ptr=d_idx.ptr)
+
+            phase(case, "decode")
+            dec_s = timed(dec, ZFP_REPS)
+            kdec = kernel_ms(dec, ("decode",))
+            rec.update({"decode_GBps": round(fld.nbytes / dec_s / 1e9, 2), "decode_ms": round(dec_s * 1e3, 3)})
             if variable:
-                idx = [ndic.make_zfp_desc(fld.shape, fld.dtype.name, cfg, arr[0].out_offset, clen, 0,
-                                          arr[0].index_offset, arr[0].index_len)]
-                dec_s = timed(lambda: ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, idx, d_index_ptr=d_idx.ptr), 5)
+                rec["block_index_bytes"] = int(ilen)
+                phase(case, "walk")
                 walk_s = timed(lambda: ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, walk), 1)
-                rec.update({"decode_GBps": round(fld.nbytes / dec_s / 1e9, 2), "decode_ms": round(dec_s * 1e3, 3),
-                            "block_index_bytes": int(arr[0].index_len),
-                            "decode_without_index_GBps": round(fld.nbytes / walk_s / 1e9, 3),
-                            "decode_without_index_ms": round(walk_s * 1e3, 1)})
-            else:
-                dec_s = timed(lambda: ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, walk), 5)
-                rec.update({"decode_GBps": round(fld.nbytes / dec_s / 1e9, 2), "decode_ms": round(dec_s * 1e3, 3)})
+                dev.profile(True)
+                ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, walk)
+                kw_ = dev.zfp_times()
+                dev.profile(False)
+                rec.update({"decode_without_index_GBps": round(fld.nbytes / walk_s / 1e9, 3),
+                            "decode_without_index_ms": round(walk_s * 1e3, 1),
+                            "walk_kernel_ms": round(kw_["walk"][0], 3),
+                            "walk_MBps": round(fld.nbytes / kw_["walk"][0] / 1e3, 1)})
             rec["decode_algorithmic_GBps"] = round((fld.nbytes + clen) / dec_s / 1e9, 2)
+            # kernels alone: the encoder reads the samples and writes the
+            # stream (and index); the decoder reads the stream (and index) and
+            # writes the samples
+            rec["kernels"] = {"encode": roof(fld.nbytes + clen + ilen, kenc["encode"] + kenc["place"]),
+                              "decode": roof(fld.nbytes + clen + ilen, kdec["decode"])}
             if mname == "reversible":
                 rec["lossless_exact"] = dev.count_mismatch(d_out.ptr, d_in.ptr, fld.nbytes) == 0
-            out[f"{name} {mname}"] = rec
-            log(f"zfp {name} {mname}: {rec}")
+            out[case] = rec
+            log(f"zfp {case}: {rec}")
         del d_in, d_enc, d_out, d_idx
-    # 2,048 independent bricks (chunks) of 16^3..64^3 f32 at rate 8
+    # 2,048 independent fields (chunks) of 16^3..64^3 f32 at rate 8
     sides = rng.integers(16, 65, size=(2048, 3))
     sizes = [int(np.prod(sd)) * 4 for sd in sides]
     offs = np.concatenate([[0], np.cumsum([(n + 255) // 256 * 256 for n in sizes])])
@@ -348,19 +418,27 @@ def zfp_bench(ndic, dev, args):
              for i, sd in enumerate(sides)]
     cap = int(offs[-1]) + 2048 * 4096
     d_enc = dev.DeviceBuffer(cap, zero=True)
+    phase(ZFP_BRICKS, "encode")
     arr = ndic.zfp_encode_batch(d_raw.ptr, d_enc.ptr, cap, descs)
     ddescs = [ndic.make_zfp_desc(tuple(int(v) for v in sd), "float32", cfg, arr[i].out_offset, arr[i].out_len,
                                  int(offs[i])) for i, sd in enumerate(sides)]
     d_out = dev.DeviceBuffer(int(offs[-1]))
-    ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, ddescs)
-    t0 = time.perf_counter()
-    for _ in range(5):
+    phase(ZFP_BRICKS, "decode")
+
+    def bdec():
         ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, ddescs)
-    bs = (time.perf_counter() - t0) / 5
-    out["2048 bricks 16^3..64^3 f32 rate8 decode"] = {"GBps": round(sum(sizes) / bs / 1e9, 2),
-                                                      "ms": round(bs * 1e3, 3)}
-    log(f"zfp bricks: {out['2048 bricks 16^3..64^3 f32 rate8 decode']}")
+
+    bs = timed(bdec, ZFP_REPS)
+    kb = kernel_ms(bdec, ("decode",))
+    comp = sum(a.out_len for a in arr)
+    out[ZFP_BRICKS + " decode"] = {"GBps": round(sum(sizes) / bs / 1e9, 2), "ms": round(bs * 1e3, 3),
+                                   "kernels": {"decode": roof(sum(sizes) + comp, kb["decode"])}}
+    log(f"zfp bricks: {out[ZFP_BRICKS + ' decode']}")
+    assert next(sched, None) is None
+    if args.no_cpu:
+        return out
     # the oracle on one core, f32 field rate 8 (bounded sample: the first 8 z-slices)
+    sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
     sample = fields[0][1][:8]
     t0 = time.perf_counter()
@@ -375,7 +453,8 @@ def zfp_bench(ndic, dev, args):
 
 
 # kernels of the decode path, as their names appear in rocprofv3's CSVs
-PROFILED = {"parse_planes_kernel": "::parse_planes_kernel(", "ht_decode_kernel": "::ht_decode_kernel<",
+PROFILED = {"parse_planes_kernel": "::parse_planes_kernel(", "ht_vlc_kernel": "::ht_vlc_kernel<",
+            "ht_magsgn_kernel": "::ht_magsgn_kernel(", "ht_decode_kernel": "::ht_decode_kernel<",
             "ht_refine_kernel": "::ht_refine_kernel(", "dwt_inv_level": "::dwt_inv_level(",
             "unshift_kernel": "::unshift_kernel(", "dwt_lift_kernel": "::dwt_lift_kernel(",
             "lift_col_kernel": "::lift_col_kernel<", "ht_decode_wave_kernel": "::ht_decode_wave_kernel("}
@@ -385,21 +464,28 @@ def write_counters(root: Path, cus: int = 256) -> dict:
     """profiles/c2_decode_counters.json from one rocprofv3 kernel trace
     (root/prof/trace: this script with --steps 2 --warmup 1 --no-autotune ...)
     and three PMC passes (root/pmc/{fetch,write,sq}: --steps 1 --warmup 0
-    --no-autotune --copy-calibration 1073741824 ...).  HBM bytes per window
-    are FETCH_SIZE / WRITE_SIZE scaled by what they read for the 1 GiB copy
-    kernel; per-window times come from the stage-profiled pass (windows one
-    after the other); GRBM_GUI_ACTIVE is summed over the 8 XCDs in the CSV."""
+    --no-autotune --copy-calibration 1073741824 ...).  Everything is read
+    from the stage-profiled pass (its batch windows one after the other),
+    which two marker copy kernels delimit: HBM bytes per window are
+    FETCH_SIZE / WRITE_SIZE scaled by what they read for the 1 GiB copy
+    kernel, times come from the trace; GRBM_GUI_ACTIVE is summed over the 8
+    XCDs in the CSV."""
     import csv
     import glob
     from collections import defaultdict
 
     def rows(pattern):
         with open(glob.glob(str(root / pattern))[0]) as f:
-            return list(csv.DictReader(f))
+            return sorted(csv.DictReader(f), key=lambda r: int(r["Dispatch_Id"]))
 
-    def decode_phase(rs):
+    def profiled_pass(rs):
+        """the rows between the two marker copies that follow the first decode"""
         first = min(int(r["Dispatch_Id"]) for r in rs if "walk_chunks_kernel" in r["Kernel_Name"])
-        return [r for r in rs if int(r["Dispatch_Id"]) >= first]
+        marks = sorted({int(r["Dispatch_Id"]) for r in rs
+                        if "copy_kernel" in r["Kernel_Name"] and int(r["Dispatch_Id"]) > first})[:2]
+        if len(marks) != 2:
+            raise SystemExit(f"no marked stage-profiled pass in {root}")
+        return [r for r in rs if marks[0] < int(r["Dispatch_Id"]) < marks[1]]
 
     def per_kernel(rs):
         vals, disp = defaultdict(lambda: defaultdict(float)), defaultdict(set)
@@ -417,14 +503,12 @@ def write_counters(root: Path, cus: int = 256) -> dict:
     fetch, write, sq = (rows(f"pmc/{p}/*counter_collection.csv") for p in ("fetch", "write", "sq"))
     fscale = (1 << 30) / 1024 / calib(fetch, "FETCH_SIZE")
     wscale = (1 << 30) / 1024 / calib(write, "WRITE_SIZE")
-    F, nF = per_kernel(decode_phase(fetch))
-    W, _ = per_kernel(decode_phase(write))
-    S, _ = per_kernel(decode_phase(sq))
-    windows = nF.get("parse_planes_kernel", 16) // 2  # a timed and a stage-profiled pass
-    dec = decode_phase(rows("prof/trace/*kernel_trace.csv"))
-    walks = [i for i, r in enumerate(dec) if "walk_chunks_kernel" in r["Kernel_Name"]]
+    F, nF = per_kernel(profiled_pass(fetch))
+    W, _ = per_kernel(profiled_pass(write))
+    S, _ = per_kernel(profiled_pass(sq))
+    windows = nF.get("parse_planes_kernel", 8)
     dur = defaultdict(float)
-    for r in dec[walks[-windows]:]:
+    for r in profiled_pass(rows("prof/trace/*kernel_trace.csv")):
         for k, pat in PROFILED.items():
             if pat in r["Kernel_Name"]:
                 dur[k] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
@@ -433,9 +517,9 @@ def write_counters(root: Path, cus: int = 256) -> dict:
     for k in PROFILED:
         if k not in F:
             continue
-        fb = F[k]["FETCH_SIZE"] * 1024 * fscale / (2 * windows)
-        wb = W[k]["WRITE_SIZE"] * 1024 * wscale / (2 * windows)
-        rec = {"dispatches_per_pass": nF[k] // 2, "fetch_bytes": round(fb), "write_bytes": round(wb),
+        fb = F[k]["FETCH_SIZE"] * 1024 * fscale / windows
+        wb = W[k]["WRITE_SIZE"] * 1024 * wscale / windows
+        rec = {"dispatches_per_pass": nF[k], "fetch_bytes": round(fb), "write_bytes": round(wb),
                "traffic_bytes": round(fb + wb), "avg_launch_ms": round(dur[k] / windows, 3)}
         s = S.get(k)
         if s and s.get("GRBM_GUI_ACTIVE"):
@@ -451,6 +535,79 @@ def write_counters(root: Path, cus: int = 256) -> dict:
     return out
 
 
+ZFP_COUNTERS = ROOT / "profiles" / "zfp_counters.json"
+ZFP_KERNELS = {"encode": ["zfp_encode_kernel", "zfp_place_kernel", "zfp_index_write_kernel"],
+               "decode": ["zfp_decode_kernel"], "walk": ["zfp_index_kernel", "zfp_decode_kernel"]}
+
+
+def write_zfp_counters(root: Path) -> dict:
+    """profiles/zfp_counters.json from a rocprofv3 kernel trace of
+    `bench.py --zfp-only` (root/zprof/trace) and two PMC passes with
+    --copy-calibration 1073741824 (root/zpmc/{fetch,write}).  The marker copy
+    kernels split the dispatches into zfp_schedule()'s phases; per phase and
+    kernel: dispatches, average duration, and HBM bytes per dispatch
+    (FETCH_SIZE / WRITE_SIZE scaled by what they read for the 1 GiB copy)."""
+    import csv
+    import glob
+    import re
+    from collections import defaultdict
+
+    def rows(pattern):
+        with open(glob.glob(str(root / pattern))[0]) as f:
+            return sorted(csv.DictReader(f), key=lambda r: int(r["Dispatch_Id"]))
+
+    def short(name):
+        m = re.search(r"(\w+_kernel|\w+)\s*[<(]", name.replace("(anonymous namespace)::", ""))
+        return m.group(1) if m else name
+
+    def phases(rs, calib_first):
+        """dispatch rows grouped per schedule phase, after the calibration copy"""
+        marks = [i for i, r in enumerate(rs) if "copy_kernel" in r["Kernel_Name"]]
+        if calib_first:
+            marks = marks[1:]
+        sched = zfp_schedule()
+        if len(marks) != len(sched):
+            raise SystemExit(f"{len(marks)} marker copies in {root}, the schedule has {len(sched)} phases")
+        bounds = marks + [len(rs)]
+        return [(sched[j], rs[bounds[j] + 1:bounds[j + 1]]) for j in range(len(sched))]
+
+    trace = phases(rows("zprof/trace/*kernel_trace.csv"), False)
+    fetch_rows = rows("zpmc/fetch/*counter_collection.csv")
+    write_rows = rows("zpmc/write/*counter_collection.csv")
+
+    def per_dispatch(rs, counter):
+        # counter rows (one per dispatch and counter) reduced to one value per dispatch
+        out = defaultdict(float)
+        names = {}
+        for r in rs:
+            if r["Counter_Name"] == counter:
+                out[int(r["Dispatch_Id"])] += float(r["Counter_Value"])
+                names[int(r["Dispatch_Id"])] = r["Kernel_Name"]
+        return [{"Dispatch_Id": d, "Kernel_Name": names[d], "v": out[d]} for d in sorted(out)]
+
+    F, W = per_dispatch(fetch_rows, "FETCH_SIZE"), per_dispatch(write_rows, "WRITE_SIZE")
+    fscale = (1 << 30) / 1024 / next(r["v"] for r in F if "copy_kernel" in r["Kernel_Name"])
+    wscale = (1 << 30) / 1024 / next(r["v"] for r in W if "copy_kernel" in r["Kernel_Name"])
+    Fp, Wp = phases(F, True), phases(W, True)
+    out = {"workload": "ZFP fields f32 64x256x256 and f64 4x64x128x128 in five modes; 2,048 f32 fields 16^3..64^3 "
+                       "at rate 8", "fetch_scale": fscale, "write_scale": wscale, "phases": []}
+    for (ph, trs), (_, frs), (_, wrs) in zip(trace, Fp, Wp):
+        case, name, calls = ph
+        kern = {}
+        for k in ZFP_KERNELS[name]:
+            ds = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in trs if short(r["Kernel_Name"]) == k]
+            fb = [r["v"] * 1024 * fscale for r in frs if short(r["Kernel_Name"]) == k]
+            wb = [r["v"] * 1024 * wscale for r in wrs if short(r["Kernel_Name"]) == k]
+            if not ds:
+                continue
+            kern[k] = {"dispatches": len(ds), "avg_ms": round(sum(ds) / len(ds), 4),
+                       "fetch_bytes": round(sum(fb) / len(fb)) if fb else None,
+                       "write_bytes": round(sum(wb) / len(wb)) if wb else None}
+        out["phases"].append({"case": case, "phase": name, "calls": calls, "kernels": kern})
+    ZFP_COUNTERS.write_text(json.dumps(out, indent=1) + "\n")
+    return out
+
+
 def counters_of(kernel: str, args, avg_ms: float):
     """The committed rocprofv3 record of a decode kernel on this same workload
     (HBM bytes per launch from FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy
@@ -459,8 +616,15 @@ def counters_of(kernel: str, args, avg_ms: float):
     than 20 %)."""
     if args.workload != "c2" or args.chunks != 4096 or args.seed != 2 or not COUNTERS.exists():
         return None
-    rec = json.loads(COUNTERS.read_text()).get("kernels", {}).get(kernel)
-    if not rec or abs(rec.get("avg_launch_ms", 0.0) - avg_ms) > 0.2 * avg_ms:
+    recs = json.loads(COUNTERS.read_text()).get("kernels", {})
+    parts = [recs.get(k) for k in kernel.split("+")]
+    if not all(parts):
+        return None
+    if len(parts) == 1:
+        rec = parts[0]
+    else:  # kernels launched back to back as one stage: their sums
+        rec = {k: sum(p[k] for p in parts) for k in ("fetch_bytes", "write_bytes", "traffic_bytes", "avg_launch_ms")}
+    if abs(rec.get("avg_launch_ms", 0.0) - avg_ms) > 0.2 * avg_ms:
         return None
     return rec
 
@@ -644,12 +808,19 @@ def main():
                     help="keep the library's built-in launch parameters instead of tuning them first")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first run one device copy kernel of this many bytes (traffic-counter scale)")
+    ap.add_argument("--zfp-only", action="store_true", help="run only the ZFP section and print its JSON")
+    ap.add_argument("--zfp-counters-from", default=None,
+                    help="only write profiles/zfp_counters.json from the rocprofv3 CSVs of --zfp-only runs under "
+                         "this directory (zprof/trace and zpmc/{fetch,write}) and exit")
     ap.add_argument("--counters-from", default=None,
                     help="only write profiles/c2_decode_counters.json from the rocprofv3 CSVs under this "
                          "directory (prof/trace and pmc/{fetch,write,sq}) and exit")
     args = ap.parse_args()
     if args.counters_from:
         print(json.dumps(write_counters(Path(args.counters_from)), indent=1))
+        return
+    if args.zfp_counters_from:
+        print(json.dumps(write_zfp_counters(Path(args.zfp_counters_from)), indent=1))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (got --gpus {args.gpus})",
@@ -668,6 +839,9 @@ def main():
         dev.synchronize()
         log(f"copy calibration: {args.copy_calibration} bytes")
         del a, b
+    if args.zfp_only:
+        print(json.dumps({"zfp": zfp_bench(ndic, dev, args)}), flush=True)
+        return
     tuning = None
     if args.autotune:
         # the project's autotuner picks the coder forms and the work-queue
@@ -715,15 +889,22 @@ def main():
         plain_gbps = raw_total / pdt / 1e9
         exact &= exact_now()
         log(f"plain decode: {pdt * 1e3:.1f} ms/step, {plain_gbps:.2f} GB/s")
-    # stage profile (one extra pass, HIP events on the library's stream)
+    # stage profile (one extra pass, HIP events on the library's stream),
+    # between two marker copy kernels that delimit it in a kernel trace
+    marker = dev.DeviceBuffer(8192, zero=True)
+    dev.copy(marker.ptr + 4096, marker.ptr, 256)
     dev.profile(True)
     run()
     dev.synchronize()
-    ms, by, windows = dev.stage_times(5)
+    dev.copy(marker.ptr + 4096, marker.ptr, 256)
+    dev.synchronize()
+    ms, by, windows = dev.stage_times(8)
     dev.profile(False)
     names = ["parse", "ht_decode", "dwt_inverse", "dc_unshift", "lift_inverse_narrow"]
     stages = {names[i]: {"ms": round(ms[i], 3), "GBps": round(by[i] / ms[i] / 1e6, 1) if ms[i] > 0 else None}
               for i in range(5)}
+    if ms[5] > 0:
+        stages["ht_decode"]["passes_ms"] = {k: round(ms[5 + j], 3) for j, k in enumerate(HT_PASSES)}
     dom = max(range(5), key=lambda i: ms[i])
     achieved = by[dom] / ms[dom] / 1e6 if ms[dom] > 0 else 0.0
     log("stages:", json.dumps(stages))
@@ -736,9 +917,18 @@ def main():
                                 "avg_launch_ms": round(per_launch[i], 3),
                                 "traffic": (counters_of(KERNELS[i], args, per_launch[i]) or {}).get("traffic_bytes")}
                    for i in (2, 4)}
-    ht_rec = counters_of("ht_decode_kernel", args, per_launch[1]) or {}
-    entropy = {k: ht_rec.get(k) for k in ("occupancy_waves_per_simd", "occupancy_pct", "valu_busy_pct",
-                                          "valu_lane_util_pct", "valu_insts_per_wave")} if ht_rec else None
+    # the HT stage's passes apart (HIP events between its launches), with
+    # their occupancy and VALU issue from the committed counters
+    entropy = {}
+    for j, k in enumerate(HT_PASSES):
+        if ms[5 + j] <= 0:
+            continue
+        rec = counters_of(k, args, ms[5 + j] / max(windows, 1)) or {}
+        entropy[k] = {"avg_launch_ms": round(ms[5 + j] / max(windows, 1), 3),
+                      **{c: rec.get(c) for c in ("traffic_bytes", "occupancy_waves_per_simd", "occupancy_pct",
+                                                 "valu_busy_pct", "valu_lane_util_pct", "valu_insts_per_wave")
+                         if rec}}
+    entropy = entropy or None
     zfp = zfp_bench(ndic, dev, args) if args.zfp else None
     single, single_cases = single_chunk_lines(ndic, dev, args)
     del d_out

@@ -38,9 +38,21 @@ ndic_status ndic_dev_ht_decode_block(const uint8_t* coded, size_t len, uint32_t*
  * the summed device milliseconds (HIP events on the library's stream) and
  * the algorithmic bytes moved, and returns the number of batch windows timed.
  * Stages: 0 container/codestream/packet parse, 1 HT block decode, 2 inverse
- * DWT, 3 DC un-shift, 4 inverse nd_lift with dtype narrowing. */
+ * DWT, 3 DC un-shift, 4 inverse nd_lift with dtype narrowing; when the HT
+ * stage runs as two passes, entries 5, 6 and 7 split stage 1 into the VLC/MEL
+ * pass, the MagSgn pass and the SigProp/MagRef passes (times only, bytes 0).
+ * n is at most 8. */
 void ndic_dev_profile(int32_t enable);
 size_t ndic_dev_stage_times(double* ms, double* bytes, size_t n);
+
+/* Device time of the ZFP coders' kernels while the profiler is on (HIP
+ * events around their launches on the call's stream): per group, the summed
+ * milliseconds and the number of timed spans.  Groups: 0 block encode
+ * (zfp_encode_kernel), 1 placement of variable-size blocks and their index
+ * (zfp_place_kernel, zfp_index_write_kernel), 2 block decode
+ * (zfp_decode_kernel), 3 serial stream walk (zfp_index_kernel).  Returns the
+ * number of groups filled (at most n). */
+size_t ndic_dev_zfp_times(double* ms, uint64_t* spans, size_t n);
 
 /* Device memory on the library's own HIP runtime and stream, for callers
  * (tests, the benchmark) that hold no other GPU framework.  kind of

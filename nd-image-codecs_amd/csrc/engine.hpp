@@ -5,6 +5,7 @@
 #include <cstring>
 #include <functional>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -119,6 +120,25 @@ struct DwtChunk {
 void level_dims(uint64_t w, uint64_t h, uint32_t level, uint64_t* ow, uint64_t* oh);
 void run_dwt_forward(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t stream);
 void run_dwt_inverse(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t stream);
+
+// Device time of the ZFP coders' kernels while the stage profiler is on
+// (ndic_dev_profile): HIP events around each group of launches, summed per
+// group once the stream has synchronised (collect).  Does nothing when the
+// profiler is off.
+class ZfpClock {
+ public:
+  enum Group { kEncode, kPlace, kDecode, kWalk, kGroups };
+  explicit ZfpClock(hipStream_t s);
+  ~ZfpClock();
+  void begin(Group g);
+  void end();
+  void collect();
+
+ private:
+  hipStream_t s_;
+  bool on_;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> spans_;
+};
 
 // Runs body, mapping a Failure to the C-ABI status and error record.
 ndic_status guard_call(ndic_error* err, const std::function<void()>& body);

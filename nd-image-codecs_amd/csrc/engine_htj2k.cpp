@@ -130,30 +130,85 @@ struct StageProfile {
   bool on = false;
   double ms[8] = {0}, bytes[8] = {0};
   uint64_t calls = 0;
+  double zfp_ms[ZfpClock::kGroups] = {0};
+  uint64_t zfp_spans[ZfpClock::kGroups] = {0};
 };
 StageProfile g_prof;
+
+ZfpClock::ZfpClock(hipStream_t s) : s_(s), on_(g_prof.on) {}
+
+ZfpClock::~ZfpClock() {
+  for (auto& sp : spans_) {
+    (void)hipEventDestroy(sp.second.first);
+    if (sp.second.second) (void)hipEventDestroy(sp.second.second);
+  }
+}
+
+void ZfpClock::begin(Group g) {
+  if (!on_) return;
+  hipEvent_t a = nullptr;
+  NDIC_HIP(hipEventCreate(&a));
+  spans_.push_back({(int)g, {a, nullptr}});
+  NDIC_HIP(hipEventRecord(a, s_));
+}
+
+void ZfpClock::end() {
+  if (!on_ || spans_.empty() || spans_.back().second.second) return;
+  hipEvent_t b = nullptr;
+  NDIC_HIP(hipEventCreate(&b));
+  spans_.back().second.second = b;
+  NDIC_HIP(hipEventRecord(b, s_));
+}
+
+void ZfpClock::collect() {
+  if (!on_) return;
+  for (auto& sp : spans_) {
+    if (!sp.second.second) continue;
+    float ms = 0;
+    NDIC_HIP(hipEventElapsedTime(&ms, sp.second.first, sp.second.second));
+    g_prof.zfp_ms[sp.first] += ms;
+    g_prof.zfp_spans[sp.first] += 1;
+  }
+  for (auto& sp : spans_) {
+    (void)hipEventDestroy(sp.second.first);
+    if (sp.second.second) (void)hipEventDestroy(sp.second.second);
+  }
+  spans_.clear();
+}
 
 namespace {
 
 struct StageTimer {
   hipEvent_t ev[8];
-  int n = 0;
+  int n = 0, nsub = 0;
   hipStream_t s;
   explicit StageTimer(hipStream_t st) : s(st) {
     if (!g_prof.on) return;
     for (auto& e : ev) NDIC_HIP(hipEventCreate(&e));
   }
   void mark() {
-    if (g_prof.on && n < 8) NDIC_HIP(hipEventRecord(ev[n++], s));
+    if (g_prof.on && n < 6) NDIC_HIP(hipEventRecord(ev[n++], s));
   }
-  // after the stream is synchronised: stage i spans marks i..i+1
+  // inside the HT stage (marks 1..2): the ends of its VLC and MagSgn passes
+  void sub() {
+    if (g_prof.on && nsub < 2) NDIC_HIP(hipEventRecord(ev[6 + nsub++], s));
+  }
+  // after the stream is synchronised: stage i spans marks i..i+1; with the
+  // two sub-marks, entries 5..7 split the HT stage into its VLC pass, its
+  // MagSgn pass and the refinement passes
   void collect(const double* bytes) {
     if (!g_prof.on) return;
-    for (int i = 0; i + 1 < n; ++i) {
+    auto add = [](int i, hipEvent_t a, hipEvent_t b, double by) {
       float ms = 0;
-      NDIC_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      NDIC_HIP(hipEventElapsedTime(&ms, a, b));
       g_prof.ms[i] += ms;
-      g_prof.bytes[i] += bytes[i];
+      g_prof.bytes[i] += by;
+    };
+    for (int i = 0; i + 1 < n; ++i) add(i, ev[i], ev[i + 1], bytes[i]);
+    if (nsub == 2 && n > 2) {
+      add(5, ev[1], ev[6], 0);
+      add(6, ev[6], ev[7], 0);
+      add(7, ev[7], ev[2], 0);
     }
     g_prof.calls += 1;
   }
@@ -307,7 +362,10 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   launch_parse_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_gp, d_gb, s);
   timer->mark();
   if (split) {
-    launch_ht_decode_split(d_tasks, ntasks, d_rec, s);
+    launch_ht_vlc(d_tasks, ntasks, d_rec, s);
+    timer->sub();
+    launch_ht_magsgn(d_tasks, ntasks, d_rec, s);
+    timer->sub();
   } else if (lane_decode) {
     if (sort) launch_ht_order(d_tasks, ntasks, d_counts, d_order, s);
     launch_ht_decode(d_tasks, d_order, ntasks, s);
@@ -873,8 +931,17 @@ extern "C" void ndic_dev_profile(int32_t enable) {
   ndic::g_prof.on = enable != 0;
 }
 
+extern "C" size_t ndic_dev_zfp_times(double* ms, uint64_t* spans, size_t n) {
+  size_t k = n < (size_t)ndic::ZfpClock::kGroups ? n : (size_t)ndic::ZfpClock::kGroups;
+  for (size_t i = 0; i < k; ++i) {
+    ms[i] = ndic::g_prof.zfp_ms[i];
+    spans[i] = ndic::g_prof.zfp_spans[i];
+  }
+  return k;
+}
+
 extern "C" size_t ndic_dev_stage_times(double* ms, double* bytes, size_t n) {
-  size_t k = n < 5 ? n : 5;
+  size_t k = n < 8 ? n : 8;
   for (size_t i = 0; i < k; ++i) {
     ms[i] = ndic::g_prof.ms[i];
     bytes[i] = ndic::g_prof.bytes[i];

@@ -464,7 +464,10 @@ void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out,
       ctx.up(d_hdr, hdr.data(), hdr.size() * sizeof(ZfpHeader), s);
       launch_zfp_headers(d_hdr, (uint32_t)hdr.size(), s);
     }
+    ZfpClock clock(s);
+    clock.begin(ZfpClock::kEncode);
     launch_zfp_encode(d_ch, (uint32_t)ch.size(), total, cls / 8, cls % 8, s);
+    clock.end();
     if (vblocks) {
       zfp_exclusive_scan(scan_tmp, scan_bytes, offs, scan, total, s);
       launch_zfp_totals(d_ch, (uint32_t)ch.size(), scan, totals, s);
@@ -511,10 +514,13 @@ void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out,
         ctx.up(d_hdr, hdr.data(), hdr.size() * sizeof(ZfpHeader), s);
         launch_zfp_headers(d_hdr, (uint32_t)hdr.size(), s);
       }
+      clock.begin(ZfpClock::kPlace);
       launch_zfp_place(d_ch, (uint32_t)ch.size(), total, scan, s);
       if (any_index) launch_zfp_index_write(d_ch, (uint32_t)ch.size(), total, scan, s);
+      clock.end();
     }
     NDIC_HIP(hipStreamSynchronize(s));
+    clock.collect();
   }
 }
 
@@ -594,10 +600,18 @@ void zfp_decode_items(Context& ctx, std::vector<ZfpItem>& items, hipStream_t s) 
     ZfpChunk* d_ch = (ZfpChunk*)meta;
     ctx.up(d_ch, ch.data(), ch.size() * sizeof(ZfpChunk), s);
     if (indexed) NDIC_HIP(hipMemsetAsync(bad, 0, ch.size() * 4, s));
-    if (walk) launch_zfp_index(d_ch, (uint32_t)ch.size(), cls / 8, cls % 8, s);
+    ZfpClock clock(s);
+    if (walk) {
+      clock.begin(ZfpClock::kWalk);
+      launch_zfp_index(d_ch, (uint32_t)ch.size(), cls / 8, cls % 8, s);
+      clock.end();
+    }
+    clock.begin(ZfpClock::kDecode);
     launch_zfp_decode(d_ch, (uint32_t)ch.size(), total, cls / 8, cls % 8, nullptr, s);
+    clock.end();
     const uint32_t* flags = indexed ? (const uint32_t*)ctx.down(bad, ch.size() * 4, s) : nullptr;
     NDIC_HIP(hipStreamSynchronize(s));
+    clock.collect();
     for (size_t j = 0; flags && j < ch.size(); ++j)
       if (flags[j]) {
         group[j]->status = NDIC_CODESTREAM;
@@ -986,9 +1000,13 @@ extern "C" ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_ou
       uint8_t* w = (uint8_t*)ctx.work.ensure(sz_ch + bc.bricks.size() * sizeof(ZfpBrick));
       ctx.up(w, bc.chunks.data(), bc.chunks.size() * sizeof(ZfpChunk), s);
       ctx.up(w + sz_ch, bc.bricks.data(), bc.bricks.size() * sizeof(ZfpBrick), s);
+      ZfpClock clock(s);
+      clock.begin(ZfpClock::kDecode);
       launch_zfp_decode((ZfpChunk*)w, (uint32_t)bc.chunks.size(), bc.bricks.size(), bc.dims, bc.scalar,
                         (ZfpBrick*)(w + sz_ch), s);
+      clock.end();
       NDIC_HIP(hipStreamSynchronize(s));
+      clock.collect();
     }
   });
   if (st != NDIC_OK) return st;

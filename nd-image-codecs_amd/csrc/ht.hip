@@ -2686,13 +2686,22 @@ template <class K> uint32_t persistent_grid(K kernel, uint64_t grains) {
 }
 }  // namespace
 
-void launch_ht_decode_split(const HtDecTask* d_tasks, uint64_t n, uint32_t* d_rec, hipStream_t stream) {
+void launch_ht_vlc(const HtDecTask* d_tasks, uint64_t n, uint32_t* d_rec, hipStream_t stream) {
   if (n == 0) return;
   hipLaunchKernelGGL(ht_vlc_kernel<4>, dim3((uint32_t)ceil_div(n, 256)), dim3(256), 0, stream, d_tasks, n, d_rec);
   NDIC_HIP(hipGetLastError());
+}
+
+void launch_ht_magsgn(const HtDecTask* d_tasks, uint64_t n, const uint32_t* d_rec, hipStream_t stream) {
+  if (n == 0) return;
   const uint32_t gx = (uint32_t)std::min<uint64_t>(n, 1u << 20), gy = (uint32_t)ceil_div(n, gx);
-  hipLaunchKernelGGL(ht_magsgn_kernel, dim3(gx, gy), dim3(64), 0, stream, d_tasks, n, (const uint32_t*)d_rec);
+  hipLaunchKernelGGL(ht_magsgn_kernel, dim3(gx, gy), dim3(64), 0, stream, d_tasks, n, d_rec);
   NDIC_HIP(hipGetLastError());
+}
+
+void launch_ht_decode_split(const HtDecTask* d_tasks, uint64_t n, uint32_t* d_rec, hipStream_t stream) {
+  launch_ht_vlc(d_tasks, n, d_rec, stream);
+  launch_ht_magsgn(d_tasks, n, d_rec, stream);
 }
 
 void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, uint64_t* queue, hipStream_t stream) {

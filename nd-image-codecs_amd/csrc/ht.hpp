@@ -62,6 +62,9 @@ void launch_ht_decode_wave(const HtDecTask* d_tasks, uint64_t n, uint64_t* queue
 // decode one wavefront per block from those records.
 constexpr uint64_t kHtRecBytes = 32 * 32 * 4;
 void launch_ht_decode_split(const HtDecTask* d_tasks, uint64_t n, uint32_t* d_rec, hipStream_t stream);
+// its two passes, for callers that time them apart
+void launch_ht_vlc(const HtDecTask* d_tasks, uint64_t n, uint32_t* d_rec, hipStream_t stream);
+void launch_ht_magsgn(const HtDecTask* d_tasks, uint64_t n, const uint32_t* d_rec, hipStream_t stream);
 // SigProp/MagRef of the tasks with more than one coding pass (after launch_ht_decode)
 void launch_ht_refine(const HtDecTask* d_tasks, uint64_t n, hipStream_t stream);
 void launch_ht_encode(const HtEncTask* d_tasks, uint64_t n, hipStream_t stream);

@@ -45,6 +45,7 @@ def _lib():
             ("ndic_dev_hash_async", C.c_int, [P, S, P, E]),
             ("ndic_dev_profile", None, [C.c_int32]),
             ("ndic_dev_stage_times", C.c_size_t, [C.POINTER(C.c_double), C.POINTER(C.c_double), S]),
+            ("ndic_dev_zfp_times", C.c_size_t, [C.POINTER(C.c_double), C.POINTER(C.c_uint64), S]),
         ]:
             fn = getattr(lib, name)
             fn.restype = res
@@ -185,6 +186,18 @@ def stage_times(n: int = 5):
     by = (C.c_double * n)()
     windows = _lib().ndic_dev_stage_times(ms, by, n)
     return list(ms), list(by), int(windows)
+
+
+ZFP_GROUPS = ("encode", "place", "decode", "walk")
+
+
+def zfp_times() -> dict:
+    """Device milliseconds of the ZFP kernel groups since profile(True):
+    {group: (ms, spans)} for encode, place, decode and walk."""
+    ms = (C.c_double * 4)()
+    sp = (C.c_uint64 * 4)()
+    _lib().ndic_dev_zfp_times(ms, sp, 4)
+    return {g: (ms[i], int(sp[i])) for i, g in enumerate(ZFP_GROUPS)}
 
 
 def as_array(buf: DeviceBuffer, dtype, offset: int = 0, count: Optional[int] = None) -> np.ndarray:
