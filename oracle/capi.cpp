@@ -119,10 +119,26 @@ int orc_encode_image(const int32_t* samples, size_t width, size_t height, unsign
   });
 }
 
-int orc_decode_codestream(const uint8_t* data, size_t len, int max_res, int prefix, int32_t** out, size_t* width,
-                          size_t* height, orc_err* err) {
+int orc_dwt97(int forward, float* plane, size_t width, size_t height, unsigned levels, orc_err* err) {
   return guard(err, [&] {
-    orc::Decoded d = orc::decode_codestream(data, len, max_res, prefix != 0);
+    if (forward) orc::dwt97_forward(plane, width, height, levels);
+    else orc::dwt97_inverse(plane, width, height, levels);
+  });
+}
+
+int orc_encode_image_irreversible(const int32_t* samples, size_t width, size_t height, unsigned depth, int is_signed,
+                                  unsigned levels, unsigned eps, unsigned mu, unsigned guard_bits, uint8_t** out,
+                                  size_t* out_len, orc_err* err) {
+  return guard(err, [&] {
+    emit(orc::encode_image_irreversible(samples, width, height, depth, is_signed != 0, levels, eps, mu, guard_bits),
+         out, out_len);
+  });
+}
+
+int orc_decode_codestream(const uint8_t* data, size_t len, int max_res, int prefix, int irreversible, int32_t** out,
+                          size_t* width, size_t* height, orc_err* err) {
+  return guard(err, [&] {
+    orc::Decoded d = orc::decode_codestream(data, len, max_res, prefix != 0, irreversible != 0);
     *out = (int32_t*)std::malloc(d.samples.size() * 4 + 4);
     std::memcpy(*out, d.samples.data(), d.samples.size() * 4);
     *width = d.width;

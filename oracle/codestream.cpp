@@ -12,6 +12,7 @@
 //   chunk core: crates/ndic-zarr/src/htj2k.rs:113-441
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstring>
 #include <sstream>
 
@@ -352,6 +353,20 @@ struct Quant {
       nb = e > 0 ? e - 1 : 0;
     }
     return (uint8_t)(nb + guard_bits);
+  }
+  // quant.rs:204-221: the irreversible step of (res, band) with the band gain
+  // (T.800 E-3, relative to the samples' nominal range 2^depth)
+  float irrev_delta(unsigned res, unsigned band) const {
+    const float gains[4] = {1.0f, 2.0f, 2.0f, 4.0f};
+    size_t idx = res == 0 ? 0 : (res - 1) * 3 + band;
+    idx = std::min(idx, values.empty() ? 0 : values.size() - 1);
+    uint16_t v = values[idx];
+    uint32_t eps = v >> 11;
+    float mantissa = (float)((v & 0x7FF) | 0x800) * gains[std::min(band, 3u)] / (float)(1u << 11);
+    float two_eps;
+    uint32_t bits = (127 + eps) << 23;
+    std::memcpy(&two_eps, &bits, 4);
+    return mantissa / two_eps;
   }
   uint32_t magb() const {
     uint32_t b = 0;
@@ -706,7 +721,9 @@ Codestream parse_codestream(const uint8_t* data, size_t len, bool strict) {
   return cs;
 }
 
-void check_supported(const Codestream& cs) {
+bool irreversible(const Codestream& cs) { return cs.cod.wavelet == 0 && cs.quant.style == 2; }
+
+void check_supported(const Codestream& cs, bool allow_irreversible = false) {
   const Siz& s = cs.siz;
   size_t tx = (s.xsiz - s.xtosiz + s.xtsiz - 1) / s.xtsiz, ty = (s.ysiz - s.ytosiz + s.ytsiz - 1) / s.ytsiz;
   if (tx != 1 || ty != 1 || s.xosiz || s.yosiz || s.xtosiz || s.ytosiz)
@@ -714,7 +731,14 @@ void check_supported(const Codestream& cs) {
   for (auto& c : s.comps)
     if (c.xr != 1 || c.yr != 1) fail(kUnsupported, "component subsampling");
   if (!(cs.cod.cb_style & 0x40)) fail(kUnsupported, "non-HT (legacy J2K-1) block coding");
-  if (cs.cod.wavelet != 1 || cs.quant.style != 0) fail(kUnsupported, "irreversible (9/7) codestreams");
+  if (allow_irreversible && irreversible(cs)) {
+    // the decode-only 9/7 path: one component, no ICT, depths the int32
+    // level-shifted samples hold
+    if (cs.siz.comps.size() != 1 || cs.cod.mct) fail(kUnsupported, "irreversible codestreams with several components");
+    if (cs.siz.comps[0].depth > 30) fail(kUnsupported, "irreversible codestreams deeper than 30 bits");
+  } else if (cs.cod.wavelet != 1 || cs.quant.style != 0) {
+    fail(kUnsupported, "irreversible (9/7) codestreams");
+  }
   if (cs.cod.layers != 1) fail(kUnsupported, "multiple quality layers");
   if ((cs.cod.progression == 3 || cs.cod.progression == 4) && (cs.cod.scod & 1))
     fail(kUnsupported, "PCRL/CPRL with explicit precinct sizes");
@@ -809,8 +833,31 @@ void precinct_block_range(size_t bw, size_t bh, unsigned ppx_log, unsigned ppy_l
 
 // reader.rs:478-645 (single component is all the htj2k chunk core accepts;
 // multi-component streams decode each component the same way)
-std::vector<std::vector<int32_t>> decode_to_resolution(const Codestream& cs, unsigned max_res, size_t* ow, size_t* oh) {
-  check_supported(cs);
+// The step that turns the block decoder's aligned magnitude (bit 30 down,
+// with the half-bin bit below the last decoded plane: decode.rs:5-6) into a
+// coefficient of the unscaled 9/7 lifting (dwt/mod.rs:7-9) for a decode to
+// resolution max_res: irrev_delta x 2^depth (T.800 E-3's 2^R_b) x the K
+// factor that lifting leaves in the band relative to the output LL
+// (K^2 per 2-D level of low-pass: LL of level L holds K^2L, HL/LH of level d
+// K^2(d-1), HH of level d K^2(d-2), all over K^2(L-max_res)) / 2^(31-K_max).
+// Double precision, rounded once to f32.
+float irrev_scale(const Quant& q, unsigned res, unsigned band, unsigned levels, unsigned max_res, unsigned depth) {
+  constexpr double kK = 1.230174104914001;  // T.800 Table F.4
+  constexpr double kK2 = kK * kK;
+  unsigned d = res == 0 ? levels : levels - res + 1;
+  int e = res == 0 ? (int)levels : band == 3 ? (int)d - 2 : (int)d - 1;
+  e -= (int)(levels - max_res);
+  double f = 1.0;
+  if (e < 0) f = 1.0 / kK2;
+  for (int i = 0; i < e; ++i) f *= kK2;
+  double s = (double)q.irrev_delta(res, band) * f;
+  return (float)std::ldexp(s, (int)depth - 31 + (int)q.k_max(res, band));
+}
+
+std::vector<std::vector<int32_t>> decode_to_resolution(const Codestream& cs, unsigned max_res, size_t* ow, size_t* oh,
+                                                       bool allow_irreversible = false) {
+  check_supported(cs, allow_irreversible);
+  const bool irrev = irreversible(cs);
   max_res = std::min<unsigned>(max_res, cs.cod.decomps);
   unsigned levels = cs.cod.decomps;
   size_t width = cs.siz.xsiz, height = cs.siz.ysiz, ncomps = cs.siz.comps.size();
@@ -820,6 +867,7 @@ std::vector<std::vector<int32_t>> decode_to_resolution(const Codestream& cs, uns
     body.insert(body.end(), cs.data + s, cs.data + e);
   }
   std::vector<std::vector<int32_t>> planes(ncomps, std::vector<int32_t>(width * height, 0));
+  std::vector<float> fplane(irrev ? width * height : 0, 0.0f);  // irreversible: dequantised coefficients
   size_t cbw_n = (size_t)1 << (cs.cod.cbw_exp + 2), cbh_n = (size_t)1 << (cs.cod.cbh_exp + 2);
   bool uses_sop = cs.cod.scod & 2, uses_eph = cs.cod.scod & 4, stripe_causal = cs.cod.cb_style & 0x08;
   std::vector<Seq> seq = packet_sequence(cs);
@@ -862,6 +910,17 @@ std::vector<std::vector<int32_t>> decode_to_resolution(const Codestream& cs, uns
       std::vector<uint32_t> out(rw * rh, 0);
       ht_decode_block(seg, seg_len, out.data(), rw, rh, rw, blk.missing_msbs, blk.num_passes, blk.len_cleanup,
                       blk.len_refinement, stripe_causal);
+      if (irrev) {
+        // T.800 E.1.1.2 with the decoder's mid-point reconstruction
+        const float scale = irrev_scale(cs.quant, band.res, band.band, levels, max_res, cs.siz.comps[0].depth);
+        for (size_t y = 0; y < rh; ++y)
+          for (size_t x = 0; x < rw; ++x) {
+            uint32_t v = out[y * rw + x];
+            float c = (float)(v & 0x7FFFFFFF) * scale;
+            fplane[(band.y0 + ry + y) * width + band.x0 + rx + x] = (v & 0x80000000u) ? -c : c;
+          }
+        continue;
+      }
       uint8_t k_max = cs.quant.k_max(band.res, band.band);
       uint32_t sh = 31 - k_max;
       std::vector<int32_t>& plane = planes[seq[i].comp];
@@ -877,6 +936,21 @@ std::vector<std::vector<int32_t>> decode_to_resolution(const Codestream& cs, uns
   size_t out_w, out_h;
   level_dims(width, height, levels - max_res, &out_w, &out_h);
   std::vector<std::vector<int32_t>> comps_out;
+  if (irrev) {
+    // inverse 9/7 over the kept levels, then to the nearest integer (ties
+    // to even) inside the level-shifted sample range
+    std::vector<float> region(out_w * out_h);
+    for (size_t y = 0; y < out_h; ++y)
+      std::copy(fplane.begin() + y * width, fplane.begin() + y * width + out_w, region.begin() + y * out_w);
+    dwt97_inverse(region.data(), out_w, out_h, max_res);
+    const unsigned depth = cs.siz.comps[0].depth;
+    const float lo = -(float)(1 << (depth - 1)), hi = (float)((1 << (depth - 1)) - 1);
+    std::vector<int32_t> ints(out_w * out_h);
+    for (size_t i = 0; i < ints.size(); ++i)
+      ints[i] = (int32_t)std::fmin(std::fmax(std::nearbyint(region[i]), lo), hi);
+    comps_out.push_back(std::move(ints));
+    planes.clear();
+  }
   for (auto& plane : planes) {
     std::vector<int32_t> region(out_w * out_h);
     for (size_t y = 0; y < out_h; ++y)
@@ -1017,10 +1091,13 @@ int32_t read_sample(const uint8_t* p, int dtype) {
 }  // namespace
 
 // writer.rs:55-323
+std::vector<uint8_t> write_codestream(const std::vector<int32_t>& plane, size_t width, size_t height,
+                                      unsigned bit_depth, bool is_signed, unsigned levels, int progression,
+                                      bool emit_tlm_plt, const Quant& quant, uint8_t wavelet);
+
 std::vector<uint8_t> encode_image_with_depth(const int32_t* samples, size_t width, size_t height, unsigned bit_depth,
                                              bool is_signed, unsigned levels, int progression, bool emit_tlm_plt) {
   if (width == 0 || height == 0) fail(kInvalidArgument, "empty image");
-  const size_t cbw = 64, cbh = 64;
   if (bit_depth < 1 || bit_depth > 38) fail(kInvalidArgument, "declared bit depth must be in 1..=38");
   int64_t lo = is_signed ? -((int64_t)1 << (bit_depth - 1)) : 0;
   int64_t hi = is_signed ? ((int64_t)1 << (bit_depth - 1)) - 1 : ((int64_t)1 << bit_depth) - 1;
@@ -1038,6 +1115,16 @@ std::vector<uint8_t> encode_image_with_depth(const int32_t* samples, size_t widt
     for (auto& s : plane) s -= half;
   }
   dwt53_forward(plane.data(), width, height, levels);
+  return write_codestream(plane, width, height, bit_depth, is_signed, levels, progression, emit_tlm_plt, quant, 1);
+}
+
+// The codestream of one transformed (and, on the irreversible path,
+// quantised) plane: packets of every resolution, PLT/TLM, SIZ/COD/QCD/CAP/COM
+// (the tail of htj2k.rs encode_image / writer.rs); wavelet 1 = 5/3, 0 = 9/7.
+std::vector<uint8_t> write_codestream(const std::vector<int32_t>& plane, size_t width, size_t height,
+                                      unsigned bit_depth, bool is_signed, unsigned levels, int progression,
+                                      bool emit_tlm_plt, const Quant& quant, uint8_t wavelet) {
+  const size_t cbw = 64, cbh = 64;
   std::vector<std::vector<BandBlocks>> per_res;
   for (unsigned r = 0; r <= levels; ++r) {
     std::vector<Band> bands = bands_of_resolution(width, height, levels, r);
@@ -1108,7 +1195,7 @@ std::vector<uint8_t> encode_image_with_depth(const int32_t* samples, size_t widt
     p.push_back(4);  // log2(64) - 2
     p.push_back(4);
     p.push_back(0x40);
-    p.push_back(1);
+    p.push_back(wavelet);
     push_segment(out, COD, p);
   }
   push_segment(out, QCD, quant.to_bytes());
@@ -1117,7 +1204,7 @@ std::vector<uint8_t> encode_image_with_depth(const int32_t* samples, size_t widt
     uint32_t bp = magb <= 8 ? 0 : (magb < 28 ? magb - 8 : 13 + (magb >> 2));
     std::vector<uint8_t> p;
     be32(p, 0x00020000u);
-    be16(p, bp & 0x1F);
+    be16(p, (wavelet == 0 ? 0x20 : 0) | (bp & 0x1F));  // Ccap15 (markers.rs:325-339)
     push_segment(out, CAP, p);
   }
   {
@@ -1137,11 +1224,50 @@ std::vector<uint8_t> encode_image_with_depth(const int32_t* samples, size_t widt
   return out;
 }
 
-Decoded decode_codestream(const uint8_t* data, size_t len, int max_res, bool prefix) {
+// A lossy codestream for the tests of the decode-only 9/7 path (the
+// reference refuses lossy encode, htj2k.rs:293-297): level shift, forward
+// 9/7 (dwt/mod.rs:205-240), a dead-zone quantiser q = sign(c) floor(|c| /
+// step) with every band at exponent eps and mantissa mu (QCD style 2,
+// guard bits `guard`) and step = irrev_scale x 2^(31 - K_max), then the
+// reversible path's block coder and writer.  Fails when a band's indices do
+// not fit K_max = guard + eps - 1 bits.
+std::vector<uint8_t> encode_image_irreversible(const int32_t* samples, size_t width, size_t height,
+                                               unsigned bit_depth, bool is_signed, unsigned levels, unsigned eps,
+                                               unsigned mu, unsigned guard) {
+  if (width == 0 || height == 0) fail(kInvalidArgument, "empty image");
+  if (bit_depth < 1 || bit_depth > 30) fail(kInvalidArgument, "bit depth outside 1..=30");
+  if (eps > 31 || mu > 0x7FF || guard > 7) fail(kInvalidArgument, "quantiser parameters out of range");
+  Quant q;
+  q.guard_bits = (uint8_t)guard;
+  q.style = 2;
+  q.values.assign(1 + 3 * levels, (uint16_t)(eps << 11 | mu));
+  int32_t half = is_signed ? 0 : 1 << (bit_depth - 1);
+  std::vector<float> f(width * height);
+  for (size_t i = 0; i < f.size(); ++i) f[i] = (float)(samples[i] - half);
+  dwt97_forward(f.data(), width, height, levels);
+  std::vector<int32_t> plane(width * height, 0);
+  for (unsigned r = 0; r <= levels; ++r) {
+    for (const Band& b : bands_of_resolution(width, height, levels, r)) {
+      unsigned km = q.k_max(b.res, b.band);
+      if (km > 30) fail(kInvalidArgument, "K_max above 30");
+      double step = std::ldexp((double)irrev_scale(q, b.res, b.band, levels, levels, bit_depth), 31 - (int)km);
+      for (size_t y = 0; y < b.h; ++y)
+        for (size_t x = 0; x < b.w; ++x) {
+          size_t at = (b.y0 + y) * width + b.x0 + x;
+          double m = std::floor(std::fabs((double)f[at]) / step);
+          if (m >= std::ldexp(1.0, (int)km)) fail(kInvalidArgument, "quantisation index exceeds K_max bits");
+          plane[at] = f[at] < 0 ? -(int32_t)m : (int32_t)m;
+        }
+    }
+  }
+  return write_codestream(plane, width, height, bit_depth, is_signed, levels, 2, true, q, 0);
+}
+
+Decoded decode_codestream(const uint8_t* data, size_t len, int max_res, bool prefix, bool allow_irreversible) {
   Codestream cs = parse_codestream(data, len, !prefix);
   Decoded d;
-  std::vector<std::vector<int32_t>> comps =
-      decode_to_resolution(cs, max_res < 0 ? cs.cod.decomps : (unsigned)max_res, &d.width, &d.height);
+  std::vector<std::vector<int32_t>> comps = decode_to_resolution(
+      cs, max_res < 0 ? cs.cod.decomps : (unsigned)max_res, &d.width, &d.height, allow_irreversible);
   if (comps.size() != 1) fail(kUnsupported, "oracle decodes single-component codestreams");
   d.samples = std::move(comps[0]);
   return d;

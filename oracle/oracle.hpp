@@ -60,6 +60,9 @@ void lift_inverse_i64(std::vector<int64_t>& chunk, const std::vector<size_t>& sh
 void level_dims(size_t w, size_t h, unsigned level, size_t* ow, size_t* oh);
 void dwt53_forward(int32_t* plane, size_t width, size_t height, unsigned levels);
 void dwt53_inverse(int32_t* plane, size_t width, size_t height, unsigned levels);
+// 9/7 float lifting without K scaling (dwt/mod.rs:103-325)
+void dwt97_forward(float* plane, size_t width, size_t height, unsigned levels);
+void dwt97_inverse(float* plane, size_t width, size_t height, unsigned levels);
 
 // ---- HT block coder (crates/ndic-htj2k/src/block) ---------------------------
 struct HtTables {
@@ -88,7 +91,13 @@ struct Decoded {
   size_t width = 0, height = 0;
   std::vector<int32_t> samples;
 };
-Decoded decode_codestream(const uint8_t* data, size_t len, int max_res /* -1 = full */, bool prefix);
+// allow_irreversible: decode 9/7 codestreams (QCD style 2) instead of
+// refusing them as the reference does (reader.rs:326-330)
+Decoded decode_codestream(const uint8_t* data, size_t len, int max_res /* -1 = full */, bool prefix,
+                          bool allow_irreversible = false);
+std::vector<uint8_t> encode_image_irreversible(const int32_t* samples, size_t width, size_t height,
+                                               unsigned bit_depth, bool is_signed, unsigned levels, unsigned eps,
+                                               unsigned mu, unsigned guard);
 std::vector<uint8_t> htj2k_encode_chunk(const uint8_t* bytes, size_t len, const std::vector<size_t>& shape,
                                         int dtype, const Htj2kConfig& cfg);
 std::vector<uint8_t> htj2k_decode_chunk(const uint8_t* chunk, size_t len, const std::vector<size_t>& shape,

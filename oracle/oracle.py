@@ -64,7 +64,9 @@ def _declare(L) -> None:
         "orc_ht_encode_block": [P, S, S, C.c_uint32, P, P, E],
         "orc_ht_decode_block": [C.c_char_p, S, P, S, S, C.c_uint32, C.c_uint32, S, S, I, E],
         "orc_encode_image": [P, S, S, U, I, U, P, P, E],
-        "orc_decode_codestream": [C.c_char_p, S, I, I, P, P, P, E],
+        "orc_decode_codestream": [C.c_char_p, S, I, I, I, P, P, P, E],
+        "orc_dwt97": [I, P, S, S, U, E],
+        "orc_encode_image_irreversible": [P, S, S, U, I, U, U, U, U, P, P, E],
         "orc_htj2k_encode_chunk": [C.c_char_p, S, P, S, I, U, I, I, I, P, P, E],
         "orc_htj2k_decode_chunk": [C.c_char_p, S, P, S, I, P, P, E],
         "orc_zfp_encode": [I, C.c_char_p, S, P, S, I, I, C.c_double, P, P, P, E],
@@ -135,6 +137,33 @@ def dwt53(forward: bool, plane: np.ndarray, levels: int) -> np.ndarray:
     return p
 
 
+def dwt97(forward: bool, plane: np.ndarray, levels: int) -> np.ndarray:
+    """The reference's unscaled 9/7 lifting (dwt/mod.rs:103-325) on an f32 plane."""
+    p = np.ascontiguousarray(plane, dtype=np.float32).copy()
+    h, w = p.shape
+    err = _Err()
+    rc = lib().orc_dwt97(int(forward), p.ctypes.data_as(C.POINTER(C.c_float)), C.c_size_t(w), C.c_size_t(h),
+                         C.c_uint(levels), C.byref(err))
+    _check(rc, err)
+    return p
+
+
+def encode_image_irreversible(samples: np.ndarray, depth: int, signed: bool, levels: int, eps: int, mu: int = 0,
+                              guard: int = 2) -> bytes:
+    """A lossy 9/7 codestream (QCD style 2, every band at eps / mu) for the
+    tests of the decode-only irreversible path."""
+    s = np.ascontiguousarray(samples, dtype=np.int32)
+    h, w = s.shape
+    out = C.POINTER(C.c_uint8)()
+    n = C.c_size_t()
+    err = _Err()
+    rc = lib().orc_encode_image_irreversible(s.ctypes.data_as(C.POINTER(C.c_int32)), C.c_size_t(w), C.c_size_t(h),
+                                             C.c_uint(depth), C.c_int(int(signed)), C.c_uint(levels), C.c_uint(eps),
+                                             C.c_uint(mu), C.c_uint(guard), C.byref(out), C.byref(n), C.byref(err))
+    _check(rc, err)
+    return _take(out, n.value)
+
+
 def ht_tables() -> dict:
     arrs = {k: np.zeros(n, np.uint16) for k, n in (("DEC_VLC_TBL0", 1024), ("DEC_VLC_TBL1", 1024),
                                                    ("ENC_VLC_TBL0", 2048), ("ENC_VLC_TBL1", 2048),
@@ -180,13 +209,14 @@ def encode_image(samples: np.ndarray, depth: int, signed: bool, levels: int) -> 
     return _take(out, n.value)
 
 
-def decode_codestream(data: bytes, max_res: int = -1, prefix: bool = False) -> np.ndarray:
+def decode_codestream(data: bytes, max_res: int = -1, prefix: bool = False, irreversible: bool = False) -> np.ndarray:
     out = C.POINTER(C.c_int32)()
     w = C.c_size_t()
     h = C.c_size_t()
     err = _Err()
     rc = lib().orc_decode_codestream(data, C.c_size_t(len(data)), C.c_int(max_res), C.c_int(int(prefix)),
-                                     C.byref(out), C.byref(w), C.byref(h), C.byref(err))
+                                     C.c_int(int(irreversible)), C.byref(out), C.byref(w), C.byref(h),
+                                     C.byref(err))
     _check(rc, err)
     arr = np.frombuffer(C.string_at(out, 4 * w.value * h.value), np.int32).reshape(h.value, w.value).copy()
     lib().orc_free(out)
