@@ -158,6 +158,23 @@ typedef struct {
 ndic_status ndic_htj2k_decode_plane_batch(const uint8_t* d_in, uint8_t* d_out, ndic_plane_desc* descs, size_t n,
                                           void* stream, ndic_error* err);
 
+/* The decode-only lossy path: as ndic_htj2k_decode_plane_batch, but a
+ * codestream coded with the irreversible 9/7 wavelet and an expounded QCD
+ * (style 2) is decoded instead of refused as Unsupported (reader.rs:326-330,
+ * the check every other decode entry point keeps).  It assembles the pieces
+ * the reference ships for that path -- the unscaled 9/7 lifting
+ * (crates/ndic-htj2k/src/dwt/mod.rs:243-325), Quant::irrev_delta
+ * (crates/ndic-codestream/src/quant.rs:204-221) and the block decoder's
+ * mid-point reconstruction (block/decode.rs:5-6) -- as T.800 Annex E/F define
+ * the decode: each coefficient is sign x magnitude x (delta x 2^depth x the K
+ * factor the unscaled lifting leaves in its band) in f32, synthesised in f32,
+ * rounded to the nearest integer (ties to even), clamped to the sample range
+ * and level shifted.  Scope: one component without a component transform,
+ * at most 30-bit samples; 5/3 codestreams in the same batch decode as usual.
+ * Samples are int32 at out_offset, as for the reversible batch. */
+ndic_status ndic_htj2k_decode_plane_batch_irreversible(const uint8_t* d_in, uint8_t* d_out, ndic_plane_desc* descs,
+                                                       size_t n, void* stream, ndic_error* err);
+
 /* ---- byte-range plans and plane-selective decodes ---------------------- */
 
 /* Replaces RangeIndex::from_chunk_prefix + thumbnail / plane / thumbnail_3d

@@ -218,10 +218,41 @@ struct BitWriter {
 // ---------------------------------------------------------------------------
 struct Hdr {
   uint32_t xsiz, ysiz, depth, is_signed;
-  uint32_t scod, decomps, cbw, cbh, cb_style, wavelet, layers;
+  uint32_t scod, decomps, cbw, cbh, cb_style, wavelet, layers, mct;
   uint32_t guard, style, nvals;
-  uint8_t vals[100];
+  uint16_t vals[100];  // style 0: exponents; style 2: (exponent << 11) | mantissa
 };
+
+// K_max of band index kidx (quant.rs:88-107): guard bits + exponent - 1
+__device__ uint32_t kmax_of(const Hdr& h, uint32_t kidx) {
+  uint32_t i = kidx < h.nvals ? kidx : h.nvals - 1;
+  uint32_t eps = h.style == 0 ? h.vals[i] : (uint32_t)(h.vals[i] >> 11);
+  return (eps == 0 ? 0 : eps - 1) + h.guard;
+}
+
+// The dequantisation step of band index kidx of a 9/7 plane decoded to
+// resolution max_res: Quant::irrev_delta (quant.rs:204-221: (1 + mu / 2^11)
+// x band gain / 2^eps) x 2^depth (T.800 E-3's 2^R_b) x the K factor the
+// unscaled lifting (dwt/mod.rs:7-9) leaves in the band relative to the output
+// LL -- K^2 per 2-D level of low-pass: LL of level L holds K^2L, HL / LH of
+// level d K^2(d-1), HH of level d K^2(d-2), all over K^2(L - max_res) -- over
+// 2^(31 - K_max), the alignment of the block decoder's magnitudes.  Double
+// precision, rounded once to f32.
+__device__ float irrev_scale(const Hdr& h, uint32_t kidx, uint32_t levels, uint32_t max_res) {
+  constexpr double kK = 1.230174104914001;  // T.800 Table F.4
+  constexpr double kK2 = kK * kK;
+  const uint32_t res = kidx == 0 ? 0 : (kidx - 1) / 3 + 1, band = kidx == 0 ? 0 : (kidx - 1) % 3 + 1;
+  const uint32_t i = kidx < h.nvals ? kidx : h.nvals - 1;
+  const uint32_t v = h.vals[i], eps = v >> 11;
+  const float gain = band == 0 ? 1.0f : band == 3 ? 4.0f : 2.0f;
+  const float delta = (float)((v & 0x7FF) | 0x800) * gain / 2048.0f / __uint_as_float((127 + eps) << 23);
+  const int d = res == 0 ? (int)levels : (int)(levels - res + 1);
+  int e = res == 0 ? (int)levels : band == 3 ? d - 2 : d - 1;
+  e -= (int)(levels - max_res);
+  double f = e < 0 ? 1.0 / kK2 : 1.0;
+  for (int k = 0; k < e; ++k) f *= kK2;
+  return (float)ldexp((double)delta * f, (int)h.depth - 31 + (int)kmax_of(h, kidx));
+}
 
 // Locates plane p's codestream inside the chunk (container.rs:287-340,
 // htj2k.rs:354-384); returns false after raising an error.
@@ -329,6 +360,7 @@ __device__ bool parse_main(const DevChunk& ch, const Bytes& cs, uint64_t len, ui
       h.cbh = cs[q + 7];
       h.cb_style = cs[q + 8];
       h.wavelet = cs[q + 9];
+      h.mct = cs[q + 4];
       has_cod = true;
     } else if (m == kQCD) {
       if (n == 0) return raise(ch, kErrCsMarker, base + pos, gid), false;
@@ -340,16 +372,15 @@ __device__ bool parse_main(const DevChunk& ch, const Bytes& cs, uint64_t len, ui
         for (uint32_t i = 0; i < h.nvals; ++i) h.vals[i] = cs[q + 1 + i] >> 3;
       } else if (h.style == 2) {
         if ((n - 1) % 2) return raise(ch, kErrCsMarker, base + pos, gid), false;
-        h.nvals = (n - 1) / 2;  // irreversible: rejected by the scope check below
+        h.nvals = (n - 1) / 2;  // irreversible: decoded only where the call allows it
+        if (h.nvals > 100) return raise(ch, kErrCsUnsupported, base + pos, gid), false;
+        for (uint32_t i = 0; i < h.nvals; ++i) h.vals[i] = (uint16_t)rd16(cs, q + 1 + 2 * i);
       } else {
         return raise(ch, kErrCsMarker, base + pos, gid), false;
       }
       if (h.nvals == 0) return raise(ch, kErrCsMarker, base + pos, gid), false;
-      if (h.style == 0)
-        for (uint32_t i = 0; i < h.nvals; ++i) {
-          uint32_t eps = h.vals[i];
-          if ((eps == 0 ? 0 : eps - 1) + h.guard > 31) return raise(ch, kErrCsMarker, base + pos, gid), false;
-        }
+      for (uint32_t i = 0; i < h.nvals; ++i)  // quant.rs parse: K_max within the 31-bit datapath
+        if (kmax_of(h, i) > 31) return raise(ch, kErrCsMarker, base + pos, gid), false;
       has_qcd = true;
     } else if (m == kCAP) {
       if (n < 6 || !(rd32(cs, q) & 0x00020000u)) return raise(ch, kErrCsMarker, base + pos, gid), false;
@@ -364,9 +395,12 @@ __device__ bool parse_main(const DevChunk& ch, const Bytes& cs, uint64_t len, ui
   }
   if (!has_siz || !has_cod || !has_qcd) return raise(ch, kErrCsMarker, base + pos, gid), false;
   // check_supported (reader.rs:304-341) and this path's scope: one HT
-  // component, reversible 5/3, one layer, maximal precincts, 64x64 blocks.
-  if (!(h.cb_style & 0x40) || h.wavelet != 1 || h.style != 0 || h.layers != 1)
-    return raise(ch, kErrCsUnsupported, base, gid), false;
+  // component, reversible 5/3 (or, where the call allows it, 9/7 with an
+  // expounded QCD, no component transform and at most 30-bit samples), one
+  // layer, maximal precincts, 64x64 blocks.
+  const bool rev = h.wavelet == 1 && h.style == 0;
+  const bool irrev = ch.irrev && h.wavelet == 0 && h.style == 2 && h.mct == 0 && h.depth <= 30;
+  if (!(h.cb_style & 0x40) || !(rev || irrev) || h.layers != 1) return raise(ch, kErrCsUnsupported, base, gid), false;
   if ((h.scod & 1) || h.cbw != 4 || h.cbh != 4)
     return raise(ch, kErrCsUnsupported, base, gid), false;
   if (h.decomps != ch.L) return raise(ch, kErrCsLevels, base, gid), false;
@@ -375,11 +409,6 @@ __device__ bool parse_main(const DevChunk& ch, const Bytes& cs, uint64_t len, ui
   return true;
 }
 
-__device__ uint32_t kmax_of(const Hdr& h, uint32_t kidx) {
-  uint32_t i = kidx < h.nvals ? kidx : h.nvals - 1;
-  uint32_t eps = h.vals[i];
-  return (eps == 0 ? 0 : eps - 1) + h.guard;
-}
 
 // Tile-part (reader.rs:178-291): SOT, PLT..., SOD; returns the body range.
 __device__ bool parse_tile(const DevChunk& ch, const Bytes& cs, uint64_t len, uint64_t pos, uint64_t base,
@@ -594,7 +623,7 @@ __device__ void parse_plane(const DevChunk& ch, uint64_t p, uint64_t gid, DevPla
           t.len2 = len2;
           t.mmsb = (uint8_t)min(mmsbs, 255u);
           t.passes = (uint8_t)min(np, 255u);  // more than 3 is refused by the block decoder
-          t.kmax = (uint8_t)kmax;
+          t.kmax = (uint8_t)(pl.irrev ? 31 : kmax);  // 9/7: the aligned magnitude itself, dequantised later
           t.flags = kTaskHeader | ((h.cb_style & 0x08) ? kTaskCausal : 0);
           nincl += 1;
         }
@@ -635,6 +664,13 @@ __device__ bool plane_headers(const DevChunk& ch, uint64_t p, uint64_t gid, DevP
   pl.depth = h.depth;
   pl.is_signed = h.is_signed;
   for (uint32_t i = 0; i < 1 + 3 * ch.L && i < 100; ++i) pl.kmax[i] = (uint8_t)kmax_of(h, i);
+  pl.irrev = h.wavelet == 0;
+  if (ch.irrev) {
+    IrrevPlane& ip = ch.irrev[p];
+    ip.on = pl.irrev;
+    if (pl.irrev)
+      for (uint32_t i = 0; i < 1 + 3 * ch.L && i < 100; ++i) ip.scale[i] = irrev_scale(h, i, ch.L, ch.max_res);
+  }
   return parse_tile(ch, cs, len, pos, pl.cs_off, gid, !(ch.no_container & 2), bs, be, plt, nplt);
 }
 
@@ -830,6 +866,74 @@ __global__ __launch_bounds__(256) void unshift_kernel(const DevChunk* __restrict
     uint64_t y = i / ow, x = i % ow;
     int64_t v = (int64_t)d[y * ch.W + x] + half;
     d[y * ch.W + x] = (int32_t)(v < 0 ? 0 : v > mx ? mx : v);
+  }
+}
+
+// Irreversible planes: aligned sign-magnitude words (the HT decoder's output
+// with K_max taken as 31) to f32 coefficients, scaled by the step of the
+// coefficient's band.  Grid: (plane, 64 strides); the band of (x, y) is found
+// from the level sizes: d = min(levels column x and row y lie inside), LL at
+// d = L, else HL / LH / HH of level d + 1 (resolution L - d).
+__global__ __launch_bounds__(256) void dequantize_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks) {
+  const uint64_t gid = blockIdx.x;
+  const DevChunk& ch = chunks[find_chunk(chunks, nchunks, gid)];
+  if (!ch.irrev) return;
+  const uint64_t p = gid - ch.plane_base;
+  const IrrevPlane& ip = ch.irrev[p];
+  if (!ip.on) return;
+  __shared__ uint32_t lw[34], lh[34];
+  __shared__ float sc[100];
+  const uint32_t L = ch.L;
+  if (threadIdx.x == 0) {
+    uint32_t w = ch.W, h = ch.H;
+    for (uint32_t l = 0; l <= L; ++l) {
+      lw[l] = w;
+      lh[l] = h;
+      w = (w + 1) / 2;
+      h = (h + 1) / 2;
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < 1 + 3 * L && i < 100; i += blockDim.x) sc[i] = ip.scale[i];
+  __syncthreads();
+  int32_t* plane = ch.coef + p * (uint64_t)ch.W * ch.H;
+  const uint64_t n = (uint64_t)ch.W * ch.H;
+  for (uint64_t i = (uint64_t)blockIdx.y * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.y * blockDim.x) {
+    const int32_t v = plane[i];
+    if (v == 0) continue;  // 0 and +0.0f share their bits
+    const uint32_t x = (uint32_t)(i % ch.W), y = (uint32_t)(i / ch.W);
+    uint32_t dx = 0, dy = 0;
+    while (dx < L && x < lw[dx + 1]) ++dx;
+    while (dy < L && y < lh[dy + 1]) ++dy;
+    const uint32_t d = dx < dy ? dx : dy;
+    uint32_t k = 0;
+    if (d < L) {
+      const uint32_t band = (dx == d && dy == d) ? 3 : (dx == d ? 1 : 2);
+      k = 3 * (L - d - 1) + band;
+    }
+    const float c = __fmul_rn((float)(uint32_t)(v < 0 ? -(int64_t)v : v), sc[k]);
+    plane[i] = __float_as_int(v < 0 ? -c : c);
+  }
+}
+
+// Irreversible planes: f32 region -> nearest int32, clamped to the
+// level-shifted range of the sample depth.
+__global__ __launch_bounds__(256) void irrev_round_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks,
+                                                          const DevPlane* __restrict__ planes,
+                                                          int32_t* const* data, const uint64_t* dims) {
+  const uint64_t gid = blockIdx.x;
+  const DevPlane& pl = planes[gid];
+  if (!pl.ok || !pl.irrev) return;
+  const uint32_t c = find_chunk(chunks, nchunks, gid);
+  if (!data[c]) return;
+  const DevChunk& ch = chunks[c];
+  const uint64_t ow = dims[2 * c], oh = dims[2 * c + 1];
+  int32_t* d = data[c] + (gid - ch.plane_base) * (uint64_t)ch.W * ch.H;
+  const float lo = -(float)(1 << (pl.depth - 1)), hi = (float)((1 << (pl.depth - 1)) - 1);
+  for (uint64_t i = (uint64_t)blockIdx.y * blockDim.x + threadIdx.x; i < ow * oh;
+       i += (uint64_t)gridDim.y * blockDim.x) {
+    const uint64_t y = i / ow, x = i % ow;
+    const float f = __int_as_float(d[y * ch.W + x]);
+    d[y * ch.W + x] = (int32_t)fminf(fmaxf(rintf(f), lo), hi);
   }
 }
 
@@ -1428,6 +1532,20 @@ void launch_unshift(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes
                     int32_t* const* d_data, const uint64_t* d_dims, hipStream_t stream) {
   if (!nplanes) return;
   hipLaunchKernelGGL(unshift_kernel, dim3((uint32_t)nplanes), dim3(256), 0, stream, d_chunks, nchunks, planes,
+                     d_data, d_dims);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_dequantize(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, hipStream_t stream) {
+  if (!nplanes) return;
+  hipLaunchKernelGGL(dequantize_kernel, dim3((uint32_t)nplanes, 64), dim3(256), 0, stream, d_chunks, nchunks);
+  NDIC_HIP(hipGetLastError());
+}
+
+void launch_irrev_round(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, const DevPlane* planes,
+                        int32_t* const* d_data, const uint64_t* d_dims, hipStream_t stream) {
+  if (!nplanes) return;
+  hipLaunchKernelGGL(irrev_round_kernel, dim3((uint32_t)nplanes, 16), dim3(256), 0, stream, d_chunks, nchunks, planes,
                      d_data, d_dims);
   NDIC_HIP(hipGetLastError());
 }

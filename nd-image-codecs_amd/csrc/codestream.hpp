@@ -21,12 +21,22 @@ struct GeoPacket {
   uint32_t block_first, nblocks;
 };
 
+// Dequantisation of one irreversible (9/7) plane, filled by the parse: the
+// step that turns the block decoder's aligned magnitude into a coefficient of
+// the unscaled 9/7 lifting, per band index (0 LL, then HL, LH, HH of each
+// resolution from the coarsest).
+struct IrrevPlane {
+  uint32_t on;  // the plane's codestream is 9/7 with an expounded QCD
+  float scale[100];
+};
+
 // Decoded per-plane facts (written by the parse kernel / the encode planner).
 struct DevPlane {
   int32_t depth;          // SIZ bit depth
   int32_t is_signed;
   uint8_t kmax[100];      // K_max per band index (1 + 3 * levels)
   uint32_t ok;
+  uint32_t irrev;         // decode: the codestream is 9/7 (allowed by DevChunk::irrev)
   uint64_t cs_off;        // codestream offset within the chunk
   uint64_t cs_len;
   uint32_t packet_len[33];    // encode: bytes per packet (header + bodies)
@@ -56,6 +66,7 @@ struct DevChunk {
   uint32_t no_container;  // decode: bit 0 `in` is one bare codestream, bit 1 it may be a prefix
   const uint32_t* plane_sel;  // decode: container plane of each decoded plane (ascending), or null: plane p
   uint32_t P_all;             // decode: planes in the container (P when plane_sel is null)
+  IrrevPlane* irrev;          // decode: 9/7 codestreams are decoded (one entry per plane), or refused (null)
   // encode
   int32_t* planes_in;     // the int32 spatial planes (P x W x H)
   int32_t sample_signed;  // the sample dtype is signed (SIZ signedness)
@@ -103,6 +114,14 @@ void launch_parse_planes(const DevChunk* d_chunks, uint32_t nchunks, uint64_t np
 void launch_walk_chunks(const DevChunk* d_chunks, uint32_t nchunks, DevPlane* planes, hipStream_t stream);
 // Unsigned planes: DC un-shift with clamping (reader.rs:626-638), in place on
 // each chunk's result planes (d_data[c], region d_dims[2c] x d_dims[2c+1]).
+// Irreversible planes (IrrevPlane::on): the HT decoder's aligned magnitudes
+// (sign and bits 30..0) to f32 coefficients of the unscaled 9/7 lifting, in
+// place (T.800 E.1.1.2, the half-bin bit included).
+void launch_dequantize(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, hipStream_t stream);
+// Irreversible planes: the synthesised f32 region of d_data[c] to the nearest
+// int32 (ties to even) in the level-shifted sample range, in place.
+void launch_irrev_round(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, const DevPlane* planes,
+                        int32_t* const* d_data, const uint64_t* d_dims, hipStream_t stream);
 void launch_unshift(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, const DevPlane* planes,
                     int32_t* const* d_data, const uint64_t* d_dims, hipStream_t stream);
 void launch_plane_minmax(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, DevPlane* planes,

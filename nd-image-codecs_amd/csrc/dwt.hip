@@ -195,6 +195,7 @@ __global__ __launch_bounds__(256) void dwt_inv_level(const DwtJob* __restrict__ 
   const uint64_t rest = local / tiles_x;
   const uint32_t ty = (uint32_t)(rest % tiles_y);
   const uint64_t plane = rest / tiles_y;
+  if (j.irrev && j.irrev[plane].on) return;  // a 9/7 plane
   const uint32_t w = j.w, h = j.h;
   const uint32_t nl = (w + 1) / 2, nh = w / 2;   // horizontal low / high widths
   const uint32_t hl = (h + 1) / 2, hh = h / 2;   // vertical low / high heights
@@ -290,6 +291,93 @@ __global__ __launch_bounds__(256) void dwt_inv_level(const DwtJob* __restrict__ 
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Irreversible 9/7 synthesis (dwt/mod.rs:243-325): the unscaled lifting
+// (T.800 Table F.4 multipliers, K folded into the dequantisation step) on f32
+// coefficients held as bits in the int32 buffers.  Each thread computes one
+// output pair (2i, 2i+1) of a line straight from the subband samples: the
+// four steps are evaluated at the neighbours each step reads, with the
+// reference's index clamps applied at every step (its symmetric extension
+// repeats the edge sample of each intermediate array), so the result equals
+// the in-place sweep of the reference sample for sample.  Sums, products and
+// updates are rounded separately (no contraction), in the reference's order.
+// ---------------------------------------------------------------------------
+constexpr float k97Alpha = -1.5861343f, k97Beta = -0.052980118f, k97Gamma = 0.8829111f, k97Delta = 0.44350685f;
+
+__device__ __forceinline__ float lstep(float base, float c, float a, float b) {
+#pragma clang fp contract(off)
+  return base + c * (a + b);
+}
+
+// One line of a level: low half at lo[k * ls] (k < nl), high half at
+// hi[k * hs] (k < nh), n >= 2.
+struct Line97 {
+  const int32_t* lo;
+  const int32_t* hi;
+  uint64_t ls, hs;
+  int nl, nh;
+  __device__ float L(int k) const { return __int_as_float(lo[(uint64_t)k * ls]); }
+  __device__ float H(int k) const { return __int_as_float(hi[(uint64_t)k * hs]); }
+  __device__ int ch(int k) const { return k < 0 ? 0 : (k > nh - 1 ? nh - 1 : k); }
+  __device__ int cl(int k) const { return k > nl - 1 ? nl - 1 : k; }
+  // after the -delta step (low from high), the -gamma step (high from low),
+  // the -beta step and the -alpha step
+  __device__ float L1(int k) const { return lstep(L(k), -k97Delta, H(ch(k - 1)), H(ch(k))); }
+  __device__ float H1(int k) const { return lstep(H(k), -k97Gamma, L1(k), L1(cl(k + 1))); }
+  __device__ float L2(int k) const { return lstep(L1(k), -k97Beta, H1(ch(k - 1)), H1(ch(k))); }
+  __device__ float H2(int k) const { return lstep(H1(k), -k97Alpha, L2(k), L2(cl(k + 1))); }
+};
+
+// Inverse horizontal 9/7 pass: pair i of row y of the level region; the low
+// half of rows y < hl comes from the LL source (the coarser level's
+// reconstruction), else from A (LH); the high half from A.
+__global__ __launch_bounds__(256) void dwt97_inv_h(const DwtJob* __restrict__ jobs, uint32_t njobs) {
+  uint64_t item = (blockIdx.x + (uint64_t)blockIdx.y * gridDim.x) * 4 + (threadIdx.x >> 6);
+  const DwtJob& j = jobs[find_job(jobs, njobs, item)];
+  uint64_t local = item - j.item_begin;
+  if (local >= j.items) return;
+  uint64_t lines = j.h, t = local % j.tiles, r = local / j.tiles;
+  uint64_t y = r % lines, plane = r / lines;
+  if (!j.irrev || !j.irrev[plane].on) return;
+  const uint32_t n = j.w, nl = (n + 1) / 2, nh = n / 2;
+  const uint32_t i = (uint32_t)t * 64 + (threadIdx.x & 63);
+  if (i >= nl) return;
+  const uint64_t base = plane * j.plane_elems + y * j.stride;
+  const int32_t* low = (y < j.hl ? j.src_ll : j.src) + base;
+  int32_t* out = j.dst + base;
+  if (n == 1) {
+    out[0] = low[0];
+    return;
+  }
+  const Line97 ln{low, j.src + base + nl, 1, 1, (int)nl, (int)nh};
+  out[2 * i] = __float_as_int(ln.L2((int)i));
+  if (i < nh) out[2 * i + 1] = __float_as_int(ln.H2((int)i));
+}
+
+// Inverse vertical 9/7 pass: pair (2i, 2i+1) of column x.
+__global__ __launch_bounds__(256) void dwt97_inv_v(const DwtJob* __restrict__ jobs, uint32_t njobs) {
+  uint64_t item = (blockIdx.x + (uint64_t)blockIdx.y * gridDim.x) * 4 + (threadIdx.x >> 6);
+  const DwtJob& j = jobs[find_job(jobs, njobs, item)];
+  uint64_t local = item - j.item_begin;
+  if (local >= j.items) return;
+  const uint32_t n = j.h, nl = (n + 1) / 2, nh = n / 2;
+  uint64_t t = local % j.tiles, r = local / j.tiles;
+  uint64_t i = r % nl, plane = r / nl;
+  if (!j.irrev || !j.irrev[plane].on) return;
+  const uint32_t x = (uint32_t)t * 64 + (threadIdx.x & 63);
+  if (x >= j.w) return;
+  const uint64_t S = j.stride;
+  const int32_t* col = j.src + plane * j.plane_elems + x;
+  int32_t* out = j.dst + plane * j.plane_elems + x;
+  if (n == 1) {
+    out[0] = col[0];
+    return;
+  }
+  const Line97 ln{col, col + (uint64_t)nl * S, S, S, (int)nl, (int)nh};
+  out[2 * i * S] = __float_as_int(ln.L2((int)i));
+  if (i < nh) out[(2 * i + 1) * S] = __float_as_int(ln.H2((int)i));
+}
 
 // ---------------------------------------------------------------------------
 // Finest synthesis level fused with the inverse nd_lift step along a plane
@@ -651,6 +739,8 @@ void launch_dwt_pass(int pass, const DwtJob* d_jobs, uint32_t njobs, uint64_t it
     case kDwtInvH: hipLaunchKernelGGL(dwt_inv_h, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs); break;
     case kDwtInvV: hipLaunchKernelGGL(dwt_inv_v, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs); break;
     case kDwtFwdV: hipLaunchKernelGGL(dwt_fwd_v, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs); break;
+    case kDwt97InvH: hipLaunchKernelGGL(dwt97_inv_h, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs); break;
+    case kDwt97InvV: hipLaunchKernelGGL(dwt97_inv_v, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs); break;
     case kDwtInvLevel: {
       // one workgroup per tile
       uint32_t tx = (uint32_t)(items < 65535 ? items : 65535);

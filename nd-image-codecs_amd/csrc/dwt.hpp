@@ -5,7 +5,9 @@
 
 namespace ndic {
 
-enum DwtPass { kDwtInvH = 0, kDwtInvV = 1, kDwtFwdV = 2, kDwtFwdH = 3, kDwtInvLevel = 4 };
+enum DwtPass { kDwtInvH = 0, kDwtInvV = 1, kDwtFwdV = 2, kDwtFwdH = 3, kDwtInvLevel = 4, kDwt97InvH = 5, kDwt97InvV = 6 };
+
+struct IrrevPlane;
 
 // One pass of one level over every plane of one chunk (all planes of a
 // chunk share their geometry).  Planes are plane_elems apart in each buffer.
@@ -20,6 +22,10 @@ struct DwtJob {
   uint32_t tiles_x;       // fused inverse level: 64x64 output tiles per tile row
   uint64_t tiles;         // 64-wide tiles per line (fused inverse level: tiles per plane)
   uint64_t items, item_begin;
+  // planes whose codestream is 9/7 (IrrevPlane::on): the 5/3 passes skip
+  // them, the 9/7 passes (f32 bits in the int32 buffers) take only them;
+  // null: every plane is 5/3
+  const IrrevPlane* irrev;
 };
 
 void launch_dwt_pass(int pass, const DwtJob* d_jobs, uint32_t njobs, uint64_t items, hipStream_t stream);

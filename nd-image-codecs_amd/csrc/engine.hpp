@@ -47,7 +47,7 @@ struct Context {
   std::recursive_mutex mu;
   hipStream_t stream = nullptr;
   int device = 0;
-  DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, fusedjobs, winjobs, small, htrec;
+  DevBuf in, out, plane, tmp, coef, work, jobs, dwtjobs, fusedjobs, winjobs, small, htrec, irrev, coef97;
   uint64_t* queues = nullptr;  // work-queue counter pairs of the persistent kernels (zeroed)
   // Counter pair k (0: HT decode, 1: HT encode) of this context's queues.
   uint64_t* queue(int k);
@@ -105,6 +105,7 @@ struct LiftChunk {
 
 // One chunk of a DWT launch sequence: nplanes planes of W x H, each buffer
 // holding them back to back.
+struct IrrevPlane;
 struct DwtChunk {
   int32_t* X = nullptr;   // forward input (spatial planes)
   int32_t* A = nullptr;   // Mallat coefficients
@@ -116,6 +117,11 @@ struct DwtChunk {
   uint64_t plane_elems = 0;            // plane spacing (0: W * H)
   uint32_t levels = 0;
   bool skip_finest = false;  // inverse: stop before the finest level (result = its LL source)
+  // inverse: planes whose codestream is 9/7 (IrrevPlane::on, one entry per
+  // plane) take the f32 9/7 synthesis instead, through S97 (a buffer laid
+  // out like A); null: every plane is 5/3
+  const IrrevPlane* irrev = nullptr;
+  int32_t* S97 = nullptr;
 };
 void level_dims(uint64_t w, uint64_t h, uint32_t level, uint64_t* ow, uint64_t* oh);
 void run_dwt_forward(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t stream);

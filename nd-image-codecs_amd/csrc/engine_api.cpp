@@ -430,8 +430,10 @@ extern "C" ndic_status ndic_device_memory(size_t* free_bytes, size_t* total_byte
   });
 }
 
-extern "C" ndic_status ndic_htj2k_decode_plane_batch(const uint8_t* d_in, uint8_t* d_out, ndic_plane_desc* descs,
-                                                     size_t n, void* stream, ndic_error* err) {
+namespace {
+
+ndic_status plane_batch(const uint8_t* d_in, uint8_t* d_out, ndic_plane_desc* descs, size_t n, void* stream,
+                        ndic_error* err, bool irreversible) {
   return ndic::guard_call(err, [&] {
     using namespace ndic;
     if (n && !descs) fail(NDIC_INVALID_ARGUMENT, "null descriptors");
@@ -449,6 +451,7 @@ extern "C" ndic_status ndic_htj2k_decode_plane_batch(const uint8_t* d_in, uint8_
       it.shape = {d.height, d.width};
       it.L = d.levels;
       it.bare = true;
+      it.irreversible = irreversible;
       it.prefix = d.prefix != 0;
       it.max_res = d.max_res < 0 ? -1 : d.max_res;
       it.d_out = d_out + d.out_offset;
@@ -468,4 +471,17 @@ extern "C" ndic_status ndic_htj2k_decode_plane_batch(const uint8_t* d_in, uint8_
       fail(items[first].status, m.str(), items[first].offset);
     }
   });
+}
+
+}  // namespace
+
+extern "C" ndic_status ndic_htj2k_decode_plane_batch(const uint8_t* d_in, uint8_t* d_out, ndic_plane_desc* descs,
+                                                     size_t n, void* stream, ndic_error* err) {
+  return plane_batch(d_in, d_out, descs, n, stream, err, false);
+}
+
+extern "C" ndic_status ndic_htj2k_decode_plane_batch_irreversible(const uint8_t* d_in, uint8_t* d_out,
+                                                                  ndic_plane_desc* descs, size_t n, void* stream,
+                                                                  ndic_error* err) {
+  return plane_batch(d_in, d_out, descs, n, stream, err, true);
 }

@@ -50,7 +50,10 @@ void add_launch(std::vector<DwtJob>& all, std::vector<PassLaunch>& launches, std
 // Synthesis, one fused launch per level (dwt_inv_level).  A level reads its
 // LL rows from the previous level's reconstruction and writes the next one,
 // so reconstructions alternate between O and T (the coarsest level reads LL
-// from A); result names the buffer the finest level wrote.
+// from A); result names the buffer the finest level wrote.  Chunks that may
+// hold 9/7 planes add the two 9/7 passes of each level (rows into S97,
+// columns into the same destination), which take only those planes while
+// dwt_inv_level skips them.
 void run_dwt_inverse(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t stream) {
   uint32_t max_l = 0;
   for (auto& c : chunks) {
@@ -60,7 +63,7 @@ void run_dwt_inverse(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t st
   std::vector<DwtJob> all;
   std::vector<PassLaunch> launches;
   for (uint32_t l = max_l; l-- > 0;) {
-    std::vector<DwtJob> jobs;
+    std::vector<DwtJob> jobs, hjobs, vjobs;
     for (auto& c : chunks) {
       if (c.levels <= l || c.nplanes == 0 || (l == 0 && c.skip_finest)) continue;
       uint64_t w, h;
@@ -78,10 +81,25 @@ void run_dwt_inverse(Context& ctx, std::vector<DwtChunk>& chunks, hipStream_t st
       j.tiles_x = (uint32_t)ceil_div(w, 64);
       j.tiles = j.tiles_x * ceil_div(h, 64);
       j.items = c.nplanes * j.tiles;
+      j.irrev = c.irrev;
       jobs.push_back(j);
+      if (c.irrev) {
+        // the chunk's 9/7 planes: rows A (+ LL) -> S97, columns S97 -> dst
+        DwtJob hj = j, vj = j;
+        hj.dst = c.S97;
+        hj.tiles = ceil_div((w + 1) / 2, 64);
+        hj.items = c.nplanes * h * hj.tiles;
+        vj.src = c.S97;
+        vj.tiles = ceil_div(w, 64);
+        vj.items = c.nplanes * ((h + 1) / 2) * vj.tiles;
+        hjobs.push_back(hj);
+        vjobs.push_back(vj);
+      }
       c.result = j.dst;
     }
     add_launch(all, launches, jobs, kDwtInvLevel);
+    add_launch(all, launches, hjobs, kDwt97InvH);
+    add_launch(all, launches, vjobs, kDwt97InvV);
   }
   submit(ctx, all, launches, stream);
 }

@@ -230,7 +230,7 @@ uint64_t planes_of(const std::vector<size_t>& shape) {
 // a plane axis (not y or x) with at most two synthesis levels, and a dtype of
 // at most 32 bits (int32 planes).
 bool fusable(const DecodeItem& it, const DevChunk& c) {
-  if (it.bare || !it.lift || !it.d_out || it.steps.size() != 1 || c.P == 0) return false;
+  if (it.bare || it.irreversible || !it.lift || !it.d_out || it.steps.size() != 1 || c.P == 0) return false;
   if (c.L == 0 || c.max_res != (int32_t)c.L || (c.W == 1 && c.H == 1)) return false;
   if (lift_wide(it.out_dtype)) return false;
   const ndic_lift_step& st = it.steps[0];
@@ -323,12 +323,19 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   int32_t* A = (int32_t*)ctx.coef.ensure(al(3 * coef * 4 + 64));
   int32_t* T = A + coef;
   int32_t* O = A + 2 * coef;
+  // items that may hold 9/7 codestreams: one IrrevPlane per plane, and a
+  // fourth coefficient buffer for the 9/7 row pass
+  bool any_irrev = false;
+  for (auto* it : items) any_irrev |= it->irreversible;
+  IrrevPlane* d_irrev = any_irrev ? (IrrevPlane*)ctx.irrev.ensure(al(nplanes * sizeof(IrrevPlane))) : nullptr;
+  int32_t* S97 = any_irrev ? (int32_t*)ctx.coef97.ensure(al(coef * 4 + 64)) : nullptr;
   uint64_t co = 0;
   for (size_t i = 0; i < n; ++i) {
     dc[i].coef = A + co;
     dc[i].err = d_err + 4 * i;
     dc[i].tag = d_tag;
     dc[i].tag_stride = tag_stride;
+    dc[i].irrev = items[i]->irreversible ? d_irrev + dc[i].plane_base : nullptr;
     co += (uint64_t)dc[i].P * dc[i].W * dc[i].H;
   }
   // With graph_windows the window's whole launch sequence, from here to the
@@ -351,6 +358,7 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
     g_parked_frees = &ctx.parked;
   }
   launch_fill32(d_planes, 0, sz_planes, s);
+  if (d_irrev) launch_fill32(d_irrev, 0, al(nplanes * sizeof(IrrevPlane)), s);
   launch_fill32(d_err, 0, sz_err + sz_ptr + sz_dims + sz_flag, s);
   ctx.up(d_chunks, dc.data(), n * sizeof(DevChunk), s);
   ctx.up(d_gp, geo.packets.data(), geo.packets.size() * sizeof(GeoPacket), s);
@@ -373,6 +381,7 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
     launch_ht_decode_wave(d_tasks, ntasks, ctx.queue(0), s);
   }
   launch_ht_refine(d_tasks, ntasks, s);
+  if (any_irrev) launch_dequantize(d_chunks, (uint32_t)n, nplanes, s);
   timer->mark();
   // fused only when the window has the work items to fill the GPU many
   // times over: each fused workgroup walks its tile's whole lift axis, so a
@@ -404,6 +413,10 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
     uint32_t lv = (uint32_t)dc[i].max_res;
     level_dims(dc[i].W, dc[i].H, dc[i].L - lv, &w.W, &w.H);
     w.levels = lv;
+    if (dc[i].irrev) {
+      w.irrev = dc[i].irrev;
+      w.S97 = S97 + off;
+    }
     items[i]->out_w = w.W;
     items[i]->out_h = w.H;
   }
@@ -419,6 +432,7 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   }
   ctx.up(d_ptr, res.data(), n * sizeof(void*), s);
   ctx.up(d_dims, dims.data(), n * 16, s);
+  if (any_irrev) launch_irrev_round(d_chunks, (uint32_t)n, nplanes, d_planes, d_ptr, d_dims, s);
   launch_unshift(d_chunks, (uint32_t)n, nplanes, d_planes, d_ptr, d_dims, s);
   timer->mark();
   // outputs

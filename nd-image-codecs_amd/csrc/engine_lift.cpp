@@ -136,7 +136,7 @@ void destroy_context(Context* c) {
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   for (void* p : c->parked) (void)hipFree(p);
   for (DevBuf* b : {&c->in, &c->out, &c->plane, &c->tmp, &c->coef, &c->work, &c->jobs, &c->dwtjobs, &c->fusedjobs,
-                    &c->winjobs, &c->small, &c->htrec})
+                    &c->winjobs, &c->small, &c->htrec, &c->irrev, &c->coef97})
     b->release();
   if (c->queues) (void)hipFree(c->queues);
   c->pin.reset();

@@ -21,6 +21,7 @@ struct DecodeItem {
   uint8_t* d_out = nullptr;
   int max_res = -1;
   bool bare = false;        // d_in is one plane codestream
+  bool irreversible = false;  // ... which may be 9/7 (decoded, not refused)
   bool prefix = false;      // ... possibly cut short (a byte-range prefix)
   uint64_t out_w = 0, out_h = 0;
   int32_t* d_result = nullptr;  // bare: the decoded region (row pitch = plane width)

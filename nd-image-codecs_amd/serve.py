@@ -77,6 +77,8 @@ def _lib():
         P, S, E = C.c_void_p, C.c_size_t, C.POINTER(_pkg.Error)
         lib.ndic_htj2k_decode_plane_batch.argtypes = [P, P, P, S, P, E]
         lib.ndic_htj2k_decode_plane_batch.restype = C.c_int
+        lib.ndic_htj2k_decode_plane_batch_irreversible.argtypes = [P, P, P, S, P, E]
+        lib.ndic_htj2k_decode_plane_batch_irreversible.restype = C.c_int
         lib.ndic_zfp_decode_bricks.argtypes = [P, P, P, S, P, E]
         lib.ndic_zfp_decode_bricks.restype = C.c_int
         _bound = True

@@ -30,9 +30,10 @@ def _arena(items: Sequence[bytes], align: int = 16) -> Tuple[bytes, List[int]]:
 
 
 def plane_batch(ndic, streams: Sequence[bytes], width: int, height: int, levels: int, max_res: int = -1,
-                prefix: bool = False) -> List[Tuple[int, Optional[np.ndarray]]]:
+                prefix: bool = False, irreversible: bool = False) -> List[Tuple[int, Optional[np.ndarray]]]:
     """Decodes every codestream (or prefix) with one ndic_htj2k_decode_plane_batch
-    call; returns (status, int32 samples or None) per stream."""
+    call (..._irreversible: 9/7 codestreams too); returns (status, int32
+    samples or None) per stream."""
     serve = importlib.import_module("nd-image-codecs_amd.serve")
     dev = importlib.import_module("nd-image-codecs_amd.device")
     lib = serve._lib()
@@ -48,8 +49,8 @@ def plane_batch(ndic, streams: Sequence[bytes], width: int, height: int, levels:
         d.max_res, d.prefix = max_res, int(prefix)
         d.status = -1
     err = ndic.Error()
-    rc = lib.ndic_htj2k_decode_plane_batch(C.c_void_p(d_in.ptr), C.c_void_p(d_out.ptr), arr, len(streams), None,
-                                           C.byref(err))
+    fn = lib.ndic_htj2k_decode_plane_batch_irreversible if irreversible else lib.ndic_htj2k_decode_plane_batch
+    rc = fn(C.c_void_p(d_in.ptr), C.c_void_p(d_out.ptr), arr, len(streams), None, C.byref(err))
     if rc != 0 and all(arr[i].status in (0, -1) for i in range(len(streams))):
         ndic._check(rc, err)  # a call-level failure, not a per-item one
     out = d_out.download()
