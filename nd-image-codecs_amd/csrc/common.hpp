@@ -49,6 +49,27 @@ inline const char* dtype_name(int d) {
 
 inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
+// XCD-aware work-group order.  The dispatcher deals work-groups to the 8 XCDs
+// of a gfx950 device round-robin by linear id, and each XCD has its own L2:
+// with items in launch order, neighbouring tiles (which share the cache lines
+// at their edges) land on different XCDs and each fetches the shared lines.
+// xcd_item deals items to the XCDs in runs of kXcdRun consecutive items
+// instead (run r on XCD r % 8), so neighbours meet in one L2 while every XCD
+// still gets a fine-grained mix of the launch's work; xcd_grid sizes a 2-D
+// grid whose work-group count is a multiple of 8 runs (x a multiple of 8).
+// The map is a bijection on the grid; ids at or past the item count are idle.
+constexpr uint32_t kXcds = 8, kXcdRun = 16;
+inline dim3 xcd_grid(uint64_t items) {
+  const uint64_t n = ceil_div(items, kXcds * kXcdRun) * kXcds * kXcdRun;
+  const uint32_t gx = (uint32_t)(n < 65408 ? n : 65408);  // 65408 = 511 * 128
+  return dim3(gx, (uint32_t)ceil_div(n, gx));
+}
+__device__ __forceinline__ uint64_t xcd_item() {
+  const uint64_t id = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  const uint64_t k = id / kXcds;  // slot on the XCD
+  return ((k / kXcdRun) * kXcds + id % kXcds) * kXcdRun + k % kXcdRun;
+}
+
 // Device-side per-item error record: a code from DevErr plus a byte offset.
 // The host maps codes to the reference's messages.
 enum DevErr : int32_t {

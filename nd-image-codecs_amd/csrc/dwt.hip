@@ -186,7 +186,8 @@ constexpr int kTile = 64;
 __global__ __launch_bounds__(256) void dwt_inv_level(const DwtJob* __restrict__ jobs, uint32_t njobs) {
   __shared__ int32_t TLo[kTile / 2 + 1][kTile];
   __shared__ int32_t THi[kTile / 2 + 2][kTile];
-  uint64_t item = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  const uint64_t item = xcd_item();
+  if (item >= jobs[njobs - 1].item_begin + jobs[njobs - 1].items) return;
   const DwtJob& j = jobs[find_job(jobs, njobs, item)];
   uint64_t local = item - j.item_begin;
   if (local >= j.items) return;
@@ -650,7 +651,8 @@ __device__ __forceinline__ uint32_t zbands(uint64_t n, uint32_t levels) {
 __global__ __launch_bounds__(256) void dwt_lift_kernel(const DwtLiftJob* __restrict__ jobs, uint32_t njobs) {
   __shared__ int32_t TLo[2][kFLo][kFx];
   __shared__ int32_t THi[2][kFHi][kFx];
-  const uint64_t item = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  const uint64_t item = xcd_item();
+  if (item >= jobs[njobs - 1].item_begin + jobs[njobs - 1].items) return;
   const DwtLiftJob& j = jobs[find_job(jobs, njobs, item)];
   const uint64_t local = item - j.item_begin;
   if (local >= j.items) return;
@@ -724,9 +726,7 @@ __global__ __launch_bounds__(256) void dwt_lift_kernel(const DwtLiftJob* __restr
 
 void launch_dwt_lift(const DwtLiftJob* d_jobs, uint32_t njobs, uint64_t items, hipStream_t stream) {
   if (items == 0) return;
-  uint32_t gx = (uint32_t)(items < 65535 ? items : 65535);
-  uint32_t gy = (uint32_t)ceil_div(items, gx);
-  hipLaunchKernelGGL(dwt_lift_kernel, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs);
+  hipLaunchKernelGGL(dwt_lift_kernel, xcd_grid(items), dim3(256), 0, stream, d_jobs, njobs);
   NDIC_HIP(hipGetLastError());
 }
 
@@ -741,13 +741,9 @@ void launch_dwt_pass(int pass, const DwtJob* d_jobs, uint32_t njobs, uint64_t it
     case kDwtFwdV: hipLaunchKernelGGL(dwt_fwd_v, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs); break;
     case kDwt97InvH: hipLaunchKernelGGL(dwt97_inv_h, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs); break;
     case kDwt97InvV: hipLaunchKernelGGL(dwt97_inv_v, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs); break;
-    case kDwtInvLevel: {
-      // one workgroup per tile
-      uint32_t tx = (uint32_t)(items < 65535 ? items : 65535);
-      uint32_t ty = (uint32_t)ceil_div(items, tx);
-      hipLaunchKernelGGL(dwt_inv_level, dim3(tx, ty), dim3(256), 0, stream, d_jobs, njobs);
+    case kDwtInvLevel:  // one workgroup per tile, XCD-aware order
+      hipLaunchKernelGGL(dwt_inv_level, xcd_grid(items), dim3(256), 0, stream, d_jobs, njobs);
       break;
-    }
     default: hipLaunchKernelGGL(dwt_fwd_h, dim3(gx, gy), dim3(256), 0, stream, d_jobs, njobs); break;
   }
   NDIC_HIP(hipGetLastError());

@@ -877,7 +877,7 @@ template <int D, class S>
 __global__ __launch_bounds__(256) void zfp_encode_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks,
                                                          uint64_t total) {
   constexpr int R = D == 4 ? 4 : 1;
-  uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  uint64_t g = xcd_item() * 4 + (threadIdx.x >> 6);  // neighbouring blocks share input lines: one L2
   if (g >= total) return;
   const ZfpChunk& c = chunks[find_chunk(chunks, nchunks, g)];
   if (c.fixed ? !c.dst : !c.slots) return;  // refused chunk
@@ -947,7 +947,7 @@ template <int D, class S>
 __global__ __launch_bounds__(256) void zfp_decode_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks,
                                                          uint64_t total, const ZfpBrick* __restrict__ bricks) {
   constexpr int R = D == 4 ? 4 : 1;
-  uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  uint64_t g = xcd_item() * 4 + (threadIdx.x >> 6);  // neighbouring blocks share output lines: one L2
   if (g >= total) return;
   uint32_t ci;
   uint64_t k;
@@ -996,11 +996,10 @@ __global__ __launch_bounds__(256) void zfp_index_write_kernel(const ZfpChunk* __
 }
 
 template <int D, class S> void enc(const ZfpChunk* d, uint32_t n, uint64_t total, hipStream_t s) {
-  hipLaunchKernelGGL((zfp_encode_kernel<D, S>), dim3((uint32_t)ceil_div(total, 4)), dim3(256), 0, s, d, n, total);
+  hipLaunchKernelGGL((zfp_encode_kernel<D, S>), xcd_grid(ceil_div(total, 4)), dim3(256), 0, s, d, n, total);
 }
 template <int D, class S> void dec(const ZfpChunk* d, uint32_t n, uint64_t total, const ZfpBrick* br, hipStream_t s) {
-  hipLaunchKernelGGL((zfp_decode_kernel<D, S>), dim3((uint32_t)ceil_div(total, 4)), dim3(256), 0, s, d, n, total,
-                     br);
+  hipLaunchKernelGGL((zfp_decode_kernel<D, S>), xcd_grid(ceil_div(total, 4)), dim3(256), 0, s, d, n, total, br);
 }
 template <int D, class S> void idx(const ZfpChunk* d, uint32_t n, hipStream_t s) {
   hipLaunchKernelGGL((zfp_index_kernel<D, S>), dim3(n), dim3(64), 0, s, d, n);
