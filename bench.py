@@ -806,6 +806,9 @@ def main():
     ap.add_argument("--no-zfp", dest="zfp", action="store_false", help="skip the ZFP section")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the library's built-in launch parameters instead of tuning them first")
+    ap.add_argument("--knob", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a library launch-parameter knob after the autotuner (repeatable), e.g. "
+                         "ht_decode_wave_max_blocks=9223372036854775807 for the queue-fed wavefront decoder everywhere")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first run one device copy kernel of this many bytes (traffic-counter scale)")
     ap.add_argument("--zfp-only", action="store_true", help="run only the ZFP section and print its JSON")
@@ -849,6 +852,11 @@ def main():
         autotune = importlib.import_module("nd-image-codecs_amd.autotune")
         tuning = autotune.tune(quick=True)
         log("autotune:", json.dumps(tuning["knobs"]), f"({tuning['seconds']} s)")
+    for kv in args.knob:
+        key, _, val = kv.partition("=")
+        ndic.tune_set(key, int(val))
+        log(f"knob {key} = {int(val)}")
+        tuning = dict(tuning or {}, overrides={**(tuning or {}).get("overrides", {}), key: int(val)})
     specs, d_raw, d_enc, ddescs, raw_total, comp_total, enc_s, padded = make_workload(dev, ndic, args)
     d_out = dev.DeviceBuffer(padded, zero=True)
 
