@@ -1338,6 +1338,23 @@ __global__ __launch_bounds__(64 * kLaneWaves) void ht_vlc_kernel(const HtDecTask
 // gives (8 - [previous byte == 0xFF]).  The next chunk's bytes are loaded one
 // refill ahead.  Past the stream's end the bits read as ones.
 constexpr uint32_t kMsRing = 512;  // words: 16 Kbit, more than the 12 Kbit live window
+static_assert((kMsRing & (kMsRing - 1)) == 0, "the ring index is a mask");
+
+// Inclusive prefix sum over the wavefront from DPP moves (GFX9 forms): a
+// Hillis-Steele scan inside each 16-lane row (row_shr 1, 2, 4, 8; lanes whose
+// source is outside the row add 0), then the row totals carried across rows
+// (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2 and 3; rows the
+// mask leaves out add 0).
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
+  return v;
+}
+
 struct MsRing {
   uint32_t* ring;
   const uint8_t* seg;
@@ -1376,14 +1393,9 @@ struct MsRing {
       sum += w[j];
       pv = pf[j];
     }
-    // exclusive prefix of the lane sums (<= 128), bit-sliced over ballots
-    uint32_t excl = 0, total = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const uint64_t bal = __ballot((sum >> k) & 1);
-      excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u)) << k;
-      total += (uint32_t)__popcll(bal) << k;
-    }
+    // exclusive prefix of the lane sums (<= 128)
+    const uint32_t incl = wave_scan_incl(sum);
+    const uint32_t excl = incl - sum, total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     // clear the words this chunk reaches past the partly filled one (their
     // previous contents are long consumed), then OR the bytes in
     const uint32_t w0 = (end >> 5) + 1, w1 = ((end + total) >> 5) + 1;
@@ -1416,10 +1428,11 @@ struct MsRing {
   __device__ void ensure(uint32_t from, uint32_t need) {
     while (end < from + need && nvalid == ~0u) refill();
   }
+  // (ring[kMsRing] holds ones: the word read for positions past the end)
   __device__ uint32_t bits32(uint32_t pos) const {
     const uint32_t wi = pos >> 5;
-    const uint32_t lo = wi < nvalid ? ring[wi % kMsRing] : ~0u;
-    const uint32_t hi = wi + 1 < nvalid ? ring[(wi + 1) % kMsRing] : ~0u;
+    const uint32_t lo = ring[wi < nvalid ? wi & (kMsRing - 1) : kMsRing];
+    const uint32_t hi = ring[wi + 1 < nvalid ? (wi + 1) & (kMsRing - 1) : kMsRing];
     return __builtin_amdgcn_alignbit(hi, lo, pos & 31);
   }
 };
@@ -1470,6 +1483,7 @@ __device__ void ht_magsgn_block(const HtDecTask* __restrict__ tasks, uint64_t i,
   MsRing ms;
   ms.init(ringbuf, t.seg, lcup - scup);
   if (lane == 0) ringbuf[0] = 0;  // the first refill clears from word 1 on
+  if (lane == 1) ringbuf[kMsRing] = ~0u;
   wave_sync();
   uint32_t msbase = 0;
   bool bad = false;
@@ -1503,30 +1517,23 @@ __device__ void ht_magsgn_block(const HtDecTask* __restrict__ tasks, uint64_t i,
       m[r] = (U - ((tq >> (12 + nbit)) & 1)) & 31;
     }
     const uint32_t c0 = sig[0] ? m[0] : 0u, c1 = sig[1] ? m[1] : 0u;
-    // exclusive prefix sum of the bit counts over the lanes, bit-sliced: for
-    // each bit of the count a ballot, and mbcnt counts the lanes below
+    // exclusive prefix sum of the bit counts over the lanes
     const uint32_t cnt = c0 + c1;  // < 64
-    uint32_t excl = 0, total = 0;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const uint64_t bal = __ballot((cnt >> k) & 1);
-      excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u)) << k;
-      total += (uint32_t)__popcll(bal) << k;
-    }
+    const uint32_t incl = wave_scan_incl(cnt);
+    const uint32_t excl = incl - cnt, total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     ms.ensure(msbase, total + 64);
     const uint32_t off = msbase + excl;
     uint32_t blnew = 0;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const uint32_t nbit = nb0 + r;
-      uint32_t val = 0, vv = 0;
-      if (sig[r]) {
-        const uint32_t ms_val = ms.bits32(off + (r ? c0 : 0u));
-        vv = ms_val & ((1u << m[r]) - 1);
-        vv |= ((tq >> (8 + nbit)) & 1) << m[r];
-        vv |= 1;
-        val = (ms_val << 31) | ((vv + 2) << ((p - 1) & 31));
-      }
+      // read for every lane (the ring index is always in bounds), kept where significant
+      const uint32_t ms_val = ms.bits32(off + (r ? c0 : 0u));
+      uint32_t vv = ms_val & ((1u << m[r]) - 1);
+      vv |= ((tq >> (8 + nbit)) & 1) << m[r];
+      vv |= 1;
+      uint32_t val = (ms_val << 31) | ((vv + 2) << ((p - 1) & 31));
+      if (!sig[r]) vv = val = 0;
       if (r == 1) blnew = bitlen(vv);
       if (col && y + r < height) {
         uint32_t o = val;
@@ -1551,7 +1558,7 @@ __device__ void ht_magsgn_block(const HtDecTask* __restrict__ tasks, uint64_t i,
 
 __global__ __launch_bounds__(64) void ht_magsgn_kernel(const HtDecTask* __restrict__ tasks, uint64_t n,
                                                        const uint32_t* __restrict__ rec) {
-  __shared__ uint32_t ring[kMsRing];
+  __shared__ uint32_t ring[kMsRing + 1];
   const uint64_t i = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
   if (i >= n) return;
   ht_magsgn_block(tasks, i, rec + i * kRecWords, ring);
