@@ -1363,6 +1363,7 @@ struct MsRing {
   uint32_t end;    // bits unstuffed so far
   uint32_t prev;   // last byte of the previous chunk (0 before the first)
   uint32_t nvalid; // words that hold stream bits, once all are in (else ~0)
+  uint32_t ones;   // once all are in: words nvalid .. ones - 1 are set to all ones
   uint32_t pf[16]; // this lane's bytes of the next chunk
 
   __device__ void load_next() {
@@ -1378,6 +1379,7 @@ struct MsRing {
     end = 0;
     prev = 0;
     nvalid = ~0u;
+    ones = 0;
     load_next();
   }
   __device__ void refill() {
@@ -1421,25 +1423,33 @@ struct MsRing {
       // the bits past the end read as ones
       if (lane == 0) atomicOr(&ring[(end >> 5) % kMsRing], ~0u << (end & 31));
       nvalid = (end >> 5) + 1;
+      ones = nvalid;
     }
     wave_sync();
   }
-  // the stream holds bits [from, from + need) (or has ended)
+  // bits [from, from + need) readable: the stream holds them, or past its
+  // end the ring words hold ones (filled ahead of the reads, never over the
+  // live window: reads stay within 2 Kbit of `from`, far less than the ring)
   __device__ void ensure(uint32_t from, uint32_t need) {
     while (end < from + need && nvalid == ~0u) refill();
+    if (nvalid != ~0u) {
+      const uint32_t last = ((from + need) >> 5) + 1;
+      if (ones <= last) {
+        for (uint32_t k = ones + (threadIdx.x & 63); k <= last; k += 64) ring[k & (kMsRing - 1)] = ~0u;
+        ones = last + 1;
+        wave_sync();
+      }
+    }
   }
-  // (ring[kMsRing] holds ones: the word read for positions past the end)
   __device__ uint32_t bits32(uint32_t pos) const {
     const uint32_t wi = pos >> 5;
-    const uint32_t lo = ring[wi < nvalid ? wi & (kMsRing - 1) : kMsRing];
-    const uint32_t hi = ring[wi + 1 < nvalid ? (wi + 1) & (kMsRing - 1) : kMsRing];
-    return __builtin_amdgcn_alignbit(hi, lo, pos & 31);
+    return __builtin_amdgcn_alignbit(ring[(wi + 1) & (kMsRing - 1)], ring[wi & (kMsRing - 1)], pos & 31);
   }
 };
 
 // Task i with its quad records (the whole wave; returns together).
 __device__ void ht_magsgn_block(const HtDecTask* __restrict__ tasks, uint64_t i, const uint32_t* __restrict__ rec,
-                                uint32_t* ringbuf) {
+                                uint32_t* ringbuf, uint32_t* recs) {
   const uint32_t lane = threadIdx.x;
   const HtDecTask t = tasks[i];
   HtDecTask* slot = (HtDecTask*)tasks + i;
@@ -1483,18 +1493,27 @@ __device__ void ht_magsgn_block(const HtDecTask* __restrict__ tasks, uint64_t i,
   MsRing ms;
   ms.init(ringbuf, t.seg, lcup - scup);
   if (lane == 0) ringbuf[0] = 0;  // the first refill clears from word 1 on
-  if (lane == 1) ringbuf[kMsRing] = ~0u;
   wave_sync();
   uint32_t msbase = 0;
   bool bad = false;
   uint32_t blprev = 0;  // bit length of this column's sample in the row above
-  uint32_t qnext = col ? ldi32u(rec + lane / 2) : 0u;
+  {
+    // the block's quad records into LDS at once (16 bytes per load), so the
+    // row loop reads them there instead of waiting on global loads that
+    // queue behind its own stores
+    const uint32_t words = ((height + 1) / 2) * 32;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      const uint32_t w = lane * 16 + 4 * k;
+      if (w < words) *(uint4*)(recs + w) = *(const uint4*)(rec + w);
+    }
+    wave_sync();
+  }
   for (uint32_t y = 0; y < height; y += 2) {
     const bool initial = y == 0;
     // ---- MagSgn of rows y, y+1, lane = column (decode.rs:302-396); the
     // next row's records are in flight meanwhile
-    const uint32_t qinfo = qnext;
-    qnext = col && y + 2 < height ? ldi32u(rec + (y / 2 + 1) * 32 + lane / 2) : 0u;
+    const uint32_t qinfo = col ? recs[(y / 2) * 32 + lane / 2] : 0u;
     const uint32_t tq = qinfo & 0xFFFF;
     uint32_t U = qinfo >> 16;
     if (!initial) {
@@ -1532,17 +1551,15 @@ __device__ void ht_magsgn_block(const HtDecTask* __restrict__ tasks, uint64_t i,
       uint32_t vv = ms_val & ((1u << m[r]) - 1);
       vv |= ((tq >> (8 + nbit)) & 1) << m[r];
       vv |= 1;
-      uint32_t val = (ms_val << 31) | ((vv + 2) << ((p - 1) & 31));
-      if (!sig[r]) vv = val = 0;
-      if (r == 1) blnew = bitlen(vv);
-      if (col && y + r < height) {
-        uint32_t o = val;
-        if (!raw) {
-          const int32_t mag = (int32_t)((val & 0x7FFFFFFFu) >> shift);
-          o = (uint32_t)((val & 0x80000000u) ? -mag : mag);
-        }
-        st32(out + (uint64_t)(y + r) * stride + lane, o);
+      const uint32_t val = (ms_val << 31) | ((vv + 2) << ((p - 1) & 31));  // the half-bin bit included
+      uint32_t o = val;
+      if (!raw) {
+        const uint32_t mag = (val & 0x7FFFFFFFu) >> shift;
+        o = (val >> 31) ? 0u - mag : mag;
       }
+      if (!sig[r]) vv = o = 0;
+      if (r == 1) blnew = bitlen(vv);
+      if (col && y + r < height) st32(out + (uint64_t)(y + r) * stride + lane, o);
     }
     msbase += total;
     blprev = blnew;
@@ -1558,10 +1575,11 @@ __device__ void ht_magsgn_block(const HtDecTask* __restrict__ tasks, uint64_t i,
 
 __global__ __launch_bounds__(64) void ht_magsgn_kernel(const HtDecTask* __restrict__ tasks, uint64_t n,
                                                        const uint32_t* __restrict__ rec) {
-  __shared__ uint32_t ring[kMsRing + 1];
+  __shared__ uint32_t ring[kMsRing];
+  __shared__ __align__(16) uint32_t recs[kRecWords];
   const uint64_t i = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
   if (i >= n) return;
-  ht_magsgn_block(tasks, i, rec + i * kRecWords, ring);
+  ht_magsgn_block(tasks, i, rec + i * kRecWords, ring, recs);
 }
 
 // SigProp and MagRef passes (decode.rs:398-600) of the blocks that carry
