@@ -64,12 +64,14 @@ __device__ __forceinline__ uint32_t log2_ceil(uint32_t x) {
 // Tag trees over an nx x ny grid (tagtree.rs:13-123): level 0 holds the
 // leaves, each level halves the grid, one extra level reads 0.  Values and
 // flags of both trees live in the plane's scratch.
+constexpr uint32_t kTreeOffs = 34;  // level offsets a tag tree keeps (levels + 1 <= 34)
 struct Tree {
   uint8_t* val;
   uint8_t* flag;
   uint32_t nx, ny, levels;
-  uint32_t off[34];
-  __device__ uint32_t init(uint8_t* mem, uint32_t w, uint32_t h, uint8_t init_val) {
+  uint32_t* off;  // kTreeOffs words the caller provides (LDS in the parsers)
+  __device__ uint32_t init(uint8_t* mem, uint32_t* offs, uint32_t w, uint32_t h, uint8_t init_val) {
+    off = offs;
     nx = w;
     ny = h;
     levels = 1 + max(log2_ceil(w), log2_ceil(h));
@@ -155,25 +157,72 @@ struct BitReader {
     unstuff = false;
     bad = false;
   }
-  __device__ uint32_t bit() {
-    if (avail == 0) {
-      if (pos >= len) {
-        bad = true;
-        return 0;
-      }
-      uint32_t t = src[start + pos];
-      pos += 1;
-      tmp = t;
-      avail = 8 - (unstuff ? 1 : 0);
-      unstuff = t == 0xFF;
+  // the next byte (7 bits after 0xFF); false, and `bad`, past the end
+  __device__ bool fetch() {
+    if (pos >= len) {
+      bad = true;
+      return false;
     }
+    uint32_t t = src[start + pos];
+    pos += 1;
+    tmp = t;
+    avail = 8 - (unstuff ? 1 : 0);
+    unstuff = t == 0xFF;
+    return true;
+  }
+  __device__ uint32_t bit() {
+    if (avail == 0 && !fetch()) return 0;
     avail -= 1;
     return (tmp >> avail) & 1;
   }
+  // n bits, most significant first, a byte's worth at a time (past the end
+  // they read as 0, as bit() by bit())
   __device__ uint32_t bits(uint32_t n) {
     uint32_t v = 0;
-    for (uint32_t i = 0; i < n; ++i) v = (v << 1) | bit();
+    while (n) {
+      if (avail == 0 && !fetch()) return v << n;
+      const uint32_t k = n < avail ? n : avail;
+      avail -= k;
+      v = (v << k) | ((tmp >> avail) & ((1u << k) - 1));
+      n -= k;
+    }
     return v;
+  }
+  // the number of 0 bits before the next 1 bit, which is consumed (a run
+  // that reaches the end stops there, `bad` set)
+  __device__ uint32_t zeros_to_one() {
+    uint32_t z = 0;
+    for (;;) {
+      if (avail == 0 && !fetch()) return z;
+      const uint32_t b = tmp & ((1u << avail) - 1);
+      if (b == 0) {
+        z += avail;
+        avail = 0;
+        continue;
+      }
+      const uint32_t hb = 31 - __clz(b);  // position of the first 1 in the byte's remaining bits
+      z += avail - 1 - hb;
+      avail = hb;
+      return z;
+    }
+  }
+  // the number of 1 bits before the next 0 bit, which is consumed (a run
+  // that reaches the end stops there, `bad` set)
+  __device__ uint32_t ones_to_zero() {
+    uint32_t c = 0;
+    for (;;) {
+      if (avail == 0 && !fetch()) return c;
+      const uint32_t b = ~tmp & ((1u << avail) - 1);
+      if (b == 0) {
+        c += avail;
+        avail = 0;
+        continue;
+      }
+      const uint32_t hb = 31 - __clz(b);
+      c += avail - 1 - hb;
+      avail = hb;
+      return c;
+    }
   }
   __device__ uint64_t terminate() {
     if (unstuff) pos += pos < len ? 1 : 0;
@@ -525,7 +574,7 @@ __global__ __launch_bounds__(256) void init_tasks_kernel(const DevChunk* __restr
 // byte source for packet r starting at codestream offset off.
 template <class Win>
 __device__ void parse_plane(const DevChunk& ch, uint64_t p, uint64_t gid, DevPlane& pl, HtDecTask* pt,
-                            const Bytes& cs, const Hdr& h, uint64_t bs, uint64_t be, uint8_t* tag,
+                            const Bytes& cs, const Hdr& h, uint64_t bs, uint64_t be, uint8_t* tag, uint32_t* toffs,
                             const GeoPacket* __restrict__ gp, const GeoBand* __restrict__ gb, Win win) {
   uint64_t blen = be - bs;
   bool sop = (h.scod & 2) != 0, eph = (h.scod & 4) != 0;
@@ -552,8 +601,8 @@ __device__ void parse_plane(const DevChunk& ch, uint64_t p, uint64_t gid, DevPla
         empty = false;
       }
       Tree inc, mm;
-      uint32_t used = inc.init(tag, bd.nx, bd.ny, 0);
-      mm.init(tag + used, bd.nx, bd.ny, 0);
+      uint32_t used = inc.init(tag, toffs, bd.nx, bd.ny, 0);
+      mm.init(tag + used, toffs + kTreeOffs, bd.nx, bd.ny, 0);
       uint32_t nl = inc.levels, kmax = pl.kmax[bd.kidx];
       for (uint32_t y = 0; y < bd.ny; ++y)
         for (uint32_t x = 0; x < bd.nx; ++x) {
@@ -578,11 +627,7 @@ __device__ void parse_plane(const DevChunk& ch, uint64_t p, uint64_t gid, DevPla
             mmsbs = mm.val[mm.idx(x >> lp1, y >> lp1, lp1)];
             uint32_t ci = mm.idx(x >> cur, y >> cur, cur);
             if (mm.flag[ci] == 0) {
-              for (;;) {
-                uint32_t bit = bb.bit();
-                if (bit == 1 || bb.bad) break;
-                mmsbs += 1;
-              }
+              mmsbs += bb.zeros_to_one();
               mm.val[ci] = (uint8_t)min(mmsbs, 255u);
               mm.flag[ci] = 1;
             }
@@ -604,11 +649,8 @@ __device__ void parse_plane(const DevChunk& ch, uint64_t p, uint64_t gid, DevPla
           mmsbs += phld;
           np -= phld * 3;
           if (mmsbs > kmax) return raise(ch, kErrPacket, pl.cs_off + bs + cursor, gid);
-          uint32_t lblock = 3;
-          while (bb.bit() == 1 && !bb.bad) {
-            lblock += 1;
-            if (lblock > 24) return raise(ch, kErrPacket, pl.cs_off + bs + cursor, gid);
-          }
+          const uint32_t lblock = 3 + bb.ones_to_zero();
+          if (lblock > 24) return raise(ch, kErrPacket, pl.cs_off + bs + cursor, gid);
           uint32_t nb = lblock + 31 - __clz(phld * 3 + 1);
           uint32_t len1 = bb.bits(nb);
           if (len1 < 2 || len1 >= 65535) return raise(ch, kErrPacket, pl.cs_off + bs + cursor, gid);
@@ -679,7 +721,13 @@ __device__ bool plane_headers(const DevChunk& ch, uint64_t p, uint64_t gid, DevP
 __global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __restrict__ chunks, uint32_t nchunks,
                                                           uint64_t nplanes, DevPlane* planes, HtDecTask* tasks,
                                                           const GeoPacket* __restrict__ gp,
-                                                          const GeoBand* __restrict__ gb) {
+                                                          const GeoBand* __restrict__ gb, uint32_t tag_lds) {
+  // per lane in LDS: the tag trees' level offsets and, when the window's
+  // trees fit tag_lds bytes, their nodes (the serial header walk reads and
+  // writes them at every code-block)
+  extern __shared__ __align__(16) uint8_t lane_lds[];
+  uint8_t* mine = lane_lds + threadIdx.x * (tag_lds + 8 * kTreeOffs);
+  uint32_t* toffs = (uint32_t*)(mine + tag_lds);
   uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   if (gid >= nplanes) return;
   const DevChunk& ch = chunks[find_chunk(chunks, nchunks, gid)];
@@ -693,7 +741,7 @@ __global__ __launch_bounds__(64) void parse_planes_kernel(const DevChunk* __rest
   Hdr h;
   uint64_t bs, be;
   if (!plane_headers(ch, p, gid, pl, cs, h, &bs, &be, nullptr, nullptr)) return;
-  parse_plane(ch, p, gid, pl, pt, cs, h, bs, be, ch.tag + gid * ch.tag_stride, gp, gb,
+  parse_plane(ch, p, gid, pl, pt, cs, h, bs, be, tag_lds ? mine : ch.tag + gid * ch.tag_stride, toffs, gp, gb,
               [&](uint32_t, uint64_t) { return cs; });
 }
 
@@ -732,6 +780,7 @@ __global__ __launch_bounds__(64) void parse_planes_wave_kernel(const DevChunk* _
                                                                const GeoBand* __restrict__ gb) {
   __shared__ __align__(16) uint8_t stage[kStageBytes];
   __shared__ __align__(16) uint8_t tagl[kTagLds];
+  __shared__ uint32_t toffl[2 * kTreeOffs];
   __shared__ uint32_t plt[kMaxPackets];
   __shared__ uint64_t wbase[kMaxPackets];
   __shared__ uint32_t woff[kMaxPackets], wlen[kMaxPackets];
@@ -791,7 +840,7 @@ __global__ __launch_bounds__(64) void parse_planes_wave_kernel(const DevChunk* _
   // first window shrinks to the headers
   const Bytes cs{g, stage, 0, use_plt ? (uint32_t)((bs + 15) & ~15ull) : n0};
   uint8_t* tag = ch.tag_stride <= kTagLds ? tagl : ch.tag + gid * ch.tag_stride;
-  parse_plane(ch, p, gid, pl, pt, cs, h, bs, s_be, tag, gp, gb, [&](uint32_t r, uint64_t off) {
+  parse_plane(ch, p, gid, pl, pt, cs, h, bs, s_be, tag, toffl, gp, gb, [&](uint32_t r, uint64_t off) {
     if (use_plt && r < npk && off == wbase[r]) return Bytes{g, stage + woff[r], wbase[r], wlen[r]};
     return cs;
   });
@@ -1179,8 +1228,9 @@ __global__ __launch_bounds__(64) void write_planes_kernel(const DevChunk* __rest
       const GeoBand& bd = gb[ch.geo_band + pk.band_first + b];
       if (bd.nx == 0 || bd.ny == 0) continue;
       Tree inc, mm;
-      uint32_t used = inc.init(tag, bd.nx, bd.ny, 255);
-      mm.init(tag + used, bd.nx, bd.ny, 255);
+      uint32_t toffs[2 * kTreeOffs];
+      uint32_t used = inc.init(tag, toffs, bd.nx, bd.ny, 255);
+      mm.init(tag + used, toffs + kTreeOffs, bd.nx, bd.ny, 255);
       uint32_t kmm = pl.kmax[bd.kidx] - 1;  // missing MSBs of every coded block
       for (uint32_t y = 0; y < bd.ny; ++y)
         for (uint32_t x = 0; x < bd.nx; ++x) {
@@ -1481,14 +1531,20 @@ __global__ __launch_bounds__(256) void assemble_kernel(const DevChunk* __restric
 uint32_t tag_scratch_bytes(uint32_t nx, uint32_t ny) { return tree_bytes_host(nx, ny) * 2; }
 
 void launch_parse_planes(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, DevPlane* planes,
-                         HtDecTask* tasks, const GeoPacket* gp, const GeoBand* gb, hipStream_t stream) {
+                         HtDecTask* tasks, const GeoPacket* gp, const GeoBand* gb, uint32_t tag_stride,
+                         hipStream_t stream) {
   if (!nplanes) return;
-  if ((int64_t)nplanes <= tune(kTuneParseWaveMaxPlanes) && nplanes <= 0xFFFFFFFFull)
+  if ((int64_t)nplanes <= tune(kTuneParseWaveMaxPlanes) && nplanes <= 0xFFFFFFFFull) {
     hipLaunchKernelGGL(parse_planes_wave_kernel, dim3((uint32_t)nplanes), dim3(64), 0, stream, d_chunks, nchunks,
                        nplanes, planes, tasks, gp, gb);
-  else
-    hipLaunchKernelGGL(parse_planes_kernel, dim3((uint32_t)ceil_div(nplanes, 64)), dim3(64), 0, stream, d_chunks,
-                       nchunks, nplanes, planes, tasks, gp, gb);
+  } else {
+    // tag trees of up to 512 bytes per plane (64x64 blocks of planes up to
+    // 2048 x 2048) live in LDS with the level offsets; larger ones in the
+    // window's global scratch
+    const uint32_t tl = tag_stride <= 512 ? tag_stride : 0;
+    hipLaunchKernelGGL(parse_planes_kernel, dim3((uint32_t)ceil_div(nplanes, 64)), dim3(64),
+                       64 * (tl + 8 * kTreeOffs), stream, d_chunks, nchunks, nplanes, planes, tasks, gp, gb, tl);
+  }
   NDIC_HIP(hipGetLastError());
 }
 

@@ -109,8 +109,10 @@ uint32_t tag_scratch_bytes(uint32_t nx, uint32_t ny);
 // launch_parse_planes, which marks the included ones).
 void launch_init_tasks(const DevChunk* d_chunks, uint32_t nchunks, uint64_t ntasks, HtDecTask* tasks,
                        const GeoPacket* gp, const GeoBand* gb, hipStream_t stream);
+// tag_stride: the window's tag-tree scratch per plane (DevChunk::tag_stride)
 void launch_parse_planes(const DevChunk* d_chunks, uint32_t nchunks, uint64_t nplanes, DevPlane* planes,
-                         HtDecTask* tasks, const GeoPacket* gp, const GeoBand* gb, hipStream_t stream);
+                         HtDecTask* tasks, const GeoPacket* gp, const GeoBand* gb, uint32_t tag_stride,
+                         hipStream_t stream);
 void launch_walk_chunks(const DevChunk* d_chunks, uint32_t nchunks, DevPlane* planes, hipStream_t stream);
 // Unsigned planes: DC un-shift with clamping (reader.rs:626-638), in place on
 // each chunk's result planes (d_data[c], region d_dims[2c] x d_dims[2c+1]).

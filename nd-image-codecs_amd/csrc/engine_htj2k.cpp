@@ -367,7 +367,7 @@ std::function<void()> decode_window_begin(Context& ctx, const std::vector<Decode
   timer->mark();
   launch_walk_chunks(d_chunks, (uint32_t)n, d_planes, s);
   launch_init_tasks(d_chunks, (uint32_t)n, ntasks, d_tasks, d_gp, d_gb, s);
-  launch_parse_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_gp, d_gb, s);
+  launch_parse_planes(d_chunks, (uint32_t)n, nplanes, d_planes, d_tasks, d_gp, d_gb, tag_stride, s);
   timer->mark();
   if (split) {
     launch_ht_vlc(d_tasks, ntasks, d_rec, s);

@@ -1165,8 +1165,12 @@ __global__ __launch_bounds__(64) void ht_decode_wave_kernel(const HtDecTask* __r
 // ---------------------------------------------------------------------------
 constexpr uint32_t kRecWords = 32 * 32;  // quad records per block: 32 quad rows x 32 quads
 
-constexpr uint32_t kRecHalf = 16;        // quads of a half row: the unit staged in LDS and stored
-constexpr uint32_t kRecPitch = kRecHalf + 1;  // LDS pitch of a lane's staged half row
+#ifndef NDIC_HT_REC_SEG
+#define NDIC_HT_REC_SEG 16
+#endif
+constexpr uint32_t kRecSeg = NDIC_HT_REC_SEG;  // quads of a row segment: the unit staged in LDS and stored
+constexpr uint32_t kRecPitch = kRecSeg + 1;    // LDS pitch of a lane's staged segment
+static_assert(kRecSeg == 8 || kRecSeg == 16 || kRecSeg == 32, "a row of 32 quads in whole segments");
 
 template <class Tab>
 struct VlcMelDec {
@@ -1309,30 +1313,32 @@ __global__ __launch_bounds__(64 * kLaneWaves) void ht_vlc_kernel(const HtDecTask
   for (uint32_t y = 0; y < 64; y += 2) {
     if (__ballot(y < h) == 0) break;
     if (y < h) d.row_begin();
-    // each half row (16 quads) staged in LDS, then stored four blocks per
-    // store instruction: lanes 16k .. 16k+15 the quads of block j + k
+    // each row segment (kRecSeg quads) staged in LDS, then stored 64 /
+    // kRecSeg blocks per store instruction: lanes kRecSeg k .. kRecSeg k +
+    // kRecSeg - 1 the quads of block j + k
+    constexpr uint32_t kPer = 64 / kRecSeg;
 #pragma unroll 1
-    for (uint32_t hx = 0; hx < 2; ++hx) {
-      const uint32_t x0 = hx * 2 * kRecHalf;  // first column of the half row
+    for (uint32_t hx = 0; hx < 32 / kRecSeg; ++hx) {
+      const uint32_t x0 = hx * 2 * kRecSeg;  // first column of the segment
       if (__ballot(y < h && x0 < w) == 0) break;
       if (y < h)
-        for (uint32_t x = x0; x < w && x < x0 + 2 * kRecHalf; x += 4) {
+        for (uint32_t x = x0; x < w && x < x0 + 2 * kRecSeg; x += 4) {
           uint32_t r0, r1;
           d.pair(x, y, w, lrho + lane, &r0, &r1);
           recst[lane * kRecPitch + (x - x0) / 2] = r0;
           recst[lane * kRecPitch + (x - x0) / 2 + 1] = r1;
         }
-      wave_sync();  // the half row of every lane's block is staged
-      const uint32_t part = lane >> 4, qi = lane & 15;
+      wave_sync();  // the segment of every lane's block is staged
+      const uint32_t part = lane / kRecSeg, qi = lane % kRecSeg;
 #pragma unroll 4
-      for (uint32_t j = 0; j < 64; j += 4) {
+      for (uint32_t j = 0; j < 64; j += kPer) {
         const uint32_t b = j + part;
         const uint32_t bh = __shfl(h, (int)b), bq = __shfl(nq, (int)b);
         const uint64_t bi = (uint64_t)__shfl((long long)i, (int)b);  // the task index of lane b
         if (y < bh && x0 / 2 + qi < bq)
           rec[bi * kRecWords + (y / 2) * 32 + x0 / 2 + qi] = recst[b * kRecPitch + qi];
       }
-      wave_sync();  // reads done before the next half row overwrites the stage
+      wave_sync();  // reads done before the next segment overwrites the stage
     }
     if (y < h) d.row_end(lrho + lane);
   }
