@@ -403,12 +403,12 @@ constexpr uint32_t kFLo = kFy / 2 + 1, kFHi = kFy / 2 + 2;     // T rows: low, h
 constexpr uint32_t kFTRows = kFLo + kFHi;
 // horizontal pass: kFx / 2 column pairs per T row, kFHRows T rows at a time
 constexpr uint32_t kFPairs = kFx / 2, kFHRows = 256 / kFPairs, kFIter = (kFTRows + kFHRows - 1) / kFHRows;
-// vertical pass: kFx columns, kFVGroups row-pair groups, 4 row pairs per thread
-constexpr uint32_t kFVGroups = 256 / kFx;
-static_assert(kFVGroups * 4 * 2 == kFy, "a thread holds 4 row pairs of one column");
+// vertical pass: kFx columns, kFVGroups row-pair groups, kFRP row pairs per thread
+constexpr uint32_t kFVGroups = 256 / kFx, kFRP = kDwtLiftRowPairs, kFS = 2 * kFRP;  // samples per thread
+static_assert(kFVGroups * kFS == kFy, "a thread holds kFRP row pairs of one column");
 
 struct V8 {
-  int32_t v[8];
+  int32_t v[kFS];
 };
 __device__ __forceinline__ int32_t wadd32(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
 __device__ __forceinline__ int32_t wsub32(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
@@ -495,7 +495,7 @@ struct FusedTile {
     V8 out;
     const uint32_t c = tid % kFx, ivb = tid / kFx;
 #pragma unroll
-    for (uint32_t kk = 0; kk < 4; ++kk) {
+    for (uint32_t kk = 0; kk < kFRP; ++kk) {
       const uint32_t iv = ivb + kFVGroups * kk, ii = i0 + iv;
       int32_t x0 = 0, x1 = 0;
       if (h == 1) {
@@ -515,7 +515,7 @@ struct FusedTile {
       const int32_t half = 1 << (pl.depth - 1);
       const int64_t mx = ((int64_t)1 << pl.depth) - 1;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
+      for (int e = 0; e < (int)kFS; ++e) {
         const int64_t v = (int64_t)out.v[e] + half;
         out.v[e] = (int32_t)(v < 0 ? 0 : v > mx ? mx : v);
       }
@@ -527,7 +527,7 @@ struct FusedTile {
   __device__ __forceinline__ void put_as(uint8_t* base, const V8& x, uint32_t c, uint32_t ivb) {
     D* d = (D*)base;
 #pragma unroll
-    for (uint32_t e = 0; e < 8; ++e) {
+    for (uint32_t e = 0; e < kFS; ++e) {
       const uint32_t y = Y0 + 2 * (ivb + kFVGroups * (e / 2)) + (e & 1), xx = X0 + c;
       if (xx < j.w && y < j.h) {
         d[(uint64_t)y * j.w + xx] = (D)x.v[e];
@@ -546,7 +546,7 @@ struct FusedTile {
     const uint32_t c = threadIdx.x % kFx, ivb = threadIdx.x / kFx;
     if (!j.dst) {
 #pragma unroll
-      for (uint32_t e = 0; e < 8; ++e) {
+      for (uint32_t e = 0; e < kFS; ++e) {
         const uint32_t y = Y0 + 2 * (ivb + kFVGroups * (e / 2)) + (e & 1), xx = X0 + c;
         if (xx < j.w && y < j.h) {
           lo = x.v[e] < lo ? x.v[e] : lo;
@@ -596,7 +596,7 @@ struct ZLevel {
       dc = t->fetch(dbase);
       const V8 s0 = src->next();
 #pragma unroll
-      for (int e = 0; e < 8; ++e) xe.v[e] = wsub32(s0.v[e], wadd32(wadd32(dc.v[e], dc.v[e]), 2) >> 2);
+      for (int e = 0; e < (int)kFS; ++e) xe.v[e] = wsub32(s0.v[e], wadd32(wadd32(dc.v[e], dc.v[e]), 2) >> 2);
     }
   }
   __device__ __forceinline__ V8 next() {
@@ -613,7 +613,7 @@ struct ZLevel {
       const V8 d = t->fetch(dbase + m);
       V8 x0;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
+      for (int e = 0; e < (int)kFS; ++e) {
         x0.v[e] = wsub32(s.v[e], d.v[e] >> 1);
         xo.v[e] = wadd32(x0.v[e], d.v[e]);
       }
@@ -629,10 +629,10 @@ struct ZLevel {
         if (m + 1 < nd) dn = t->fetch(dbase + m + 1);
         const V8 s = src->next();
 #pragma unroll
-        for (int e = 0; e < 8; ++e) xn.v[e] = wsub32(s.v[e], wadd32(wadd32(dc.v[e], dn.v[e]), 2) >> 2);
+        for (int e = 0; e < (int)kFS; ++e) xn.v[e] = wsub32(s.v[e], wadd32(wadd32(dc.v[e], dn.v[e]), 2) >> 2);
       }
 #pragma unroll
-      for (int e = 0; e < 8; ++e) xo.v[e] = wadd32(dc.v[e], wadd32(wadd32(xe.v[e], xn.v[e]), 1) >> 1);
+      for (int e = 0; e < (int)kFS; ++e) xo.v[e] = wadd32(dc.v[e], wadd32(wadd32(xe.v[e], xn.v[e]), 1) >> 1);
       has_odd = true;
       xe = xn;
       dc = dn;
@@ -685,7 +685,7 @@ __global__ __launch_bounds__(256) void dwt_lift_kernel(const DwtLiftJob* __restr
       V8 v = ft.fetch(k);
       if (j.kind == 0) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v.v[e] = wadd32(v.v[e], acc.v[e]);
+        for (int e = 0; e < (int)kFS; ++e) v.v[e] = wadd32(v.v[e], acc.v[e]);
       }
       ft.put(k, v);
       acc = v;

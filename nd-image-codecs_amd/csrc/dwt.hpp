@@ -54,7 +54,13 @@ struct DwtLiftJob {
 #ifndef NDIC_DWT_LIFT_TILE_W
 #define NDIC_DWT_LIFT_TILE_W 64
 #endif
-constexpr uint32_t kDwtLiftTileW = NDIC_DWT_LIFT_TILE_W, kDwtLiftTileH = 2048 / kDwtLiftTileW;
+#ifndef NDIC_DWT_LIFT_ROW_PAIRS
+#define NDIC_DWT_LIFT_ROW_PAIRS 2
+#endif
+// the fused pass's output tile: 256 threads, each one column and
+// kDwtLiftRowPairs row pairs of it
+constexpr uint32_t kDwtLiftTileW = NDIC_DWT_LIFT_TILE_W, kDwtLiftRowPairs = NDIC_DWT_LIFT_ROW_PAIRS;
+constexpr uint32_t kDwtLiftTileH = 2 * kDwtLiftRowPairs * (256 / kDwtLiftTileW);
 void launch_dwt_lift(const DwtLiftJob* d_jobs, uint32_t njobs, uint64_t items, hipStream_t stream);
 
 }  // namespace ndic
