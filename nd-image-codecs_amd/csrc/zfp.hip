@@ -37,11 +37,13 @@ template <class S> struct ZT;
 template <> struct ZT<int32_t> {
   using Int = int32_t;
   using UInt = uint32_t;
+  static constexpr int ebits = 0, ebias = 0;
   static constexpr bool fp = false;
 };
 template <> struct ZT<int64_t> {
   using Int = int64_t;
   using UInt = uint64_t;
+  static constexpr int ebits = 0, ebias = 0;
   static constexpr bool fp = false;
 };
 template <> struct ZT<float> {
@@ -924,23 +926,312 @@ __global__ __launch_bounds__(256) void zfp_place_kernel(const ZfpChunk* __restri
   }
 }
 
+// ---------------------------------------------------------------------------
+// The serial walk of a variable-size stream (which has no block index): where
+// each block begins, found by parsing the blocks' bits without decoding them.
+// One wavefront; every value but the window is wave-uniform.
+//
+// The stream is read through a window of 127 words: lane l holds words
+// first + l (cur) and first + 64 + l (nxt, loaded while cur is parsed).  When
+// a read reaches cur's last word the window moves on by 63 words, cur taking
+// its own last word and nxt's first 63.  The walk is one loop with one read of
+// 64 stream bits at its head, each turn consuming at most those 64 bits, so
+// the window registers change in one place only.
+//
+// A bit plane's group tests are parsed a word at a time rather than a test at
+// a time: a run of significance tokens "1 0^z 1" ends at the first token whose
+// closing one is followed by a zero, which a prefix parity of the ones finds.
+// Where that cannot decide (a token longer than the word, the last
+// coefficient, a bit budget below 64) a turn parses one test.  The walk
+// consumes exactly the bits the block decoder does (decode_float_block /
+// decode_int_block / decode_ints with Parse), budgets and minimum sizes
+// included.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t i) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)i);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)i);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// inclusive prefix parity: bit p of the result is the parity of x's bits 0..p
+__device__ __forceinline__ uint64_t prefix_parity(uint64_t x) {
+  x ^= x << 1;
+  x ^= x << 2;
+  x ^= x << 4;
+  x ^= x << 8;
+  x ^= x << 16;
+  x ^= x << 32;
+  return x;
+}
+
+struct WalkWindow {
+  GlobalSrc g;
+  uint64_t first;     // first word of cur (wave-uniform)
+  uint64_t cur, nxt;  // lane l: words first + l and first + 64 + l
+
+  // cur's load is waited for at once (the empty asm reads it), so that only
+  // nxt's is ever in flight: a wait for cur, where the compiler copies it
+  // between registers, then never waits for nxt as well
+  __device__ void load_cur(uint64_t w) {
+    cur = g.load(w + lane());
+    __asm__ volatile("" : : "v"(cur));
+  }
+  __device__ void init(const uint8_t* base, uint64_t nbytes, uint64_t bit) {
+    g.base = base;
+    g.nbytes = nbytes;
+    first = bit >> 6;
+    load_cur(first);
+    nxt = g.load(first + 64 + lane());
+  }
+  // the 64 stream bits from bit `pos` on, the first in bit 0 (pos never
+  // moves back, and never by 2^38 bits at once)
+  __device__ uint64_t peek(uint64_t pos) {
+    const uint64_t w = pos >> 6;
+    uint32_t i = (uint32_t)(w - first);
+    if (i >= 63) {     // word w + 1 is past cur
+      if (i < 126) {   // within the window: move on by 63 words
+        const uint64_t last = readlane64(cur, 63);
+        const uint64_t up = __shfl_up(nxt, 1);
+        cur = lane() == 0 ? last : up;
+        first += 63;
+      } else {
+        first = w;
+        load_cur(w);
+      }
+      nxt = g.load(first + 64 + lane());
+      i = (uint32_t)(w - first);
+    }
+    return peek_at(pos, i);
+  }
+  // the same where word pos / 64 is word i < 63 of cur (any i < 64 reads
+  // within cur)
+  __device__ uint64_t peek_at(uint64_t pos, uint32_t i) const {
+    const uint64_t lo = readlane64(cur, i);
+    const uint64_t hi = readlane64(cur, (i + 1) & 63);
+    const uint32_t s = (uint32_t)pos & 63;
+    return (lo >> s) | ((hi << 1) << (63 - s));
+  }
+};
+
+// One word of a bit plane's group tests, from coefficient count n.  Ones at
+// odd ranks open tokens "1 0^z 1", ones at even ranks close them; a closing
+// one followed by a zero ends the plane, the zero being its last test.  The
+// prefix parity of the ones is taken across the wave: lane j holds the count
+// of ones in bits 0..j (mlo / mhi are lane j's masks of bits 0..j).
+struct TestWord {
+  uint64_t closing;      // the closing ones (bits 0..62)
+  uint32_t n2, used, p;  // count after the word's tests, bits they take, last closing one taken
+  bool z0, end;          // first test negative; the plane ends in the word
+  __device__ __forceinline__ TestWord(uint64_t x, uint32_t n, uint32_t mlo, uint32_t mhi) {
+    z0 = !(x & 1);
+    const uint32_t cnt = __builtin_popcount((uint32_t)x & mlo) + __builtin_popcount((uint32_t)(x >> 32) & mhi);
+    closing = x & ~__ballot(cnt & 1) & ~(1ull << 63);
+    const uint64_t stop = closing & ~(x >> 1);
+    const uint32_t pa = (uint32_t)__builtin_ctzll(stop | (1ull << 63));
+    const uint32_t pb = 63 - (uint32_t)__builtin_clzll(closing | 1);
+    p = stop ? pa : pb;
+    const uint32_t tokens = (uint32_t)__builtin_amdgcn_readlane((int)cnt, (int)p) >> 1;
+    n2 = z0 ? n : n + p + 1 - tokens;
+    used = z0 ? 1 : (stop ? p + 2 : p + 1);
+    end = z0 | (stop != 0);
+  }
+};
+
 // Variable-size modes: one wavefront per field walks its blocks in stream
-// order and records where each begins (the format has no block index).
+// order and records where each begins (the format has no block index).  Lane
+// j keeps the offset of block 64 q + j; the wave stores 64 offsets at a time.
+//
+// The walk is a loop of turns, each reading the 64 stream bits at pos.  The
+// common turn, branch-free, takes one word of a plane's group tests, and when
+// the plane ends, the next plane's verbatim bits (a run of negative first
+// tests with no significant coefficients ends as many planes at once).  The
+// general turn takes the rest: block headers, window moves, the last
+// coefficient becoming significant (the plane's tests end there and the
+// planes left are verbatim), ends of blocks, and single tests where the word
+// cannot decide (a token longer than the word, a bit budget below 64).
 template <int D, class S>
 __global__ __launch_bounds__(64) void zfp_index_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks) {
-  constexpr int R = D == 4 ? 4 : 1;
+  typedef __attribute__((address_space(1))) uint64_t* g64;
+  constexpr bool fp = ZT<S>::fp;
+  using BInt = typename ZT<S>::Int;  // the coded integers
+  constexpr uint32_t SZ = 1u << (2 * D);
+  constexpr uint32_t intprec = sizeof(BInt) * 8;
+  constexpr uint32_t pbits = sizeof(BInt) == 4 ? 5 : 6;
+  constexpr uint32_t ebits = ZT<S>::ebits;
+  constexpr int ebias = ZT<S>::ebias;
+  enum { kHeader, kTests, kZeroRun };
+
   uint32_t ci = blockIdx.x;
   if (ci >= nchunks) return;
   const ZfpChunk& c = chunks[ci];
   if (c.fixed || c.index) return;  // computed addressing, or a block index
-  BitReaderT<WindowSrc> r;
-  r.src.init(c.src, c.src_len, c.header_bits >> 6);
-  r.seek(c.header_bits);
-  S vals[R];
-  for (uint64_t k = 0; k < c.nblocks; ++k) {
-    if (lane() == 0) c.offs[c.block_begin + k] = r.pos();
-    decode_block<D, S, true>(r, c, vals);
+  const int minbits = (int)c.minbits, maxbits = (int)c.maxbits, cmaxprec = (int)c.maxprec, minexp = c.minexp;
+  const bool rev = minexp < kZfpMinExp;
+  const uint64_t nblocks = c.nblocks;
+  const g64 offs = (g64)(uintptr_t)(c.offs + c.block_begin);
+  const uint32_t l = lane();
+  const uint32_t mlo = l < 32 ? (2u << l) - 1 : ~0u;
+  const uint32_t mhi = l < 32 ? 0u : (2u << (l - 32)) - 1;
+  WalkWindow win;
+  win.init(c.src, c.src_len, c.header_bits);
+
+  uint64_t pos = c.header_bits, blk_start = pos, kb = 0, mine = 0;
+  uint32_t bits = 0, n = 0, k = 0, kmin = 0, zacc = 0, zlim = 0;
+  int st = kHeader;
+  for (;;) {
+    while (st == kTests) {  // common turns (n < SZ here)
+      const uint32_t i = (uint32_t)((pos >> 6) - win.first);
+      const uint64_t x = win.peek_at(pos, i & 63);
+      const TestWord tw(x, n, mlo, mhi);
+      const bool zrun = tw.z0 & (n == 0);  // negative first tests ending empty planes
+      const uint32_t tz = (uint32_t)__builtin_ctzll(x | (1ull << 63));
+      const uint32_t ends = tw.end ? (zrun ? tz : 1u) : 0u;
+      const uint32_t used = zrun ? tz : tw.used;
+      const uint32_t bits1 = bits - used;
+      const uint32_t m = tw.n2 < bits1 ? tw.n2 : bits1;
+      const bool decided = tw.z0 | ((tw.closing != 0) & (tw.n2 < SZ));
+      if (!((i < 63) & (bits >= 64) & decided & (!tw.end | ((bits1 != 0) & (k - kmin >= ends))))) break;
+      n = tw.n2;
+      pos += tw.end ? used + m : used;
+      bits = tw.end ? bits1 - m : bits1;
+      k -= ends;
+    }
+    // the general turn
+    if (st == kHeader) {
+      if (kb == nblocks) break;
+      const uint32_t j = (uint32_t)kb & 63;
+      if (l == j) mine = pos;
+      if (j == 63) offs[kb - 63 + l] = mine;
+      blk_start = pos;
+    }
+    const uint64_t x = win.peek(pos);
+    bool plane_end = false;
+    if (st == kHeader) {
+      int hdr = 0, prec = cmaxprec;
+      if constexpr (fp) {
+        if (!(x & 1)) {  // every value zero: one bit (reversible) or minbits
+          pos += rev || minbits <= 1 ? 1u : (uint32_t)minbits;
+          ++kb;
+          continue;
+        }
+        if (rev) {  // 1 1: raw bits; 1 0: a block exponent follows
+          hdr = (x & 2) ? 2 : 2 + (int)ebits;
+          prec = (int)((x >> hdr) & ((1u << pbits) - 1)) + 1;
+          hdr += (int)pbits;
+        } else {
+          const int emax = (int)((x >> 1) & ((1u << ebits) - 1)) - ebias;
+          const int dp = emax - minexp + 2 * (D + 1);
+          prec = cmaxprec < (dp > 0 ? dp : 0) ? cmaxprec : (dp > 0 ? dp : 0);
+          hdr = 1 + (int)ebits;
+        }
+      } else if (rev) {
+        prec = (int)(x & ((1u << pbits) - 1)) + 1;
+        hdr = (int)pbits;
+      }
+      pos += (uint32_t)hdr;
+      bits = maxbits - hdr > 0 ? (uint32_t)(maxbits - hdr) : 0u;
+      n = 0;
+      k = intprec;
+      kmin = intprec > (uint32_t)prec ? intprec - (uint32_t)prec : 0;
+      plane_end = true;
+    } else if (st == kZeroRun) {  // a test's zeros that ran past the last word
+      const uint32_t t = x ? (uint32_t)__builtin_ctzll(x) : 64u;
+      if (t < zlim && t < 64) {  // the closing one
+        pos += t + 1;
+        bits -= t + 1;
+        n += zacc + t + 1;
+        st = kTests;
+      } else if (zlim <= 64) {  // the zeros reach the limit: no closing one
+        pos += zlim;
+        bits -= zlim;
+        n += zacc + zlim + 1;
+        st = kTests;
+      } else {
+        pos += 64;
+        bits -= 64;
+        zacc += 64;
+        zlim -= 64;
+      }
+    } else if (n >= SZ || !bits) {
+      plane_end = true;
+    } else {
+      const TestWord tw(x, n, mlo, mhi);
+      if (bits >= 64 && tw.z0) {
+        pos += 1;
+        bits -= 1;
+        plane_end = true;
+      } else if (bits >= 64 && tw.closing && tw.n2 >= SZ) {
+        // the last coefficient becomes significant inside the word: its
+        // token is the one at which the count of bits other than closing
+        // ones reaches SZ - n (lim zeros after the opening one, no closing)
+        const uint64_t other = ~tw.closing;
+        const uint32_t cnt =
+            __builtin_popcount((uint32_t)other & mlo) + __builtin_popcount((uint32_t)(other >> 32) & mhi);
+        const uint32_t q = (uint32_t)__builtin_ctzll(__ballot(cnt == SZ - n) & other);
+        pos += q + 1;
+        bits -= q + 1;
+        n = SZ;
+        plane_end = true;
+      } else if (bits >= 64 && tw.closing) {
+        n = tw.n2;
+        pos += tw.used;
+        bits -= tw.used;
+        plane_end = tw.end;
+      } else {  // one test: a budget below 64, or a token longer than the word
+        bits--;
+        if (!(x & 1)) {
+          pos += 1;
+          plane_end = true;
+        } else {
+          uint32_t lim = SZ - 1 - n;
+          if (lim > bits) lim = bits;
+          const uint64_t y = x >> 1;
+          const uint32_t t = y ? (uint32_t)__builtin_ctzll(y) : 64u;
+          if (t < lim && t < 64) {  // the closing one, in view
+            pos += t + 2;
+            bits -= t + 1;
+            n += t + 1;
+          } else if (lim <= 63) {  // the zeros reach the limit in view
+            pos += lim + 1;
+            bits -= lim;
+            n += lim + 1;
+          } else {  // 63 zeros in view, more to come
+            pos += 64;
+            bits -= 63;
+            zacc = 63;
+            zlim = lim - 63;
+            st = kZeroRun;
+          }
+        }
+      }
+    }
+    plane_end |= (st == kTests) & (n >= SZ);  // the tests end with the last coefficient
+    if (plane_end) {
+      bool block_end = true;
+      if (bits && k > kmin) {
+        --k;
+        if (n == SZ) {  // every coefficient significant: the planes left are SZ verbatim bits each
+          const uint64_t need = (uint64_t)(k + 1 - kmin) * SZ;
+          const uint32_t take = need < bits ? (uint32_t)need : bits;
+          pos += take;
+          bits -= take;
+        } else {
+          const uint32_t m = n < bits ? n : bits;
+          bits -= m;
+          pos += m;
+          st = kTests;
+          block_end = false;
+        }
+      }
+      if (block_end) {
+        if (pos - blk_start < (uint64_t)(minbits > 0 ? minbits : 0)) pos = blk_start + (uint32_t)minbits;
+        ++kb;
+        st = kHeader;
+      }
+    }
   }
+  const uint64_t tail = nblocks & 63;
+  if (l < tail) offs[nblocks - tail + l] = mine;
 }
 
 template <int D, class S>
