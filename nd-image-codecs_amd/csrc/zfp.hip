@@ -1037,20 +1037,69 @@ struct TestWord {
   }
 };
 
-// Variable-size modes: one wavefront per field walks its blocks in stream
-// order and records where each begins (the format has no block index).  Lane
-// j keeps the offset of block 64 q + j; the wave stores 64 offsets at a time.
+// The test word at every stream position, computed ahead of the walk.  For
+// the group tests of a plane starting at bit t, what the word at t decides
+// does not depend on the coefficient count n (only its last-coefficient
+// check does), so other waves compute it for every t in parallel while the
+// walking wave only chains lookups.  Entry (16 bits): kind in bits 14..15 --
+// 0: the first test is negative (bits 7..13: the run of zeros, at most 63),
+// 1: the plane's tests end in the word, 2: tokens run past the word, 3: a
+// token longer than the word; bits 7..13 the bits taken, bits 0..6 the count
+// of coefficients they make significant.
+enum : uint32_t { kWordZero = 0, kWordEnd = 1, kWordOn = 2, kWordLong = 3 };
+
+__device__ __forceinline__ uint32_t test_word_entry(GlobalSrc& g, uint64_t t) {
+  const uint64_t w = t >> 6;
+  const uint32_t sh = (uint32_t)t & 63;
+  const uint64_t x = (g.load(w) >> sh) | ((g.load(w + 1) << 1) << (63 - sh));
+  if (!(x & 1)) return (kWordZero << 14) | ((uint32_t)__builtin_ctzll(x | (1ull << 63)) << 7);
+  const uint64_t closing = x & ~prefix_parity(x) & ~(1ull << 63);
+  if (!closing) return kWordLong << 14;
+  const uint64_t stop = closing & ~(x >> 1);
+  const uint32_t p = stop ? (uint32_t)__builtin_ctzll(stop) : 63 - (uint32_t)__builtin_clzll(closing);
+  const uint32_t tokens = (uint32_t)__builtin_popcountll(x & (~0ull >> (63 - p))) >> 1;
+  return ((stop ? kWordEnd : kWordOn) << 14) | ((stop ? p + 2 : p + 1) << 7) | (p + 1 - tokens);
+}
+
+// The ring of test-word entries in LDS: kRingSlots slots of kSlotBits
+// positions; chunk c (positions base + c kSlotBits ..) goes to slot
+// c % kRingSlots.  Producers take 64-position batches from a shared counter
+// (so all of them work on the chunks nearest the walk) and write a batch
+// only while its chunk is within kRingSlots of the chunk the walk reads
+// (`reading`); when the walk moves on, it raises the counter past the
+// batches it no longer needs.  Each entry carries its chunk number in its
+// upper 32 bits, and the walk takes an entry only when the number is its
+// own, so that an entry not yet written, or one overwritten by a producer
+// that fell behind, can cost time (a short wait, then that turn parses the
+// stream words) but never the result.
+constexpr uint32_t kWalkProducers = 15;
+constexpr uint32_t kWalkThreads = 64 * (1 + kWalkProducers);
+constexpr uint32_t kRingSlots = 16, kSlotBits = 512;
+constexpr uint32_t kWalkSpins = 1u << 22;
+
+__device__ __forceinline__ uint32_t lds_acquire(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_release(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Variable-size modes: one workgroup per field walks its blocks in stream
+// order and records where each begins (the format has no block index).  Wave
+// 0 walks; waves 1.. fill the ring of test words ahead of it.  Lane j of the
+// walking wave keeps the offset of block 64 q + j; it stores 64 at a time.
 //
-// The walk is a loop of turns, each reading the 64 stream bits at pos.  The
-// common turn, branch-free, takes one word of a plane's group tests, and when
-// the plane ends, the next plane's verbatim bits (a run of negative first
-// tests with no significant coefficients ends as many planes at once).  The
-// general turn takes the rest: block headers, window moves, the last
-// coefficient becoming significant (the plane's tests end there and the
-// planes left are verbatim), ends of blocks, and single tests where the word
-// cannot decide (a token longer than the word, a bit budget below 64).
+// The walk is a loop of turns.  The common turn, branch-free, takes one word
+// of a plane's group tests from the ring, and when the plane ends, the next
+// plane's verbatim bits (a run of negative first tests with no significant
+// coefficients ends as many planes at once).  The general turn reads the
+// stream words and takes the rest: block headers, the last coefficient
+// becoming significant (the plane's tests end there and the planes left are
+// verbatim), ends of blocks, and single tests where the word cannot decide (a
+// token longer than the word, a bit budget below 64).
 template <int D, class S>
-__global__ __launch_bounds__(64) void zfp_index_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks) {
+__global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk* __restrict__ chunks,
+                                                                 uint32_t nchunks) {
   typedef __attribute__((address_space(1))) uint64_t* g64;
   constexpr bool fp = ZT<S>::fp;
   using BInt = typename ZT<S>::Int;  // the coded integers
@@ -1070,33 +1119,88 @@ __global__ __launch_bounds__(64) void zfp_index_kernel(const ZfpChunk* __restric
   const uint64_t nblocks = c.nblocks;
   const g64 offs = (g64)(uintptr_t)(c.offs + c.block_begin);
   const uint32_t l = lane();
+  const uint64_t base = c.header_bits;
+  const uint64_t stream_bits = c.src_len * 8;
+  const uint32_t nring = stream_bits > base ? (uint32_t)((stream_bits - base + kSlotBits - 1) / kSlotBits) : 0u;
+
+  __shared__ uint64_t ring[kRingSlots * kSlotBits];
+  __shared__ uint32_t reading, next_batch, done;
+  for (uint32_t i = threadIdx.x; i < kRingSlots * kSlotBits; i += kWalkThreads)
+    ring[i] = ~0ull << 32 | kWordLong << 14;  // no chunk's; undecided
+  if (threadIdx.x == 0) reading = next_batch = done = 0;
+  __syncthreads();
+
+  constexpr uint32_t kBatches = kSlotBits / 64;  // batches per chunk
+  if (threadIdx.x >= 64) {  // a producer
+    GlobalSrc g{c.src, c.src_len};
+    const uint64_t nbatches = (uint64_t)nring * kBatches;
+    for (;;) {
+      uint32_t b = 0;
+      if (l == 0) b = __hip_atomic_fetch_add(&next_batch, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      b = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+      if (b >= nbatches || lds_acquire(&done)) break;
+      const uint32_t ch = b / kBatches;
+      bool go = false;
+      for (uint32_t spin = 0; spin < kWalkSpins; ++spin) {
+        const uint32_t at = lds_acquire(&reading);
+        if (at > ch || lds_acquire(&done)) break;  // passed by the walk: not needed
+        if (at + kRingSlots > ch) {
+          go = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (!go) continue;
+      const uint32_t r = (b % kBatches) * 64 + l;
+      __hip_atomic_store(&ring[(ch % kRingSlots) * kSlotBits + r],
+                         (uint64_t)ch << 32 | test_word_entry(g, base + (uint64_t)ch * kSlotBits + r),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    return;
+  }
+
   const uint32_t mlo = l < 32 ? (2u << l) - 1 : ~0u;
   const uint32_t mhi = l < 32 ? 0u : (2u << (l - 32)) - 1;
   WalkWindow win;
   win.init(c.src, c.src_len, c.header_bits);
 
-  uint64_t pos = c.header_bits, blk_start = pos, kb = 0, mine = 0;
+  uint64_t pos = base, blk_start = pos, kb = 0, mine = 0;
   uint32_t bits = 0, n = 0, k = 0, kmin = 0, zacc = 0, zlim = 0;
+  uint32_t cur_ch = ~0u;
   int st = kHeader;
   for (;;) {
-    while (st == kTests) {  // common turns (n < SZ here)
-      const uint32_t i = (uint32_t)((pos >> 6) - win.first);
-      const uint64_t x = win.peek_at(pos, i & 63);
-      const TestWord tw(x, n, mlo, mhi);
-      const bool zrun = tw.z0 & (n == 0);  // negative first tests ending empty planes
-      const uint32_t tz = (uint32_t)__builtin_ctzll(x | (1ull << 63));
-      const uint32_t ends = tw.end ? (zrun ? tz : 1u) : 0u;
-      const uint32_t used = zrun ? tz : tw.used;
-      const uint32_t bits1 = bits - used;
-      const uint32_t m = tw.n2 < bits1 ? tw.n2 : bits1;
-      const bool decided = tw.z0 | ((tw.closing != 0) & (tw.n2 < SZ));
-      if (!((i < 63) & (bits >= 64) & decided & (!tw.end | ((bits1 != 0) & (k - kmin >= ends))))) break;
-      n = tw.n2;
-      pos += tw.end ? used + m : used;
-      bits = tw.end ? bits1 - m : bits1;
-      k -= ends;
+    if (st == kTests) {  // common turns (n < SZ here)
+      for (;;) {
+        const uint64_t rel = pos - base;
+        const uint64_t te = __hip_atomic_load(&ring[(uint32_t)rel % (kRingSlots * kSlotBits)], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t e = (uint32_t)te, kind = e >> 14, uf = (e >> 7) & 127;
+        const bool z0 = kind == kWordZero;
+        const uint32_t used = z0 & (n != 0) ? 1u : uf;
+        const uint32_t n2 = n + (e & 127);
+        const uint32_t ends = z0 ? used : 2 - kind;  // planes ended: kWordEnd 1, kWordOn 0
+        const uint32_t bits1 = bits - used;
+        const uint32_t madd = kind <= kWordEnd ? (n2 < bits1 ? n2 : bits1) : 0u;  // the next plane's verbatim bits
+        // the entry is this chunk's and decides; the budget is above the
+        // word; the planes ended exist
+        if (!(((uint32_t)(te >> 32) == (uint32_t)(rel / kSlotBits)) & (kind != kWordLong) & (n2 < SZ) &
+              (bits > 64) & (k - kmin >= ends)))
+          break;
+        n = n2;
+        pos += used + madd;
+        bits = bits1 - madd;
+        k -= ends;
+      }
     }
     // the general turn
+    {
+      const uint32_t ch = (uint32_t)((pos - base) / kSlotBits);
+      if (ch != cur_ch) {  // let the producers move on
+        lds_release(&reading, ch);
+        __hip_atomic_fetch_max(&next_batch, ch * kBatches, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        cur_ch = ch;
+      }
+    }
     if (st == kHeader) {
       if (kb == nblocks) break;
       const uint32_t j = (uint32_t)kb & 63;
@@ -1230,6 +1334,7 @@ __global__ __launch_bounds__(64) void zfp_index_kernel(const ZfpChunk* __restric
       }
     }
   }
+  lds_release(&done, 1);
   const uint64_t tail = nblocks & 63;
   if (l < tail) offs[nblocks - tail + l] = mine;
 }
@@ -1293,7 +1398,7 @@ template <int D, class S> void dec(const ZfpChunk* d, uint32_t n, uint64_t total
   hipLaunchKernelGGL((zfp_decode_kernel<D, S>), xcd_grid(ceil_div(total, 4)), dim3(256), 0, s, d, n, total, br);
 }
 template <int D, class S> void idx(const ZfpChunk* d, uint32_t n, hipStream_t s) {
-  hipLaunchKernelGGL((zfp_index_kernel<D, S>), dim3(n), dim3(64), 0, s, d, n);
+  hipLaunchKernelGGL((zfp_index_kernel<D, S>), dim3(n), dim3(kWalkThreads), 0, s, d, n);
 }
 
 #define NDIC_ZFP_DISPATCH(FN, ...)                                                     \
