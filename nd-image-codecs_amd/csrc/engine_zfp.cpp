@@ -607,7 +607,7 @@ void zfp_decode_items(Context& ctx, std::vector<ZfpItem>& items, hipStream_t s) 
       clock.end();
     }
     clock.begin(ZfpClock::kDecode);
-    launch_zfp_decode(d_ch, (uint32_t)ch.size(), total, cls / 8, cls % 8, nullptr, s);
+    launch_zfp_decode(d_ch, (uint32_t)ch.size(), total, cls / 8, cls % 8, nullptr, zfp_fixed_maxbits(ch), s);
     clock.end();
     const uint32_t* flags = indexed ? (const uint32_t*)ctx.down(bad, ch.size() * 4, s) : nullptr;
     NDIC_HIP(hipStreamSynchronize(s));
@@ -771,7 +771,7 @@ extern "C" ndic_status ndic_zfp_decode_brick(const uint8_t* stream, size_t len, 
     b.out = d_out;
     NDIC_HIP(hipMemcpyAsync(d_ch, &c, sizeof(c), hipMemcpyHostToDevice, s));
     NDIC_HIP(hipMemcpyAsync(d_br, &b, sizeof(b), hipMemcpyHostToDevice, s));
-    launch_zfp_decode(d_ch, 1, 1, pl.dims, pl.scalar, d_br, s);
+    launch_zfp_decode(d_ch, 1, 1, pl.dims, pl.scalar, d_br, zfp_fixed_maxbits({c}), s);
     emit_host(d_out, nout, out, s);
   });
 }
@@ -1003,7 +1003,7 @@ extern "C" ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_ou
       ZfpClock clock(s);
       clock.begin(ZfpClock::kDecode);
       launch_zfp_decode((ZfpChunk*)w, (uint32_t)bc.chunks.size(), bc.bricks.size(), bc.dims, bc.scalar,
-                        (ZfpBrick*)(w + sz_ch), s);
+                        (ZfpBrick*)(w + sz_ch), zfp_fixed_maxbits(bc.chunks), s);
       clock.end();
       NDIC_HIP(hipStreamSynchronize(s));
       clock.collect();

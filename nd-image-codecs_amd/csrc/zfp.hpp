@@ -1,6 +1,9 @@
-// ZFP (LLNL ZFP stream format, codec version 5) on the device: one wavefront
-// per 4^d block.  Definitions in zfp.hip, orchestration in engine_zfp.cpp.
+// ZFP (LLNL ZFP stream format, codec version 5) on the device: one lane per
+// 4^d block for 1D-3D fields (zfp_lane.hip), one wavefront per block for 4D
+// fields (zfp.hip); orchestration in engine_zfp.cpp.
 #pragma once
+
+#include <vector>
 
 #include "common.hpp"
 
@@ -82,8 +85,23 @@ void launch_zfp_place(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total
                       hipStream_t stream);
 void launch_zfp_index(const ZfpChunk* d_chunks, uint32_t nchunks, uint32_t dims, uint32_t scalar,
                       hipStream_t stream);
+// fixed_maxbits: the largest block size when every chunk is fixed-rate (it
+// sizes the lane decoder's LDS stage), else 0.
 void launch_zfp_decode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, uint32_t dims, uint32_t scalar,
-                       const ZfpBrick* d_bricks, hipStream_t stream);
+                       const ZfpBrick* d_bricks, uint32_t fixed_maxbits, hipStream_t stream);
+// One lane per block for the 1D-3D classes (zfp_lane.hip); false for 4D.
+bool launch_zfp_lane_encode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, uint32_t dims,
+                            uint32_t scalar, hipStream_t stream);
+bool launch_zfp_lane_decode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, uint32_t dims,
+                            uint32_t scalar, const ZfpBrick* d_bricks, uint32_t fixed_maxbits, hipStream_t stream);
+inline uint32_t zfp_fixed_maxbits(const std::vector<ZfpChunk>& ch) {
+  uint32_t m = 0;
+  for (const ZfpChunk& c : ch) {
+    if (!c.fixed) return 0;
+    m = c.maxbits > m ? c.maxbits : m;
+  }
+  return m;
+}
 // Variable-size modes, after the scan: writes each indexed chunk's block index.
 void launch_zfp_index_write(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, const uint64_t* d_scan,
                             hipStream_t stream);

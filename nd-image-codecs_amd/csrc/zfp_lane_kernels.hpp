@@ -1,0 +1,858 @@
+// ZFP block codec on the device, one lane per 4^d block (d = 1..3).
+//
+// Each lane runs the whole block through the coder: gather (pad rule of
+// partial blocks), block-floating-point cast, decorrelating lifts, negabinary
+// ranks, embedded group-test bit-plane coder, and the inverse of all that.
+// The 4^d coefficients of a lane's block live in its registers: every index
+// into them is a compile-time constant (the lifts walk fixed lines, the rank
+// order is a constexpr table resolved per rank), so nothing spills.  Bit
+// planes are the coder's unit: the decoder fills plane words (bit r = the
+// coefficient of rank r) one plane at a time, the encoder reads them, and
+// one in-register bit-matrix transpose converts between plane words and
+// coefficient magnitudes.
+//
+// A wave decodes 64 neighbouring blocks.  When their bits are one contiguous
+// stretch of one stream (fixed-rate, or variable-size with a block index or
+// walked offsets) the wave first copies that stretch into LDS with
+// coalesced loads, and the lanes' bit readers take their words from there;
+// otherwise (bricks of different streams, a stretch larger than the stage)
+// they read global memory.
+//
+// The 4D class (256 coefficients a block) stays with one wavefront per block
+// (zfp.hip).  Streams and samples equal that coder's bit for bit.  The
+// algorithm and stream layout are the published ZFP ones (Lindstrom, IEEE
+// TVCG 2014; LLNL ZFP stream format, codec 5), which the reference reaches
+// through zfp-rs 0.1.1 (crates/ndic-zfp/src/lib.rs:249-376).
+#pragma once
+
+#include <utility>
+
+#include "zfp.hpp"
+#include "zfp_dev.hpp"
+
+namespace ndic {
+
+namespace {
+
+using namespace zfpdev;
+
+constexpr uint32_t kLaneThreads = 64;
+
+template <int D> constexpr int perm_at(int r) {
+  return D == 1 ? r : D == 2 ? (int)kPerm2[r] : (int)kPerm3[r];
+}
+
+__device__ __forceinline__ uint32_t ctz64(uint64_t x) { return x ? (uint32_t)__builtin_ctzll(x) : 64u; }
+__device__ __forceinline__ uint64_t lowmask64(uint32_t n) { return n >= 64 ? ~0ull : ((1ull << n) - 1); }
+
+// ---------------------------------------------------------------------------
+// bit-matrix transpose of 32 words of 32 bits, in place: bit i of a[k]
+// becomes bit k of a[i] (recursive exchange of off-diagonal blocks)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void transpose32(uint32_t* a) {
+#pragma unroll
+  for (int j = 16; j >= 1; j >>= 1) {
+    const uint32_t m = 0xffffffffu / ((1u << j) + 1u);  // bits whose index has bit j clear
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      if (k & j) continue;
+      const uint32_t t = ((a[k] >> j) ^ a[k + j]) & m;
+      a[k + j] ^= t;
+      a[k] ^= t << j;
+    }
+  }
+}
+
+// Plane words <-> coefficient magnitudes for S = 4^D coefficients of IP bits.
+// Planes: lo[k] holds bit k of coefficients 0..31 (bit r = rank r), hi[k] of
+// coefficients 32..63 (S = 64 only).  Coefficients: u[r], r < S.
+template <int S, int IP> struct PlaneSet {
+  static constexpr int kHi = S > 32 ? IP : 1;
+  uint32_t lo[IP];
+  uint32_t hi[kHi];
+};
+
+template <int S, class UInt> __device__ __forceinline__ void planes_to_coeffs(PlaneSet<S, sizeof(UInt) * 8>& p,
+                                                                               UInt* u) {
+  constexpr int IP = sizeof(UInt) * 8;
+  constexpr int S0 = S < 32 ? S : 32;  // coefficients in lo
+  transpose32(p.lo);
+  if constexpr (IP == 64) transpose32(p.lo + 32);
+  if constexpr (S > 32) {
+    transpose32(p.hi);
+    if constexpr (IP == 64) transpose32(p.hi + 32);
+  }
+  // 64 planes: planes 32..63 (the upper coefficient bits) are their own
+  // 32x32 blocks
+#pragma unroll
+  for (int r = 0; r < S0; ++r) {
+    if constexpr (IP == 64) u[r] = ((uint64_t)p.lo[32 + r] << 32) | p.lo[r];
+    else u[r] = p.lo[r];
+  }
+  if constexpr (S > 32) {
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+      if constexpr (IP == 64) u[32 + r] = ((uint64_t)p.hi[32 + r] << 32) | p.hi[r];
+      else u[32 + r] = p.hi[r];
+    }
+  }
+}
+
+template <int S, class UInt> __device__ __forceinline__ void coeffs_to_planes(const UInt* u,
+                                                                               PlaneSet<S, sizeof(UInt) * 8>& p) {
+  constexpr int IP = sizeof(UInt) * 8;
+  constexpr int S0 = S < 32 ? S : 32;
+#pragma unroll
+  for (int r = 0; r < IP; ++r) p.lo[r] = 0;
+#pragma unroll
+  for (int r = 0; r < S0; ++r) {
+    p.lo[r] = (uint32_t)u[r];
+    if constexpr (IP == 64) p.lo[32 + r] = (uint32_t)((uint64_t)u[r] >> 32);
+  }
+  if constexpr (S > 32) {
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+      p.hi[r] = (uint32_t)u[32 + r];
+      if constexpr (IP == 64) p.hi[32 + r] = (uint32_t)((uint64_t)u[32 + r] >> 32);
+    }
+  }
+  transpose32(p.lo);
+  if constexpr (IP == 64) transpose32(p.lo + 32);
+  if constexpr (S > 32) {
+    transpose32(p.hi);
+    if constexpr (IP == 64) transpose32(p.hi + 32);
+  }
+}
+
+// ranks <-> coefficients in block order (negabinary), resolved per rank at
+// compile time
+template <int D, class Int, class UInt, int... R>
+__device__ __forceinline__ void ranks_to_block(const UInt* u, Int* v, std::integer_sequence<int, R...>) {
+  const UInt m = nbmask<UInt>();
+  ((v[std::integral_constant<int, perm_at<D>(R)>::value] = (Int)((u[R] ^ m) - m)), ...);
+}
+template <int D, class Int, class UInt, int... R>
+__device__ __forceinline__ void block_to_ranks(const Int* v, UInt* u, std::integer_sequence<int, R...>) {
+  const UInt m = nbmask<UInt>();
+  ((u[R] = ((UInt)v[std::integral_constant<int, perm_at<D>(R)>::value] + m) ^ m), ...);
+}
+
+// separable lifts along x, y, z of a block in registers (forward: x first)
+template <int D, class Int, bool Fwd, bool Rev> __device__ __forceinline__ void lane_xform(Int* v) {
+  constexpr int S = 1 << (2 * D);
+#pragma unroll
+  for (int t = 0; t < D; ++t) {
+    const int a = Fwd ? t : D - 1 - t;
+    const int s = 1 << (2 * a);
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+      if ((p / s) % 4) continue;
+      lift4<Int, Fwd, Rev>(v[p], v[p + s], v[p + 2 * s], v[p + 3 * s]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-lane bit reader: words from the wave's LDS stage or global memory
+// ---------------------------------------------------------------------------
+extern __shared__ uint64_t zstage[];
+
+__device__ __forceinline__ uint64_t global_word(const uint8_t* base, uint64_t nbytes, uint64_t w) {
+  typedef const __attribute__((address_space(1))) uint64_t* g64;
+  typedef const __attribute__((address_space(1))) uint8_t* g8;
+  const uint64_t b = w * 8;
+  if (b + 8 <= nbytes) return *(g64)(uintptr_t)(base + b);
+  uint64_t v = 0;
+  for (uint64_t i = b; i < nbytes && i < b + 8; ++i) v |= (uint64_t)*(g8)(uintptr_t)(base + i) << (8 * (i - b));
+  return v;
+}
+
+// Keeps the 128 stream bits at its position in two words and the word after
+// them in flight, so a word is asked for one word of parsing before it is
+// needed.
+struct LaneReader {
+  const uint8_t* base;
+  uint64_t nbytes;
+  uint64_t s0, sn;   // the stage holds words s0 .. s0 + sn - 1
+  uint64_t next;     // index of the word after w2
+  uint64_t cur, w1, w2;
+  uint32_t left;     // valid bits of cur (low end), 1..64
+
+  __device__ __forceinline__ uint64_t load(uint64_t w) const {
+    if (w - s0 < sn) return zstage[w - s0];
+    return global_word(base, nbytes, w);
+  }
+  __device__ __forceinline__ void seek(uint64_t bit) {
+    const uint64_t w = bit >> 6;
+    const uint32_t o = (uint32_t)bit & 63;
+    cur = load(w) >> o;
+    left = 64 - o;
+    w1 = load(w + 1);
+    w2 = load(w + 2);
+    next = w + 3;
+  }
+  // the next 64 bits, first bit lowest
+  __device__ __forceinline__ uint64_t peek() const { return left == 64 ? cur : cur | (w1 << left); }
+  // consumes n <= 64 bits
+  __device__ __forceinline__ void skip(uint32_t n) {
+    if (n < left) {
+      cur >>= n;
+      left -= n;
+      return;
+    }
+    const uint32_t r = n - left;  // < 64: left >= 1
+    cur = w1 >> r;
+    left = 64 - r;
+    w1 = w2;
+    w2 = load(next++);
+  }
+  __device__ __forceinline__ uint64_t get(uint32_t n) {
+    const uint64_t v = peek() & lowmask64(n);
+    skip(n);
+    return v;
+  }
+  __device__ __forceinline__ uint32_t bit() { return (uint32_t)get(1); }
+};
+
+// ---------------------------------------------------------------------------
+// per-lane bit writer (LSB-first 64-bit words)
+// ---------------------------------------------------------------------------
+// Fixed-rate blocks share their edge words with the neighbouring blocks (OR
+// them in); the stream is zeroed beforehand, so zero words need no store.
+// A variable-size block writes every word of its private slot.
+struct LaneWriter {
+  uint64_t* out;
+  uint64_t word;
+  uint64_t acc;
+  uint32_t fill;
+  bool shared, first;
+  uint64_t count;
+
+  __device__ __forceinline__ void init(uint64_t* base, uint64_t bitpos, bool share) {
+    out = base;
+    word = bitpos >> 6;
+    fill = (uint32_t)bitpos & 63;
+    acc = 0;
+    shared = share;
+    first = true;
+    count = 0;
+  }
+  __device__ __forceinline__ void flush() {
+    if (shared && first) atomicOr((unsigned long long*)&out[word], (unsigned long long)acc);
+    else if (!shared || acc) out[word] = acc;
+    first = false;
+    ++word;
+    acc = 0;
+  }
+  // appends the low n <= 64 bits of v (v has no bits at or above n)
+  __device__ __forceinline__ void put(uint64_t v, uint32_t n) {
+    if (!n) return;
+    count += n;
+    acc |= v << fill;
+    const uint32_t f = fill + n;
+    if (f >= 64) {
+      const uint32_t used = 64 - fill;
+      flush();
+      acc = used < 64 ? v >> used : 0;
+      fill = f - 64;
+    } else {
+      fill = f;
+    }
+  }
+  __device__ __forceinline__ void zeros(uint64_t n) {
+    if (!n) return;
+    count += n;
+    if (fill + n < 64) {
+      fill += (uint32_t)n;
+      return;
+    }
+    n -= 64 - fill;
+    flush();
+    while (n >= 64) {
+      if (!shared) out[word] = 0;
+      ++word;
+      n -= 64;
+    }
+    fill = (uint32_t)n;
+  }
+  __device__ __forceinline__ void finish() {
+    if (!fill) return;
+    if (shared) atomicOr((unsigned long long*)&out[word], (unsigned long long)acc);
+    else out[word] = acc;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// embedded bit-plane coder over one lane's ranked coefficients
+// ---------------------------------------------------------------------------
+// Planes KHI down to KLO (at most 32 of them, so that the loop unrolls and
+// every plane word has a register of its own).
+template <int S, int IP, int KHI, int KLO>
+__device__ __forceinline__ void lane_decode_planes(LaneReader& r, uint32_t kmin, uint32_t& bits, uint32_t& n,
+                                                   PlaneSet<S, IP>& p) {
+#pragma unroll
+  for (int k = KHI; k >= KLO; --k) {
+    uint64_t x = 0;
+    if ((uint32_t)k >= kmin && bits) {
+      const uint32_t m = n < bits ? n : bits;
+      bits -= m;
+      if (m) x = r.get(m);
+      while (n < (uint32_t)S && bits) {
+        bits--;
+        const uint64_t w = r.peek();
+        if (!(w & 1)) {
+          r.skip(1);
+          break;
+        }
+        // a positive group test: zeros up to the next significant
+        // coefficient (at most lim of them), then its one (implied at lim)
+        uint32_t lim = S - 1 - n;
+        if (lim > bits) lim = bits;
+        uint32_t z = ctz64(w >> 1);
+        if (z < lim) {
+          r.skip(z + 2);
+          bits -= z + 1;
+        } else {
+          z = lim;
+          r.skip(lim + 1);
+          bits -= lim;
+        }
+        n += z;
+        x |= 1ull << n;
+        n++;
+      }
+    }
+    p.lo[k] = (uint32_t)x;
+    if constexpr (S > 32) p.hi[k] = (uint32_t)(x >> 32);
+  }
+}
+
+template <int S, int IP>
+__device__ __forceinline__ uint32_t lane_decode_ints(LaneReader& r, uint32_t maxbits, uint32_t maxprec,
+                                                     PlaneSet<S, IP>& p) {
+  const uint32_t kmin = IP > (int)maxprec ? IP - maxprec : 0;
+  uint32_t bits = maxbits, n = 0;
+  if constexpr (IP == 64) lane_decode_planes<S, IP, 63, 32>(r, kmin, bits, n, p);
+  lane_decode_planes<S, IP, 31, 0>(r, kmin, bits, n, p);
+  return maxbits - bits;
+}
+
+template <int S, int IP, int KHI, int KLO>
+__device__ __forceinline__ void lane_encode_planes(LaneWriter& w, uint32_t kmin, uint32_t& bits, uint32_t& n,
+                                                   const PlaneSet<S, IP>& p) {
+#pragma unroll
+  for (int k = KHI; k >= KLO; --k) {
+    if ((uint32_t)k >= kmin && bits) {
+      uint64_t x = p.lo[k];
+      if constexpr (S > 32) x |= (uint64_t)p.hi[k] << 32;
+      const uint32_t m = n < bits ? n : bits;
+      bits -= m;
+      w.put(x & lowmask64(m), m);
+      while (n < (uint32_t)S && bits) {
+        bits--;
+        const uint64_t rest = x >> n;  // n < S <= 64
+        if (!rest) {
+          w.put(0, 1);
+          break;
+        }
+        const uint32_t z = (uint32_t)__builtin_ctzll(rest);
+        w.put(1, 1);
+        if (bits <= z) {
+          w.zeros(bits);
+          bits = 0;
+          break;
+        }
+        bits -= z;
+        n += z;
+        if (n < (uint32_t)S - 1) {
+          bits--;
+          w.put(1ull << z, z + 1);
+        } else {
+          w.zeros(z);
+        }
+        n++;
+      }
+    }
+  }
+}
+
+template <int S, int IP>
+__device__ __forceinline__ uint32_t lane_encode_ints(LaneWriter& w, uint32_t maxbits, uint32_t maxprec,
+                                                     const PlaneSet<S, IP>& p) {
+  const uint32_t kmin = IP > (int)maxprec ? IP - maxprec : 0;
+  uint32_t bits = maxbits, n = 0;
+  if constexpr (IP == 64) lane_encode_planes<S, IP, 63, 32>(w, kmin, bits, n, p);
+  lane_encode_planes<S, IP, 31, 0>(w, kmin, bits, n, p);
+  return maxbits - bits;
+}
+
+// ---------------------------------------------------------------------------
+// integer blocks
+// ---------------------------------------------------------------------------
+template <int D, class Int>
+__device__ __forceinline__ uint32_t lane_decode_int_block(LaneReader& r, int minbits, int maxbits, int maxprec,
+                                                          bool rev, Int* v) {
+  using UInt = typename std::make_unsigned<Int>::type;
+  constexpr int S = 1 << (2 * D);
+  constexpr int IP = sizeof(Int) * 8;
+  constexpr int pbits = sizeof(Int) == 4 ? 5 : 6;
+  int bits = 0;
+  uint32_t prec = (uint32_t)maxprec;
+  if (rev) {
+    prec = (uint32_t)r.get(pbits) + 1;
+    bits += pbits;
+  }
+  PlaneSet<S, IP> p;
+  bits += (int)lane_decode_ints<S, IP>(r, maxbits - bits > 0 ? (uint32_t)(maxbits - bits) : 0u, prec, p);
+  if (bits < minbits) bits = minbits;
+  UInt u[S];
+  planes_to_coeffs<S, UInt>(p, u);
+  ranks_to_block<D, Int, UInt>(u, v, std::make_integer_sequence<int, S>{});
+  if (rev) lane_xform<D, Int, false, true>(v);
+  else lane_xform<D, Int, false, false>(v);
+  return (uint32_t)bits;
+}
+
+template <int D, class Int>
+__device__ __forceinline__ uint32_t lane_encode_int_block(LaneWriter& w, int minbits, int maxbits, int maxprec,
+                                                          bool rev, Int* v) {
+  using UInt = typename std::make_unsigned<Int>::type;
+  constexpr int S = 1 << (2 * D);
+  constexpr int IP = sizeof(Int) * 8;
+  constexpr int pbits = sizeof(Int) == 4 ? 5 : 6;
+  if (rev) lane_xform<D, Int, true, true>(v);
+  else lane_xform<D, Int, true, false>(v);
+  UInt u[S];
+  block_to_ranks<D, Int, UInt>(v, u, std::make_integer_sequence<int, S>{});
+  int bits = 0;
+  uint32_t prec = (uint32_t)maxprec;
+  if (rev) {
+    // precision: the planes down to the lowest nonzero bit of any coefficient
+    UInt m = 0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) m |= u[i];
+    prec = m ? IP - (uint32_t)__builtin_ctzll((unsigned long long)m) : 0;
+    if (prec > (uint32_t)maxprec) prec = (uint32_t)maxprec;
+    if (prec < 1) prec = 1;
+    w.put(prec - 1, pbits);
+    bits = pbits;
+  }
+  PlaneSet<S, IP> p;
+  coeffs_to_planes<S, UInt>(u, p);
+  bits += (int)lane_encode_ints<S, IP>(w, maxbits - bits > 0 ? (uint32_t)(maxbits - bits) : 0u, prec, p);
+  if (bits < minbits) {
+    w.zeros((uint64_t)(minbits - bits));
+    bits = minbits;
+  }
+  return (uint32_t)bits;
+}
+
+// ---------------------------------------------------------------------------
+// floating-point blocks
+// ---------------------------------------------------------------------------
+// x = m 2^e, 0.5 <= m < 1, of the block's largest |value| (NaN ignored,
+// infinity -> 0)
+template <int S, class F> __device__ __forceinline__ int lane_exponent(const F* f) {
+  using U = typename ZT<F>::UInt;
+  constexpr int bias = ZT<F>::ebias, mb = ZT<F>::mbits;
+  constexpr U absmask = ~(U)0 >> 1, inf = (U)(2 * bias + 1) << mb;
+  U mx = 0;
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    U a = fbits(f[j]) & absmask;
+    if (a > inf) a = 0;
+    mx = a > mx ? a : mx;
+  }
+  if (mx == 0) return -bias;
+  const int be = (int)(mx >> mb);
+  if (be == 0) return 1 - bias;
+  if (be == 2 * bias + 1) return 0;
+  return be - (bias - 1);
+}
+
+template <int D, class F> __device__ __forceinline__ uint32_t lane_decode_float_block(LaneReader& r,
+                                                                                   const ZfpChunk& c, F* f) {
+  using Int = typename ZT<F>::Int;
+  using U = typename ZT<F>::UInt;
+  constexpr int S = 1 << (2 * D);
+  constexpr int ebits = ZT<F>::ebits, ebias = ZT<F>::ebias;
+  Int v[S];
+  if (c.minexp < kZfpMinExp) {  // reversible
+    uint32_t bits = 1;
+    if (r.bit()) {
+      bits++;
+      if (r.bit()) {  // the samples' bits, coded losslessly
+        bits += lane_decode_int_block<D, Int>(r, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits,
+                                              (int)c.maxprec, true, v);
+        const U mx = ~(U)0 >> 1;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+          const Int x = v[j] < 0 ? (Int)((U)v[j] ^ mx) : v[j];
+          f[j] = fromb<F>((U)x);
+        }
+      } else {  // an exact block-floating-point cast
+        bits += ebits;
+        const int emax = (int)r.get(ebits) - ebias;
+        bits += lane_decode_int_block<D, Int>(r, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits,
+                                              (int)c.maxprec, true, v);
+        const F s = pow2<F>(emax - ((int)sizeof(F) * 8 - 2));
+#pragma unroll
+        for (int j = 0; j < S; ++j) f[j] = s * (F)v[j];
+      }
+      return bits;
+    }
+#pragma unroll
+    for (int j = 0; j < S; ++j) f[j] = 0;
+    return bits;
+  }
+  if (r.bit()) {
+    uint32_t bits = 1 + ebits;
+    const int emax = (int)r.get(ebits) - ebias;
+    const int dp = emax - c.minexp + 2 * (D + 1);
+    const int maxprec = (int)c.maxprec < (dp > 0 ? dp : 0) ? (int)c.maxprec : (dp > 0 ? dp : 0);
+    bits += lane_decode_int_block<D, Int>(r, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits, maxprec, false,
+                                          v);
+    const F s = pow2<F>(emax - ((int)sizeof(F) * 8 - 2));
+#pragma unroll
+    for (int j = 0; j < S; ++j) f[j] = s * (F)v[j];
+    return bits;
+  }
+#pragma unroll
+  for (int j = 0; j < S; ++j) f[j] = 0;
+  return c.minbits > 1 ? c.minbits : 1;
+}
+
+template <int D, class F> __device__ __forceinline__ uint32_t lane_encode_float_block(LaneWriter& w,
+                                                                                   const ZfpChunk& c,
+                                                                                   const F* f) {
+  using Int = typename ZT<F>::Int;
+  using U = typename ZT<F>::UInt;
+  constexpr int S = 1 << (2 * D);
+  constexpr int ebits = ZT<F>::ebits, ebias = ZT<F>::ebias;
+  Int v[S];
+  const int emax = lane_exponent<S, F>(f);
+  if (c.minexp < kZfpMinExp) {  // reversible
+    uint32_t bits = 0;
+    bool same = true;
+    if (emax != -ebias) {
+      const F s = pow2<F>((int)sizeof(F) * 8 - 2 - emax), si = pow2<F>(emax - ((int)sizeof(F) * 8 - 2));
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        v[j] = cast_int<F>(s * f[j]);
+        same &= fbits(si * (F)v[j]) == fbits(f[j]);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        v[j] = 0;
+        same &= fbits(f[j]) == 0;
+      }
+    }
+    if (same) {
+      const uint32_t e = (uint32_t)(emax + ebias);
+      if (!e) {
+        w.put(0, 1);
+        return 1;
+      }
+      w.put(1, 2);
+      w.put(e, ebits);
+      bits = 2 + ebits;
+    } else {
+      const U mx = ~(U)0 >> 1;
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const U b = fbits(f[j]);
+        v[j] = (Int)b < 0 ? (Int)(b ^ mx) : (Int)b;
+      }
+      w.put(3, 2);
+      bits = 2;
+    }
+    return bits + lane_encode_int_block<D, Int>(w, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits,
+                                                 (int)c.maxprec, true, v);
+  }
+  const int dp = emax - c.minexp + 2 * (D + 1);
+  const int maxprec = (int)c.maxprec < (dp > 0 ? dp : 0) ? (int)c.maxprec : (dp > 0 ? dp : 0);
+  const uint32_t e = maxprec ? (uint32_t)(emax + ebias) : 0;
+  if (e) {
+    const uint32_t bits = 1 + ebits;
+    w.put(2 * (uint64_t)e + 1, bits);
+    const F s = pow2<F>((int)sizeof(F) * 8 - 2 - emax);
+#pragma unroll
+    for (int j = 0; j < S; ++j) v[j] = cast_int<F>(s * f[j]);
+    return bits + lane_encode_int_block<D, Int>(w, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits,
+                                                 maxprec, false, v);
+  }
+  w.put(0, 1);
+  if (c.minbits > 1) {
+    w.zeros(c.minbits - 1);
+    return c.minbits;
+  }
+  return 1;
+}
+
+template <int D, class S> __device__ __forceinline__ uint32_t lane_decode_block(LaneReader& r, const ZfpChunk& c,
+                                                                               S* vals) {
+  if constexpr (ZT<S>::fp) {
+    return lane_decode_float_block<D, S>(r, c, vals);
+  } else {
+    return lane_decode_int_block<D, S>(r, (int)c.minbits, (int)c.maxbits, (int)c.maxprec, c.minexp < kZfpMinExp,
+                                       vals);
+  }
+}
+
+template <int D, class S> __device__ __forceinline__ uint32_t lane_encode_block(LaneWriter& w, const ZfpChunk& c,
+                                                                               S* vals) {
+  if constexpr (ZT<S>::fp) {
+    return lane_encode_float_block<D, S>(w, c, vals);
+  } else {
+    return lane_encode_int_block<D, S>(w, (int)c.minbits, (int)c.maxbits, (int)c.maxprec, c.minexp < kZfpMinExp,
+                                       vals);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// gather / scatter of one lane's block (x fastest; rows of four samples)
+// ---------------------------------------------------------------------------
+template <class S> __device__ __forceinline__ bool vector_rows(const ZfpChunk& c, const uint8_t* p) {
+  // whole 4-sample rows of a native 4- or 8-byte type start 16-byte aligned
+  const bool native = !std::is_same<S, int32_t>::value || c.dtype == NDIC_I32;
+  return native && (c.n[0] & 3) == 0 && ((uintptr_t)p & 15) == 0;
+}
+
+template <class S> __device__ __forceinline__ void load_row4(const S* p, S* o) {
+  if constexpr (sizeof(S) == 4) {
+    typedef __attribute__((ext_vector_type(4))) uint32_t u4;
+    const u4 t = *(const u4*)p;
+    uint32_t a[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) __builtin_memcpy(&o[i], &a[i], 4);
+  } else {
+    typedef __attribute__((ext_vector_type(2))) uint64_t u2;
+    const u2 t0 = *(const u2*)p, t1 = *(const u2*)(p + 2);
+    uint64_t a[4] = {t0.x, t0.y, t1.x, t1.y};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) __builtin_memcpy(&o[i], &a[i], 8);
+  }
+}
+
+template <class S> __device__ __forceinline__ void store_row4(S* p, const S* v) {
+  if constexpr (sizeof(S) == 4) {
+    typedef __attribute__((ext_vector_type(4))) uint32_t u4;
+    uint32_t a[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) __builtin_memcpy(&a[i], &v[i], 4);
+    *(u4*)p = u4{a[0], a[1], a[2], a[3]};
+  } else {
+    typedef __attribute__((ext_vector_type(2))) uint64_t u2;
+    uint64_t a[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) __builtin_memcpy(&a[i], &v[i], 8);
+    *(u2*)p = u2{a[0], a[1]};
+    *(u2*)(p + 2) = u2{a[2], a[3]};
+  }
+}
+
+template <int D> __device__ __forceinline__ void block_extents(const ZfpChunk& c, const uint32_t* b, uint32_t* ext) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const uint32_t e = a < D ? c.n[a] - 4 * b[a] : 1;
+    ext[a] = e < 4 ? e : 4;
+  }
+}
+
+template <int D, class S> __device__ __forceinline__ void lane_gather(const ZfpChunk& c, const uint32_t* b, S* vals) {
+  uint32_t ext[3];
+  block_extents<D>(c, b, ext);
+  const bool vec = vector_rows<S>(c, c.src) && ext[0] == 4;
+#pragma unroll
+  for (int zz = 0; zz < (D > 2 ? 4 : 1); ++zz)
+#pragma unroll
+    for (int yy = 0; yy < (D > 1 ? 4 : 1); ++yy) {
+      const uint64_t sz = D > 2 ? 4ull * b[2] + pad_src(zz, ext[2]) : 0;
+      const uint64_t sy = D > 1 ? 4ull * b[1] + pad_src(yy, ext[1]) : 0;
+      const uint64_t row = (sz * c.n[1] + sy) * c.n[0] + 4ull * b[0];
+      S* o = vals + 4 * yy + 16 * zz;
+      if (vec) {
+        load_row4<S>((const S*)c.src + row, o);
+      } else {
+#pragma unroll
+        for (int xx = 0; xx < 4; ++xx) o[xx] = load_elem<S>(c.src, row + pad_src(xx, ext[0]), c.dtype);
+      }
+    }
+}
+
+template <int D, class S>
+__device__ __forceinline__ void lane_scatter(const ZfpChunk& c, const uint32_t* b, const S* vals, uint8_t* brick) {
+  uint32_t ext[3];
+  block_extents<D>(c, b, ext);
+  uint8_t* dst = brick ? brick : c.dst;
+  const bool vec = !brick && vector_rows<S>(c, dst) && ext[0] == 4;
+#pragma unroll
+  for (int zz = 0; zz < (D > 2 ? 4 : 1); ++zz)
+#pragma unroll
+    for (int yy = 0; yy < (D > 1 ? 4 : 1); ++yy) {
+      if ((uint32_t)zz >= ext[2] || (uint32_t)yy >= ext[1]) continue;
+      const S* v = vals + 4 * yy + 16 * zz;
+      uint64_t row;
+      if (brick) row = ((uint64_t)zz * ext[1] + yy) * ext[0];
+      else row = ((4ull * b[2] + zz) * c.n[1] + 4ull * b[1] + yy) * c.n[0] + 4ull * b[0];
+      if (vec) {
+        store_row4<S>((S*)dst + row, v);
+      } else {
+#pragma unroll
+        for (int xx = 0; xx < 4; ++xx)
+          if ((uint32_t)xx < ext[0]) store_elem<S>(dst, row + xx, c.dtype, v[xx]);
+      }
+    }
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const uint64_t t = __shfl_xor(v, o);
+    v = t < v ? t : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const uint64_t t = __shfl_xor(v, o);
+    v = t > v ? t : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t lane_block_id() {
+  return (blockIdx.x + (uint64_t)blockIdx.y * gridDim.x) * kLaneThreads + threadIdx.x;
+}
+
+// ---------------------------------------------------------------------------
+// kernels: one wave per workgroup, one block per lane
+// ---------------------------------------------------------------------------
+template <int D, class S>
+__global__ __launch_bounds__(kLaneThreads) void zfp_lane_encode_kernel(const ZfpChunk* __restrict__ chunks,
+                                                                      uint32_t nchunks, uint64_t total) {
+  constexpr int SZ = 1 << (2 * D);
+  const uint64_t g = lane_block_id();
+  if (g >= total) return;
+  const ZfpChunk& c = chunks[find_chunk(chunks, nchunks, g)];
+  if (c.fixed ? !c.dst : !c.slots) return;  // refused chunk
+  const uint64_t k = g - c.block_begin;
+  uint32_t b[4];
+  block_coords(c, k, b);
+  S vals[SZ];
+  lane_gather<D, S>(c, b, vals);
+  LaneWriter w;
+  if (c.fixed) w.init((uint64_t*)c.dst, c.header_bits + k * c.maxbits, true);
+  else w.init(c.slots + g * c.slot_words, 0, false);
+  lane_encode_block<D, S>(w, c, vals);
+  w.finish();
+  if (!c.fixed) c.offs[g] = w.count;
+}
+
+template <int D, class S>
+__global__ __launch_bounds__(kLaneThreads) void zfp_lane_decode_kernel(const ZfpChunk* __restrict__ chunks,
+                                                                      uint32_t nchunks, uint64_t total,
+                                                                      const ZfpBrick* __restrict__ bricks,
+                                                                      uint32_t stage_words) {
+  constexpr int SZ = 1 << (2 * D);
+  const uint64_t g = lane_block_id();
+  const bool valid = g < total;
+  uint32_t ci = 0;
+  uint64_t k = 0;
+  uint8_t* brick = nullptr;
+  if (valid) {
+    if (bricks) {
+      ci = bricks[g].chunk;
+      k = bricks[g].block;
+      brick = bricks[g].out;
+    } else {
+      ci = find_chunk(chunks, nchunks, g);
+      k = g - chunks[ci].block_begin;
+    }
+  }
+  const ZfpChunk& c = chunks[ci];
+  const bool indexed = !c.fixed && c.index;
+  // this block's bits: [start, end)
+  uint64_t start = 0, end = 0;
+  if (valid) {
+    if (c.fixed) {
+      start = c.header_bits + k * c.maxbits;
+      end = start + c.maxbits;
+    } else if (indexed) {
+      start = c.header_bits + c.index[k];
+      end = c.header_bits + c.index[k + 1];
+    } else {
+      start = c.offs[c.block_begin + k];
+      end = k + 1 < c.nblocks ? c.offs[c.block_begin + k + 1] : 8 * c.src_len;
+    }
+  }
+  // Stage the wave's stretch of the stream in LDS when its blocks are one
+  // stretch of one stream that fits (words lo .. hi + 2: the readers keep
+  // two words ahead).
+  const uint32_t ci0 = (uint32_t)__shfl((int)ci, 0);
+  const bool one = __ballot(valid && ci != ci0) == 0;
+  const uint64_t lo = wave_min_u64(valid ? start : ~0ull) >> 6;
+  const uint64_t hi = ((wave_max_u64(valid ? end : 0) + 63) >> 6) + 2;
+  const bool staged = one && hi > lo && hi - lo <= stage_words;
+  LaneReader r;
+  r.base = c.src;
+  r.nbytes = c.src_len;
+  r.s0 = 0;
+  r.sn = 0;
+  if (staged) {
+    const ZfpChunk& c0 = chunks[ci0];
+    for (uint64_t i = threadIdx.x; i < hi - lo; i += kLaneThreads) zstage[i] = global_word(c0.src, c0.src_len, lo + i);
+    r.s0 = lo;
+    r.sn = hi - lo;
+  }
+  __syncthreads();
+  if (!valid) return;
+  r.seek(start);
+  S vals[SZ];
+  const uint32_t used = lane_decode_block<D, S>(r, c, vals);
+  if (indexed) {
+    // the block must take exactly the bits the index gives it, the index
+    // must start at the first block and end inside the stream
+    const uint32_t ilo = c.index[k], ihi = c.index[k + 1];
+    if (ihi < ilo || ihi - ilo != used || (k == 0 && ilo != 0) ||
+        (k + 1 == c.nblocks && c.header_bits + (uint64_t)ihi > 8 * c.src_len))
+      atomicOr(c.bad, 1u);
+  }
+  uint32_t b[4];
+  block_coords(c, k, b);
+  lane_scatter<D, S>(c, b, vals, brick);
+}
+
+inline dim3 lane_grid(uint64_t total) {
+  const uint64_t wgs = ceil_div(total, kLaneThreads);
+  const uint32_t gx = (uint32_t)(wgs < 65536 ? wgs : 65536);
+  return dim3(gx, (uint32_t)ceil_div(wgs, gx));
+}
+
+}  // namespace
+
+// Launchers of one scalar type's lane coders (instantiated per type in
+// zfp_lane_<type>.hip, so that the four types compile in parallel).
+template <class S> void zfp_lane_encode_launch(uint32_t dims, const ZfpChunk* d, uint32_t n, uint64_t total,
+                                               hipStream_t s) {
+  switch (dims) {
+    case 1: hipLaunchKernelGGL((zfp_lane_encode_kernel<1, S>), lane_grid(total), dim3(kLaneThreads), 0, s, d, n, total); break;
+    case 2: hipLaunchKernelGGL((zfp_lane_encode_kernel<2, S>), lane_grid(total), dim3(kLaneThreads), 0, s, d, n, total); break;
+    case 3: hipLaunchKernelGGL((zfp_lane_encode_kernel<3, S>), lane_grid(total), dim3(kLaneThreads), 0, s, d, n, total); break;
+    default: break;
+  }
+}
+template <class S> void zfp_lane_decode_launch(uint32_t dims, const ZfpChunk* d, uint32_t n, uint64_t total,
+                                               const ZfpBrick* br, uint32_t words, hipStream_t s) {
+  const size_t lds = (size_t)words * 8;
+  switch (dims) {
+    case 1: hipLaunchKernelGGL((zfp_lane_decode_kernel<1, S>), lane_grid(total), dim3(kLaneThreads), lds, s, d, n, total, br, words); break;
+    case 2: hipLaunchKernelGGL((zfp_lane_decode_kernel<2, S>), lane_grid(total), dim3(kLaneThreads), lds, s, d, n, total, br, words); break;
+    case 3: hipLaunchKernelGGL((zfp_lane_decode_kernel<3, S>), lane_grid(total), dim3(kLaneThreads), lds, s, d, n, total, br, words); break;
+    default: break;
+  }
+}
+
+}  // namespace ndic

@@ -96,7 +96,7 @@ extern "C" ndic_status ndic_zfp_decode_bricks_noalloc(const uint8_t* d_in, uint8
       NDIC_HIP(hipMemcpyAsync(ch, bc.chunks.data(), sz_ch, hipMemcpyHostToDevice, s));
       NDIC_HIP(hipMemcpyAsync(br, bc.bricks.data(), sz_br, hipMemcpyHostToDevice, s));
       launch_zfp_decode((const ZfpChunk*)ch, (uint32_t)bc.chunks.size(), bc.bricks.size(), bc.dims, bc.scalar,
-                        (const ZfpBrick*)br, s);
+                        (const ZfpBrick*)br, zfp_fixed_maxbits(bc.chunks), s);
     }
     NDIC_HIP(hipStreamSynchronize(s));
   });
