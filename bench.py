@@ -237,6 +237,8 @@ def single_chunk_lines(ndic, dev, args):
              ("u16 8x1x32x512x512 series", np.ascontiguousarray(datagen.series_5d(seed=1).transpose(1, 0, 2, 3, 4)),
               [{"dimension": 1, "kind": "lift53", "levels": 2, "group": 0},
                {"dimension": 2, "kind": "haar", "levels": 2, "group": 0}])]
+    if args.single_reps <= 0:
+        return out, cases
     for name, vol, steps in cases:
         d_raw = dev.DeviceBuffer.from_bytes(vol.tobytes())
         cap = 2 * vol.nbytes + (16 << 20)

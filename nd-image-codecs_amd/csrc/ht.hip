@@ -1358,6 +1358,14 @@ static_assert((kMsRing & (kMsRing - 1)) == 0, "the ring index is a mask");
 // source is outside the row add 0), then the row totals carried across rows
 // (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2 and 3; rows the
 // mask leaves out add 0).
+// DPP lane moves (GFX9 encodings): source lanes outside the wave read 0.
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;  // lane l reads lane l + 1 / l - 1
+constexpr int kDppSwapPair = 0xB1, kDppEvenOfPair = 0xA0, kDppOddOfPair = 0xF5;  // quad_perm [1,0,3,2] [0,0,2,2] [1,1,3,3]
+template <int Ctrl> __device__ __forceinline__ uint32_t dpp_mov(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, Ctrl, 0xF, 0xF, true);
+}
+#define dpp(v, ctrl) dpp_mov<ctrl>(v)
+
 __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
   v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);
   v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);
@@ -1522,6 +1530,7 @@ __device__ void ht_magsgn_block(const HtDecTask* __restrict__ tasks, uint64_t i,
     }
     wave_sync();
   }
+  uint32_t* orow = out + lane;  // row y of this lane's column
   for (uint32_t y = 0; y < height; y += 2) {
     const bool initial = y == 0;
     // ---- MagSgn of rows y, y+1, lane = column (decode.rs:302-396); the
@@ -1532,11 +1541,14 @@ __device__ void ht_magsgn_block(const HtDecTask* __restrict__ tasks, uint64_t i,
     if (!initial) {
       uint32_t gamma = tq & 0xF0;
       gamma &= gamma - 0x10;
-      // columns 2q-1 .. 2q+2 of the row above (q = lane / 2); past either edge 0
-      const int q2 = (int)(lane & ~1u);
-      const uint32_t b0 = __shfl(blprev, q2 - 1), b1 = __shfl(blprev, q2), b2 = __shfl(blprev, q2 + 1),
-                     b3 = __shfl(blprev, q2 + 2);
-      const uint32_t e4 = max(max(q2 >= 1 ? b0 : 0u, b1), max(b2, q2 + 2 < 64 ? b3 : 0u));
+      // columns 2q-1 .. 2q+2 of the row above (q = lane / 2); past either
+      // edge 0: the pair's own two columns, the column left of the pair
+      // (wave shift right by one, from the pair's even lane) and the one
+      // right of it (wave shift left by one, from the pair's odd lane)
+      const uint32_t pair = max(blprev, dpp(blprev, kDppSwapPair));
+      const uint32_t left = dpp(dpp(blprev, kDppWaveShr1), kDppEvenOfPair);
+      const uint32_t right = dpp(dpp(blprev, kDppWaveShl1), kDppOddOfPair);
+      const uint32_t e4 = max(pair, max(left, right));
       U += gamma != 0 ? max(e4, 2u) - 1 : 1;
     }
     if (col && !(lane & 1) && U > mmsbp2) bad = true;
@@ -1572,10 +1584,11 @@ __device__ void ht_magsgn_block(const HtDecTask* __restrict__ tasks, uint64_t i,
       }
       if (!sig[r]) vv = o = 0;
       if (r == 1) blnew = bitlen(vv);
-      if (col && y + r < height) st32(out + (uint64_t)(y + r) * stride + lane, o);
+      if (col && y + r < height) st32(r ? orow + stride : orow, o);
     }
     msbase += total;
     blprev = blnew;
+    orow += 2 * stride;
   }
   // a magnitude bound violated mid-block: the whole block reads as zero
   if (__ballot(bad)) {
