@@ -1385,12 +1385,22 @@ struct MsRing {
   uint32_t prev;   // last byte of the previous chunk (0 before the first)
   uint32_t nvalid; // words that hold stream bits, once all are in (else ~0)
   uint32_t ones;   // once all are in: words nvalid .. ones - 1 are set to all ones
-  uint32_t pf[16]; // this lane's bytes of the next chunk
+  uint32_t q[4];   // this lane's 16 bytes of the next chunk (byte j: bits 8 (j % 4) of q[j / 4])
 
   __device__ void load_next() {
     const uint32_t k0 = next + 16 * (threadIdx.x & 63);
+    if (k0 + 16 <= n) {
+      const u32x4 v = ld128(seg + k0);  // a 16-byte read need not be aligned
+      q[0] = v.x;
+      q[1] = v.y;
+      q[2] = v.z;
+      q[3] = v.w;
+    } else {  // the stream's last bytes; past it zeros
+      q[0] = q[1] = q[2] = q[3] = 0;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) pf[j] = k0 + j < n ? ld8(seg + k0 + j) : 0u;
+      for (uint32_t j = 0; j < 16; ++j)
+        if (k0 + j < n) q[j >> 2] |= ld8(seg + k0 + j) << (8 * (j & 3));
+    }
   }
   __device__ void init(uint32_t* r, const uint8_t* s, uint32_t nbytes) {
     ring = r;
@@ -1403,38 +1413,62 @@ struct MsRing {
     ones = 0;
     load_next();
   }
+  // Unstuffs the next 1024 bytes into the ring.  A byte after 0xFF brings 7
+  // bits, any other 8, and each byte is OR-ed in whole at its bit position
+  // (a stuffed byte's spare top bit lands on the next byte's first bit, as
+  // the serial reader has it).  A lane packs its 16 bytes in four registers
+  // -- their plain concatenation, with the bytes after each 7-bit byte moved
+  // down one bit -- and ORs them into the ring at its offset (a wave prefix
+  // sum of the lanes' bit counts).
   __device__ void refill() {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t k0 = next + 16 * lane;
-    uint32_t prev_l = __shfl(pf[15], (int)lane - 1);
+    uint32_t prev_l = __shfl(q[3] >> 24, (int)lane - 1);
     if (lane == 0) prev_l = prev;
-    uint32_t w[16], sum = 0;
-    uint32_t pv = prev_l;
+    const uint32_t nin = k0 >= n ? 0u : (n - k0 < 16 ? n - k0 : 16u);
+    // bit j: byte j is 0xFF (no carries cross bytes: low seven bits + 1 <= 0x80)
+    uint32_t ff = 0;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      w[j] = k0 + j < n ? 8 - uint32_t(pv == 0xFF) : 0u;
-      sum += w[j];
-      pv = pf[j];
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t t = ((q[k] & 0x7F7F7F7Fu) + 0x01010101u) & q[k] & 0x80808080u;
+      ff |= ((t >> 7) & 1) << (4 * k) | ((t >> 15) & 1) << (4 * k + 1) | ((t >> 23) & 1) << (4 * k + 2) |
+            ((t >> 31) & 1) << (4 * k + 3);
+    }
+    // bit j: byte j brings 7 bits (the byte before it is 0xFF)
+    const uint32_t inmask = nin >= 16 ? 0xFFFFu : (1u << nin) - 1;
+    uint32_t seven = ((ff << 1) | uint32_t(prev_l == 0xFF)) & inmask;
+    const uint32_t sum = 8 * nin - (uint32_t)__builtin_popcount(seven);
+    uint32_t v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3];
+    while (seven) {  // highest first: bytes above byte i move down one bit
+      const uint32_t i = 31 - (uint32_t)__builtin_clz(seven);
+      seven &= ~(1u << i);
+      const uint32_t bb = 8 * (i + 1);  // bits kept in place
+      const uint32_t m0 = bb >= 32 ? ~0u : (1u << bb) - 1;
+      const uint32_t m1 = bb >= 64 ? ~0u : bb <= 32 ? 0u : (1u << (bb - 32)) - 1;
+      const uint32_t m2 = bb >= 96 ? ~0u : bb <= 64 ? 0u : (1u << (bb - 64)) - 1;
+      const uint32_t m3 = bb >= 128 ? ~0u : bb <= 96 ? 0u : (1u << (bb - 96)) - 1;
+      const uint32_t h0 = v0 & ~m0, h1 = v1 & ~m1, h2 = v2 & ~m2, h3 = v3 & ~m3;
+      v0 = (v0 & m0) | __builtin_amdgcn_alignbit(h1, h0, 1);
+      v1 = (v1 & m1) | __builtin_amdgcn_alignbit(h2, h1, 1);
+      v2 = (v2 & m2) | __builtin_amdgcn_alignbit(h3, h2, 1);
+      v3 = (v3 & m3) | (h3 >> 1);
     }
     // exclusive prefix of the lane sums (<= 128)
     const uint32_t incl = wave_scan_incl(sum);
     const uint32_t excl = incl - sum, total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     // clear the words this chunk reaches past the partly filled one (their
-    // previous contents are long consumed), then OR the bytes in
+    // previous contents are long consumed), then OR the lanes' bits in
     const uint32_t w0 = (end >> 5) + 1, w1 = ((end + total) >> 5) + 1;
     for (uint32_t k = w0 + lane; k <= w1; k += 64) ring[k % kMsRing] = 0;
     wave_sync();
-    uint32_t pos = end + excl;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      if (w[j]) {
-        const uint32_t sh = pos & 31, wi = pos >> 5;
-        atomicOr(&ring[wi % kMsRing], pf[j] << sh);
-        if (sh > 24) atomicOr(&ring[(wi + 1) % kMsRing], pf[j] >> (32 - sh));
-      }
-      pos += w[j];
-    }
-    prev = __shfl(pf[15], 63);
+    const uint32_t pos = end + excl, sh = pos & 31, wi = pos >> 5;
+    const uint32_t r = 32 - sh;  // sh == 0: each word lands whole
+    atomicOr(&ring[wi % kMsRing], v0 << sh);
+    atomicOr(&ring[(wi + 1) % kMsRing], sh ? __builtin_amdgcn_alignbit(v1, v0, r) : v1);
+    atomicOr(&ring[(wi + 2) % kMsRing], sh ? __builtin_amdgcn_alignbit(v2, v1, r) : v2);
+    atomicOr(&ring[(wi + 3) % kMsRing], sh ? __builtin_amdgcn_alignbit(v3, v2, r) : v3);
+    atomicOr(&ring[(wi + 4) % kMsRing], sh ? v3 >> r : 0u);
+    prev = __shfl(q[3] >> 24, 63);
     if (next + 1024 > n) prev = 0;  // unused: the stream ends in this chunk
     end += total;
     next += 1024;
