@@ -204,8 +204,7 @@ __device__ __forceinline__ uint32_t find_chunk(const ZfpChunk* chunks, uint32_t 
 }
 
 // Coefficient orders: by total sequency i+j+k+l, then by sum of squares.
-// The 1D-3D orders (rank -> index), written out per (i, j, k); the 4D one is
-// in zfp.hip.
+// The 1D-3D orders (rank -> index), written out per (i, j, k).
 #define P2(i, j) ((i) + 4 * (j))
 #define P3(i, j, k) ((i) + 4 * (j) + 16 * (k))
 constexpr uint8_t kPerm2[16] = {P2(0, 0), P2(1, 0), P2(0, 1), P2(1, 1), P2(2, 0), P2(0, 2), P2(2, 1), P2(1, 2),
@@ -221,6 +220,21 @@ constexpr uint8_t kPerm3[64] = {
     P3(2, 2, 3), P3(1, 3, 3), P3(3, 1, 3), P3(3, 3, 1), P3(2, 3, 3), P3(3, 2, 3), P3(3, 3, 2), P3(3, 3, 3)};
 #undef P2
 #undef P3
+// The 4D coefficient order (rank -> index): the order the LLNL C library 0.5.5
+// emits, ranks measured from its streams.
+constexpr uint8_t kPerm4[256] = {
+    0,   1,   4,   16,  64,  5,   80,  17,  68,  65,  20,  2,   8,   32,  128, 84,  81,  69,  21,  6,   18,  66,
+    24,  72,  9,   96,  33,  36,  129, 132, 144, 3,   12,  48,  192, 85,  82,  70,  22,  73,  25,  88,  37,  100,
+    97,  148, 145, 133, 10,  160, 34,  136, 130, 40,  7,   19,  67,  28,  76,  13,  112, 49,  52,  193, 196, 208,
+    86,  89,  101, 149, 161, 137, 41,  134, 38,  164, 26,  152, 146, 104, 98,  74,  83,  71,  23,  77,  29,  92,
+    53,  116, 113, 212, 209, 197, 11,  35,  131, 44,  140, 14,  176, 50,  56,  194, 200, 224, 90,  165, 102, 153,
+    150, 105, 168, 162, 138, 42,  87,  93,  117, 213, 27,  75,  99,  39,  135, 147, 108, 45,  141, 156, 30,  78,
+    177, 180, 54,  114, 120, 57,  198, 210, 216, 201, 225, 228, 15,  240, 51,  204, 195, 60,  169, 166, 154, 106,
+    91,  103, 151, 109, 157, 94,  181, 118, 121, 214, 217, 229, 163, 139, 43,  142, 46,  172, 58,  184, 178, 232,
+    226, 202, 241, 205, 61,  199, 55,  244, 31,  220, 211, 124, 115, 79,  170, 167, 155, 107, 158, 110, 173, 122,
+    185, 182, 233, 230, 218, 95,  245, 119, 221, 215, 125, 242, 206, 62,  203, 59,  248, 47,  236, 227, 188, 179,
+    143, 171, 174, 186, 234, 246, 222, 126, 219, 123, 249, 111, 237, 231, 189, 183, 159, 252, 243, 207, 63,  175,
+    250, 187, 238, 235, 190, 253, 247, 223, 127, 254, 251, 239, 191, 255};
 
 }  // namespace zfpdev
 }  // namespace ndic

@@ -21,7 +21,7 @@ uint32_t stage_words_for(uint32_t fixed_maxbits) {
 
 bool launch_zfp_lane_encode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, uint32_t dims,
                             uint32_t scalar, hipStream_t stream) {
-  if (dims < 1 || dims > 3) return false;
+  if (dims < 1 || dims > 4) return false;
   if (total == 0) return true;
   switch (scalar) {
     case kZfpInt32: zfp_lane_encode_launch<int32_t>(dims, d_chunks, nchunks, total, stream); break;
@@ -36,7 +36,7 @@ bool launch_zfp_lane_encode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t
 
 bool launch_zfp_lane_decode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, uint32_t dims,
                             uint32_t scalar, const ZfpBrick* d_bricks, uint32_t fixed_maxbits, hipStream_t stream) {
-  if (dims < 1 || dims > 3) return false;
+  if (dims < 1 || dims > 4) return false;
   if (total == 0) return true;
   const uint32_t words = stage_words_for(fixed_maxbits);
   switch (scalar) {

@@ -1,5 +1,5 @@
 """The ZFP block coders on the GPU at sizes where many waves run at once:
-one lane per block (1D-3D fields) and one wavefront per block (4D fields).
+one lane per block (1D-3D fields) and four lanes per block (4D fields).
 
 Streams must equal the oracle's byte for byte and decode to the oracle's
 samples: every dtype and mode on shapes with partial blocks, fields large
@@ -57,7 +57,7 @@ def check(ndic, oracle, a, dtype, cfg, encodes=1):
 @pytest.mark.parametrize("dtype", list(NP))
 def test_small_fields_match_oracle(ndic, oracle, dtype):
     rng = np.random.default_rng(sum(map(ord, dtype)))
-    for shape in ([6], [70], [3, 5], [9, 21], [2, 3, 5], [7, 9, 19]):
+    for shape in ([6], [70], [3, 5], [9, 21], [2, 3, 5], [7, 9, 19], [2, 3, 5, 6], [5, 2, 9, 7]):
         for kind in ("smooth", "noise"):
             a = field(rng, shape, dtype, kind)
             for cfg in MODES:
