@@ -377,7 +377,6 @@ uint32_t class_of(const Plan& p) { return p.dims * 8 + p.scalar; }
 void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out, uint64_t capacity, hipStream_t s,
                       uint8_t* d_index, uint64_t index_capacity) {
   ctx.release_kept();  // earlier calls on this context ended synchronised
-  zfp_init_tables();
   for (auto& it : items) {
     if (it.status != NDIC_OK) continue;
     try {
@@ -526,7 +525,6 @@ void zfp_encode_items(Context& ctx, std::vector<ZfpItem>& items, uint8_t* d_out,
 
 void zfp_decode_items(Context& ctx, std::vector<ZfpItem>& items, hipStream_t s) {
   ctx.release_kept();  // earlier calls on this context ended synchronised
-  zfp_init_tables();
   // header bytes of every stream, for the host-side checks
   std::vector<uint8_t> heads(items.size() * kZfpHeadBytes, 0);
   {
@@ -752,8 +750,7 @@ extern "C" ndic_status ndic_zfp_decode_brick(const uint8_t* stream, size_t len, 
     }
     Context& ctx = Context::get();
     std::lock_guard<std::recursive_mutex> lock(ctx.mu);
-    zfp_init_tables();
-    hipStream_t s = ctx.stream;
+      hipStream_t s = ctx.stream;
     uint64_t nout = cnt * dtype_size(dtype);
     uint64_t sz_in = al256(len + 8), sz_out = al256(nout), sz_ch = al256(sizeof(ZfpChunk));
     uint8_t* m = (uint8_t*)ctx.in.ensure(sz_in + sz_out + sz_ch + sizeof(ZfpBrick));
@@ -978,8 +975,7 @@ extern "C" ndic_status ndic_zfp_decode_bricks(const uint8_t* d_in, uint8_t* d_ou
     if (n && !descs) fail(NDIC_INVALID_ARGUMENT, "null descriptors");
     Context& ctx = Context::get();
     std::lock_guard<std::recursive_mutex> lock(ctx.mu);
-    zfp_init_tables();
-    ctx.release_kept();
+      ctx.release_kept();
     hipStream_t s = stream ? (hipStream_t)stream : ctx.stream;
     // header bytes of every stream for the checks of BrickReader::fixed_rate
     std::vector<ZfpHead> hv(n);

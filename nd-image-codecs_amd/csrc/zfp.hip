@@ -1,24 +1,12 @@
-// ZFP block codec on the device for 4D fields, one wavefront per 4^4 block,
-// and the serial walk of variable-size streams without a block index (every
-// dimensionality).
+// ZFP streams on the device, the parts around the block coders
+// (zfp_lane_kernels.hpp): the serial walk of a variable-size stream without a
+// block index (where each block begins), the placement of variable-size
+// blocks from their scratch slots into the stream, the block index written
+// beside a stream, stream headers and the batch's bookkeeping kernels.
 //
-// Layout of a block across the wave: lane l holds coefficient l (x = l & 3,
-// y = (l >> 2) & 3, z = l >> 4) for d <= 3; for d = 4 each lane holds the four
-// w-slices of its (x, y, z) position in registers v[0..3].  The separable
-// decorrelating transform runs along x, y, z with lane shuffles (each lane
-// rebuilds its 4-line and keeps its own output) and along w in registers.
-// After the negabinary reorder, lane r holds the coefficient of rank r
-// (ranks 64q + l in register q for d = 4); one ballot then yields a whole
-// bit plane, so the embedded group-test coder runs wave-uniform on 64-bit
-// plane words (ctz finds the next significant coefficient instead of a
-// per-position loop) and every lane deposits its own bit when decoding.
-//
-// The algorithm and the stream layout are the published ZFP ones (Lindstrom,
-// IEEE TVCG 2014; LLNL ZFP stream format, codec 5), which the reference
-// reaches through zfp-rs 0.1.1 (crates/ndic-zfp/src/lib.rs:249-376): block
-// order x fastest, partial blocks padded by the pad rule, block-floating-
-// point cast at 2^(p-2-emax), integer lifting, negabinary, embedded coder,
-// reversible variant (Lorenzo lifting, precision prefix, float reinterpret).
+// The stream layout is the published ZFP one (Lindstrom, IEEE TVCG 2014;
+// LLNL ZFP stream format, codec 5), which the reference reaches through
+// zfp-rs 0.1.1 (crates/ndic-zfp/src/lib.rs:249-376).
 #include "zfp.hpp"
 #include "zfp_dev.hpp"
 
@@ -32,689 +20,7 @@ namespace {
 
 using namespace zfpdev;
 
-// coefficient order per dimensionality (rank -> index) and its inverse
-__constant__ uint8_t c_perm[4][256];
-__constant__ uint8_t c_iperm[4][256];
-
 __device__ __forceinline__ uint32_t lane() { return __lane_id(); }
-
-template <class Int, bool Fwd, bool Rev> __device__ __forceinline__ Int lift_lanes(Int v, uint32_t axis) {
-  uint32_t l = lane(), sh = 2 * axis, s = 1u << sh;
-  uint32_t pos = (l >> sh) & 3, base = l - pos * s;
-  Int x = __shfl(v, (int)base), y = __shfl(v, (int)(base + s)), z = __shfl(v, (int)(base + 2 * s)),
-      w = __shfl(v, (int)(base + 3 * s));
-  lift4<Int, Fwd, Rev>(x, y, z, w);
-  return pos == 0 ? x : pos == 1 ? y : pos == 2 ? z : w;
-}
-
-template <int D, class Int, bool Fwd, bool Rev> __device__ __forceinline__ void xform(Int* v) {
-  constexpr int R = D == 4 ? 4 : 1;
-  if (Fwd) {
-#pragma unroll
-    for (int a = 0; a < (D < 3 ? D : 3); ++a)
-#pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = lift_lanes<Int, Fwd, Rev>(v[j], a);
-    if (D == 4) lift4<Int, Fwd, Rev>(v[0], v[1], v[2], v[R - 1]);
-  } else {
-    if (D == 4) lift4<Int, Fwd, Rev>(v[0], v[1], v[2], v[R - 1]);
-#pragma unroll
-    for (int a = (D < 3 ? D : 3) - 1; a >= 0; --a)
-#pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = lift_lanes<Int, Fwd, Rev>(v[j], a);
-  }
-}
-
-// coefficients -> ranked negabinary (lane r, register q holds rank 64q + r)
-template <int D, class Int, class UInt> __device__ __forceinline__ void to_ranks(const Int* v, UInt* u) {
-  constexpr uint32_t S = 1u << (2 * D);
-  const UInt m = nbmask<UInt>();
-  uint32_t l = lane();
-  if constexpr (D < 4) {
-    uint32_t c = l < S ? c_perm[D - 1][l] : 0;
-    Int t = __shfl(v[0], (int)c);
-    u[0] = l < S ? (((UInt)t + m) ^ m) : 0;
-  } else {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      uint32_t c = c_perm[3][l + 64 * q], sl = c & 63, sr = c >> 6;
-      Int t0 = __shfl(v[0], (int)sl), t1 = __shfl(v[1], (int)sl), t2 = __shfl(v[2], (int)sl),
-          t3 = __shfl(v[3], (int)sl);
-      Int t = sr == 0 ? t0 : sr == 1 ? t1 : sr == 2 ? t2 : t3;
-      u[q] = ((UInt)t + m) ^ m;
-    }
-  }
-}
-
-template <int D, class Int, class UInt> __device__ __forceinline__ void from_ranks(const UInt* u, Int* v) {
-  constexpr uint32_t S = 1u << (2 * D);
-  const UInt m = nbmask<UInt>();
-  uint32_t l = lane();
-  if constexpr (D < 4) {
-    uint32_t r = l < S ? c_iperm[D - 1][l] : 0;
-    UInt t = __shfl(u[0], (int)r);
-    v[0] = (Int)((t ^ m) - m);
-  } else {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      uint32_t r = c_iperm[3][l + 64 * j], sl = r & 63, sq = r >> 6;
-      UInt t0 = __shfl(u[0], (int)sl), t1 = __shfl(u[1], (int)sl), t2 = __shfl(u[2], (int)sl),
-           t3 = __shfl(u[3], (int)sl);
-      UInt t = sq == 0 ? t0 : sq == 1 ? t1 : sq == 2 ? t2 : t3;
-      v[j] = (Int)((t ^ m) - m);
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// wave-uniform bit I/O (LSB-first 64-bit words)
-// ---------------------------------------------------------------------------
-struct BitWriter {
-  uint64_t* out;
-  uint64_t word;
-  uint64_t acc;
-  uint32_t fill;
-  bool shared;  // edge words may be shared with neighbouring blocks (OR them in)
-  bool first;
-  uint64_t count;
-
-  __device__ void init(uint64_t* base, uint64_t bitpos, bool share) {
-    out = base;
-    word = bitpos >> 6;
-    fill = (uint32_t)(bitpos & 63);
-    acc = 0;
-    shared = share;
-    first = true;
-    count = 0;
-  }
-  __device__ void store_word() {
-    if (lane() == 0) {
-      if (shared && first) atomicOr((unsigned long long*)&out[word], (unsigned long long)acc);
-      else out[word] = acc;
-    }
-    first = false;
-    ++word;
-    acc = 0;
-  }
-  // appends the low n bits of v (v has no bits at or above n)
-  __device__ void put(uint64_t v, uint32_t n) {
-    if (n == 0) return;
-    count += n;
-    acc |= v << fill;
-    uint32_t f = fill + n;
-    if (f >= 64) {
-      uint32_t used = 64 - fill;
-      store_word();
-      acc = used < 64 ? v >> used : 0;
-      fill = f - 64;
-    } else {
-      fill = f;
-    }
-  }
-  __device__ void zeros(uint64_t n) {
-    while (n) {
-      uint32_t k = n < 64 ? (uint32_t)n : 64;
-      put(0, k);
-      n -= k;
-    }
-  }
-  __device__ void finish() {
-    if (fill && lane() == 0) {
-      if (shared) atomicOr((unsigned long long*)&out[word], (unsigned long long)acc);
-      else out[word] = acc;
-    }
-  }
-};
-
-// instead of once per word.
-struct WindowSrc {
-  GlobalSrc g;
-  uint64_t first;      // first word of the window (wave-uniform)
-  uint64_t cur, nxt;   // lane l: words first + l and first + 64 + l
-  __device__ void init(const uint8_t* base, uint64_t nbytes, uint64_t w) {
-    g.base = base;
-    g.nbytes = nbytes;
-    first = w;
-    cur = g.load(w + lane());
-    nxt = g.load(w + 64 + lane());
-  }
-  __device__ uint64_t load(uint64_t w) {
-    if (w - first >= 64) {
-      if (w - first < 128) {
-        cur = nxt;
-        first += 64;
-      } else {
-        first = w;
-        cur = g.load(w + lane());
-      }
-      nxt = g.load(first + 64 + lane());
-    }
-    const int i = (int)(w - first);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)cur, i);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(cur >> 32), i);
-    return ((uint64_t)hi << 32) | lo;
-  }
-};
-
-template <class Src> struct BitReaderT {
-  Src src;
-  uint64_t next;
-  uint64_t cur;
-  uint32_t left;
-
-  __device__ uint64_t load(uint64_t w) { return src.load(w); }
-  __device__ void seek(uint64_t bit) {
-    next = bit >> 6;
-    uint32_t o = (uint32_t)(bit & 63);
-    cur = load(next++) >> o;
-    left = 64 - o;
-  }
-  __device__ uint64_t pos() const { return next * 64 - left; }
-  __device__ uint64_t get(uint32_t n) {
-    if (n == 0) return 0;
-    if (n <= left) {
-      uint64_t v = n == 64 ? cur : cur & ((1ull << n) - 1);
-      cur = n == 64 ? 0 : cur >> n;
-      left -= n;
-      return v;
-    }
-    uint64_t v = cur;
-    uint32_t got = left;
-    uint64_t w = load(next++);
-    uint32_t rem = n - got;
-    v |= w << got;
-    cur = rem == 64 ? 0 : w >> rem;
-    left = 64 - rem;
-    return n == 64 ? v : v & ((1ull << n) - 1);
-  }
-  __device__ uint32_t bit() { return (uint32_t)get(1); }
-  // consumes up to lim consecutive zero bits; stops before a one bit
-  __device__ uint32_t zeros(uint32_t lim) {
-    uint32_t z = 0;
-    while (z < lim) {
-      if (left == 0) {
-        cur = load(next++);
-        left = 64;
-      }
-      uint32_t t = cur ? (uint32_t)__builtin_ctzll(cur) : 64;
-      if (t > left) t = left;
-      uint32_t take = t < lim - z ? t : lim - z;
-      cur = take == 64 ? 0 : cur >> take;
-      left -= take;
-      z += take;
-      if (take < t || left > 0) break;
-    }
-    return z;
-  }
-  __device__ void skip(uint64_t n) { seek(pos() + n); }
-};
-using BitReader = BitReaderT<GlobalSrc>;
-
-// ---------------------------------------------------------------------------
-// embedded bit-plane coder over ranked coefficients
-// ---------------------------------------------------------------------------
-template <int R> __device__ __forceinline__ uint32_t next_one(const uint64_t* x, uint32_t n, uint32_t S) {
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    if (64u * q + 63 < n) continue;
-    uint64_t v = x[q];
-    if (64u * q < n) v &= ~0ull << (n - 64u * q);
-    if (v) return 64u * q + (uint32_t)__builtin_ctzll(v);
-  }
-  return S;
-}
-
-template <int D, class UInt>
-__device__ uint32_t encode_ints(BitWriter& w, uint32_t maxbits, uint32_t maxprec, const UInt* u) {
-  constexpr int R = D == 4 ? 4 : 1;
-  constexpr uint32_t S = 1u << (2 * D);
-  constexpr uint32_t intprec = sizeof(UInt) * 8;
-  uint32_t kmin = intprec > maxprec ? intprec - maxprec : 0;
-  uint32_t bits = maxbits, n = 0;
-  for (uint32_t k = intprec; bits && k-- > kmin;) {
-    uint64_t x[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) x[q] = __ballot((int)((u[q] >> k) & 1));
-    uint32_t m = n < bits ? n : bits;
-    bits -= m;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      if (m > 64u * q) {
-        uint32_t c = m - 64u * q;
-        if (c >= 64) w.put(x[q], 64);
-        else w.put(x[q] & ((1ull << c) - 1), c);
-      }
-    }
-    while (n < S && bits) {
-      bits--;
-      uint32_t p = next_one<R>(x, n, S);
-      if (p >= S) {
-        w.put(0, 1);
-        break;
-      }
-      w.put(1, 1);
-      uint32_t z = p - n;
-      if (bits <= z) {
-        w.zeros(bits);
-        bits = 0;
-        break;
-      }
-      w.zeros(z);
-      bits -= z;
-      n = p;
-      if (n < S - 1) {
-        bits--;
-        w.put(1, 1);
-      }
-      n++;
-    }
-  }
-  return maxbits - bits;
-}
-
-template <int D, class UInt, bool Parse, class Rd>
-__device__ uint32_t decode_ints(Rd& r, uint32_t maxbits, uint32_t maxprec, UInt* u) {
-  constexpr int R = D == 4 ? 4 : 1;
-  constexpr uint32_t S = 1u << (2 * D);
-  constexpr uint32_t intprec = sizeof(UInt) * 8;
-  uint32_t kmin = intprec > maxprec ? intprec - maxprec : 0;
-  uint32_t bits = maxbits, n = 0;
-#pragma unroll
-  for (int q = 0; q < R; ++q) u[q] = 0;
-  uint32_t l = lane();
-  for (uint32_t k = intprec; bits && k-- > kmin;) {
-    if (Parse && n == S) {
-      // every coefficient is significant: each plane left (k .. kmin) is S
-      // verbatim bits and nothing else, up to the bit budget
-      const uint64_t need = (uint64_t)(k + 1 - kmin) * S;
-      const uint32_t take = need < bits ? (uint32_t)need : bits;
-      r.skip(take);
-      bits -= take;
-      break;
-    }
-    uint64_t x[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) x[q] = 0;
-    uint32_t m = n < bits ? n : bits;
-    bits -= m;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      if (m > 64u * q) {
-        uint32_t c = m - 64u * q;
-        x[q] = r.get(c >= 64 ? 64 : c);
-      }
-    }
-    while (n < S && bits) {
-      bits--;
-      if (!r.bit()) break;
-      uint32_t lim = S - 1 - n;
-      if (lim > bits) lim = bits;
-      uint32_t z = r.zeros(lim);
-      if (z < lim) {
-        (void)r.get(1);
-        bits -= z + 1;
-      } else {
-        bits -= z;
-      }
-      n += z;
-      if (!Parse) {
-#pragma unroll
-        for (int q = 0; q < R; ++q)
-          if ((n >> 6) == (uint32_t)q) x[q] |= 1ull << (n & 63);
-      }
-      n++;
-    }
-    if (!Parse) {
-#pragma unroll
-      for (int q = 0; q < R; ++q) u[q] |= (UInt)((x[q] >> l) & 1) << k;
-    }
-  }
-  return maxbits - bits;
-}
-
-// ---------------------------------------------------------------------------
-// integer blocks
-// ---------------------------------------------------------------------------
-template <int D, class Int>
-__device__ uint32_t encode_int_block(BitWriter& w, int minbits, int maxbits, int maxprec, Int* v) {
-  using UInt = typename std::make_unsigned<Int>::type;
-  constexpr int R = D == 4 ? 4 : 1;
-  xform<D, Int, true, false>(v);
-  UInt u[R];
-  to_ranks<D, Int, UInt>(v, u);
-  int bits = (int)encode_ints<D, UInt>(w, maxbits > 0 ? (uint32_t)maxbits : 0u, (uint32_t)maxprec, u);
-  if (bits < minbits) {
-    w.zeros((uint64_t)(minbits - bits));
-    bits = minbits;
-  }
-  return (uint32_t)bits;
-}
-
-template <class UInt> __device__ __forceinline__ UInt wave_or(UInt v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o);
-  return v;
-}
-
-template <int D, class Int>
-__device__ uint32_t rev_encode_int_block(BitWriter& w, int minbits, int maxbits, int maxprec, Int* v) {
-  using UInt = typename std::make_unsigned<Int>::type;
-  constexpr int R = D == 4 ? 4 : 1;
-  constexpr int pbits = sizeof(Int) == 4 ? 5 : 6;
-  constexpr uint32_t intprec = sizeof(Int) * 8;
-  xform<D, Int, true, true>(v);
-  UInt u[R];
-  to_ranks<D, Int, UInt>(v, u);
-  UInt m = 0;
-#pragma unroll
-  for (int q = 0; q < R; ++q) m |= u[q];
-  m = wave_or(m);
-  uint32_t prec = m ? intprec - (uint32_t)__builtin_ctzll((unsigned long long)m) : 0;
-  if (prec > (uint32_t)maxprec) prec = (uint32_t)maxprec;
-  if (prec < 1) prec = 1;
-  w.put(prec - 1, pbits);
-  int bits = pbits;
-  bits += (int)encode_ints<D, UInt>(w, maxbits - bits > 0 ? (uint32_t)(maxbits - bits) : 0u, prec, u);
-  if (bits < minbits) {
-    w.zeros((uint64_t)(minbits - bits));
-    bits = minbits;
-  }
-  return (uint32_t)bits;
-}
-
-template <int D, class Int, bool Parse, class Rd>
-__device__ uint32_t decode_int_block(Rd& r, int minbits, int maxbits, int maxprec, bool rev, Int* v) {
-  using UInt = typename std::make_unsigned<Int>::type;
-  constexpr int R = D == 4 ? 4 : 1;
-  constexpr int pbits = sizeof(Int) == 4 ? 5 : 6;
-  int bits = 0;
-  uint32_t prec = (uint32_t)maxprec;
-  if (rev) {
-    prec = (uint32_t)r.get(pbits) + 1;
-    bits += pbits;
-  }
-  UInt u[R];
-  bits += (int)decode_ints<D, UInt, Parse>(r, maxbits - bits > 0 ? (uint32_t)(maxbits - bits) : 0u, prec, u);
-  if (bits < minbits) {
-    r.skip((uint64_t)(minbits - bits));
-    bits = minbits;
-  }
-  if (!Parse) {
-    from_ranks<D, Int, UInt>(u, v);
-    if (rev) xform<D, Int, false, true>(v);
-    else xform<D, Int, false, false>(v);
-  }
-  return (uint32_t)bits;
-}
-
-// ---------------------------------------------------------------------------
-// floating-point blocks
-// ---------------------------------------------------------------------------
-
-template <class UInt> __device__ __forceinline__ UInt wave_max(UInt v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    UInt t = __shfl_xor(v, o);
-    v = t > v ? t : v;
-  }
-  return v;
-}
-
-// x = m 2^e, 0.5 <= m < 1, of the largest |value| (NaN ignored, infinity -> 0)
-template <int D, class S> __device__ __forceinline__ int block_exponent(const S* f) {
-  using U = typename ZT<S>::UInt;
-  constexpr int R = D == 4 ? 4 : 1;
-  constexpr int bias = ZT<S>::ebias, mb = ZT<S>::mbits;
-  constexpr U absmask = ~(U)0 >> 1, inf = (U)(2 * bias + 1) << mb;
-  U mx = 0;
-#pragma unroll
-  for (int j = 0; j < R; ++j) {
-    U a = fbits(f[j]) & absmask;
-    if (a > inf) a = 0;
-    mx = a > mx ? a : mx;
-  }
-  mx = wave_max(mx);
-  if (mx == 0) return -bias;
-  int be = (int)(mx >> mb);
-  if (be == 0) return 1 - bias;
-  if (be == 2 * bias + 1) return 0;
-  return be - (bias - 1);
-}
-
-
-template <int D, class S> __device__ __forceinline__ void fwd_cast(const S* f, typename ZT<S>::Int* v, int emax) {
-  constexpr int R = D == 4 ? 4 : 1;
-  S s = pow2<S>((int)sizeof(S) * 8 - 2 - emax);
-#pragma unroll
-  for (int j = 0; j < R; ++j) v[j] = cast_int<S>(s * f[j]);
-}
-template <int D, class S> __device__ __forceinline__ void inv_cast(const typename ZT<S>::Int* v, S* f, int emax) {
-  constexpr int R = D == 4 ? 4 : 1;
-  S s = pow2<S>(emax - ((int)sizeof(S) * 8 - 2));
-#pragma unroll
-  for (int j = 0; j < R; ++j) f[j] = s * (S)v[j];
-}
-
-template <int D, class S> __device__ uint32_t encode_float_block(BitWriter& w, const ZfpChunk& c, const S* f) {
-  using Int = typename ZT<S>::Int;
-  using U = typename ZT<S>::UInt;
-  constexpr int R = D == 4 ? 4 : 1;
-  constexpr uint32_t SZ = 1u << (2 * D);
-  constexpr int ebits = ZT<S>::ebits, ebias = ZT<S>::ebias;
-  Int v[R];
-  int emax = block_exponent<D, S>(f);
-  if (c.minexp < kZfpMinExp) {  // reversible
-    uint32_t bits = 0;
-    if (emax != -ebias) fwd_cast<D, S>(f, v, emax);
-    else
-#pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = 0;
-    S g[R];
-    if (emax != -ebias) inv_cast<D, S>(v, g, emax);
-    else
-#pragma unroll
-      for (int j = 0; j < R; ++j) g[j] = 0;
-    bool bad = false;
-#pragma unroll
-    for (int j = 0; j < R; ++j) bad |= (D == 4 || lane() < SZ) && fbits(g[j]) != fbits(f[j]);
-    bool same = __ballot(bad) == 0;
-    if (same) {
-      uint32_t e = (uint32_t)(emax + ebias);
-      if (!e) {
-        w.put(0, 1);
-        return 1;
-      }
-      w.put(1, 2);
-      w.put(e, ebits);
-      bits = 2 + ebits;
-    } else {
-      const U mx = ~(U)0 >> 1;
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        U b = fbits(f[j]);
-        v[j] = (Int)b < 0 ? (Int)(b ^ mx) : (Int)b;
-      }
-      w.put(3, 2);
-      bits = 2;
-    }
-    return bits + rev_encode_int_block<D, Int>(w, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits,
-                                                (int)c.maxprec, v);
-  }
-  int dp = emax - c.minexp + 2 * (D + 1);
-  int maxprec = (int)c.maxprec < (dp > 0 ? dp : 0) ? (int)c.maxprec : (dp > 0 ? dp : 0);
-  uint32_t e = maxprec ? (uint32_t)(emax + ebias) : 0;
-  if (e) {
-    uint32_t bits = 1 + ebits;
-    w.put(2 * (uint64_t)e + 1, bits);
-    fwd_cast<D, S>(f, v, emax);
-    return bits + encode_int_block<D, Int>(w, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits, maxprec, v);
-  }
-  w.put(0, 1);
-  if (c.minbits > 1) {
-    w.zeros(c.minbits - 1);
-    return c.minbits;
-  }
-  return 1;
-}
-
-template <int D, class S, bool Parse, class Rd> __device__ uint32_t decode_float_block(Rd& r, const ZfpChunk& c, S* f) {
-  using Int = typename ZT<S>::Int;
-  using U = typename ZT<S>::UInt;
-  constexpr int R = D == 4 ? 4 : 1;
-  constexpr int ebits = ZT<S>::ebits, ebias = ZT<S>::ebias;
-  Int v[R];
-  if (c.minexp < kZfpMinExp) {  // reversible
-    uint32_t bits = 1;
-    if (r.bit()) {
-      bits++;
-      if (r.bit()) {
-        bits += decode_int_block<D, Int, Parse>(r, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits,
-                                                (int)c.maxprec, true, v);
-        if (!Parse) {
-          const U mx = ~(U)0 >> 1;
-#pragma unroll
-          for (int j = 0; j < R; ++j) {
-            Int x = v[j] < 0 ? (Int)((U)v[j] ^ mx) : v[j];
-            f[j] = fromb<S>((U)x);
-          }
-        }
-      } else {
-        bits += ebits;
-        int emax = (int)r.get(ebits) - ebias;
-        bits += decode_int_block<D, Int, Parse>(r, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits,
-                                                (int)c.maxprec, true, v);
-        if (!Parse) inv_cast<D, S>(v, f, emax);
-      }
-      return bits;
-    }
-    if (!Parse)
-#pragma unroll
-      for (int j = 0; j < R; ++j) f[j] = 0;
-    return bits;
-  }
-  if (r.bit()) {
-    uint32_t bits = 1 + ebits;
-    int emax = (int)r.get(ebits) - ebias;
-    int dp = emax - c.minexp + 2 * (D + 1);
-    int maxprec = (int)c.maxprec < (dp > 0 ? dp : 0) ? (int)c.maxprec : (dp > 0 ? dp : 0);
-    bits += decode_int_block<D, Int, Parse>(r, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits, maxprec,
-                                            false, v);
-    if (!Parse) inv_cast<D, S>(v, f, emax);
-    return bits;
-  }
-  if (!Parse)
-#pragma unroll
-    for (int j = 0; j < R; ++j) f[j] = 0;
-  if (c.minbits > 1) {
-    r.skip(c.minbits - 1);
-    return c.minbits;
-  }
-  return 1;
-}
-
-template <int D, class S> __device__ uint32_t encode_block(BitWriter& w, const ZfpChunk& c, S* vals) {
-  if constexpr (ZT<S>::fp) {
-    return encode_float_block<D, S>(w, c, vals);
-  } else {
-    if (c.minexp < kZfpMinExp)
-      return rev_encode_int_block<D, S>(w, (int)c.minbits, (int)c.maxbits, (int)c.maxprec, vals);
-    return encode_int_block<D, S>(w, (int)c.minbits, (int)c.maxbits, (int)c.maxprec, vals);
-  }
-}
-
-template <int D, class S, bool Parse, class Rd> __device__ uint32_t decode_block(Rd& r, const ZfpChunk& c, S* vals) {
-  if constexpr (ZT<S>::fp) {
-    return decode_float_block<D, S, Parse>(r, c, vals);
-  } else {
-    return decode_int_block<D, S, Parse>(r, (int)c.minbits, (int)c.maxbits, (int)c.maxprec,
-                                         c.minexp < kZfpMinExp, vals);
-  }
-}
-
-
-template <int D, class S> __device__ void gather(const ZfpChunk& c, const uint32_t* b, S* vals) {
-  constexpr int R = D == 4 ? 4 : 1;
-  constexpr uint32_t SZ = 1u << (2 * D);
-  uint32_t l = lane();
-  uint32_t cc[3] = {l & 3, (l >> 2) & 3, (l >> 4) & 3};
-  uint32_t ext[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    uint32_t e = a < D ? c.n[a] - 4 * b[a] : 1;
-    ext[a] = e < 4 ? e : 4;
-  }
-  uint64_t sx = 4ull * b[0] + pad_src(cc[0], ext[0]);
-  uint64_t sy = D > 1 ? 4ull * b[1] + pad_src(cc[1], ext[1]) : 0;
-  uint64_t sz = D > 2 ? 4ull * b[2] + pad_src(cc[2], ext[2]) : 0;
-#pragma unroll
-  for (int j = 0; j < R; ++j) {
-    if (D < 4 && l >= SZ) {
-      vals[j] = 0;
-      continue;
-    }
-    uint64_t sw = D > 3 ? 4ull * b[3] + pad_src((uint32_t)j, ext[3]) : 0;
-    uint64_t idx = ((sw * c.n[2] + sz) * c.n[1] + sy) * c.n[0] + sx;
-    vals[j] = load_elem<S>(c.src, idx, c.dtype);
-  }
-}
-
-// Writes the block's samples that fall inside the field (or, for a brick,
-// into its own clipped C-ordered array).
-template <int D, class S> __device__ void scatter(const ZfpChunk& c, const uint32_t* b, const S* vals, uint8_t* brick) {
-  constexpr int R = D == 4 ? 4 : 1;
-  constexpr uint32_t SZ = 1u << (2 * D);
-  uint32_t l = lane();
-  if (D < 4 && l >= SZ) return;
-  uint32_t cc[3] = {l & 3, (l >> 2) & 3, (l >> 4) & 3};
-  uint32_t ext[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    uint32_t e = a < D ? c.n[a] - 4 * b[a] : 1;
-    ext[a] = e < 4 ? e : 4;
-  }
-  for (int a = 0; a < (D < 3 ? D : 3); ++a)
-    if (cc[a] >= ext[a]) return;
-#pragma unroll
-  for (int j = 0; j < R; ++j) {
-    if (D == 4 && (uint32_t)j >= ext[3]) continue;
-    uint64_t idx;
-    if (brick) {
-      uint64_t ey = D > 1 ? ext[1] : 1, ez = D > 2 ? ext[2] : 1;
-      idx = ((uint64_t)(D > 3 ? j : 0) * ez + (D > 2 ? cc[2] : 0)) * ey * ext[0] + (D > 1 ? cc[1] : 0) * ext[0] +
-            cc[0];
-      store_elem<S>(brick, idx, c.dtype, vals[j]);
-    } else {
-      uint64_t x = 4ull * b[0] + cc[0], y = D > 1 ? 4ull * b[1] + cc[1] : 0, z = D > 2 ? 4ull * b[2] + cc[2] : 0,
-               w = D > 3 ? 4ull * b[3] + j : 0;
-      idx = ((w * c.n[2] + z) * c.n[1] + y) * c.n[0] + x;
-      store_elem<S>(c.dst, idx, c.dtype, vals[j]);
-    }
-  }
-}
-
-
-// ---------------------------------------------------------------------------
-// kernels: four wavefronts (blocks) per workgroup
-// ---------------------------------------------------------------------------
-template <int D, class S>
-__global__ __launch_bounds__(256) void zfp_encode_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks,
-                                                         uint64_t total) {
-  constexpr int R = D == 4 ? 4 : 1;
-  uint64_t g = xcd_item() * 4 + (threadIdx.x >> 6);  // neighbouring blocks share input lines: one L2
-  if (g >= total) return;
-  const ZfpChunk& c = chunks[find_chunk(chunks, nchunks, g)];
-  if (c.fixed ? !c.dst : !c.slots) return;  // refused chunk
-  uint64_t k = g - c.block_begin;
-  uint32_t b[4];
-  block_coords(c, k, b);
-  S vals[R];
-  gather<D, S>(c, b, vals);
-  BitWriter w;
-  if (c.fixed) w.init((uint64_t*)c.dst, c.header_bits + k * c.maxbits, true);
-  else w.init(c.slots + g * c.slot_words, 0, false);
-  encode_block<D, S>(w, c, vals);
-  w.finish();
-  if (!c.fixed && lane() == 0) c.offs[g] = w.count;
-}
 
 // Variable-size modes: moves block g's bits from its scratch slot to its
 // place in the stream (header + exclusive prefix sum of the lengths).
@@ -1159,44 +465,6 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
   if (l < tail) offs[nblocks - tail + l] = mine;
 }
 
-template <int D, class S>
-__global__ __launch_bounds__(256) void zfp_decode_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks,
-                                                         uint64_t total, const ZfpBrick* __restrict__ bricks) {
-  constexpr int R = D == 4 ? 4 : 1;
-  uint64_t g = xcd_item() * 4 + (threadIdx.x >> 6);  // neighbouring blocks share output lines: one L2
-  if (g >= total) return;
-  uint32_t ci;
-  uint64_t k;
-  uint8_t* brick = nullptr;
-  if (bricks) {
-    ci = bricks[g].chunk;
-    k = bricks[g].block;
-    brick = bricks[g].out;
-  } else {
-    ci = find_chunk(chunks, nchunks, g);
-    k = g - chunks[ci].block_begin;
-  }
-  const ZfpChunk& c = chunks[ci];
-  BitReader r;
-  r.src.base = c.src;
-  r.src.nbytes = c.src_len;
-  const bool indexed = !c.fixed && c.index;
-  r.seek(c.fixed ? c.header_bits + k * c.maxbits : indexed ? c.header_bits + c.index[k] : c.offs[c.block_begin + k]);
-  S vals[R];
-  const uint32_t used = decode_block<D, S, false>(r, c, vals);
-  if (indexed && lane() == 0) {
-    // the block must take exactly the bits the index gives it, the index
-    // must start at the first block and end inside the stream
-    const uint32_t lo = c.index[k], hi = c.index[k + 1];
-    if (hi < lo || hi - lo != used || (k == 0 && lo != 0) ||
-        (k + 1 == c.nblocks && c.header_bits + (uint64_t)hi > 8 * c.src_len))
-      atomicOr(c.bad, 1u);
-  }
-  uint32_t b[4];
-  block_coords(c, k, b);
-  scatter<D, S>(c, b, vals, brick);
-}
-
 // Block index of each variable-size chunk that asks for one: entry k is
 // block k's bit offset from the first block, entry nblocks the end.
 __global__ __launch_bounds__(256) void zfp_index_write_kernel(const ZfpChunk* __restrict__ chunks, uint32_t nchunks,
@@ -1211,12 +479,6 @@ __global__ __launch_bounds__(256) void zfp_index_write_kernel(const ZfpChunk* __
   if (k + 1 == c.nblocks) c.index[k + 1] = (uint32_t)(at + c.offs[g]);
 }
 
-template <int D, class S> void enc(const ZfpChunk* d, uint32_t n, uint64_t total, hipStream_t s) {
-  hipLaunchKernelGGL((zfp_encode_kernel<D, S>), xcd_grid(ceil_div(total, 4)), dim3(256), 0, s, d, n, total);
-}
-template <int D, class S> void dec(const ZfpChunk* d, uint32_t n, uint64_t total, const ZfpBrick* br, hipStream_t s) {
-  hipLaunchKernelGGL((zfp_decode_kernel<D, S>), xcd_grid(ceil_div(total, 4)), dim3(256), 0, s, d, n, total, br);
-}
 template <int D, class S> void idx(const ZfpChunk* d, uint32_t n, hipStream_t s) {
   hipLaunchKernelGGL((zfp_index_kernel<D, S>), dim3(n), dim3(kWalkThreads), 0, s, d, n);
 }
@@ -1241,19 +503,6 @@ template <int D, class S> void idx(const ZfpChunk* d, uint32_t n, hipStream_t s)
     case 4 * 8 + kZfpDouble: FN<4, double>(__VA_ARGS__); break;                        \
     default: fail(NDIC_INVALID_ARGUMENT, "zfp: unsupported field class");              \
   }
-
-// The block coders of 4D fields (256 coefficients a block: one wavefront per
-// block); 1D-3D fields take one lane per block (zfp_lane.hip).
-#define NDIC_ZFP_DISPATCH_4D(FN, ...)                                                  \
-  switch (dims * 8 + scalar) {                                                         \
-    case 4 * 8 + kZfpInt32: FN<4, int32_t>(__VA_ARGS__); break;                        \
-    case 4 * 8 + kZfpInt64: FN<4, int64_t>(__VA_ARGS__); break;                        \
-    case 4 * 8 + kZfpFloat: FN<4, float>(__VA_ARGS__); break;                          \
-    case 4 * 8 + kZfpDouble: FN<4, double>(__VA_ARGS__); break;                        \
-    default: fail(NDIC_INVALID_ARGUMENT, "zfp: unsupported field class");              \
-  }
-
-
 
 __global__ void zfp_headers_kernel(const ZfpHeader* __restrict__ h, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1315,29 +564,11 @@ void launch_zfp_heads(const ZfpHead* d_heads, uint32_t n, uint8_t* d_out, hipStr
   NDIC_HIP(hipGetLastError());
 }
 
-void zfp_init_tables() {
-  static std::mutex mu;
-  static bool done = false;
-  std::lock_guard<std::mutex> lock(mu);
-  if (done) return;
-  uint8_t perm[4][256] = {}, iperm[4][256] = {};
-  for (int i = 0; i < 4; ++i) perm[0][i] = (uint8_t)i;
-  for (int i = 0; i < 16; ++i) perm[1][i] = kPerm2[i];
-  for (int i = 0; i < 64; ++i) perm[2][i] = kPerm3[i];
-  for (int i = 0; i < 256; ++i) perm[3][i] = kPerm4[i];
-  for (int d = 0; d < 4; ++d)
-    for (int r = 0; r < (1 << (2 * (d + 1))); ++r) iperm[d][perm[d][r]] = (uint8_t)r;
-  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_perm), perm, sizeof(perm)));
-  NDIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_iperm), iperm, sizeof(iperm)));
-  done = true;
-}
-
 void launch_zfp_encode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, uint32_t dims, uint32_t scalar,
                        hipStream_t stream) {
   if (total == 0) return;
-  if (launch_zfp_lane_encode(d_chunks, nchunks, total, dims, scalar, stream)) return;
-  fail(NDIC_INVALID_ARGUMENT, "zfp: unsupported field class");
-  NDIC_HIP(hipGetLastError());
+  if (!launch_zfp_lane_encode(d_chunks, nchunks, total, dims, scalar, stream))
+    fail(NDIC_INVALID_ARGUMENT, "zfp: unsupported field class");
 }
 
 void launch_zfp_place(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, const uint64_t* d_scan,
@@ -1366,9 +597,8 @@ void launch_zfp_index_write(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t
 void launch_zfp_decode(const ZfpChunk* d_chunks, uint32_t nchunks, uint64_t total, uint32_t dims, uint32_t scalar,
                        const ZfpBrick* d_bricks, uint32_t fixed_maxbits, hipStream_t stream) {
   if (total == 0) return;
-  if (launch_zfp_lane_decode(d_chunks, nchunks, total, dims, scalar, d_bricks, fixed_maxbits, stream)) return;
-  fail(NDIC_INVALID_ARGUMENT, "zfp: unsupported field class");
-  NDIC_HIP(hipGetLastError());
+  if (!launch_zfp_lane_decode(d_chunks, nchunks, total, dims, scalar, d_bricks, fixed_maxbits, stream))
+    fail(NDIC_INVALID_ARGUMENT, "zfp: unsupported field class");
 }
 
 }  // namespace ndic

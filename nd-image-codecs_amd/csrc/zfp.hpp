@@ -1,6 +1,7 @@
-// ZFP (LLNL ZFP stream format, codec version 5) on the device: one lane per
-// 4^d block for 1D-3D fields (zfp_lane.hip), one wavefront per block for 4D
-// fields (zfp.hip); orchestration in engine_zfp.cpp.
+// ZFP (LLNL ZFP stream format, codec version 5) on the device: the block
+// coders (zfp_lane_kernels.hpp: one lane per 4^d block for 1D-3D fields,
+// four lanes per block for 4D fields), the stream walk, placement and block
+// index (zfp.hip); orchestration in engine_zfp.cpp.
 #pragma once
 
 #include <vector>
@@ -69,7 +70,6 @@ struct ZfpHead {
 };
 constexpr uint32_t kZfpHeadBytes = 24;  // 148 header bits at most
 
-void zfp_init_tables();
 size_t zfp_scan_temp_bytes(uint64_t n);
 void zfp_exclusive_scan(void* temp, size_t temp_bytes, const uint64_t* d_in, uint64_t* d_out, uint64_t n,
                         hipStream_t stream);

@@ -1,6 +1,5 @@
-// Device helpers shared by the two ZFP block-coder layouts (zfp.hip: one
-// wavefront per block; zfp_lane.hip: one lane per block): scalar traits,
-// the 4-point lifting steps, IEEE-754 helpers, element loads and stores with
+// Device helpers of the ZFP kernels (zfp.hip, zfp_lane_kernels.hpp): scalar
+// traits, the 4-point lifting steps, IEEE-754 helpers, element loads and stores with
 // the chunk layer's dtype promotions, block geometry.  The algorithm is the
 // published ZFP one (Lindstrom, IEEE TVCG 2014; LLNL ZFP stream format).
 #pragma once

@@ -1,4 +1,5 @@
-// ZFP block codec on the device, one lane per 4^d block (d = 1..3).
+// ZFP block codec on the device: one lane per 4^d block (d = 1..3), four
+// lanes per block for 4D fields.
 //
 // Each lane runs the whole block through the coder: gather (pad rule of
 // partial blocks), block-floating-point cast, decorrelating lifts, negabinary
@@ -18,9 +19,8 @@
 // otherwise (bricks of different streams, a stretch larger than the stage)
 // they read global memory.
 //
-// The 4D class (256 coefficients a block) stays with one wavefront per block
-// (zfp.hip).  Streams and samples equal that coder's bit for bit.  The
-// algorithm and stream layout are the published ZFP ones (Lindstrom, IEEE
+// 4D blocks (256 coefficients) are shared by the four lanes of a quad (see
+// the 4D section below).  The algorithm and stream layout are the published ZFP ones (Lindstrom, IEEE
 // TVCG 2014; LLNL ZFP stream format, codec 5), which the reference reaches
 // through zfp-rs 0.1.1 (crates/ndic-zfp/src/lib.rs:249-376).
 #pragma once

@@ -71,8 +71,7 @@ extern "C" ndic_status ndic_zfp_decode_bricks_noalloc(const uint8_t* d_in, uint8
                                       std::to_string(NDIC_ZFP_NOALLOC_MAX_BRICKS));
     std::lock_guard<std::mutex> lock(g_mu);
     hipStream_t s = stream ? (hipStream_t)stream : Context::get().stream;
-    zfp_init_tables();
-    uint8_t* base = scratch_ptr();
+      uint8_t* base = scratch_ptr();
     std::vector<uint8_t> heads(n * kZfpHeadBytes);
     if (n) {
       // header bytes of every stream for the checks of BrickReader::fixed_rate
