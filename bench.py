@@ -845,6 +845,10 @@ def main():
         log(f"copy calibration: {args.copy_calibration} bytes")
         del a, b
     if args.zfp_only:
+        for kv in args.knob:
+            key, _, val = kv.partition("=")
+            ndic.tune_set(key, int(val))
+            log(f"knob {key} = {int(val)}")
         print(json.dumps({"zfp": zfp_bench(ndic, dev, args)}), flush=True)
         return
     tuning = None
