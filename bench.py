@@ -288,10 +288,11 @@ ZFP_BRICKS = "2048 fields 16^3..64^3 f32 rate8"
 
 def zfp_schedule():
     """The ZFP section's phases in launch order, each opened by one marker
-    copy kernel: (case, phase, calls).  An encode call launches
-    zfp_encode_kernel (+ zfp_place_kernel and zfp_index_write_kernel in the
-    variable-size modes), a decode call zfp_decode_kernel, a walk call
-    zfp_index_kernel then zfp_decode_kernel."""
+    copy kernel: (case, phase, calls).  An encode call launches the block
+    coder's encode kernel (zfp_lane_encode_kernel, zfp_quad_encode_kernel for
+    4D fields; + zfp_place_kernel and zfp_index_write_kernel in the
+    variable-size modes), a decode call its decode kernel, a walk call
+    zfp_index_kernel then the decode kernel."""
     out = []
     for fname, *_ in ZFP_FIELDS:
         for mname, mode, _ in ZFP_MODES:
@@ -538,8 +539,10 @@ def write_counters(root: Path, cus: int = 256) -> dict:
 
 
 ZFP_COUNTERS = ROOT / "profiles" / "zfp_counters.json"
-ZFP_KERNELS = {"encode": ["zfp_encode_kernel", "zfp_place_kernel", "zfp_index_write_kernel"],
-               "decode": ["zfp_decode_kernel"], "walk": ["zfp_index_kernel", "zfp_decode_kernel"]}
+ZFP_KERNELS = {"encode": ["zfp_lane_encode_kernel", "zfp_quad_encode_kernel", "zfp_place_kernel",
+                          "zfp_index_write_kernel"],
+               "decode": ["zfp_lane_decode_kernel", "zfp_quad_decode_kernel"],
+               "walk": ["zfp_index_kernel", "zfp_lane_decode_kernel", "zfp_quad_decode_kernel"]}
 
 
 def write_zfp_counters(root: Path) -> dict:
