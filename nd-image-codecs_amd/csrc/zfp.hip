@@ -165,26 +165,35 @@ struct TestWord {
 
 // The test word at every stream position, computed ahead of the walk.  For
 // the group tests of a plane starting at bit t, what the word at t decides
-// does not depend on the coefficient count n (only its last-coefficient
-// check does), so other waves compute it for every t in parallel while the
-// walking wave only chains lookups.  Entry (16 bits): kind in bits 14..15 --
-// 0: the first test is negative (bits 7..13: the run of zeros, at most 63),
-// 1: the plane's tests end in the word, 2: tokens run past the word, 3: a
-// token longer than the word; bits 7..13 the bits taken, bits 0..6 the count
-// of coefficients they make significant.
-enum : uint32_t { kWordZero = 0, kWordEnd = 1, kWordOn = 2, kWordLong = 3 };
+// does not depend on the coefficient count n (only its last-coefficient check
+// does), so other waves compute it for every t in parallel while the walking
+// wave only chains lookups.  A ring slot holds the number of the chunk of
+// kSlotBits positions it belongs to (high 32 bits) and the entry (low 32):
+//   bits 0..8   C, the coefficients the word's tests make significant
+//   bits 9..16  A, the bits the tests take, plus C where E is set
+//   bit 17      E, set where the plane's tests end in the word (a negative
+//               first test ends them too)
+// so that a turn from count n takes A + (E ? n : 0) bits -- the tests, then
+// the next plane's n + C verbatim bits -- and ends E planes.  A token longer
+// than the word has C = 511 (its zeros may reach the last coefficient), or,
+// in 4D blocks, whose tokens reach past the word, no entry: its slot carries
+// a chunk number no walk reads.
+constexpr uint32_t kEntryEnds = 1u << 17;
 
-__device__ __forceinline__ uint32_t test_word_entry(GlobalSrc& g, uint64_t t) {
+template <int D>
+__device__ __forceinline__ uint64_t test_word_slot(GlobalSrc& g, uint64_t t, uint32_t chunk) {
   const uint64_t w = t >> 6;
   const uint32_t sh = (uint32_t)t & 63;
   const uint64_t x = (g.load(w) >> sh) | ((g.load(w + 1) << 1) << (63 - sh));
-  if (!(x & 1)) return (kWordZero << 14) | ((uint32_t)__builtin_ctzll(x | (1ull << 63)) << 7);
+  const uint64_t tag = (uint64_t)chunk << 32;
+  if (!(x & 1)) return tag | kEntryEnds | (1u << 9);  // C = 0, A = 1
   const uint64_t closing = x & ~prefix_parity(x) & ~(1ull << 63);
-  if (!closing) return kWordLong << 14;
+  if (!closing) return D < 4 ? tag | 511u : (uint64_t)(chunk ^ 0x80000000u) << 32;
   const uint64_t stop = closing & ~(x >> 1);
   const uint32_t p = stop ? (uint32_t)__builtin_ctzll(stop) : 63 - (uint32_t)__builtin_clzll(closing);
   const uint32_t tokens = (uint32_t)__builtin_popcountll(x & (~0ull >> (63 - p))) >> 1;
-  return ((stop ? kWordEnd : kWordOn) << 14) | ((stop ? p + 2 : p + 1) << 7) | (p + 1 - tokens);
+  const uint32_t cnt = p + 1 - tokens;
+  return tag | (stop ? kEntryEnds | (p + 2 + cnt) << 9 : (p + 1) << 9) | cnt;
 }
 
 // The ring of test-word entries in LDS: kRingSlots slots of kSlotBits
@@ -193,11 +202,10 @@ __device__ __forceinline__ uint32_t test_word_entry(GlobalSrc& g, uint64_t t) {
 // (so all of them work on the chunks nearest the walk) and write a batch
 // only while its chunk is within kRingSlots of the chunk the walk reads
 // (`reading`); when the walk moves on, it raises the counter past the
-// batches it no longer needs.  Each entry carries its chunk number in its
-// upper 32 bits, and the walk takes an entry only when the number is its
-// own, so that an entry not yet written, or one overwritten by a producer
-// that fell behind, can cost time (a short wait, then that turn parses the
-// stream words) but never the result.
+// batches it no longer needs.  The walk takes an entry only when its chunk
+// number is the one the walk reads, so that an entry not yet written, or one
+// overwritten by a producer that fell behind, can cost time (that turn parses
+// the stream words) but never the result.
 constexpr uint32_t kWalkProducers = 15;
 constexpr uint32_t kWalkThreads = 64 * (1 + kWalkProducers);
 constexpr uint32_t kRingSlots = 16, kSlotBits = 512;
@@ -215,14 +223,16 @@ __device__ __forceinline__ void lds_release(uint32_t* p, uint32_t v) {
 // 0 walks; waves 1.. fill the ring of test words ahead of it.  Lane j of the
 // walking wave keeps the offset of block 64 q + j; it stores 64 at a time.
 //
-// The walk is a loop of turns.  The common turn, branch-free, takes one word
-// of a plane's group tests from the ring, and when the plane ends, the next
-// plane's verbatim bits (a run of negative first tests with no significant
-// coefficients ends as many planes at once).  The general turn reads the
-// stream words and takes the rest: block headers, the last coefficient
-// becoming significant (the plane's tests end there and the planes left are
-// verbatim), ends of blocks, and single tests where the word cannot decide (a
-// token longer than the word, a bit budget below 64).
+// A block takes three kinds of turn.  Its header is parsed from the stream
+// words.  Its group tests are taken from the ring a word at a time, each turn
+// reading the entry its predecessor points at before it checks its own (the
+// next plane's verbatim bits come with the entry that ends a plane).  When an
+// entry says that the last coefficient becomes significant in its word, the
+// word is parsed for where (the tests end there and the planes left are
+// verbatim) and the block ends.  Everything else -- an entry that is missing,
+// a token longer than the word, a bit budget near its end, a block that runs
+// out of planes before every coefficient is significant -- goes through the
+// general turn, which parses one word (or one test) of the stream.
 template <int D, class S>
 __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk* __restrict__ chunks,
                                                                  uint32_t nchunks) {
@@ -234,6 +244,7 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
   constexpr uint32_t pbits = sizeof(BInt) == 4 ? 5 : 6;
   constexpr uint32_t ebits = ZT<S>::ebits;
   constexpr int ebias = ZT<S>::ebias;
+  constexpr uint32_t kRing = kRingSlots * kSlotBits;
   enum { kHeader, kTests, kZeroRun };
 
   uint32_t ci = blockIdx.x;
@@ -249,10 +260,9 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
   const uint64_t stream_bits = c.src_len * 8;
   const uint32_t nring = stream_bits > base ? (uint32_t)((stream_bits - base + kSlotBits - 1) / kSlotBits) : 0u;
 
-  __shared__ uint64_t ring[kRingSlots * kSlotBits];
+  __shared__ uint64_t ring[kRing];
   __shared__ uint32_t reading, next_batch, done;
-  for (uint32_t i = threadIdx.x; i < kRingSlots * kSlotBits; i += kWalkThreads)
-    ring[i] = ~0ull << 32 | kWordLong << 14;  // no chunk's; undecided
+  for (uint32_t i = threadIdx.x; i < kRing; i += kWalkThreads) ring[i] = ~0ull << 32;  // no chunk's
   if (threadIdx.x == 0) reading = next_batch = done = 0;
   __syncthreads();
 
@@ -279,8 +289,8 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
       if (!go) continue;
       const uint32_t r = (b % kBatches) * 64 + l;
       __hip_atomic_store(&ring[(ch % kRingSlots) * kSlotBits + r],
-                         (uint64_t)ch << 32 | test_word_entry(g, base + (uint64_t)ch * kSlotBits + r),
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                         test_word_slot<D>(g, base + (uint64_t)ch * kSlotBits + r, ch), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     return;
   }
@@ -294,52 +304,35 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
   uint32_t bits = 0, n = 0, k = 0, kmin = 0, zacc = 0, zlim = 0;
   uint32_t cur_ch = ~0u;
   int st = kHeader;
+  const auto follow = [&]() {  // let the producers move on to the chunk the walk reads
+    const uint32_t ch = (uint32_t)((pos - base) / kSlotBits);
+    if (ch != cur_ch) {
+      lds_release(&reading, ch);
+      __hip_atomic_fetch_max(&next_batch, ch * kBatches, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      cur_ch = ch;
+    }
+  };
+  const auto slot_at = [&](uint32_t off) {  // the ring slot at byte offset off (modulo the ring)
+    uint64_t* const at = (uint64_t*)__builtin_assume_aligned((char*)ring + (off & (kRing * 8 - 8)), 8);
+    return __hip_atomic_load(at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  const auto end_block = [&]() {
+    if (pos - blk_start < (uint64_t)(minbits > 0 ? minbits : 0)) pos = blk_start + (uint32_t)minbits;
+    ++kb;
+    st = kHeader;
+  };
   for (;;) {
-    if (st == kTests) {  // common turns (n < SZ here)
-      for (;;) {
-        const uint64_t rel = pos - base;
-        const uint64_t te = __hip_atomic_load(&ring[(uint32_t)rel % (kRingSlots * kSlotBits)], __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint32_t e = (uint32_t)te, kind = e >> 14, uf = (e >> 7) & 127;
-        const bool z0 = kind == kWordZero;
-        const uint32_t used = (z0 & (n != 0)) ? 1u : uf;
-        const uint32_t n2 = n + (e & 127);
-        const uint32_t ends = z0 ? used : 2 - kind;  // planes ended: kWordEnd 1, kWordOn 0
-        const uint32_t bits1 = bits - used;
-        const uint32_t madd = kind <= kWordEnd ? (n2 < bits1 ? n2 : bits1) : 0u;  // the next plane's verbatim bits
-        // the entry is this chunk's and decides; the budget is above the
-        // word; the planes ended exist
-        if (!(((uint32_t)(te >> 32) == (uint32_t)(rel / kSlotBits)) & (kind != kWordLong) & (n2 < SZ) &
-              (bits > 64) & (k - kmin >= ends)))
-          break;
-        n = n2;
-        pos += used + madd;
-        bits = bits1 - madd;
-        k -= ends;
-      }
-    }
-    // the general turn
-    {
-      const uint32_t ch = (uint32_t)((pos - base) / kSlotBits);
-      if (ch != cur_ch) {  // let the producers move on
-        lds_release(&reading, ch);
-        __hip_atomic_fetch_max(&next_batch, ch * kBatches, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        cur_ch = ch;
-      }
-    }
+    uint64_t te = 0;  // the slot of the turn at pos, when the header read it
+    bool have_te = false;
     if (st == kHeader) {
       if (kb == nblocks) break;
-      const uint32_t j = (uint32_t)kb & 63;
-      if (l == j) mine = pos;
-      if (j == 63) offs[kb - 63 + l] = mine;
-      blk_start = pos;
-    }
-    const uint64_t x = win.peek(pos);
-    bool plane_end = false;
-    if (st == kHeader) {
+      const uint64_t x = win.peek(pos);
       int hdr = 0, prec = cmaxprec;
       if constexpr (fp) {
         if (!(x & 1)) {  // every value zero: one bit (reversible) or minbits
+          const uint32_t j = (uint32_t)kb & 63;
+          if (l == j) mine = pos;
+          if (j == 63) offs[kb - 63 + l] = mine;
           pos += rev || minbits <= 1 ? 1u : (uint32_t)minbits;
           ++kb;
           continue;
@@ -358,13 +351,98 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
         prec = (int)(x & ((1u << pbits) - 1)) + 1;
         hdr = (int)pbits;
       }
+      te = slot_at((uint32_t)(pos + (uint32_t)hdr - base) << 3);  // the first plane's tests, read early
+      have_te = true;
+      follow();
+      const uint32_t j = (uint32_t)kb & 63;
+      if (l == j) mine = pos;
+      if (j == 63) offs[kb - 63 + l] = mine;
+      blk_start = pos;
       pos += (uint32_t)hdr;
       bits = maxbits - hdr > 0 ? (uint32_t)(maxbits - hdr) : 0u;
       n = 0;
       k = intprec;
       kmin = intprec > (uint32_t)prec ? intprec - (uint32_t)prec : 0;
-      plane_end = true;
-    } else if (st == kZeroRun) {  // a test's zeros that ran past the last word
+      if (!bits || k <= kmin) {  // no bit plane
+        end_block();
+        continue;
+      }
+      --k;  // the first plane: no verbatim bits, its tests follow
+      st = kTests;
+    }
+    if (st == kTests && n < SZ) {  // turns from the ring
+      // Each turn reads the slot its entry points at before it checks the
+      // entry.  Where the bits the block can still take stay above the
+      // budget less a word and a plane, the budget cannot end it early.
+      uint64_t rel = pos - base;
+      const uint64_t rel0 = rel;
+      uint32_t o = (uint32_t)rel << 3;  // rel's slot, in bytes (modulo the ring)
+      uint32_t kl = k - kmin;           // planes after this one
+      if (!have_te) te = slot_at(o);    // read into vector registers; made scalar where used
+      bool last = false;
+      const auto turns = [&](auto bounded) {
+        const uint32_t lim = bits > 65 + SZ ? bits - 65 - SZ : 0u;
+        for (;;) {
+          const uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)te);
+          const uint32_t tag = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(te >> 32));
+          const uint32_t step = ((e >> 9) & 255) + (n & (uint32_t)((int32_t)(e << 14) >> 31));
+          const uint32_t o2 = o + (step << 3);
+          const uint64_t te2 = slot_at(o2);
+          if (tag != (uint32_t)(rel / kSlotBits)) break;
+          if (decltype(bounded)::value && (uint32_t)(rel - rel0) >= lim) break;
+          const uint32_t n2 = n + (e & 511);
+          if (n2 >= SZ) {  // the last coefficient becomes significant in the word
+            last = true;
+            break;
+          }
+          const uint32_t ends = (e >> 17) & 1;
+          if (kl < ends) break;  // the block ends with this plane
+          n = n2;
+          rel += step;
+          o = o2;
+          kl -= ends;
+          te = te2;
+        }
+      };
+      // the rest of the block takes at most kl SZ verbatim bits and 2 SZ + kl + 1 tests
+      const uint32_t rest = (kl + 2) * (SZ + 1) + SZ;
+      if (bits > rest + 65 + SZ) turns(std::false_type{});
+      else turns(std::true_type{});
+      pos = base + rel;
+      bits -= (uint32_t)(rel - rel0);
+      k = kmin + kl;
+      if (last) {
+        // The word's tests reach the last coefficient: its token is the one at
+        // which the count of bits other than closing ones (the even-ranked
+        // ones) reaches SZ - n; in a token longer than the word, the zeros
+        // reach it (the last one implied).  Where neither is in the word's
+        // first 63 bits, the general turn takes the test.  The planes left
+        // are SZ verbatim bits each.
+        const uint64_t x = win.peek(pos);
+        const uint32_t ones = __builtin_popcount((uint32_t)x & mlo) + __builtin_popcount((uint32_t)(x >> 32) & mhi);
+        const uint64_t hit = __ballot(l + 1 - (ones >> 1) == SZ - n);
+        if (hit) {
+          const uint32_t q = (uint32_t)__builtin_ctzll(hit);
+          pos += q + 1;
+          bits -= q + 1;
+          n = SZ;
+          if (k > kmin) {
+            --k;
+            const uint64_t need = (uint64_t)(k + 1 - kmin) * SZ;
+            const uint32_t take = need < bits ? (uint32_t)need : bits;
+            pos += take;
+            bits -= take;
+          }
+          end_block();
+          continue;
+        }
+      }
+    }
+    // the general turn
+    follow();
+    const uint64_t x = win.peek(pos);
+    bool plane_end = false;
+    if (st == kZeroRun) {  // a test's zeros that ran past the last word
       const uint32_t t = x ? (uint32_t)__builtin_ctzll(x) : 64u;
       if (t < zlim && t < 64) {  // the closing one
         pos += t + 1;
@@ -391,9 +469,6 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
         bits -= 1;
         plane_end = true;
       } else if (bits >= 64 && tw.closing && tw.n2 >= SZ) {
-        // the last coefficient becomes significant inside the word: its
-        // token is the one at which the count of bits other than closing
-        // ones reaches SZ - n (lim zeros after the opening one, no closing)
         const uint64_t other = ~tw.closing;
         const uint32_t cnt =
             __builtin_popcount((uint32_t)other & mlo) + __builtin_popcount((uint32_t)(other >> 32) & mhi);
@@ -453,11 +528,7 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
           block_end = false;
         }
       }
-      if (block_end) {
-        if (pos - blk_start < (uint64_t)(minbits > 0 ? minbits : 0)) pos = blk_start + (uint32_t)minbits;
-        ++kb;
-        st = kHeader;
-      }
+      if (block_end) end_block();
     }
   }
   lds_release(&done, 1);
