@@ -261,6 +261,47 @@ def single_chunk_lines(ndic, dev, args):
                      "encode_GBps": round(vol.nbytes / me / 1e9, 2), "decode_GBps": round(vol.nbytes / md / 1e9, 2),
                      "compressed_bytes": int(e[0].out_len), "bit_exact": exact, "reps": args.single_reps}
         log(f"single chunk {name}: {out[name]}")
+    # the reference's ZFP chunk fixture, one chunk per call: encode (with a
+    # block index in the variable-size mode), decode through the index and,
+    # as the reference decodes its streams, by walking the stream
+    fx = datagen.zfp_bench_fixture()
+    d_fx = dev.DeviceBuffer.from_bytes(fx.tobytes())
+    cap, icap = 2 * fx.nbytes + (1 << 20), 1 << 20
+    d_enc, d_idx = dev.DeviceBuffer(cap, zero=True), dev.DeviceBuffer(icap, zero=True)
+    d_out = dev.DeviceBuffer(fx.nbytes)
+    for mname, mode, kw, (ref_e, ref_d) in ZFP_FIXTURE_MODES:
+        cfg = ndic.zfp_config(mode, **kw)
+        desc = [ndic.make_zfp_desc(fx.shape, "float32", cfg, 0, fx.nbytes)]
+        te, td, tw = [], [], []
+        for _ in range(args.single_reps):
+            dev.synchronize()
+            t0 = time.perf_counter()
+            arr = ndic.zfp_encode_batch(d_fx.ptr, d_enc.ptr, cap, desc, d_index_ptr=d_idx.ptr, index_capacity=icap)
+            te.append(time.perf_counter() - t0)
+            a0 = arr[0]
+            idx = [ndic.make_zfp_desc(fx.shape, "float32", cfg, a0.out_offset, a0.out_len, 0, a0.index_offset,
+                                      a0.index_len)]
+            t0 = time.perf_counter()
+            ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, idx, d_index_ptr=d_idx.ptr)
+            td.append(time.perf_counter() - t0)
+            if a0.index_len:
+                walk = [ndic.make_zfp_desc(fx.shape, "float32", cfg, a0.out_offset, a0.out_len, 0)]
+                t0 = time.perf_counter()
+                ndic.zfp_decode_batch(d_enc.ptr, d_out.ptr, walk)
+                tw.append(time.perf_counter() - t0)
+        med = lambda ts: sorted(ts)[len(ts) // 2]  # noqa: E731
+        me, md = med(te), med(td)
+        dref = med(tw) if tw else md  # the reference's streams carry no block index
+        rec = {"encode_ms": round(me * 1e3, 3), "decode_ms": round(md * 1e3, 3),
+               "compressed_bytes": int(a0.out_len), "reference_encode_ms": ref_e, "reference_decode_ms": ref_d,
+               "encode_vs_reference": round(ref_e / (me * 1e3), 2),
+               "decode_vs_reference": round(ref_d / (dref * 1e3), 2), "reps": args.single_reps}
+        if tw:
+            rec["decode_without_index_ms"] = round(dref * 1e3, 3)
+            rec["lossless_exact"] = dev.count_mismatch(d_out.ptr, d_fx.ptr, fx.nbytes) == 0
+        name = f"zfp f32 32x64x64 {mname} (reference fixture)"
+        out[name] = rec
+        log(f"single chunk {name}: {rec}")
     return out, cases
 
 
@@ -283,6 +324,11 @@ ZFP_MODES = [("rate8", "fixed_rate", {"rate": 8.0}), ("rate16", "fixed_rate", {"
              ("reversible", "reversible", {}), ("fixed_accuracy 1e-3", "fixed_accuracy", {"tolerance": 1e-3}),
              ("fixed_precision 20", "fixed_precision", {"precision": 20})]
 ZFP_REPS = 5
+# the reference's ZFP chunk benchmark fixture (f32 32x64x64) and its recorded
+# single-chunk medians in ms, encode and decode
+# (bench/baselines/main/zfp-{rate8,reversible}/zfp__chunk_{encode,decode}_f32_zyx_32x64x64.json:6)
+ZFP_FIXTURE_MODES = [("rate8", "fixed_rate", {"rate": 8.0}, (1.3954, 1.3851)),
+                     ("reversible", "reversible", {}, (1.5686, 2.4715))]
 ZFP_BRICKS = "2048 fields 16^3..64^3 f32 rate8"
 
 

@@ -277,14 +277,14 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
       if (b >= nbatches || lds_acquire(&done)) break;
       const uint32_t ch = b / kBatches;
       bool go = false;
-      for (uint32_t spin = 0; spin < kWalkSpins; ++spin) {
+      for (uint32_t spin = 0; spin < kWalkSpins; ++spin) {  // waits lightly: the walk reads the same LDS
         const uint32_t at = lds_acquire(&reading);
-        if (at > ch || lds_acquire(&done)) break;  // passed by the walk: not needed
+        if (at > ch || ((spin & 7) == 7 && lds_acquire(&done))) break;  // passed by the walk: not needed
         if (at + kRingSlots > ch) {
           go = true;
           break;
         }
-        __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_s_sleep(4);
       }
       if (!go) continue;
       const uint32_t r = (b % kBatches) * 64 + l;
@@ -351,9 +351,9 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
         prec = (int)(x & ((1u << pbits) - 1)) + 1;
         hdr = (int)pbits;
       }
+      follow();
       te = slot_at((uint32_t)(pos + (uint32_t)hdr - base) << 3);  // the first plane's tests, read early
       have_te = true;
-      follow();
       const uint32_t j = (uint32_t)kb & 63;
       if (l == j) mine = pos;
       if (j == 63) offs[kb - 63 + l] = mine;
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
       bits = maxbits - hdr > 0 ? (uint32_t)(maxbits - hdr) : 0u;
       n = 0;
       k = intprec;
-      kmin = intprec > (uint32_t)prec ? intprec - (uint32_t)prec : 0;
+      kmin = (uint32_t)__builtin_amdgcn_readfirstlane(max((int)intprec - prec, 0));
       if (!bits || k <= kmin) {  // no bit plane
         end_block();
         continue;

@@ -90,3 +90,22 @@ def series_5d(shape=(8, 1, 32, 512, 512), n_blobs=24, sigma=(3.0, 30.0), amp=(20
     flat = out.reshape(-1)
     flat[rng.choice(flat.size, size=int(flat.size * saturated), replace=False)] = 65535
     return out
+
+
+def zfp_bench_fixture(shape=(32, 64, 64)) -> np.ndarray:
+    """The f32 chunk the reference's ZFP chunk benchmarks encode and decode
+    (bench/rs/ndic-bench-cli/src/workloads/zfp.rs:58-81): at (z, y, x),
+    (4 z (nz - z) + y (ny - y) + x (nx - x) + 500 + n) / 3 in f32, n the
+    next value of a 32-bit xorshift (13, 17, 5; seed 0x9e3779b9) mod 5,
+    drawn in z, y, x order."""
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    f = (4 * z * (nz - z) + y * (ny - y) + x * (nx - x)).reshape(-1)
+    noise = np.empty(f.size, np.int64)
+    s = 0x9E3779B9
+    for i in range(f.size):
+        s ^= (s << 13) & 0xFFFFFFFF
+        s ^= s >> 17
+        s ^= (s << 5) & 0xFFFFFFFF
+        noise[i] = s % 5
+    return ((f + 500 + noise).astype(np.float32) / np.float32(3.0)).reshape(shape)

@@ -104,3 +104,19 @@ def test_bricks_match_oracle(ndic, oracle):
         got = ndic.nd_zfp_decode_brick(streams[i], fields[i].shape, "float32", br, rate=12.0)
         exp_bytes, exp_shape = oracle.zfp_decode_brick(streams[i], fields[i].shape, "f32", 12.0, br)
         assert got[0] == exp_bytes and got[1] == exp_shape, (i, br)
+
+
+def test_reference_bench_fixture(ndic, oracle):
+    """The reference's ZFP chunk benchmark fixture (f32 32x64x64): the device
+    streams have the lengths the reference's baseline records give (347,456
+    bytes lossless, 131,088 at rate 8), equal the oracle's, and decode --
+    the lossless one by walking the stream -- to the oracle's samples."""
+    import datagen
+
+    a = datagen.zfp_bench_fixture()
+    for cfg, length in (({"mode": "reversible"}, 347456), ({"mode": "fixed_rate", "rate": 8.0}, 131088)):
+        got = ndic.nd_zfp_encode(a.tobytes(), a.shape, "float32", **kw(cfg))
+        assert len(got) == length, cfg
+        check(ndic, oracle, a, "f32", cfg)
+    s = ndic.nd_zfp_encode(a.tobytes(), a.shape, "float32", mode="reversible")
+    assert ndic.nd_zfp_decode(s, a.shape, "float32", mode="reversible") == a.tobytes()

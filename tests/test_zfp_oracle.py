@@ -11,7 +11,11 @@ Pins:
   hashes too, plus that library's measured 4D coefficient order;
 - closed forms: fixed-rate stream length (crates/ndic-zfp/src/lib.rs:380-384),
   reversible identity, error bound of fixed accuracy, promotion/demotion of
-  narrow integers (crates/ndic-zfp/src/chunk.rs:309-356).
+  narrow integers (crates/ndic-zfp/src/chunk.rs:309-356);
+- the stream lengths the reference's ZFP chunk benchmarks record for their
+  f32 32x64x64 fixture: 347,456 bytes lossless, 131,088 bytes at rate 8
+  (bench/baselines/main/zfp-reversible/zfp__chunk_encode_f32_zyx_32x64x64.json:32,
+  bench/baselines/main/zfp-rate8/zfp__chunk_encode_f32_zyx_32x64x64.json:32).
 """
 
 from __future__ import annotations
@@ -171,3 +175,21 @@ def test_decode_rejects_mismatched_streams(oracle):
         oracle.zfp_decompress(s, a.shape, np.float32, {"mode": "fixed_rate", "rate": 4.0})
     with pytest.raises(oracle.OracleError):
         oracle.zfp_decompress(s + bytes(8), a.shape, np.float32, cfg)
+
+
+def test_reference_bench_fixture_stream_lengths(oracle):
+    """The reference's ZFP chunk benchmarks encode datagen.zfp_bench_fixture()
+    (bench/rs/ndic-bench-cli/src/workloads/zfp.rs:58-81) to 347,456 bytes
+    losslessly and to 131,088 bytes at 8 bits per value (their baseline
+    records' bytes_out); a lossless stream of that exact length leaves little
+    room for a wrong plane, group test or header.  The lossless stream decodes
+    to the fixture."""
+    import datagen
+
+    a = datagen.zfp_bench_fixture()
+    assert a.nbytes == 524288
+    s = oracle.zfp_encode_chunk(a.tobytes(), a.shape, "f32", {"mode": "reversible"})
+    assert len(s) == 347456
+    assert oracle.zfp_decode_chunk(s, a.shape, "f32", {"mode": "reversible"}) == a.tobytes()
+    r = oracle.zfp_encode_chunk(a.tobytes(), a.shape, "f32", {"mode": "fixed_rate", "rate": 8.0})
+    assert len(r) == 131088
