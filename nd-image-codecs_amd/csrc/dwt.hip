@@ -183,7 +183,10 @@ __global__ __launch_bounds__(256) void dwt_fwd_h(const DwtJob* __restrict__ jobs
 // and boundary rules as dwt_inv_h + dwt_inv_v, one read and one write of the
 // level instead of two of each.
 constexpr int kTile = 64;
-__global__ __launch_bounds__(256) void dwt_inv_level(const DwtJob* __restrict__ jobs, uint32_t njobs) {
+// Seven waves per SIMD (at most 72 registers a lane, no spills) keep more
+// tile loads in flight than the six of the unconstrained allocation.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void dwt_inv_level(
+    const DwtJob* __restrict__ jobs, uint32_t njobs) {
   __shared__ int32_t TLo[kTile / 2 + 1][kTile];
   __shared__ int32_t THi[kTile / 2 + 2][kTile];
   const uint64_t item = xcd_item();
@@ -648,7 +651,10 @@ __device__ __forceinline__ uint32_t zbands(uint64_t n, uint32_t levels) {
   return c;
 }
 
-__global__ __launch_bounds__(256) void dwt_lift_kernel(const DwtLiftJob* __restrict__ jobs, uint32_t njobs) {
+// Six waves per SIMD (at most 80 registers a lane) hide more of the z-row
+// reads than the five the unconstrained allocation leaves room for.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void dwt_lift_kernel(
+    const DwtLiftJob* __restrict__ jobs, uint32_t njobs) {
   __shared__ int32_t TLo[2][kFLo][kFx];
   __shared__ int32_t THi[2][kFHi][kFx];
   const uint64_t item = xcd_item();
