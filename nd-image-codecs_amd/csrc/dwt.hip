@@ -183,9 +183,9 @@ __global__ __launch_bounds__(256) void dwt_fwd_h(const DwtJob* __restrict__ jobs
 // and boundary rules as dwt_inv_h + dwt_inv_v, one read and one write of the
 // level instead of two of each.
 constexpr int kTile = 64;
-// Seven waves per SIMD (at most 72 registers a lane, no spills) keep more
+// Eight waves per SIMD (at most 64 registers a lane, no spills) keep more
 // tile loads in flight than the six of the unconstrained allocation.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void dwt_inv_level(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void dwt_inv_level(
     const DwtJob* __restrict__ jobs, uint32_t njobs) {
   __shared__ int32_t TLo[kTile / 2 + 1][kTile];
   __shared__ int32_t THi[kTile / 2 + 2][kTile];
