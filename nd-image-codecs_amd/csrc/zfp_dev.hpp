@@ -97,10 +97,6 @@ struct GlobalSrc {
   }
 };
 
-// The stream seen through a window of 64 words, one per lane, read with
-// readlane; the next 64 words are loaded as the window moves, so the serial
-// block walk of the variable-size modes waits on memory once per 512 bytes
-
 template <class S> __device__ __forceinline__ typename ZT<S>::UInt fbits(S x) {
   typename ZT<S>::UInt b;
   __builtin_memcpy(&b, &x, sizeof(b));
