@@ -181,10 +181,7 @@ struct TestWord {
 constexpr uint32_t kEntryEnds = 1u << 17;
 
 template <int D>
-__device__ __forceinline__ uint64_t test_word_slot(GlobalSrc& g, uint64_t t, uint32_t chunk) {
-  const uint64_t w = t >> 6;
-  const uint32_t sh = (uint32_t)t & 63;
-  const uint64_t x = (g.load(w) >> sh) | ((g.load(w + 1) << 1) << (63 - sh));
+__device__ __forceinline__ uint64_t test_word_slot(uint64_t x, uint32_t chunk) {  // x: the 64 bits from t on
   const uint64_t tag = (uint64_t)chunk << 32;
   if (!(x & 1)) return tag | kEntryEnds | (1u << 9);  // C = 0, A = 1
   const uint64_t closing = x & ~prefix_parity(x) & ~(1ull << 63);
@@ -198,8 +195,10 @@ __device__ __forceinline__ uint64_t test_word_slot(GlobalSrc& g, uint64_t t, uin
 
 // The ring of test-word entries in LDS: kRingSlots slots of kSlotBits
 // positions; chunk c (positions base + c kSlotBits ..) goes to slot
-// c % kRingSlots.  Producers take 64-position batches from a shared counter
-// (so all of them work on the chunks nearest the walk) and write a batch
+// c % kRingSlots.  Producers take batches of 64 lanes x kPerLane<D>
+// consecutive positions from a shared counter (so all of them work on the
+// chunks nearest the walk; a lane's positions share its three stream words)
+// and write a batch
 // only while its chunk is within kRingSlots of the chunk the walk reads
 // (`reading`); when the walk moves on, it raises the counter past the
 // batches it no longer needs.  The walk takes an entry only when its chunk
@@ -210,6 +209,10 @@ constexpr uint32_t kWalkProducers = 15;
 constexpr uint32_t kWalkThreads = 64 * (1 + kWalkProducers);
 constexpr uint32_t kRingSlots = 16, kSlotBits = 512;
 constexpr uint32_t kWalkSpins = 1u << 22;
+// positions a producer lane takes per batch: four where blocks are short
+// enough that the walk reads most positions; one for 4D blocks, whose long
+// verbatim runs the walk jumps over
+template <int D> constexpr uint32_t kPerLane = D < 4 ? 4 : 1;
 
 __device__ __forceinline__ uint32_t lds_acquire(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -266,7 +269,8 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
   if (threadIdx.x == 0) reading = next_batch = done = 0;
   __syncthreads();
 
-  constexpr uint32_t kBatches = kSlotBits / 64;  // batches per chunk
+  constexpr uint32_t kPL = kPerLane<D>;
+  constexpr uint32_t kBatches = kSlotBits / (64 * kPL);  // batches per chunk
   if (threadIdx.x >= 64) {  // a producer
     GlobalSrc g{c.src, c.src_len};
     const uint64_t nbatches = (uint64_t)nring * kBatches;
@@ -287,10 +291,18 @@ __global__ __launch_bounds__(kWalkThreads) void zfp_index_kernel(const ZfpChunk*
         __builtin_amdgcn_s_sleep(4);
       }
       if (!go) continue;
-      const uint32_t r = (b % kBatches) * 64 + l;
-      __hip_atomic_store(&ring[(ch % kRingSlots) * kSlotBits + r],
-                         test_word_slot<D>(g, base + (uint64_t)ch * kSlotBits + r, ch), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_WORKGROUP);
+      const uint32_t r = (b % kBatches) * (64 * kPL) + kPL * l;
+      const uint64_t t0 = base + (uint64_t)ch * kSlotBits + r;
+      const uint64_t w = t0 >> 6;
+      const uint32_t sh = (uint32_t)t0 & 63;
+      const uint64_t w0 = g.load(w), w1 = g.load(w + 1), w2 = kPL > 1 ? g.load(w + 2) : 0;
+      const uint64_t lo = (w0 >> sh) | ((w1 << 1) << (63 - sh));  // bits t0 .. t0 + 63
+      const uint64_t hi = (w1 >> sh) | ((w2 << 1) << (63 - sh));  // bits t0 + 64 ..
+      uint64_t* const slots = &ring[(ch % kRingSlots) * kSlotBits + r];
+#pragma unroll
+      for (uint32_t q = 0; q < kPL; ++q)
+        __hip_atomic_store(slots + q, test_word_slot<D>(q ? (lo >> q) | (hi << (64 - q)) : lo, ch), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     return;
   }
