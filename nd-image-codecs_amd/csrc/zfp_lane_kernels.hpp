@@ -495,9 +495,12 @@ template <int D, class F> __device__ __forceinline__ uint32_t lane_decode_float_
         const int emax = (int)r.get(ebits) - ebias;
         bits += lane_decode_int_block<D, Int>(r, (int)c.minbits - (int)bits, (int)c.maxbits - (int)bits,
                                               (int)c.maxprec, true, v);
+        // the lowest exponent marks an all-zero block: +0 whatever the
+        // integers (the reversible inverse cast of the ZFP algorithm)
         const F s = pow2<F>(emax - ((int)sizeof(F) * 8 - 2));
+        const bool zero = emax == -ebias;
 #pragma unroll
-        for (int j = 0; j < S; ++j) f[j] = s * (F)v[j];
+        for (int j = 0; j < S; ++j) f[j] = zero ? (F)0 : s * (F)v[j];
       }
       return bits;
     }
@@ -1210,9 +1213,11 @@ __device__ __forceinline__ uint32_t quad_decode_block(LaneReader& r, const ZfpCh
         f[j] = fromb<S>((U)x);
       }
     } else if (coded) {
+      // reversible: the lowest exponent marks an all-zero block (+0)
       const S sc = pow2<S>(emax - ((int)sizeof(S) * 8 - 2));
+      const bool zero = rev && emax == -ZT<S>::ebias;
 #pragma unroll
-      for (int j = 0; j < 64; ++j) f[j] = sc * (S)v[j];
+      for (int j = 0; j < 64; ++j) f[j] = zero ? (S)0 : sc * (S)v[j];
     } else {
 #pragma unroll
       for (int j = 0; j < 64; ++j) f[j] = 0;

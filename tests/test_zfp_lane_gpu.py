@@ -120,3 +120,37 @@ def test_reference_bench_fixture(ndic, oracle):
         check(ndic, oracle, a, "f32", cfg)
     s = ndic.nd_zfp_encode(a.tobytes(), a.shape, "float32", mode="reversible")
     assert ndic.nd_zfp_decode(s, a.shape, "float32", mode="reversible") == a.tobytes()
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3, 4])
+def test_walk_of_random_streams_matches_oracle(ndic, oracle, dims):
+    """Streams whose blocks are random bits (ones at densities 1/2, 1/10 and
+    1/50) behind a valid header, decoded by walking them: the samples equal
+    the oracle's decode of the same bits.  Sparse ones make long tokens, zero
+    runs and late last coefficients, dense ones short tokens and full
+    budgets -- the cases the walk's ring turns, last-coefficient turns and
+    general turns divide between them."""
+    rng = np.random.default_rng(40 + dims)
+    shape = {1: [300], 2: [20, 24], 3: [8, 12, 16], 4: [4, 8, 8, 8]}[dims]
+    nblocks = int(np.prod([(n + 3) // 4 for n in shape]))
+    modes = [{"mode": "reversible"}, {"mode": "fixed_precision", "precision": 20},
+             {"mode": "fixed_accuracy", "tolerance": 1e-3}]
+    for dtype in ("f32", "f64", "i32", "i64"):
+        a = np.zeros(shape, NP[dtype])
+        for cfg in modes:
+            if cfg["mode"] == "fixed_accuracy" and dtype[0] == "i":
+                continue
+            head = oracle.zfp_encode_chunk(a.tobytes(), shape, dtype, cfg)[:12]  # 96 header bits
+            # the longest stream the configuration allows (zfp_stream_maximum_size,
+            # crates/ndic-zfp/src/lib.rs:380-384): longer ones are refused
+            prec = 32 if dtype in ("f32", "i32") else 64
+            values = 4 ** dims
+            mb = 1 + {"f32": 8, "f64": 11}.get(dtype, 0) + values - 1
+            mb += (5 if prec == 32 else 6) if cfg["mode"] == "reversible" and dtype[0] == "f" else 0
+            mb += values * min(cfg.get("precision", 64), prec)
+            size = ((148 + nblocks * min(mb, 16658) + 63) // 64) * 8
+            for density in (0.5, 0.1, 0.02):
+                bits = rng.random((size - 12) * 8) < density
+                s = head + np.packbits(bits, bitorder="little").tobytes()
+                got = ndic.nd_zfp_decode(s, shape, LONG[dtype], **kw(cfg))
+                assert got == oracle.zfp_decode_chunk(s, shape, dtype, cfg), (dtype, cfg, density)
