@@ -154,3 +154,24 @@ def test_walk_of_random_streams_matches_oracle(ndic, oracle, dims):
                 s = head + np.packbits(bits, bitorder="little").tobytes()
                 got = ndic.nd_zfp_decode(s, shape, LONG[dtype], **kw(cfg))
                 assert got == oracle.zfp_decode_chunk(s, shape, dtype, cfg), (dtype, cfg, density)
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3, 4])
+def test_fixed_rate_random_streams_match_oracle(ndic, oracle, dims):
+    """Fixed-rate streams whose blocks are random bits (the header and the
+    length of a real stream): raw-bit blocks, lowest exponents, negative
+    zeros and full plane budgets decode as the oracle decodes them."""
+    rng = np.random.default_rng(50 + dims)
+    shape = {1: [300], 2: [20, 24], 3: [8, 12, 16], 4: [4, 8, 8, 8]}[dims]
+    for dtype in ("f32", "f64", "i32", "i64"):
+        a = np.zeros(shape, NP[dtype])
+        for rate in (3.25, 8.0, 16.0, 31.0):
+            cfg = {"mode": "fixed_rate", "rate": rate}
+            real = oracle.zfp_encode_chunk(a.tobytes(), shape, dtype, cfg)
+            for density in (0.5, 0.1):
+                # keep 19 bytes: the 96- or 148-bit header (a rate above 2,048
+                # bits per block takes the long mode word)
+                bits = rng.random((len(real) - 19) * 8) < density
+                s = real[:19] + np.packbits(bits, bitorder="little").tobytes()
+                got = ndic.nd_zfp_decode(s, shape, LONG[dtype], **kw(cfg))
+                assert got == oracle.zfp_decode_chunk(s, shape, dtype, cfg), (dtype, rate, density)
