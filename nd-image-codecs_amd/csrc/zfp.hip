@@ -55,23 +55,23 @@ __global__ __launch_bounds__(256) void zfp_place_kernel(const ZfpChunk* __restri
 // ---------------------------------------------------------------------------
 // The serial walk of a variable-size stream (which has no block index): where
 // each block begins, found by parsing the blocks' bits without decoding them.
-// One wavefront; every value but the window is wave-uniform.
+// One walking wavefront, every value of it but the window wave-uniform, beside
+// producer waves that evaluate the group tests at every stream position ahead
+// of it (zfp_index_kernel below).
 //
-// The stream is read through a window of 127 words: lane l holds words
-// first + l (cur) and first + 64 + l (nxt, loaded while cur is parsed).  When
-// a read reaches cur's last word the window moves on by 63 words, cur taking
-// its own last word and nxt's first 63.  The walk is one loop with one read of
-// 64 stream bits at its head, each turn consuming at most those 64 bits, so
-// the window registers change in one place only.
+// The walking wave reads the stream through a window of 127 words: lane l
+// holds words first + l (cur) and first + 64 + l (nxt, loaded while cur is
+// parsed).  When a read reaches cur's last word the window moves on by 63
+// words, cur taking its own last word and nxt's first 63; reads only move
+// forward, 64 stream bits at a time (WalkWindow::peek).
 //
 // A bit plane's group tests are parsed a word at a time rather than a test at
 // a time: a run of significance tokens "1 0^z 1" ends at the first token whose
 // closing one is followed by a zero, which a prefix parity of the ones finds.
-// Where that cannot decide (a token longer than the word, the last
-// coefficient, a bit budget below 64) a turn parses one test.  The walk
-// consumes exactly the bits the block decoder does (decode_float_block /
-// decode_int_block / decode_ints with Parse), budgets and minimum sizes
-// included.
+// Where that cannot decide (a token longer than the word, a bit budget below
+// 64) a turn parses one test.  The walk consumes exactly the bits the block
+// decoder does (the oracle's decode_float_block / decode_int_block /
+// decode_ints), budgets and minimum sizes included.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t i) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)i);
