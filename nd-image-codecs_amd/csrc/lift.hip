@@ -367,8 +367,12 @@ __device__ __forceinline__ bool store_rows(D* p, uint64_t inner, const int32_t (
 // stored once, narrowed to the dtype on the last step.  No LDS, no barriers:
 // occupancy is set by registers alone and each lane has up to N loads in
 // flight.
+// Register limits (amdgpu_waves_per_eu): the 64-row form at four waves per
+// SIMD (128 registers; three at the unconstrained 133) and the 32-row form at
+// eight (64; seven at 68) keep more rows in flight than they spill (32 bytes).
 template <uint32_t N>
-__global__ __launch_bounds__(256) void lift_col_kernel(const LiftJob* __restrict__ jobs, uint32_t njobs) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N >= 64 ? 4 : N >= 32 ? 8 : 1))) void
+lift_col_kernel(const LiftJob* __restrict__ jobs, uint32_t njobs) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint64_t item = (blockIdx.x + (uint64_t)blockIdx.y * gridDim.x) * 4 + wave;
